@@ -1,0 +1,190 @@
+/*
+ * ssim_hip.h -- C ABI of the MI355X-native FEM soft-body time step (libssim_hip.so).
+ *
+ * One context = one tetrahedral mesh with its state, living on the current HIP device.  Every entry point
+ * returns an ssim_status_t; on failure ssim_last_error() names the cause.  No C++ exception crosses the ABI.
+ * Arrays are plain host pointers with the reference's row-major layouts (vertices [nv,3] float64, cells
+ * [nt,4] int32, vertex arrays [nv,3] flattened to 3·nv doubles, dof index 3v+c), except where an entry point
+ * says "device".  All GPU work of a context runs on its stream (ssim_set_stream; default: the null stream).
+ *
+ * Each declaration cites the reference interface it replaces (paths relative to the reference tree,
+ * Adversarr/ssim).
+ */
+#ifndef SSIM_HIP_H
+#define SSIM_HIP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ssim_ctx ssim_ctx;
+
+typedef enum {
+  SSIM_OK = 0,
+  SSIM_ERR_INVALID_ARGUMENT = 1, /* the reference throws std::invalid_argument here */
+  SSIM_ERR_HIP = 2,              /* a HIP runtime call failed (out of memory, no device, ...) */
+  SSIM_ERR_STATE = 3,            /* call order violated, e.g. stepping before reset */
+  SSIM_ERR_NUMERIC = 4           /* NaN/Inf in the state, or a solver breakdown */
+} ssim_status_t;
+
+typedef enum { SSIM_STEPPER_NEWTON = 0, SSIM_STEPPER_LBFGS = 1 } ssim_stepper_t;
+
+/* vertex-shaped arrays addressable through ssim_get_array / ssim_set_array / ssim_device_array
+ * (accessors of finite_elements/time_step.hpp:275-311) */
+typedef enum {
+  SSIM_ARRAY_DEFORMATION = 0,   /* u at the current step         (deformation_)   */
+  SSIM_ARRAY_PREV_DEFORM = 1,   /* u at the previous step        (prev_deform_)   */
+  SSIM_ARRAY_INERTIA = 2,       /* predictor û                   (inertia_deform_)*/
+  SSIM_ARRAY_NEXT_DEFORM = 3,   /* solver iterate / next u       (next_deform_)   */
+  SSIM_ARRAY_VELOCITY = 4,      /*                               (velocity_)      */
+  SSIM_ARRAY_EXT_ACCEL = 5,     /* external acceleration         (ext_accel_)     */
+  SSIM_ARRAY_FORCES = 6,        /* gradient of the incremental potential (forces_) */
+  SSIM_ARRAY_DBC_VALUES = 7,    /* Dirichlet values              (dbc_values_)    */
+  SSIM_ARRAY_COUNT = 8
+} ssim_array_t;
+
+/* Solver parameters.  Reference: stepper/backward_euler.hpp:20-22 (Newton: tol = threshold·mass, Armijo 0.3)
+ * and the benchmark configuration (≤10 Newton iterations, Jacobi-PCG rtol 1e-8). */
+typedef struct {
+  int max_newton;      /* default 10 */
+  double pcg_rtol;     /* default 1e-8: stop when ‖r‖₂ ≤ rtol·‖b‖₂ */
+  int pcg_maxit;       /* default 10000 */
+  double ls_c;         /* Armijo constant, default 0.3 */
+  double ls_beta;      /* backtracking factor, default 0.5 */
+  int ls_max;          /* maximum trial steps per line search, default 20 */
+  int lbfgs_m;         /* L-BFGS history, default 8 */
+  int lbfgs_maxit;     /* default 1000 */
+  double lbfgs_c;      /* Armijo constant for L-BFGS, default 1e-4 */
+  int lbfgs_ls_max;    /* default 30 */
+  int fp32_elements;   /* 1: element energy/stress/Hessian in FP32 (reductions and PCG stay FP64) */
+} ssim_solver_params;
+
+/* Counters of the last compute_step (mathprim optim_result of stepper/backward_euler.hpp:28-32, plus
+ * per-step work counts). */
+typedef struct {
+  int converged;
+  int newton_iterations; /* Newton (or L-BFGS) iterations */
+  int pcg_iterations;    /* total PCG iterations (SpMVs) */
+  int energy_evals;      /* energy + gradient evaluations (line-search trials included) */
+  int hessian_evals;
+  int ls_failures;       /* line searches that exhausted ls_max trials */
+  double grad_norm;      /* ‖forces‖₂ after the step */
+  double energy;         /* incremental potential at the result */
+} ssim_step_stats;
+
+const char* ssim_last_error(void);
+
+/* basic_time_step(mesh, dt, E, ν, ρ) + reset(): finite_elements/time_step.hpp:260-265, :368-494.
+ * Validates E ≥ 0, 0 < ν < 0.5, ρ ≥ 0, dt > 0 and every cell index. */
+ssim_status_t ssim_create(ssim_ctx** out, const double* vertices, int nv, const int* cells, int nt,
+                          double time_step, double youngs, double poisson, double density);
+ssim_status_t ssim_destroy(ssim_ctx* ctx);
+/* all later GPU work of ctx goes to this hipStream_t (passed as void*) */
+ssim_status_t ssim_set_stream(ssim_ctx* ctx, void* hip_stream);
+
+/* setters: finite_elements/time_step.hpp:316-347 (same validation; take effect at the next reset) */
+ssim_status_t ssim_set_time_step(ssim_ctx* ctx, double dt);
+ssim_status_t ssim_set_youngs(ssim_ctx* ctx, double youngs);
+ssim_status_t ssim_set_poisson(ssim_ctx* ctx, double poisson);
+ssim_status_t ssim_set_density(ssim_ctx* ctx, double density);
+ssim_status_t ssim_set_damping(ssim_ctx* ctx, double damping);
+/* set_threshold (time_step.hpp:343-347) / set_rtol (bindings/src/fem_host.cpp:146); like the reference, the
+ * absolute tolerance rtol·mass is recomputed at the next reset */
+ssim_status_t ssim_set_threshold(ssim_ctx* ctx, double rtol);
+/* per-tet Lamé parameters (extension: the reference's per-call (λ, μ) overloads,
+ * elast/basic_elast.hpp:187-194); NULL restores the uniform (E, ν) material */
+ssim_status_t ssim_set_material_per_tet(ssim_ctx* ctx, const double* lambda, const double* mu);
+ssim_status_t ssim_set_solver_params(ssim_ctx* ctx, const ssim_solver_params* params);
+ssim_status_t ssim_get_solver_params(const ssim_ctx* ctx, ssim_solver_params* params);
+/* Newton (time_step_solver_backward_euler) or L-BFGS (time_step_solver_lbfgs): stepper/backward_euler.hpp */
+ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper);
+
+/* mark_dirichlet_batched / mark_general_batched: bindings/src/fem_host.cpp:123-144 (targets [count,3]) */
+ssim_status_t ssim_mark_dirichlet(ssim_ctx* ctx, const int* verts, int count, const double* targets);
+ssim_status_t ssim_mark_general(ssim_ctx* ctx, const int* verts, int count);
+/* add_ext_force_dof: finite_elements/time_step.hpp:550-557 (add_gravity = three calls, fem_host.cpp:148-152) */
+ssim_status_t ssim_add_ext_force_dof(ssim_ctx* ctx, int dof_idx, double value);
+
+/* reset(solver): recompute rest data, sparsity, assembly maps and thresholds (time_step.hpp:368-494) */
+ssim_status_t ssim_reset(ssim_ctx* ctx);
+/* prepare_step / compute_step / step_next: finite_elements/time_step.hpp:499-543 */
+ssim_status_t ssim_prepare_step(ssim_ctx* ctx);
+ssim_status_t ssim_compute_step(ssim_ctx* ctx, double* residual_out);
+ssim_status_t ssim_step_next(ssim_ctx* ctx);
+/* step(): prepare + compute + step_next (bindings/src/fem_host.cpp:102-111) */
+ssim_status_t ssim_step(ssim_ctx* ctx, double* residual_out);
+/* restore(): u_prev, u ← DBC values; v = (u − u_prev)/dt (time_step.hpp:560-569) */
+ssim_status_t ssim_restore(ssim_ctx* ctx);
+ssim_status_t ssim_get_step_stats(const ssim_ctx* ctx, ssim_step_stats* stats);
+
+/* update_energy_and_gradients at next_deform: time_step.hpp:577-624 (forces_ ← g) */
+ssim_status_t ssim_update_energy_and_gradients(ssim_ctx* ctx, double* energy_out);
+/* update_hessian(make_spsd, filter_dbc) at next_deform: time_step.hpp:626-658 */
+ssim_status_t ssim_update_hessian(ssim_ctx* ctx, int make_spsd, int filter_dbc);
+/* ‖forces‖₂ < threshold·mass: time_step.hpp:660-664 */
+ssim_status_t ssim_check_convergence(ssim_ctx* ctx, int* converged_out);
+
+/* host copies of the vertex arrays (3·nv doubles) */
+ssim_status_t ssim_get_array(ssim_ctx* ctx, ssim_array_t which, double* host_out);
+ssim_status_t ssim_set_array(ssim_ctx* ctx, ssim_array_t which, const double* host_in);
+/* device pointer of a vertex array (3·nv doubles, owned by ctx, valid until destroy/reset) */
+ssim_status_t ssim_device_array(ssim_ctx* ctx, ssim_array_t which, double** device_ptr);
+/* Dirichlet flags per dof (dof_type_, 0 = general, 1 = dirichlet; boundary.hpp:8-11) */
+ssim_status_t ssim_get_dof_type(const ssim_ctx* ctx, unsigned char* host_out);
+
+ssim_status_t ssim_num_vertices(const ssim_ctx* ctx, int* nv);
+ssim_status_t ssim_num_cells(const ssim_ctx* ctx, int* nt);
+ssim_status_t ssim_total_mass(const ssim_ctx* ctx, double* mass);
+ssim_status_t ssim_grad_threshold_abs(const ssim_ctx* ctx, double* tol);
+
+/* scalar CSR with the full 3x3 block pattern, rows and columns ascending (the CSR of
+ * finite_elements/global_composer.hpp:80-92 and time_step.hpp:456-461).  Any output may be NULL.
+ * hessian(): the system matrix of the last update_hessian / Newton iteration (sysmatrix_, fem_host.cpp:169-171);
+ * mass(): mass_matrix_ (fem_host.cpp:167). */
+ssim_status_t ssim_csr_nnz(const ssim_ctx* ctx, long long* nnz);
+ssim_status_t ssim_export_hessian_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
+ssim_status_t ssim_export_mass_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
+
+/* Element-level outputs at deformation u (host, 3·nv doubles): Ψ [nt], P [nt,3,3], H [nt,9,9] (SPD-projected
+ * with floor 1e-3 when make_spsd; wrapped_pfpx, time_step.hpp:41-59) and the local stiffness blocks
+ * K [nt,12,12] = pfpxᵀ·H·pfpx.  Outputs may be NULL.  stable_neohookean.hpp:23-91. */
+ssim_status_t ssim_eval_elements(ssim_ctx* ctx, const double* u, int make_spsd, double* psi, double* P,
+                                 double* H, double* K);
+/* Jacobi-PCG on the current system matrix: solve H x = b (host arrays, 3·nv); mathprim cg +
+ * diagonal_preconditioner (tests/fem_cu/main.cu:52-54) */
+ssim_status_t ssim_pcg_solve(ssim_ctx* ctx, const double* b, double* x, double rtol, int maxit, int* iters_out);
+/* y = H x with the current system matrix (host arrays) */
+ssim_status_t ssim_spmv(ssim_ctx* ctx, const double* x, double* y);
+
+/* ---- instrumentation (the reference times whole steps with std::chrono, bindings/src/fem_host.cpp:102-111;
+ *      here kernels are timed on the GPU with hipEvents) ---- */
+typedef enum {
+  SSIM_KERNEL_ELEM_GRAD = 0,   /* per-tet energy + gradient contributions            */
+  SSIM_KERNEL_GRAD_REDUCE = 1, /* per-vertex segmented reduce of the gradient        */
+  SSIM_KERNEL_ELEM_HESS = 2,   /* per-tet SNH Hessian, SPD projection, 3x3 blocks    */
+  SSIM_KERNEL_HESS_REDUCE = 3, /* per-block segmented reduce into the blocked ELL    */
+  SSIM_KERNEL_PCG_SPMV = 4,    /* q = A p with p·q (one PCG iteration's SpMV)        */
+  SSIM_KERNEL_PCG_UPDATE = 5,  /* x, r, z updates with r·z and r·r                   */
+  SSIM_KERNEL_PCG_P = 6,       /* p = z + β p                                        */
+  SSIM_KERNEL_COUNT = 7
+} ssim_kernel_t;
+
+/* Enable (1) or disable (0) sampled kernel timing and clear its accumulators.  When enabled, one launch of each
+ * kernel per call site and per PCG chunk is bracketed by two hipEvents on the context's stream. */
+ssim_status_t ssim_set_kernel_timing(ssim_ctx* ctx, int enabled);
+/* average duration in milliseconds and number of sampled launches of one kernel */
+ssim_status_t ssim_get_kernel_timing(const ssim_ctx* ctx, ssim_kernel_t kernel, double* avg_ms, long long* samples);
+
+/* sizes of the device layout (for byte accounting) */
+typedef struct {
+  int nv, nt;
+  long long nonzero_blocks;  /* 3x3 blocks of the sparsity pattern (nnz / 9) */
+  long long padded_slots;    /* blocked-ELL slots including padding */
+  int upper_blocks;          /* blocks with row ≤ column */
+  int slices;                /* 64-row slices */
+} ssim_layout_info;
+ssim_status_t ssim_get_layout_info(const ssim_ctx* ctx, ssim_layout_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,289 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/ssim_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker.  The product (ssim_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libssim_oracle.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_U8 = ctypes.POINTER(ctypes.c_ubyte)
+_V = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "orc_mu": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+            "orc_lambda": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+            "orc_snh_energy": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double]),
+            "orc_snh_stress": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, _D]),
+            "orc_snh_hessian": (None, [_D, ctypes.c_double, ctypes.c_double, _D]),
+            "orc_sym_eig": (ctypes.c_int, [ctypes.c_int, _D, _D, _D]),
+            "orc_spd_project9": (None, [_D, ctypes.c_double, _D]),
+            "orc_local_force": (None, [_D, _D, _D]),
+            "orc_local_stiffness": (None, [_D, _D, _D]),
+            "orc_pfpx": (None, [_D, _D]),
+            "orc_ts_create": (_V, [_D, ctypes.c_int, _I, ctypes.c_int, ctypes.c_double, _D, _D, ctypes.c_double]),
+            "orc_ts_destroy": (None, [_V]),
+            "orc_ts_nnz": (ctypes.c_int, [_V]),
+            "orc_ts_csr_pattern": (None, [_V, _I, _I]),
+            "orc_ts_mass_values": (None, [_V, _D]),
+            "orc_ts_total_mass": (ctypes.c_double, [_V]),
+            "orc_ts_rest_data": (None, [_V, _D, _D]),
+            "orc_ts_set_dirichlet": (None, [_V, _U8, _D]),
+            "orc_ts_set_ext_accel": (None, [_V, _D]),
+            "orc_ts_set_threshold": (None, [_V, ctypes.c_double]),
+            "orc_ts_set_state": (None, [_V, _D, _D, _D]),
+            "orc_ts_get_state": (None, [_V, _D, _D, _D]),
+            "orc_ts_set_vectors": (None, [_V, _D, _D]),
+            "orc_ts_get_vectors": (None, [_V, _D, _D, _D]),
+            "orc_ts_set_solver": (None, [_V, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int]),
+            "orc_ts_set_lbfgs": (None, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int]),
+            "orc_ts_elements": (None, [_V, _D, _D, _D, _D, _D, _D, ctypes.c_int]),
+            "orc_ts_energy_grad": (ctypes.c_double, [_V, _D, _D]),
+            "orc_ts_hessian": (None, [_V, _D, ctypes.c_int, ctypes.c_int, _D]),
+            "orc_pcg": (ctypes.c_int, [ctypes.c_int, _I, _I, _D, _D, _D, ctypes.c_double, ctypes.c_int, _D]),
+            "orc_ts_prepare_step": (None, [_V]),
+            "orc_ts_compute_step": (ctypes.c_double, [_V, _D]),
+            "orc_ts_compute_step_lbfgs": (ctypes.c_double, [_V, _D]),
+            "orc_ts_step_next": (None, [_V]),
+            "orc_ts_rest_hessian_diag": (None, [_V, _D]),
+            "orc_ts_spmv": (None, [_V, _D, _D, _D]),
+            "orc_num_threads": (ctypes.c_int, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---- element level ----
+def snh_energy(F, lam, mu):
+    F = _f64(F).reshape(9)
+    return lib().orc_snh_energy(_dp(F), lam, mu)
+
+
+def snh_stress(F, lam, mu):
+    F = _f64(F).reshape(9)
+    P = np.zeros(9)
+    e = lib().orc_snh_stress(_dp(F), lam, mu, _dp(P))
+    return e, P.reshape(3, 3)
+
+
+def snh_hessian(F, lam, mu):
+    F = _f64(F).reshape(9)
+    H = np.zeros(81)
+    lib().orc_snh_hessian(_dp(F), lam, mu, _dp(H))
+    return H.reshape(9, 9)
+
+
+def sym_eig(A):
+    A = _f64(A)
+    n = A.shape[0]
+    w = np.zeros(n)
+    Q = np.zeros((n, n))
+    lib().orc_sym_eig(n, _dp(A.reshape(-1)), _dp(w), _dp(Q.reshape(-1)))
+    return w, Q
+
+
+def spd_project9(H, floor=1e-3):
+    H = _f64(H).reshape(81)
+    Hp = np.zeros(81)
+    lib().orc_spd_project9(_dp(H), floor, _dp(Hp))
+    return Hp.reshape(9, 9)
+
+
+def local_force(P, B):
+    f = np.zeros(12)
+    lib().orc_local_force(_dp(_f64(P).reshape(9)), _dp(_f64(B).reshape(9)), _dp(f))
+    return f.reshape(4, 3)
+
+
+def local_stiffness(H, B):
+    K = np.zeros(144)
+    lib().orc_local_stiffness(_dp(_f64(H).reshape(81)), _dp(_f64(B).reshape(9)), _dp(K))
+    return K.reshape(12, 12)
+
+
+def pcg(rowptr, colidx, vals, b, rtol=1e-8, maxit=10000):
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+    colidx = np.ascontiguousarray(colidx, dtype=np.int32)
+    vals = _f64(vals)
+    b = _f64(b)
+    x = np.zeros_like(b)
+    rel = ctypes.c_double(0)
+    it = lib().orc_pcg(b.shape[0], _ip(rowptr), _ip(colidx), _dp(vals), _dp(b), _dp(x), rtol, maxit,
+                       ctypes.byref(rel))
+    return x, it, rel.value
+
+
+class TimeStep:
+    """Oracle mirror of ssim::fem::basic_time_step (finite_elements/time_step.hpp:171-715)."""
+
+    def __init__(self, X, cells, dt, lam, mu, rho):
+        self.X = _f64(X).reshape(-1, 3)
+        self.cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+        self.nv, self.nt = self.X.shape[0], self.cells.shape[0]
+        self.n = 3 * self.nv
+        lam = np.broadcast_to(_f64(lam), (self.nt,)).copy()
+        mu = np.broadcast_to(_f64(mu), (self.nt,)).copy()
+        self._h = lib().orc_ts_create(_dp(self.X), self.nv, _ip(self.cells), self.nt, dt, _dp(lam), _dp(mu), rho)
+        self.nnz = lib().orc_ts_nnz(self._h)
+
+    @classmethod
+    def from_case(cls, case):
+        ts = cls(case.X, case.cells, case.dt, case.lam, case.mu, case.rho)
+        ts.set_dirichlet(case.dof_type, case.dbc)
+        ts.set_ext_accel(case.accel)
+        ts.set_state(case.u0, np.zeros_like(case.u0), case.v0)
+        return ts
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_ts_destroy(self._h)
+            self._h = None
+
+    def csr_pattern(self):
+        rowptr = np.zeros(self.n + 1, dtype=np.int32)
+        colidx = np.zeros(self.nnz, dtype=np.int32)
+        lib().orc_ts_csr_pattern(self._h, _ip(rowptr), _ip(colidx))
+        return rowptr, colidx
+
+    def mass_values(self):
+        v = np.zeros(self.nnz)
+        lib().orc_ts_mass_values(self._h, _dp(v))
+        return v
+
+    def total_mass(self):
+        return lib().orc_ts_total_mass(self._h)
+
+    def rest_data(self):
+        B = np.zeros((self.nt, 3, 3))
+        V = np.zeros(self.nt)
+        lib().orc_ts_rest_data(self._h, _dp(B), _dp(V))
+        return B, V
+
+    def set_dirichlet(self, dof_type, dbc):
+        d = np.ascontiguousarray(dof_type, dtype=np.uint8).reshape(-1)
+        lib().orc_ts_set_dirichlet(self._h, d.ctypes.data_as(_U8), _dp(_f64(dbc).reshape(-1)))
+
+    def set_ext_accel(self, a):
+        lib().orc_ts_set_ext_accel(self._h, _dp(_f64(a).reshape(-1)))
+
+    def set_threshold(self, rtol):
+        lib().orc_ts_set_threshold(self._h, rtol)
+
+    def set_state(self, u=None, u_prev=None, vel=None):
+        conv = lambda a: None if a is None else _f64(a).reshape(-1)
+        u, u_prev, vel = conv(u), conv(u_prev), conv(vel)
+        lib().orc_ts_set_state(self._h, _dp(u), _dp(u_prev), _dp(vel))
+
+    def get_state(self):
+        u, up, v = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        lib().orc_ts_get_state(self._h, _dp(u), _dp(up), _dp(v))
+        return u, up, v
+
+    def set_vectors(self, u_next=None, u_hat=None):
+        conv = lambda a: None if a is None else _f64(a).reshape(-1)
+        u_next, u_hat = conv(u_next), conv(u_hat)
+        lib().orc_ts_set_vectors(self._h, _dp(u_next), _dp(u_hat))
+
+    def get_vectors(self):
+        a, b, c = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        lib().orc_ts_get_vectors(self._h, _dp(a), _dp(b), _dp(c))
+        return a, b, c
+
+    def set_solver(self, max_newton=10, pcg_rtol=1e-8, pcg_maxit=10000, ls_c=0.3, ls_beta=0.5, ls_max=20):
+        lib().orc_ts_set_solver(self._h, max_newton, pcg_rtol, pcg_maxit, ls_c, ls_beta, ls_max)
+
+    def set_lbfgs(self, m=8, max_iter=1000, ls_c=1e-4, ls_beta=0.5, ls_max=30):
+        lib().orc_ts_set_lbfgs(self._h, m, max_iter, ls_c, ls_beta, ls_max)
+
+    def elements(self, u, make_spsd=True, want_K=False):
+        u = _f64(u).reshape(-1)
+        nt = self.nt
+        psi, P = np.zeros(nt), np.zeros((nt, 3, 3))
+        H, Hp = np.zeros((nt, 9, 9)), np.zeros((nt, 9, 9))
+        K = np.zeros((nt, 12, 12)) if want_K else None
+        lib().orc_ts_elements(self._h, _dp(u), _dp(psi), _dp(P), _dp(H), _dp(Hp), _dp(K), int(make_spsd))
+        return dict(psi=psi, P=P, H=H, Hp=Hp, K=K)
+
+    def energy_grad(self, u):
+        g = np.zeros(self.n)
+        E = lib().orc_ts_energy_grad(self._h, _dp(_f64(u).reshape(-1)), _dp(g))
+        return E, g
+
+    def hessian(self, u, make_spsd=True, filter_dbc=True):
+        vals = np.zeros(self.nnz)
+        lib().orc_ts_hessian(self._h, _dp(_f64(u).reshape(-1)), int(make_spsd), int(filter_dbc), _dp(vals))
+        return vals
+
+    def hessian_csr(self, u, make_spsd=True, filter_dbc=True):
+        import scipy.sparse as sp
+
+        rowptr, colidx = self.csr_pattern()
+        return sp.csr_matrix((self.hessian(u, make_spsd, filter_dbc), colidx, rowptr), shape=(self.n, self.n))
+
+    def spmv(self, vals, x):
+        y = np.zeros(self.n)
+        lib().orc_ts_spmv(self._h, _dp(_f64(vals)), _dp(_f64(x)), _dp(y))
+        return y
+
+    def prepare_step(self):
+        lib().orc_ts_prepare_step(self._h)
+
+    def compute_step(self, method="newton"):
+        st = np.zeros(6)
+        fn = lib().orc_ts_compute_step if method == "newton" else lib().orc_ts_compute_step_lbfgs
+        gn = fn(self._h, _dp(st))
+        return gn, dict(newton=int(st[0]), pcg=int(st[1]), evals=int(st[2]), converged=bool(st[3]),
+                        grad_norm=float(st[4]), ls_fail=int(st[5]))
+
+    def step_next(self):
+        lib().orc_ts_step_next(self._h)
+
+    def step(self, method="newton"):
+        self.prepare_step()
+        out = self.compute_step(method)
+        self.step_next()
+        return out
+
+    def rest_hessian_diag(self):
+        d = np.zeros(self.n)
+        lib().orc_ts_rest_hessian_diag(self._h, _dp(d))
+        return d

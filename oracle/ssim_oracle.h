@@ -1,0 +1,83 @@
+/*
+ * ssim_oracle.h -- CPU FP64 restatement of the reference FEM soft-body time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (ssim_amd/, include/) may link, load or call this
+ * code.  It is the checker that the HIP path is compared against: tests/ and bench.py's cpu_baseline
+ * leg load it through ctypes (oracle/oracle.py).
+ *
+ * Every routine restates one piece of the reference (citations are relative to the reference tree,
+ * Adversarr/ssim).  The reference cannot be built here (its mathprim dependency is not vendored and is
+ * fetched from the network by get-mathprim.sh), so the solver routines (PCG, Newton line search, L-BFGS)
+ * restate the textbook algorithms with the parameters the benchmark configurations name; see the header of
+ * ssim_oracle.c for which functions are pinned by which check.
+ */
+#ifndef SSIM_ORACLE_H
+#define SSIM_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- element level (row-major 3x3 matrices, 9x9 Hessian index 3a+c <-> F[a][c]) ---- */
+double orc_mu(double E, double nu);
+double orc_lambda(double E, double nu);
+double orc_snh_energy(const double F[9], double lam, double mu);
+double orc_snh_stress(const double F[9], double lam, double mu, double P[9]);
+void orc_snh_hessian(const double F[9], double lam, double mu, double H[81]);
+int orc_sym_eig(int n, const double* A, double* w, double* Q); /* Q columns are eigenvectors */
+void orc_spd_project9(const double H[81], double floor_, double Hp[81]);
+void orc_dminv(const double x0[3], const double x1[3], const double x2[3], const double x3[3], double B[9],
+               double* V);
+void orc_def_grad(const double x0[3], const double x1[3], const double x2[3], const double x3[3],
+                  const double B[9], double F[9]);
+void orc_pfpx(const double B[9], double pfpx[108]);
+void orc_local_force(const double P[9], const double B[9], double f[12]);
+void orc_local_stiffness(const double H[81], const double B[9], double K[144]);
+
+/* ---- problem level: mirrors ssim::fem::basic_time_step (time_step.hpp:171-715) ---- */
+typedef struct orc_ts orc_ts;
+
+orc_ts* orc_ts_create(const double* X, int nv, const int* cells, int nt, double dt, const double* lam,
+                      const double* mu, double rho);
+void orc_ts_destroy(orc_ts* ts);
+int orc_ts_nnz(const orc_ts* ts);
+void orc_ts_csr_pattern(const orc_ts* ts, int* rowptr, int* colidx);
+void orc_ts_mass_values(const orc_ts* ts, double* vals);
+double orc_ts_total_mass(const orc_ts* ts);
+void orc_ts_rest_data(const orc_ts* ts, double* B, double* V);
+
+void orc_ts_set_dirichlet(orc_ts* ts, const unsigned char* dof_type, const double* dbc_values);
+void orc_ts_set_ext_accel(orc_ts* ts, const double* accel);
+void orc_ts_set_threshold(orc_ts* ts, double rtol);
+void orc_ts_set_state(orc_ts* ts, const double* u, const double* u_prev, const double* vel);
+void orc_ts_get_state(const orc_ts* ts, double* u, double* u_prev, double* vel);
+void orc_ts_set_vectors(orc_ts* ts, const double* u_next, const double* u_hat);
+void orc_ts_get_vectors(const orc_ts* ts, double* u_next, double* u_hat, double* forces);
+void orc_ts_set_solver(orc_ts* ts, int max_newton, double pcg_rtol, int pcg_maxit, double ls_c,
+                       double ls_beta, int ls_max);
+void orc_ts_set_lbfgs(orc_ts* ts, int m, int max_iter, double ls_c, double ls_beta, int ls_max);
+
+/* element outputs at deformation u (any NULL output is skipped) */
+void orc_ts_elements(const orc_ts* ts, const double* u, double* psi, double* P, double* H, double* Hp,
+                     double* K, int make_spsd);
+double orc_ts_energy_grad(orc_ts* ts, const double* u, double* g);
+void orc_ts_hessian(orc_ts* ts, const double* u, int make_spsd, int filter_dbc, double* vals);
+
+int orc_pcg(int n, const int* rowptr, const int* colidx, const double* vals, const double* b, double* x,
+            double rtol, int maxit, double* out_rel_resid);
+
+void orc_ts_prepare_step(orc_ts* ts);
+/* stats: [0]=newton iterations, [1]=pcg iterations total, [2]=energy evaluations, [3]=converged flag,
+ *        [4]=final grad norm, [5]=line-search failures */
+double orc_ts_compute_step(orc_ts* ts, double* stats);
+double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats);
+void orc_ts_step_next(orc_ts* ts);
+void orc_ts_rest_hessian_diag(orc_ts* ts, double* diag);
+
+/* timing helpers for the CPU baseline: one PCG iteration-equivalent (SpMV + vector work) etc. */
+void orc_ts_spmv(const orc_ts* ts, const double* vals, const double* x, double* y);
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

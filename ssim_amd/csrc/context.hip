@@ -1,0 +1,1152 @@
+// context.hip -- host side of libssim_hip.so: context state, setup of the assembly maps and the blocked-ELL
+// matrix layout, the Newton / L-BFGS drivers and the extern "C" ABI declared in include/ssim_hip.h.
+//
+// The host drives control flow only (Newton iterations, line-search decisions, PCG chunking); every
+// floating-point operation of a step runs in the HIP kernels of kernels.hip.  PCG scalars stay on the device
+// and the host polls a done-flag once per chunk of iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ssim_hip.h"
+#include "internal.h"
+
+using namespace ssim_hip;
+
+namespace {
+
+thread_local std::string g_err = "";
+
+struct Fail {
+  ssim_status_t code;
+};
+
+[[noreturn]] void fail(ssim_status_t code, const std::string& msg) {
+  g_err = msg;
+  throw Fail{code};
+}
+
+#define HIPCHK(x)                                                                               \
+  do {                                                                                          \
+    hipError_t e_ = (x);                                                                        \
+    if (e_ != hipSuccess) fail(SSIM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+  void alloc(size_t n) {
+    if (n == count && p) return;
+    release();
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    HIPCHK(hipMalloc(&p, bytes));
+    count = n;
+  }
+  void upload(const T* h, size_t n, hipStream_t s) {
+    alloc(n);
+    if (n) HIPCHK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+};
+
+}  // namespace
+
+struct ssim_ctx {
+  int nv = 0, nt = 0, n = 0;
+  double dt = 1e-3, youngs = 1e6, poisson = 0.33, density = 1e3, damping = 0.0;
+  double rtol = 1e-2, tol_abs = 1e-2, total_mass = 0.0;
+  double rho_reset = 0.0;  // density the mass terms use: captured at reset, as the reference's mass matrix
+  bool per_tet = false;
+  bool has_reset = false;
+  ssim_stepper_t stepper = SSIM_STEPPER_NEWTON;
+  ssim_solver_params prm{};
+  ssim_step_stats stats{};
+  hipStream_t stream = nullptr;
+
+  std::vector<double> hX;
+  std::vector<int> hcells;
+  std::vector<double> hlam, hmu;  // per-tet material when per_tet
+
+  // host topology (block adjacency, rows and columns ascending)
+  std::vector<int> brow, bcol, kdiag;
+  std::vector<int> hwidth, hcoloff;
+  long long ncolslots = 0;
+  int nslices = 0, nub = 0;
+
+  // device: mesh and rest data
+  DBuf<double> X, B, V, lam, mu;
+  DBuf<int4> cells;
+  // device: vertex arrays
+  DBuf<double> arr[SSIM_ARRAY_COUNT];
+  DBuf<unsigned char> dof;
+  DBuf<double> utry, gtry, dir, r, z, p, q, tmp, dinv, hrest;
+  // device: assembly maps
+  DBuf<int> gperm, gseg, hperm, hseg, width, coloff, col;
+  DBuf<int2> ub_rc, ub_pos;
+  DBuf<double> gcontrib, hcontrib, val, massval;
+  // device: scalars
+  DBuf<PcgScalars> pcg;
+  DBuf<EvalScalars> ev;
+  DBuf<double> partials, lb_S, lb_Y, lb_scal;
+  double* pin = nullptr;  // pinned host mirror for scalar read-backs
+
+  // sampled kernel timing: an event pair per kernel, armed at one launch and harvested at the next sync
+  struct KTimer {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double sum_ms = 0;
+    long long cnt = 0;
+    bool pending = false;
+  };
+  bool timing = false;
+  KTimer kt[SSIM_KERNEL_COUNT];
+
+  ~ssim_ctx() {
+    if (pin) (void)hipHostFree(pin);
+    for (auto& t : kt)
+      for (auto& e : t.ev)
+        if (e) (void)hipEventDestroy(e);
+  }
+
+  MeshView mesh() const {
+    MeshView m;
+    m.X = X.p;
+    m.cells = cells.p;
+    m.B = B.p;
+    m.V = V.p;
+    m.lam = lam.p;
+    m.mu = mu.p;
+    m.nv = nv;
+    m.nt = nt;
+    m.rho = rho_reset;
+    return m;
+  }
+  BellView bell(const double* values) const {
+    BellView A;
+    A.width = width.p;
+    A.col_off = coloff.p;
+    A.col = col.p;
+    A.val = values;
+    A.nrows = nv;
+    A.nslices = nslices;
+    return A;
+  }
+  double* a(ssim_array_t w) { return arr[w].p; }
+};
+
+namespace {
+
+void default_params(ssim_solver_params& p) {
+  p.max_newton = 10;
+  p.pcg_rtol = 1e-8;
+  p.pcg_maxit = 10000;
+  p.ls_c = 0.3;
+  p.ls_beta = 0.5;
+  p.ls_max = 20;
+  p.lbfgs_m = 8;
+  p.lbfgs_maxit = 1000;
+  p.lbfgs_c = 1e-4;
+  p.lbfgs_ls_max = 30;
+  p.fp32_elements = 0;
+}
+
+void check_material(double youngs, double poisson, double density) {
+  // time_step.hpp:369-374
+  if (!(youngs >= 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Young's modulus must be positive, got " + std::to_string(youngs));
+  if (!(poisson > 0 && poisson < 0.5))
+    fail(SSIM_ERR_INVALID_ARGUMENT, "Poisson's ratio must be in (0, 0.5), got " + std::to_string(poisson));
+  if (!(density >= 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Density must be positive, got " + std::to_string(density));
+}
+
+void harvest_timers(ssim_ctx* c) {
+  for (auto& t : c->kt) {
+    if (!t.pending) continue;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, t.ev[0], t.ev[1]));
+    t.sum_ms += ms;
+    t.cnt++;
+    t.pending = false;
+  }
+}
+
+void sync(ssim_ctx* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->timing) harvest_timers(c);
+}
+
+// Runs launch(); when timing is on and this kernel's timer is free, brackets it with the timer's events.
+template <typename F>
+void timed(ssim_ctx* c, ssim_kernel_t k, F&& launch) {
+  auto& t = c->kt[k];
+  const bool arm = c->timing && !t.pending;
+  if (arm) {
+    if (!t.ev[0]) {
+      HIPCHK(hipEventCreate(&t.ev[0]));
+      HIPCHK(hipEventCreate(&t.ev[1]));
+    }
+    HIPCHK(hipEventRecord(t.ev[0], c->stream));
+  }
+  launch();
+  if (arm) {
+    HIPCHK(hipEventRecord(t.ev[1], c->stream));
+    t.pending = true;
+  }
+}
+
+// Sparsity, blocked-ELL layout and the two assembly permutations.
+void build_topology(ssim_ctx* c) {
+  const int nv = c->nv, nt = c->nt;
+  const int* cells = c->hcells.data();
+  // block adjacency: neighbours (self included) of every vertex, ascending
+  std::vector<int> cnt(nv + 1, 0);
+  for (int e = 0; e < nt; ++e)
+    for (int i = 0; i < 4; ++i) cnt[cells[4 * e + i] + 1] += 4;
+  for (int v = 0; v < nv; ++v) cnt[v + 1] += cnt[v];
+  std::vector<int> cand(std::max(cnt[nv], 1));
+  std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+  for (int e = 0; e < nt; ++e)
+    for (int i = 0; i < 4; ++i) {
+      const int v = cells[4 * e + i];
+      for (int j = 0; j < 4; ++j) cand[fillp[v]++] = cells[4 * e + j];
+    }
+  c->brow.assign(nv + 1, 0);
+  std::vector<int> len(nv, 0);
+  for (int v = 0; v < nv; ++v) {
+    int* b = cand.data() + cnt[v];
+    int* e = cand.data() + cnt[v + 1];
+    std::sort(b, e);
+    len[v] = (int)(std::unique(b, e) - b);
+    c->brow[v + 1] = c->brow[v] + len[v];
+  }
+  c->bcol.resize(std::max(c->brow[nv], 1));
+  for (int v = 0; v < nv; ++v) std::copy(cand.begin() + cnt[v], cand.begin() + cnt[v] + len[v], c->bcol.begin() + c->brow[v]);
+  cand.clear();
+  cand.shrink_to_fit();
+  auto slot = [&](int v, int w) -> int {  // index of w in the sorted neighbour list of v
+    const int* b = c->bcol.data() + c->brow[v];
+    const int* e = c->bcol.data() + c->brow[v + 1];
+    const int* it = std::lower_bound(b, e, w);
+    if (it == e || *it != w) fail(SSIM_ERR_STATE, "internal: missing block in sparsity pattern");
+    return (int)(it - b);
+  };
+  c->kdiag.resize(nv);
+  for (int v = 0; v < nv; ++v) c->kdiag[v] = slot(v, v);
+
+  // blocked ELL: slices of 64 rows, width = longest row in the slice
+  c->nslices = (nv + kSlice - 1) / kSlice;
+  c->hwidth.assign(std::max(c->nslices, 1), 0);
+  c->hcoloff.assign(std::max(c->nslices, 1) + 1, 0);
+  long long off = 0;
+  for (int s = 0; s < c->nslices; ++s) {
+    int w = 0;
+    for (int l = 0; l < kSlice && s * kSlice + l < nv; ++l) w = std::max(w, len[s * kSlice + l]);
+    c->hwidth[s] = w;
+    c->hcoloff[s] = (int)off;
+    off += (long long)w * kSlice;
+  }
+  c->ncolslots = off;
+  if (9 * off >= (1LL << 31)) fail(SSIM_ERR_INVALID_ARGUMENT, "mesh too large for 32-bit blocked-ELL offsets");
+  std::vector<int> hcol(std::max<long long>(off, 1), -1);
+  for (int v = 0; v < nv; ++v) {
+    const int s = v / kSlice, lane = v % kSlice;
+    for (int k = 0; k < len[v]; ++k) hcol[c->hcoloff[s] + k * kSlice + lane] = c->bcol[c->brow[v] + k];
+  }
+  auto valpos = [&](int v, int k) -> int {
+    const int s = v / kSlice, lane = v % kSlice;
+    return 9 * c->hcoloff[s] + k * 9 * kSlice + lane;
+  };
+
+  // upper blocks (r ≤ c) in row-major order, with the value positions of (r,c) and (c,r)
+  std::vector<int> ubptr(nv + 1, 0);
+  for (int v = 0; v < nv; ++v) ubptr[v + 1] = ubptr[v] + (len[v] - c->kdiag[v]);
+  c->nub = ubptr[nv];
+  std::vector<int2> hub_rc(std::max(c->nub, 1)), hub_pos(std::max(c->nub, 1));
+  for (int v = 0; v < nv; ++v)
+    for (int k = c->kdiag[v]; k < len[v]; ++k) {
+      const int w = c->bcol[c->brow[v] + k];
+      const int ub = ubptr[v] + (k - c->kdiag[v]);
+      hub_rc[ub] = make_int2(v, w);
+      hub_pos[ub] = make_int2(valpos(v, k), valpos(w, slot(w, v)));
+    }
+
+  // Hessian contributions: 10 local pairs (v ≤ w) per tet, sorted by (upper block, tet, pair)
+  std::vector<int> hcount(c->nub + 1, 0);
+  std::vector<int> hub_of(10 * (size_t)nt);
+  for (int e = 0; e < nt; ++e)
+    for (int w = 0; w < 4; ++w)
+      for (int v = 0; v <= w; ++v) {
+        const int a = cells[4 * e + v], b = cells[4 * e + w];
+        const int r = std::min(a, b), cc = std::max(a, b);
+        const int ub = ubptr[r] + (slot(r, cc) - c->kdiag[r]);
+        hub_of[10 * (size_t)e + w * (w + 1) / 2 + v] = ub;
+        hcount[ub + 1]++;
+      }
+  for (int u = 0; u < c->nub; ++u) hcount[u + 1] += hcount[u];
+  std::vector<int> hperm(10 * (size_t)nt);
+  {
+    std::vector<int> pos(hcount.begin(), hcount.end() - 1);
+    for (size_t k = 0; k < 10 * (size_t)nt; ++k) hperm[k] = pos[hub_of[k]]++;
+  }
+  // gradient contributions: 4 per tet, sorted by (vertex, tet, corner)
+  std::vector<int> gseg(nv + 1, 0);
+  for (int e = 0; e < nt; ++e)
+    for (int i = 0; i < 4; ++i) gseg[cells[4 * e + i] + 1]++;
+  for (int v = 0; v < nv; ++v) gseg[v + 1] += gseg[v];
+  std::vector<int> gperm(4 * (size_t)nt);
+  {
+    std::vector<int> pos(gseg.begin(), gseg.end() - 1);
+    for (int e = 0; e < nt; ++e)
+      for (int i = 0; i < 4; ++i) gperm[4 * e + i] = pos[cells[4 * e + i]]++;
+  }
+  hipStream_t s = c->stream;
+  c->width.upload(c->hwidth.data(), std::max(c->nslices, 1), s);
+  c->coloff.upload(c->hcoloff.data(), std::max(c->nslices, 1), s);
+  c->col.upload(hcol, s);
+  c->ub_rc.upload(hub_rc, s);
+  c->ub_pos.upload(hub_pos, s);
+  c->hseg.upload(hcount, s);
+  c->hperm.upload(hperm, s);
+  c->gseg.upload(gseg, s);
+  c->gperm.upload(gperm, s);
+  c->val.alloc(9 * (size_t)std::max<long long>(off, 1));
+  HIPCHK(hipMemsetAsync(c->val.p, 0, c->val.count * sizeof(double), s));
+  c->gcontrib.alloc(12 * (size_t)std::max(nt, 1));
+  c->hcontrib.alloc(90 * (size_t)std::max(nt, 1));
+  sync(c);  // host staging vectors go out of scope
+}
+
+void upload_material(ssim_ctx* c) {
+  std::vector<double> lam(c->nt), mu(c->nt);
+  if (c->per_tet) {
+    lam = c->hlam;
+    mu = c->hmu;
+  } else {
+    // compute_lambda / compute_mu: elast/basic_elast.hpp:11-20
+    const double l = c->youngs * c->poisson / ((1 + c->poisson) * (1 - 2 * c->poisson));
+    const double m = c->youngs / (2 * (1 + c->poisson));
+    std::fill(lam.begin(), lam.end(), l);
+    std::fill(mu.begin(), mu.end(), m);
+  }
+  c->lam.upload(lam, c->stream);
+  c->mu.upload(mu, c->stream);
+  sync(c);
+}
+
+// reset(): finite_elements/time_step.hpp:368-494
+void do_reset(ssim_ctx* c) {
+  check_material(c->youngs, c->poisson, c->density);
+  if (0.5 - c->poisson < 1e-4)
+    fprintf(stderr, "Warning: got extremely high Poisson's ratio, may cause numerical instability. %lf\n", c->poisson);
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  c->rho_reset = c->density;
+  // vertex buffers: inertia, next, forces zeroed; the others kept if already allocated (:384-413)
+  for (int w = 0; w < SSIM_ARRAY_COUNT; ++w) {
+    const bool keep = (w == SSIM_ARRAY_EXT_ACCEL || w == SSIM_ARRAY_DBC_VALUES || w == SSIM_ARRAY_PREV_DEFORM ||
+                       w == SSIM_ARRAY_DEFORMATION || w == SSIM_ARRAY_VELOCITY) &&
+                      c->arr[w].p && c->arr[w].count == (size_t)n;
+    if (!keep) {
+      c->arr[w].alloc(n);
+      HIPCHK(hipMemsetAsync(c->arr[w].p, 0, sizeof(double) * std::max(n, 1), s));
+    }
+  }
+  if (!(c->dof.p && c->dof.count == (size_t)n)) {
+    c->dof.alloc(n);
+    HIPCHK(hipMemsetAsync(c->dof.p, 0, std::max(n, 1), s));
+  }
+  for (DBuf<double>* b : {&c->utry, &c->gtry, &c->dir, &c->r, &c->z, &c->p, &c->q, &c->tmp, &c->dinv}) {
+    b->alloc(n);
+    HIPCHK(hipMemsetAsync(b->p, 0, sizeof(double) * std::max(n, 1), s));
+  }
+  c->hrest.release();
+  c->B.alloc(9 * (size_t)std::max(c->nt, 1));
+  c->V.alloc(std::max(c->nt, 1));
+  upload_material(c);
+  launch_rest_shape(c->X.p, c->cells.p, c->nt, c->B.p, c->V.p, s);
+  HIPCHK(hipGetLastError());
+  build_topology(c);
+  const int np = max_partials(std::max({c->nt, c->n, c->nub, 1}));
+  c->partials.alloc(2 * (size_t)np + 16);
+  c->pcg.alloc(1);
+  c->ev.alloc(1);
+  HIPCHK(hipMemsetAsync(c->pcg.p, 0, sizeof(PcgScalars), s));
+  HIPCHK(hipMemsetAsync(c->ev.p, 0, sizeof(EvalScalars), s));
+  // total mass Σ ρV (= asum(M.values)/3, time_step.hpp:490)
+  std::vector<double> hV(c->nt);
+  if (c->nt) HIPCHK(hipMemcpyAsync(hV.data(), c->V.p, sizeof(double) * c->nt, hipMemcpyDeviceToHost, s));
+  sync(c);
+  double mass = 0;
+  for (int e = 0; e < c->nt; ++e) mass += c->density * hV[e];
+  c->total_mass = mass;
+  c->tol_abs = c->rtol * mass;
+  c->has_reset = true;
+}
+
+struct EvalOut {
+  double E, gg;
+};
+
+// update_energy_and_gradients at deformation u → gradient into g; returns E and ‖g‖²
+EvalOut eval_energy_grad(ssim_ctx* c, const double* u, double* g) {
+  hipStream_t s = c->stream;
+  const double dt2 = c->dt * c->dt;
+  timed(c, SSIM_KERNEL_ELEM_GRAD, [&] {
+    launch_elem_grad(c->mesh(), u, c->a(SSIM_ARRAY_INERTIA), dt2, c->gperm.p, c->gcontrib.p, c->partials.p,
+                     &c->ev.p->cnt[0], &c->ev.p->energy, c->prm.fp32_elements != 0, s);
+  });
+  timed(c, SSIM_KERNEL_GRAD_REDUCE, [&] {
+    launch_grad_reduce(c->nv, c->gseg.p, c->gcontrib.p, c->dof.p, g, c->partials.p, &c->ev.p->cnt[1], &c->ev.p->gg,
+                       s);
+  });
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->pin, c->ev.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  sync(c);
+  c->stats.energy_evals++;
+  return EvalOut{c->pin[0], c->pin[1]};
+}
+
+double device_dot(ssim_ctx* c, const double* a, const double* b) {
+  launch_dot(c->n, a, b, c->partials.p, &c->ev.p->cnt[2], &c->ev.p->dot, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->pin, &c->ev.p->dot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  return c->pin[0];
+}
+
+// update_hessian at deformation u into the blocked-ELL values and the Jacobi diagonal
+void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_dbc, double dt2, double* values) {
+  hipStream_t s = c->stream;
+  timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
+    launch_elem_hess(c->mesh(), u, dt2, make_spsd, c->hperm.p, c->hcontrib.p, c->prm.fp32_elements != 0, s);
+  });
+  timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
+    launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, filter_dbc, values,
+                       c->dinv.p, nullptr, s);
+  });
+  HIPCHK(hipGetLastError());
+  c->stats.hessian_evals++;
+}
+
+// Jacobi-PCG on the current matrix: x = H⁻¹ (sign·b).  Returns iterations.
+int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit) {
+  hipStream_t s = c->stream;
+  const BellView A = c->bell(c->val.p);
+  launch_pcg_init(c->n, b, sign, c->dinv.p, x, c->r.p, c->z.p, c->p.p, rtol, maxit, c->pcg.p, c->partials.p, s);
+  HIPCHK(hipGetLastError());
+  int chunk = 8;
+  int launched = 0;
+  for (;;) {
+    for (int k = 0; k < chunk; ++k) {
+      // the first iteration of a chunk always does work (the previous chunk ended unconverged): sample it
+      if (k == 0) {
+        timed(c, SSIM_KERNEL_PCG_SPMV, [&] { launch_pcg_spmv(A, c->p.p, c->q.p, c->pcg.p, c->partials.p, s); });
+        timed(c, SSIM_KERNEL_PCG_UPDATE, [&] {
+          launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->pcg.p, c->partials.p, s);
+        });
+        timed(c, SSIM_KERNEL_PCG_P, [&] { launch_pcg_p(c->n, c->z.p, c->p.p, c->pcg.p, s); });
+      } else {
+        launch_pcg_spmv(A, c->p.p, c->q.p, c->pcg.p, c->partials.p, s);
+        launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->pcg.p, c->partials.p, s);
+        launch_pcg_p(c->n, c->z.p, c->p.p, c->pcg.p, s);
+      }
+    }
+    launched += chunk;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->pin, &c->pcg.p->done, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    sync(c);
+    const int* flags = reinterpret_cast<const int*>(c->pin);
+    if (flags[0]) {
+      if (flags[3] == 1) fail(SSIM_ERR_NUMERIC, "PCG breakdown: pᵀAp ≤ 0 (matrix not positive definite)");
+      if (flags[3] == 2) fail(SSIM_ERR_NUMERIC, "PCG breakdown: NaN residual");
+      return flags[1];
+    }
+    if (launched > maxit + 8) fail(SSIM_ERR_STATE, "internal: PCG did not stop");
+    chunk = std::min(chunk * 2, 64);
+  }
+}
+
+// Newton with backtracking Armijo line search: stepper/backward_euler.hpp:15-33 (tol = threshold·mass,
+// Armijo 0.3) with the iteration cap of the benchmark configuration.
+void newton_solve(ssim_ctx* c) {
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* g = c->a(SSIM_ARRAY_FORCES);
+  EvalOut cur = eval_energy_grad(c, u, g);
+  int it = 0;
+  for (; it < c->prm.max_newton; ++it) {
+    if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+    if (std::sqrt(cur.gg) < c->tol_abs) {
+      c->stats.converged = 1;
+      break;
+    }
+    assemble_hessian(c, u, 1, 1, c->dt * c->dt, c->val.p);
+    c->stats.pcg_iterations += pcg_solve(c, g, -1.0, c->dir.p, c->prm.pcg_rtol, c->prm.pcg_maxit);
+    const double gd = device_dot(c, g, c->dir.p);
+    double alpha = 1.0;
+    EvalOut trial{0, 0};
+    bool ok = false;
+    for (int ls = 0; ls < c->prm.ls_max; ++ls) {
+      launch_axpy_to(n, u, alpha, c->dir.p, c->utry.p, s);
+      trial = eval_energy_grad(c, c->utry.p, c->gtry.p);
+      if (trial.E <= cur.E + c->prm.ls_c * alpha * gd) {
+        ok = true;
+        break;
+      }
+      alpha *= c->prm.ls_beta;
+    }
+    if (!ok) c->stats.ls_failures++;
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    cur = trial;
+  }
+  c->stats.newton_iterations = it;
+}
+
+// L-BFGS two-loop recursion (Nocedal & Wright, Alg. 7.4) with H0 = diag(H_rest)⁻¹ and backtracking Armijo;
+// stepper/backward_euler.hpp:48-60 (tol = threshold·mass).  The two-loop coefficients stay on the device;
+// ρ_i = 1/(y_iᵀs_i) is read back once per accepted pair.
+void lbfgs_solve(ssim_ctx* c) {
+  const int n = c->n, m = std::max(1, c->prm.lbfgs_m);
+  hipStream_t s = c->stream;
+  if (!c->hrest.p) {
+    // H0: diagonal of the SPD-projected, Dirichlet-filtered Hessian at rest (u = 0)
+    c->hrest.alloc(n);
+    HIPCHK(hipMemsetAsync(c->tmp.p, 0, sizeof(double) * n, s));
+    const double dt2 = c->dt * c->dt;
+    launch_elem_hess(c->mesh(), c->tmp.p, dt2, 1, c->hperm.p, c->hcontrib.p, c->prm.fp32_elements != 0, s);
+    launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->val.p, c->dinv.p,
+                       c->hrest.p, s);
+    HIPCHK(hipGetLastError());
+  }
+  c->lb_S.alloc((size_t)m * n);
+  c->lb_Y.alloc((size_t)m * n);
+  c->lb_scal.alloc(2 * (size_t)m + 8);  // a[m], b[m]
+  double* av = c->lb_scal.p;
+  double* bv = av + m;
+  std::vector<double> rho_h(m, 0.0);
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* g = c->a(SSIM_ARRAY_FORCES);
+  double* d = c->dir.p;  // holds r ≈ H⁻¹g; the search direction is −r
+  EvalOut cur = eval_energy_grad(c, u, g);
+  int it = 0, hist = 0, head = 0;
+  for (; it < c->prm.lbfgs_maxit; ++it) {
+    if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+    if (std::sqrt(cur.gg) < c->tol_abs) {
+      c->stats.converged = 1;
+      break;
+    }
+    HIPCHK(hipMemcpyAsync(c->q.p, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    for (int k = 0; k < hist; ++k) {
+      const int idx = (head - 1 - k + m) % m;
+      // a_i = ρ_i s_iᵀq ; q −= a_i y_i
+      launch_dot(n, c->lb_S.p + (size_t)idx * n, c->q.p, c->partials.p, &c->ev.p->cnt[3], av + idx, s);
+      launch_axpy_coef(n, -rho_h[idx], av + idx, 0.0, nullptr, c->lb_Y.p + (size_t)idx * n, c->q.p, s);
+    }
+    launch_scale_div(n, c->q.p, c->hrest.p, d, s);  // r = H0 q
+    for (int k = hist - 1; k >= 0; --k) {
+      const int idx = (head - 1 - k + m) % m;
+      // b = ρ_i y_iᵀr ; r += s_i (a_i − b)
+      launch_dot(n, c->lb_Y.p + (size_t)idx * n, d, c->partials.p, &c->ev.p->cnt[3], bv + idx, s);
+      launch_axpy_coef(n, rho_h[idx], av + idx, rho_h[idx], bv + idx, c->lb_S.p + (size_t)idx * n, d, s);
+    }
+    HIPCHK(hipGetLastError());
+    const double gd = -device_dot(c, g, d);
+    double alpha = 1.0;
+    EvalOut trial{0, 0};
+    bool ok = false;
+    for (int ls = 0; ls < c->prm.lbfgs_ls_max; ++ls) {
+      launch_axpy_to(n, u, -alpha, d, c->utry.p, s);
+      trial = eval_energy_grad(c, c->utry.p, c->gtry.p);
+      if (trial.E <= cur.E + c->prm.lbfgs_c * alpha * gd) {
+        ok = true;
+        break;
+      }
+      alpha *= c->prm.ls_beta;
+    }
+    if (!ok) c->stats.ls_failures++;
+    double* sv = c->lb_S.p + (size_t)head * n;
+    double* yv = c->lb_Y.p + (size_t)head * n;
+    launch_sub(n, c->utry.p, u, sv, s);
+    launch_sub(n, c->gtry.p, g, yv, s);
+    const double ys = device_dot(c, yv, sv);
+    if (ys > 1e-300) {
+      rho_h[head] = 1.0 / ys;
+      head = (head + 1) % m;
+      if (hist < m) hist++;
+    }
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    cur = trial;
+  }
+  c->stats.newton_iterations = it;
+}
+
+ssim_status_t guard(ssim_ctx* c) {
+  if (!c) {
+    g_err = "null context";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  return SSIM_OK;
+}
+
+template <typename F>
+ssim_status_t run(F&& f) {
+  try {
+    f();
+    return SSIM_OK;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host out of memory";
+    return SSIM_ERR_HIP;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return SSIM_ERR_STATE;
+  }
+}
+
+void require_reset(ssim_ctx* c) {
+  if (!c->has_reset) fail(SSIM_ERR_STATE, "context not reset");
+}
+
+}  // namespace
+
+// ======================================================================================================
+// extern "C" ABI
+
+extern "C" {
+
+const char* ssim_last_error(void) { return g_err.c_str(); }
+
+ssim_status_t ssim_create(ssim_ctx** out, const double* vertices, int nv, const int* cells, int nt, double time_step,
+                          double youngs, double poisson, double density) {
+  if (!out) {
+    g_err = "out is null";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  ssim_ctx* c = nullptr;
+  const ssim_status_t st = run([&] {
+    if (nv < 0 || nt < 0) fail(SSIM_ERR_INVALID_ARGUMENT, "negative mesh size");
+    if ((nv > 0 && !vertices) || (nt > 0 && !cells)) fail(SSIM_ERR_INVALID_ARGUMENT, "null mesh arrays");
+    if (!(time_step > 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Time step must be positive.");
+    if ((long long)3 * nv >= (1LL << 31) || (long long)12 * nt >= (1LL << 31))
+      fail(SSIM_ERR_INVALID_ARGUMENT, "mesh too large for 32-bit indices");
+    for (long long k = 0; k < 4LL * nt; ++k)
+      if (cells[k] < 0 || cells[k] >= nv)
+        fail(SSIM_ERR_INVALID_ARGUMENT, "cell " + std::to_string(k / 4) + " references vertex " + std::to_string(cells[k]) +
+                                            " outside [0, " + std::to_string(nv) + ")");
+    check_material(youngs, poisson, density);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) fail(SSIM_ERR_HIP, "no HIP device available");
+    c = new ssim_ctx();
+    c->nv = nv;
+    c->nt = nt;
+    c->n = 3 * nv;
+    c->dt = time_step;
+    c->youngs = youngs;
+    c->poisson = poisson;
+    c->density = density;
+    default_params(c->prm);
+    check_material(youngs, poisson, density);
+    c->hX.assign(vertices, vertices + 3 * (size_t)nv);
+    c->hcells.assign(cells, cells + 4 * (size_t)nt);
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->pin), 64 * sizeof(double), hipHostMallocDefault));
+    c->X.upload(c->hX, c->stream);
+    c->cells.upload(reinterpret_cast<const int4*>(c->hcells.data()), nt, c->stream);
+    do_reset(c);  // the constructor resets (time_step.hpp:260-265)
+  });
+  if (st != SSIM_OK) {
+    delete c;
+    return st;
+  }
+  *out = c;
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_destroy(ssim_ctx* ctx) {
+  if (!ctx) return SSIM_OK;
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_set_stream(ssim_ctx* ctx, void* hip_stream) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    sync(ctx);
+    ctx->stream = reinterpret_cast<hipStream_t>(hip_stream);
+  });
+}
+
+ssim_status_t ssim_set_time_step(ssim_ctx* ctx, double dt) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!(dt > 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Time step must be positive.");
+    ctx->dt = dt;
+  });
+}
+ssim_status_t ssim_set_youngs(ssim_ctx* ctx, double youngs) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!(youngs > 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Young's modulus must be positive.");
+    ctx->youngs = youngs;
+  });
+}
+ssim_status_t ssim_set_poisson(ssim_ctx* ctx, double poisson) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!(poisson > 0 && poisson < 0.5)) fail(SSIM_ERR_INVALID_ARGUMENT, "Poisson's ratio must be in (0, 0.5).");
+    ctx->poisson = poisson;
+  });
+}
+ssim_status_t ssim_set_density(ssim_ctx* ctx, double density) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!(density > 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Density must be positive.");
+    ctx->density = density;
+  });
+}
+ssim_status_t ssim_set_damping(ssim_ctx* ctx, double damping) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!(0 <= damping && damping < 1)) fail(SSIM_ERR_INVALID_ARGUMENT, "Damping must be in [0, 1).");
+    ctx->damping = damping;
+  });
+}
+ssim_status_t ssim_set_threshold(ssim_ctx* ctx, double rtol) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    // stored only; the absolute tolerance is rtol·mass recomputed at reset (time_step.hpp:343-347, :482-492)
+    if (!(0 < rtol && rtol < 1)) fail(SSIM_ERR_INVALID_ARGUMENT, "Threshold must be positive.");
+    ctx->rtol = rtol;
+  });
+}
+ssim_status_t ssim_set_material_per_tet(ssim_ctx* ctx, const double* lambda, const double* mu) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!lambda || !mu) {
+      ctx->per_tet = false;
+      ctx->hlam.clear();
+      ctx->hmu.clear();
+    } else {
+      for (int e = 0; e < ctx->nt; ++e)
+        if (!(lambda[e] > 0) || !(mu[e] > 0) || !std::isfinite(lambda[e]) || !std::isfinite(mu[e]))
+          fail(SSIM_ERR_INVALID_ARGUMENT, "per-tet Lamé parameters must be positive and finite (tet " + std::to_string(e) + ")");
+      ctx->per_tet = true;
+      ctx->hlam.assign(lambda, lambda + ctx->nt);
+      ctx->hmu.assign(mu, mu + ctx->nt);
+    }
+    upload_material(ctx);
+    ctx->hrest.release();
+  });
+}
+ssim_status_t ssim_set_solver_params(ssim_ctx* ctx, const ssim_solver_params* p) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!p) fail(SSIM_ERR_INVALID_ARGUMENT, "params is null");
+    if (p->max_newton < 0 || p->pcg_maxit < 0 || !(p->pcg_rtol >= 0) || !(p->ls_c > 0 && p->ls_c < 1) ||
+        !(p->ls_beta > 0 && p->ls_beta < 1) || p->ls_max < 1 || p->lbfgs_m < 1 || p->lbfgs_maxit < 0 ||
+        !(p->lbfgs_c > 0 && p->lbfgs_c < 1) || p->lbfgs_ls_max < 1)
+      fail(SSIM_ERR_INVALID_ARGUMENT, "solver parameter out of range");
+    const bool fp32_changed = (p->fp32_elements != 0) != (ctx->prm.fp32_elements != 0);
+    ctx->prm = *p;
+    if (fp32_changed) ctx->hrest.release();
+  });
+}
+ssim_status_t ssim_get_solver_params(const ssim_ctx* ctx, ssim_solver_params* p) {
+  if (!ctx || !p) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  *p = ctx->prm;
+  return SSIM_OK;
+}
+ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  if (stepper != SSIM_STEPPER_NEWTON && stepper != SSIM_STEPPER_LBFGS) {
+    g_err = "unknown stepper";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  ctx->stepper = stepper;
+  return SSIM_OK;
+}
+
+static void edit_dirichlet(ssim_ctx* c, const int* verts, int count, const double* targets, bool mark) {
+  if (count < 0 || (count > 0 && !verts)) fail(SSIM_ERR_INVALID_ARGUMENT, "bad vertex list");
+  for (int i = 0; i < count; ++i)
+    if (verts[i] < 0 || verts[i] >= c->nv) fail(SSIM_ERR_INVALID_ARGUMENT, "vertex index out of range");
+  std::vector<unsigned char> dof(c->n);
+  std::vector<double> dbc(c->n);
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(dof.data(), c->dof.p, c->n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(dbc.data(), c->a(SSIM_ARRAY_DBC_VALUES), sizeof(double) * c->n, hipMemcpyDeviceToHost, s));
+  sync(c);
+  for (int i = 0; i < count; ++i)
+    for (int j = 0; j < 3; ++j) {
+      dof[3 * (size_t)verts[i] + j] = mark ? 1 : 0;
+      dbc[3 * (size_t)verts[i] + j] = mark ? (targets ? targets[3 * (size_t)i + j] : 0.0) : 0.0;
+    }
+  HIPCHK(hipMemcpyAsync(c->dof.p, dof.data(), c->n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->a(SSIM_ARRAY_DBC_VALUES), dbc.data(), sizeof(double) * c->n, hipMemcpyHostToDevice, s));
+  sync(c);
+  c->hrest.release();
+}
+
+ssim_status_t ssim_mark_dirichlet(ssim_ctx* ctx, const int* verts, int count, const double* targets) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] { edit_dirichlet(ctx, verts, count, targets, true); });
+}
+ssim_status_t ssim_mark_general(ssim_ctx* ctx, const int* verts, int count) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] { edit_dirichlet(ctx, verts, count, nullptr, false); });
+}
+
+ssim_status_t ssim_add_ext_force_dof(ssim_ctx* ctx, int dof_idx, double value) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!(-3 < dof_idx && dof_idx < 3)) fail(SSIM_ERR_INVALID_ARGUMENT, "|dof_idx| must be less than PhysicalDim.");
+    if (dof_idx < 0) dof_idx += 3;
+    std::vector<double> acc(ctx->n);
+    HIPCHK(hipMemcpyAsync(acc.data(), ctx->a(SSIM_ARRAY_EXT_ACCEL), sizeof(double) * ctx->n, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    sync(ctx);
+    for (int v = 0; v < ctx->nv; ++v) acc[3 * (size_t)v + dof_idx] += value;
+    HIPCHK(hipMemcpyAsync(ctx->a(SSIM_ARRAY_EXT_ACCEL), acc.data(), sizeof(double) * ctx->n, hipMemcpyHostToDevice,
+                          ctx->stream));
+    sync(ctx);
+  });
+}
+
+ssim_status_t ssim_reset(ssim_ctx* ctx) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] { do_reset(ctx); });
+}
+
+ssim_status_t ssim_prepare_step(ssim_ctx* ctx) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    if (!(0 <= ctx->damping && ctx->damping < 1)) fail(SSIM_ERR_INVALID_ARGUMENT, "velocity damping must be in [0, 1).");
+    launch_prepare(ctx->n, ctx->a(SSIM_ARRAY_DEFORMATION), ctx->a(SSIM_ARRAY_VELOCITY), ctx->a(SSIM_ARRAY_EXT_ACCEL),
+                   ctx->dof.p, ctx->a(SSIM_ARRAY_DBC_VALUES), ctx->dt, ctx->a(SSIM_ARRAY_INERTIA),
+                   ctx->a(SSIM_ARRAY_NEXT_DEFORM), ctx->stream);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ssim_status_t ssim_compute_step(ssim_ctx* ctx, double* residual_out) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    ctx->stats = ssim_step_stats{};
+    if (ctx->stepper == SSIM_STEPPER_NEWTON)
+      newton_solve(ctx);
+    else
+      lbfgs_solve(ctx);
+    // compute_step: final update_energy_and_gradients and ‖forces‖ (time_step.hpp:520-528)
+    const EvalOut fin = eval_energy_grad(ctx, ctx->a(SSIM_ARRAY_NEXT_DEFORM), ctx->a(SSIM_ARRAY_FORCES));
+    ctx->stats.grad_norm = std::sqrt(fin.gg);
+    ctx->stats.energy = fin.E;
+    if (!std::isfinite(fin.E) || !std::isfinite(fin.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+    if (residual_out) *residual_out = ctx->stats.grad_norm;
+  });
+}
+
+ssim_status_t ssim_step_next(ssim_ctx* ctx) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    launch_step_next(ctx->n, ctx->a(SSIM_ARRAY_NEXT_DEFORM), ctx->a(SSIM_ARRAY_DEFORMATION),
+                     ctx->a(SSIM_ARRAY_PREV_DEFORM), ctx->a(SSIM_ARRAY_VELOCITY), ctx->dt, ctx->damping, ctx->stream);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ssim_status_t ssim_step(ssim_ctx* ctx, double* residual_out) {
+  ssim_status_t st = ssim_prepare_step(ctx);
+  if (st) return st;
+  st = ssim_compute_step(ctx, residual_out);
+  if (st) return st;
+  return ssim_step_next(ctx);
+}
+
+ssim_status_t ssim_restore(ssim_ctx* ctx) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    launch_restore(ctx->n, ctx->dof.p, ctx->a(SSIM_ARRAY_DBC_VALUES), ctx->a(SSIM_ARRAY_DEFORMATION),
+                   ctx->a(SSIM_ARRAY_PREV_DEFORM), ctx->a(SSIM_ARRAY_VELOCITY), ctx->dt, ctx->stream);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ssim_status_t ssim_get_step_stats(const ssim_ctx* ctx, ssim_step_stats* stats) {
+  if (!ctx || !stats) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  *stats = ctx->stats;
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_update_energy_and_gradients(ssim_ctx* ctx, double* energy_out) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    const EvalOut r = eval_energy_grad(ctx, ctx->a(SSIM_ARRAY_NEXT_DEFORM), ctx->a(SSIM_ARRAY_FORCES));
+    if (energy_out) *energy_out = r.E;
+  });
+}
+
+ssim_status_t ssim_update_hessian(ssim_ctx* ctx, int make_spsd, int filter_dbc) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    assemble_hessian(ctx, ctx->a(SSIM_ARRAY_NEXT_DEFORM), make_spsd ? 1 : 0, filter_dbc ? 1 : 0, ctx->dt * ctx->dt,
+                     ctx->val.p);
+    sync(ctx);
+  });
+}
+
+ssim_status_t ssim_check_convergence(ssim_ctx* ctx, int* converged_out) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    launch_dot(ctx->n, ctx->a(SSIM_ARRAY_FORCES), ctx->a(SSIM_ARRAY_FORCES), ctx->partials.p, &ctx->ev.p->cnt[2],
+               &ctx->ev.p->dot, ctx->stream);
+    HIPCHK(hipMemcpyAsync(ctx->pin, &ctx->ev.p->dot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    if (converged_out) *converged_out = std::sqrt(ctx->pin[0]) < ctx->tol_abs;
+  });
+}
+
+ssim_status_t ssim_get_array(ssim_ctx* ctx, ssim_array_t which, double* host_out) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (which < 0 || which >= SSIM_ARRAY_COUNT || !host_out) fail(SSIM_ERR_INVALID_ARGUMENT, "bad array request");
+    HIPCHK(hipMemcpyAsync(host_out, ctx->a(which), sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+  });
+}
+
+ssim_status_t ssim_set_array(ssim_ctx* ctx, ssim_array_t which, const double* host_in) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (which < 0 || which >= SSIM_ARRAY_COUNT || !host_in) fail(SSIM_ERR_INVALID_ARGUMENT, "bad array request");
+    HIPCHK(hipMemcpyAsync(ctx->a(which), host_in, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    sync(ctx);
+  });
+}
+
+ssim_status_t ssim_device_array(ssim_ctx* ctx, ssim_array_t which, double** device_ptr) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  if (which < 0 || which >= SSIM_ARRAY_COUNT || !device_ptr) {
+    g_err = "bad array request";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  *device_ptr = ctx->a(which);
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_get_dof_type(const ssim_ctx* ctx, unsigned char* host_out) {
+  if (!ctx || !host_out) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  return run([&] {
+    HIPCHK(hipMemcpyAsync(host_out, ctx->dof.p, ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+ssim_status_t ssim_num_vertices(const ssim_ctx* ctx, int* nv) {
+  if (!ctx || !nv) return SSIM_ERR_INVALID_ARGUMENT;
+  *nv = ctx->nv;
+  return SSIM_OK;
+}
+ssim_status_t ssim_num_cells(const ssim_ctx* ctx, int* nt) {
+  if (!ctx || !nt) return SSIM_ERR_INVALID_ARGUMENT;
+  *nt = ctx->nt;
+  return SSIM_OK;
+}
+ssim_status_t ssim_total_mass(const ssim_ctx* ctx, double* mass) {
+  if (!ctx || !mass) return SSIM_ERR_INVALID_ARGUMENT;
+  *mass = ctx->total_mass;
+  return SSIM_OK;
+}
+ssim_status_t ssim_grad_threshold_abs(const ssim_ctx* ctx, double* tol) {
+  if (!ctx || !tol) return SSIM_ERR_INVALID_ARGUMENT;
+  *tol = ctx->tol_abs;
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_csr_nnz(const ssim_ctx* ctx, long long* nnz) {
+  if (!ctx || !nnz) return SSIM_ERR_INVALID_ARGUMENT;
+  *nnz = 9LL * (ctx->brow.empty() ? 0 : ctx->brow[ctx->nv]);
+  return SSIM_OK;
+}
+
+// blocked-ELL values (device) → scalar CSR in row/column order
+static void export_csr(ssim_ctx* c, const double* dval, int* rowptr, int* colidx, double* values) {
+  const int nv = c->nv;
+  if (rowptr) {
+    rowptr[0] = 0;
+    for (int v = 0; v < nv; ++v) {
+      const int len = 3 * (c->brow[v + 1] - c->brow[v]);
+      for (int b = 0; b < 3; ++b) rowptr[3 * v + b + 1] = rowptr[3 * v + b] + len;
+    }
+  }
+  if (colidx) {
+    long long p = 0;
+    for (int v = 0; v < nv; ++v)
+      for (int b = 0; b < 3; ++b)
+        for (int k = c->brow[v]; k < c->brow[v + 1]; ++k)
+          for (int d = 0; d < 3; ++d) colidx[p++] = 3 * c->bcol[k] + d;
+  }
+  if (values) {
+    std::vector<double> hv(9 * (size_t)std::max<long long>(c->ncolslots, 1));
+    HIPCHK(hipMemcpyAsync(hv.data(), dval, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    long long p = 0;
+    for (int v = 0; v < nv; ++v) {
+      const int s = v / kSlice, lane = v % kSlice;
+      const size_t base = 9 * (size_t)c->hcoloff[s] + lane;
+      const int len = c->brow[v + 1] - c->brow[v];
+      for (int b = 0; b < 3; ++b)
+        for (int k = 0; k < len; ++k)
+          for (int d = 0; d < 3; ++d) values[p++] = hv[base + ((size_t)k * 9 + 3 * b + d) * kSlice];
+    }
+  }
+}
+
+ssim_status_t ssim_export_hessian_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    export_csr(ctx, ctx->val.p, rowptr, colidx, values);
+  });
+}
+
+ssim_status_t ssim_export_mass_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    if (values) {
+      // the mass matrix is the Hessian assembly with dt = 0 and no Dirichlet filter
+      ctx->massval.alloc(ctx->val.count);
+      HIPCHK(hipMemsetAsync(ctx->massval.p, 0, sizeof(double) * ctx->val.count, ctx->stream));
+      launch_elem_hess(ctx->mesh(), ctx->a(SSIM_ARRAY_NEXT_DEFORM), 0.0, 0, ctx->hperm.p, ctx->hcontrib.p, false,
+                       ctx->stream);
+      launch_hess_reduce(ctx->nub, ctx->hseg.p, ctx->hcontrib.p, ctx->ub_rc.p, ctx->ub_pos.p, ctx->dof.p, 0,
+                         ctx->massval.p, ctx->tmp.p, nullptr, ctx->stream);
+      HIPCHK(hipGetLastError());
+    }
+    export_csr(ctx, ctx->massval.p, rowptr, colidx, values);
+  });
+}
+
+ssim_status_t ssim_eval_elements(ssim_ctx* ctx, const double* u, int make_spsd, double* psi, double* P, double* H,
+                                 double* K) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    if (!u) fail(SSIM_ERR_INVALID_ARGUMENT, "u is null");
+    const size_t nt = ctx->nt;
+    HIPCHK(hipMemcpyAsync(ctx->tmp.p, u, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    DBuf<double> dpsi, dP, dH, dK;
+    if (psi) dpsi.alloc(nt);
+    if (P) dP.alloc(9 * nt);
+    if (H) dH.alloc(81 * nt);
+    if (K) dK.alloc(144 * nt);
+    launch_elem_eval(ctx->mesh(), ctx->tmp.p, make_spsd ? 1 : 0, dpsi.p, dP.p, dH.p, dK.p, ctx->prm.fp32_elements != 0,
+                     ctx->stream);
+    HIPCHK(hipGetLastError());
+    if (psi) HIPCHK(hipMemcpyAsync(psi, dpsi.p, sizeof(double) * nt, hipMemcpyDeviceToHost, ctx->stream));
+    if (P) HIPCHK(hipMemcpyAsync(P, dP.p, sizeof(double) * 9 * nt, hipMemcpyDeviceToHost, ctx->stream));
+    if (H) HIPCHK(hipMemcpyAsync(H, dH.p, sizeof(double) * 81 * nt, hipMemcpyDeviceToHost, ctx->stream));
+    if (K) HIPCHK(hipMemcpyAsync(K, dK.p, sizeof(double) * 144 * nt, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+  });
+}
+
+ssim_status_t ssim_pcg_solve(ssim_ctx* ctx, const double* b, double* x, double rtol, int maxit, int* iters_out) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    if (!b || !x) fail(SSIM_ERR_INVALID_ARGUMENT, "null vector");
+    HIPCHK(hipMemcpyAsync(ctx->tmp.p, b, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    const int it = pcg_solve(ctx, ctx->tmp.p, 1.0, ctx->dir.p, rtol, maxit);
+    HIPCHK(hipMemcpyAsync(x, ctx->dir.p, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+    if (iters_out) *iters_out = it;
+  });
+}
+
+ssim_status_t ssim_spmv(ssim_ctx* ctx, const double* x, double* y) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    if (!x || !y) fail(SSIM_ERR_INVALID_ARGUMENT, "null vector");
+    HIPCHK(hipMemcpyAsync(ctx->tmp.p, x, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+    launch_spmv(ctx->bell(ctx->val.p), ctx->tmp.p, ctx->q.p, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(y, ctx->q.p, sizeof(double) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+    sync(ctx);
+  });
+}
+
+ssim_status_t ssim_set_kernel_timing(ssim_ctx* ctx, int enabled) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (auto& t : ctx->kt) {
+      t.sum_ms = 0;
+      t.cnt = 0;
+      t.pending = false;
+    }
+    ctx->timing = enabled != 0;
+  });
+}
+
+ssim_status_t ssim_get_kernel_timing(const ssim_ctx* ctx, ssim_kernel_t kernel, double* avg_ms, long long* samples) {
+  if (!ctx || kernel < 0 || kernel >= SSIM_KERNEL_COUNT) {
+    g_err = "bad kernel timing request";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  const auto& t = ctx->kt[kernel];
+  if (avg_ms) *avg_ms = t.cnt ? t.sum_ms / t.cnt : 0.0;
+  if (samples) *samples = t.cnt;
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_get_layout_info(const ssim_ctx* ctx, ssim_layout_info* info) {
+  if (!ctx || !info) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  info->nv = ctx->nv;
+  info->nt = ctx->nt;
+  info->nonzero_blocks = ctx->brow.empty() ? 0 : ctx->brow[ctx->nv];
+  info->padded_slots = ctx->ncolslots;
+  info->upper_blocks = ctx->nub;
+  info->slices = ctx->nslices;
+  return SSIM_OK;
+}
+
+}  // extern "C"

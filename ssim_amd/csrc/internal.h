@@ -1,0 +1,96 @@
+// internal.h -- device data views and kernel launchers shared by the HIP sources of libssim_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ssim_hip {
+
+constexpr int kBlock = 256;      // threads per workgroup for every kernel (4 wave64)
+constexpr int kSlice = 64;       // rows per blocked-ELL slice (one wavefront)
+constexpr double kSpdFloor = 1e-3;  // eigenvalue floor of the SPD projection (time_step.hpp:52)
+
+// Per-tet rest data in HBM.  cells are int4 (one 16-byte load per tet); B = Dm⁻¹ is structure-of-arrays
+// B[k·nt + e] for k = 0..8 so that a wavefront's loads of one component are contiguous.
+struct MeshView {
+  const double* X;      // [nv*3] rest positions
+  const int4* cells;    // [nt]
+  const double* B;      // [9][nt]
+  const double* V;      // [nt]  rest volumes
+  const double* lam;    // [nt]
+  const double* mu;     // [nt]
+  int nv, nt;
+  double rho;
+};
+
+// Blocked ELLPACK of 3x3 blocks.  Rows (vertices) are grouped in slices of 64; slice s has width[s] slots.
+// col[col_off[s] + k*64 + lane] = column vertex of slot k of row 64 s + lane (−1 = padding);
+// val[9*col_off[s] + (k*9 + 3b + d)*64 + lane] = entry (b, d) of that block.
+struct BellView {
+  const int* width;
+  const int* col_off;
+  const int* col;
+  const double* val;
+  int nrows, nslices;
+};
+
+// Scalars of one PCG solve, resident in device memory.  Counters implement last-workgroup reductions.
+struct PcgScalars {
+  double rz, pq, alpha, beta, rr, bb, tol2, pad;
+  int done, iters, maxit, breakdown;
+  unsigned cnt[8];
+};
+
+// Reduction results of element / vector kernels.
+struct EvalScalars {
+  double energy, gg, gd, dot;
+  unsigned cnt[8];
+};
+
+// ---- launchers (kernels.hip) ----
+void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, double* V, hipStream_t s);
+// energy + gradient contributions at deformation u (predictor uhat); energy written to *E_out
+void launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
+                      double* gcontrib, double* partials, unsigned* counter, double* E_out, bool fp32,
+                      hipStream_t s);
+// g[v] = Σ contributions of v, zeroed on Dirichlet dofs; ‖g‖² written to *gg_out
+void launch_grad_reduce(int nv, const int* gseg, const double* gcontrib, const unsigned char* dof, double* g,
+                        double* partials, unsigned* counter, double* gg_out, hipStream_t s);
+// Hessian contributions (10 oriented 3x3 blocks per tet, dt²V-scaled, consistent mass on the diagonal blocks)
+void launch_elem_hess(const MeshView& m, const double* u, double dt2, int make_spsd, const int* hperm,
+                      double* hcontrib, bool fp32, hipStream_t s);
+void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const int2* ub_rc, const int2* ub_pos,
+                        const unsigned char* dof, int filter_dbc, double* bell_val, double* dinv, double* diag,
+                        hipStream_t s);
+void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double* psi, double* P, double* H,
+                      double* K, bool fp32, hipStream_t s);
+
+void launch_spmv(const BellView& A, const double* x, double* y, hipStream_t s);
+void launch_dot(int n, const double* a, const double* b, double* partials, unsigned* counter, double* out,
+                hipStream_t s);
+void launch_prepare(int n, const double* u, const double* vel, const double* accel, const unsigned char* dof,
+                    const double* dbc, double dt, double* uhat, double* unext, hipStream_t s);
+void launch_step_next(int n, const double* unext, double* u, double* uprev, double* vel, double dt,
+                      double damping, hipStream_t s);
+void launch_restore(int n, const unsigned char* dof, const double* dbc, double* u, double* uprev, double* vel,
+                    double dt, hipStream_t s);
+void launch_axpy_to(int n, const double* x, double alpha, const double* d, double* out, hipStream_t s);
+
+// PCG on H x = sign·b with Jacobi preconditioner dinv
+void launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
+                     double* p, double rtol, int maxit, PcgScalars* sc, double* partials, hipStream_t s);
+void launch_pcg_spmv(const BellView& A, const double* p, double* q, PcgScalars* sc, double* partials, hipStream_t s);
+void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
+                       PcgScalars* sc, double* partials, hipStream_t s);
+void launch_pcg_p(int n, const double* z, double* p, const PcgScalars* sc, hipStream_t s);
+
+// L-BFGS helpers
+void launch_scale_div(int n, const double* q, const double* diag, double* out, hipStream_t s);
+// y += (c1·(*p1) − c2·(*p2))·x, coefficients read from device memory (p2 may be null)
+void launch_axpy_coef(int n, double c1, const double* p1, double c2, const double* p2, const double* x, double* y,
+                      hipStream_t s);
+void launch_sub(int n, const double* a, const double* b, double* out, hipStream_t s);
+
+int max_partials(int n);
+
+}  // namespace ssim_hip

@@ -1,0 +1,768 @@
+// kernels.hip -- HIP kernels of the FEM soft-body time step for gfx950 (MI355X).
+//
+// The path is memory-bound gather/scatter work, so no MFMA: kernels are laid out for coalesced HBM access
+// (structure-of-arrays per-tet data, blocked-ELL matrix with one row per lane), deterministic reductions
+// (wavefront shuffles, a fixed-order workgroup tree and a last-workgroup final pass) and atomic-free
+// assembly (every element writes its contributions to precomputed, target-sorted slots; a segmented reduce
+// sums each target's contiguous run).
+#include <hip/hip_runtime.h>
+
+#include "elem_math.h"
+#include "internal.h"
+
+namespace ssim_hip {
+
+// ------------------------------------------------------------------------------------------------------
+// reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Sums K values over the grid.  Every workgroup stores its partial; the workgroup that arrives last sums all
+// partials in workgroup order (a fixed order, so the result is bitwise reproducible) and returns true with the
+// totals in tot[] (valid in every thread of that workgroup).
+template <int K>
+__device__ __forceinline__ bool grid_reduce(double (&v)[K], double* partials, unsigned* counter, double (&tot)[K]) {
+  __shared__ double sm[K][kBlock / 64];
+  __shared__ double res[K];
+  __shared__ int is_last;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double w = wave_sum(v[k]);
+    if (lane == 0) sm[k][wid] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; ++w) s += sm[k][w];
+      __hip_atomic_store(&partials[(size_t)blockIdx.x * K + k], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    is_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      acc[k] += __hip_atomic_load(&partials[(size_t)b * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double w = wave_sum(acc[k]);
+    if (lane == 0) sm[k][wid] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; ++w) s += sm[k][w];
+      res[k] = s;
+    }
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) tot[k] = res[k];
+  return true;
+}
+
+int max_partials(int n) { return (n + kBlock - 1) / kBlock + 1; }
+
+static inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// ------------------------------------------------------------------------------------------------------
+// rest shape
+
+__global__ __launch_bounds__(kBlock) void k_rest_shape(const double* __restrict__ X, const int4* __restrict__ cells,
+                                                       int nt, double* __restrict__ B, double* __restrict__ V) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nt) return;
+  const int4 c = cells[e];
+  const int id[4] = {c.x, c.y, c.z, c.w};
+  double x[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) x[i][a] = X[3 * (size_t)id[i] + a];
+  double Bl[9], Vl;
+  rest_shape(x[0], x[1], x[2], x[3], Bl, &Vl);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) B[(size_t)k * nt + e] = Bl[k];
+  V[e] = Vl;
+}
+
+void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, double* V, hipStream_t s) {
+  if (nt <= 0) return;
+  k_rest_shape<<<nblocks(nt), kBlock, 0, s>>>(X, cells, nt, B, V);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// element gathers
+
+struct TetLocal {
+  int id[4];
+  double B[9];
+  double V, lam, mu;
+};
+
+__device__ __forceinline__ TetLocal load_tet(const MeshView& m, int e) {
+  TetLocal t;
+  const int4 c = m.cells[e];
+  t.id[0] = c.x;
+  t.id[1] = c.y;
+  t.id[2] = c.z;
+  t.id[3] = c.w;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) t.B[k] = m.B[(size_t)k * m.nt + e];
+  t.V = m.V[e];
+  t.lam = m.lam[e];
+  t.mu = m.mu[e];
+  return t;
+}
+
+// F = Ds·B with Ds from positions X + u (def_grad.hpp:270-288); differences formed in FP64
+__device__ __forceinline__ void def_grad(const double x[4][3], const double B[9], double F[9]) {
+  double Ds[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    Ds[a * 3 + 0] = x[1][a] - x[0][a];
+    Ds[a * 3 + 1] = x[2][a] - x[0][a];
+    Ds[a * 3 + 2] = x[3][a] - x[0][a];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      F[a * 3 + c] = Ds[a * 3 + 0] * B[0 * 3 + c] + Ds[a * 3 + 1] * B[1 * 3 + c] + Ds[a * 3 + 2] * B[2 * 3 + c];
+}
+
+// local force f[v][b] = Σ_c P[b][c]·B[v−1][c], f[0] = −Σ f[1..3] (time_step.hpp:73-89)
+template <typename T>
+__device__ __forceinline__ void local_force(const T P[9], const T B[9], T f[4][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    T tot = 0;
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+      const T val = P[j * 3 + 0] * B[(i - 1) * 3 + 0] + P[j * 3 + 1] * B[(i - 1) * 3 + 1] + P[j * 3 + 2] * B[(i - 1) * 3 + 2];
+      f[i][j] = val;
+      tot += val;
+    }
+    f[0][j] = -tot;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// energy + gradient (update_energy_and_gradients, time_step.hpp:577-624), element-wise:
+//   e = ½·(ρV/20)·(Σ|Δ_i|² + |ΣΔ_i|²) + dt²·V·Ψ,   g_i = (ρV/20)·(Δ_i + ΣΔ_j) + dt²·V·f_i,   Δ = u − û.
+// The consistent mass M_e = ρV(1+δ_ij)/20 (bilinears.hpp:140-148) is applied element by element instead of
+// through a CSR SpMV.
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_elem_grad(MeshView m, const double* __restrict__ u,
+                                                      const double* __restrict__ uhat, double dt2,
+                                                      const int* __restrict__ gperm, double* __restrict__ gcontrib,
+                                                      double* partials, unsigned* counter, double* E_out) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  double energy[1] = {0.0};
+  if (e < m.nt) {
+    const TetLocal t = load_tet(m, e);
+    double x[4][3], d[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const size_t k = 3 * (size_t)t.id[i] + a;
+        const double uu = u[k];
+        x[i][a] = m.X[k] + uu;
+        d[i][a] = uu - uhat[k];
+      }
+    double Fd[9];
+    def_grad(x, t.B, Fd);
+    T F[9], Bt[9], C[9], P[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      F[k] = (T)Fd[k];
+      Bt[k] = (T)t.B[k];
+    }
+    const SnhScalars<T> s = snh_scalars(F, (T)t.lam, (T)t.mu);
+    cofactor(F, C);
+    snh_stress(F, C, s, P);
+    T f[4][3];
+    local_force(P, Bt, f);
+    const double mm = m.rho * t.V / 20.0;
+    const double sc = dt2 * t.V;
+    double S[3], dd = 0, ss = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) S[a] = d[0][a] + d[1][a] + d[2][a] + d[3][a];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const size_t pos = 3 * (size_t)gperm[4 * (size_t)e + i];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        dd += d[i][a] * d[i][a];
+        gcontrib[pos + a] = mm * (d[i][a] + S[a]) + sc * (double)f[i][a];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ss += S[a] * S[a];
+    energy[0] = 0.5 * mm * (dd + ss) + sc * (double)s.psi;
+  }
+  double tot[1];
+  if (grid_reduce<1>(energy, partials, counter, tot) && threadIdx.x == 0) *E_out = tot[0];
+}
+
+void launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
+                      double* gcontrib, double* partials, unsigned* counter, double* E_out, bool fp32,
+                      hipStream_t s) {
+  const int nb = nblocks(m.nt > 0 ? m.nt : 1);
+  if (fp32)
+    k_elem_grad<float><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, partials, counter, E_out);
+  else
+    k_elem_grad<double><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, partials, counter, E_out);
+}
+
+// segmented reduce of the vertex-sorted gradient contributions; Dirichlet dofs zeroed (boundary.hpp:45-51)
+__global__ __launch_bounds__(kBlock) void k_grad_reduce(int nv, const int* __restrict__ gseg,
+                                                        const double* __restrict__ gcontrib,
+                                                        const unsigned char* __restrict__ dof, double* __restrict__ g,
+                                                        double* partials, unsigned* counter, double* gg_out) {
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  double gg[1] = {0.0};
+  if (v < nv) {
+    double acc[3] = {0, 0, 0};
+    const int lo = gseg[v], hi = gseg[v + 1];
+    for (int q = lo; q < hi; ++q) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) acc[a] += gcontrib[3 * (size_t)q + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double val = dof[3 * (size_t)v + a] ? 0.0 : acc[a];
+      g[3 * (size_t)v + a] = val;
+      gg[0] += val * val;
+    }
+  }
+  double tot[1];
+  if (grid_reduce<1>(gg, partials, counter, tot) && threadIdx.x == 0) *gg_out = tot[0];
+}
+
+void launch_grad_reduce(int nv, const int* gseg, const double* gcontrib, const unsigned char* dof, double* g,
+                        double* partials, unsigned* counter, double* gg_out, hipStream_t s) {
+  k_grad_reduce<<<nblocks(nv > 0 ? nv : 1), kBlock, 0, s>>>(nv, gseg, gcontrib, dof, g, partials, counter, gg_out);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Hessian (update_hessian, time_step.hpp:626-658; wrapped_pfpx :41-59)
+
+// Block (v, w), v ≤ w, of K = pfpxᵀ·H·pfpx: K_vw[b][d] = Σ_{c,e} D[v][c]·H[3b+c][3d+e]·D[w][e].
+template <typename T, typename Emit>
+__device__ __forceinline__ void pullback_blocks(const T H[45], const T D[4][3], Emit&& emit) {
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    T G[9][3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd)
+        G[i][dd] = H[sym9(i, 3 * dd + 0)] * D[w][0] + H[sym9(i, 3 * dd + 1)] * D[w][1] + H[sym9(i, 3 * dd + 2)] * D[w][2];
+#pragma unroll
+    for (int v = 0; v <= w; ++v) {
+      T K[9];
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd)
+          K[b * 3 + dd] = D[v][0] * G[3 * b + 0][dd] + D[v][1] * G[3 * b + 1][dd] + D[v][2] * G[3 * b + 2][dd];
+      emit(v, w, K);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void shape_grads(const T B[9], T D[4][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    D[0][c] = -B[0 * 3 + c] - B[1 * 3 + c] - B[2 * 3 + c];
+    D[1][c] = B[0 * 3 + c];
+    D[2][c] = B[1 * 3 + c];
+    D[3][c] = B[2 * 3 + c];
+  }
+}
+
+// pair index of local blocks (v ≤ w): w(w+1)/2 + v
+__host__ __device__ constexpr int pair_index(int v, int w) { return w * (w + 1) / 2 + v; }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_elem_hess(MeshView m, const double* __restrict__ u, double dt2,
+                                                      int make_spsd, const int* __restrict__ hperm,
+                                                      double* __restrict__ hcontrib) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= m.nt) return;
+  const TetLocal t = load_tet(m, e);
+  double x[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const size_t k = 3 * (size_t)t.id[i] + a;
+      x[i][a] = m.X[k] + u[k];
+    }
+  double Fd[9];
+  def_grad(x, t.B, Fd);
+  T F[9], Bt[9], C[9], H[45], D[4][3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    F[k] = (T)Fd[k];
+    Bt[k] = (T)t.B[k];
+  }
+  const T lam = (T)t.lam;
+  const SnhScalars<T> s = snh_scalars(F, lam, (T)t.mu);
+  cofactor(F, C);
+  snh_hessian_packed(F, C, lam, s, H);
+  if (make_spsd) snh_project_spd(F, lam, s, (T)kSpdFloor, H);
+  shape_grads(Bt, D);
+  const double sc = dt2 * t.V;
+  const double mm = m.rho * t.V / 20.0;
+  const int* pp = hperm + 10 * (size_t)e;
+  pullback_blocks(H, D, [&](int v, int w, const T K[9]) {
+    const size_t pos = 9 * (size_t)pp[pair_index(v, w)];
+    const double mvw = (v == w) ? 2.0 * mm : mm;
+    const bool tr = t.id[v] > t.id[w];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd) {
+        const double kv = sc * (double)(tr ? K[dd * 3 + b] : K[b * 3 + dd]) + (b == dd ? mvw : 0.0);
+        hcontrib[pos + b * 3 + dd] = kv;
+      }
+  });
+}
+
+void launch_elem_hess(const MeshView& m, const double* u, double dt2, int make_spsd, const int* hperm,
+                      double* hcontrib, bool fp32, hipStream_t s) {
+  if (m.nt <= 0) return;
+  if (fp32)
+    k_elem_hess<float><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, dt2, make_spsd, hperm, hcontrib);
+  else
+    k_elem_hess<double><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, dt2, make_spsd, hperm, hcontrib);
+}
+
+// One thread per upper block (r ≤ c): sum its contiguous contribution run, apply the Dirichlet identity
+// filter (identity_hessian_impl, boundary.hpp:53-67), write block (r,c) and its transpose (c,r) into the
+// blocked-ELL values, and the Jacobi inverse diagonal for r == c.
+__global__ __launch_bounds__(kBlock) void k_hess_reduce(int nub, const int* __restrict__ hseg,
+                                                        const double* __restrict__ hcontrib,
+                                                        const int2* __restrict__ ub_rc, const int2* __restrict__ ub_pos,
+                                                        const unsigned char* __restrict__ dof, int filter_dbc,
+                                                        double* __restrict__ val, double* __restrict__ dinv,
+                                                        double* __restrict__ diag) {
+  const int ub = blockIdx.x * kBlock + threadIdx.x;
+  if (ub >= nub) return;
+  double S[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) S[k] = 0;
+  const int lo = hseg[ub], hi = hseg[ub + 1];
+  for (int q = lo; q < hi; ++q) {
+    const double* src = hcontrib + 9 * (size_t)q;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) S[k] += src[k];
+  }
+  const int2 rc = ub_rc[ub];
+  if (filter_dbc) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd)
+        if (dof[3 * (size_t)rc.x + b] || dof[3 * (size_t)rc.y + dd]) S[b * 3 + dd] = (rc.x == rc.y && b == dd) ? 1.0 : 0.0;
+  }
+  const int2 pos = ub_pos[ub];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) val[(size_t)pos.x + (size_t)k * kSlice] = S[k];
+  if (rc.x != rc.y) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd) val[(size_t)pos.y + (size_t)(dd * 3 + b) * kSlice] = S[b * 3 + dd];
+  } else {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      dinv[3 * (size_t)rc.x + b] = 1.0 / S[b * 4];
+      if (diag) diag[3 * (size_t)rc.x + b] = S[b * 4];
+    }
+  }
+}
+
+void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const int2* ub_rc, const int2* ub_pos,
+                        const unsigned char* dof, int filter_dbc, double* bell_val, double* dinv, double* diag,
+                        hipStream_t s) {
+  if (nub <= 0) return;
+  k_hess_reduce<<<nblocks(nub), kBlock, 0, s>>>(nub, hseg, hcontrib, ub_rc, ub_pos, dof, filter_dbc, bell_val, dinv,
+                                                diag);
+}
+
+// per-element outputs for parity checks: Ψ, P, H (full 9x9, projected if make_spsd), K (full 12x12, unscaled)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_elem_eval(MeshView m, const double* __restrict__ u, int make_spsd,
+                                                      double* psi, double* Pout, double* Hout, double* Kout) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= m.nt) return;
+  const TetLocal t = load_tet(m, e);
+  double x[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const size_t k = 3 * (size_t)t.id[i] + a;
+      x[i][a] = m.X[k] + u[k];
+    }
+  double Fd[9];
+  def_grad(x, t.B, Fd);
+  T F[9], Bt[9], C[9], P[9], H[45], D[4][3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    F[k] = (T)Fd[k];
+    Bt[k] = (T)t.B[k];
+  }
+  const T lam = (T)t.lam;
+  const SnhScalars<T> s = snh_scalars(F, lam, (T)t.mu);
+  cofactor(F, C);
+  snh_stress(F, C, s, P);
+  if (psi) psi[e] = (double)s.psi;
+  if (Pout)
+    for (int k = 0; k < 9; ++k) Pout[9 * (size_t)e + k] = (double)P[k];
+  if (Hout || Kout) {
+    snh_hessian_packed(F, C, lam, s, H);
+    if (make_spsd) snh_project_spd(F, lam, s, (T)kSpdFloor, H);
+    if (Hout)
+      for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j) Hout[81 * (size_t)e + i * 9 + j] = (double)H[sym9(i, j)];
+    if (Kout) {
+      shape_grads(Bt, D);
+      double* Ke = Kout + 144 * (size_t)e;
+      pullback_blocks(H, D, [&](int v, int w, const T K[9]) {
+        for (int b = 0; b < 3; ++b)
+          for (int dd = 0; dd < 3; ++dd) {
+            Ke[(3 * v + b) * 12 + 3 * w + dd] = (double)K[b * 3 + dd];
+            Ke[(3 * w + dd) * 12 + 3 * v + b] = (double)K[b * 3 + dd];
+          }
+      });
+    }
+  }
+}
+
+void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double* psi, double* P, double* H,
+                      double* K, bool fp32, hipStream_t s) {
+  if (m.nt <= 0) return;
+  if (fp32)
+    k_elem_eval<float><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, make_spsd, psi, P, H, K);
+  else
+    k_elem_eval<double><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, make_spsd, psi, P, H, K);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// blocked-ELL SpMV: one lane per block row, coalesced 64-wide loads of every value component
+
+__device__ __forceinline__ void bell_row(const BellView& A, int v, const double* __restrict__ x, double y[3]) {
+  const int s = v >> 6, lane = v & 63;
+  const int W = A.width[s];
+  const int off = A.col_off[s];
+  const int* col = A.col + off + lane;
+  const double* val = A.val + 9 * (size_t)off + lane;
+  double y0 = 0, y1 = 0, y2 = 0;
+  for (int k = 0; k < W; ++k) {
+    const int w = col[k * kSlice];
+    if (w >= 0) {
+      const double x0 = x[3 * (size_t)w], x1 = x[3 * (size_t)w + 1], x2 = x[3 * (size_t)w + 2];
+      const double* vk = val + (size_t)k * 9 * kSlice;
+      y0 += vk[0 * kSlice] * x0 + vk[1 * kSlice] * x1 + vk[2 * kSlice] * x2;
+      y1 += vk[3 * kSlice] * x0 + vk[4 * kSlice] * x1 + vk[5 * kSlice] * x2;
+      y2 += vk[6 * kSlice] * x0 + vk[7 * kSlice] * x1 + vk[8 * kSlice] * x2;
+    }
+  }
+  y[0] = y0;
+  y[1] = y1;
+  y[2] = y2;
+}
+
+__global__ __launch_bounds__(kBlock) void k_spmv(BellView A, const double* __restrict__ x, double* __restrict__ y) {
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= A.nrows) return;
+  double r[3];
+  bell_row(A, v, x, r);
+  y[3 * (size_t)v] = r[0];
+  y[3 * (size_t)v + 1] = r[1];
+  y[3 * (size_t)v + 2] = r[2];
+}
+
+void launch_spmv(const BellView& A, const double* x, double* y, hipStream_t s) {
+  if (A.nrows <= 0) return;
+  k_spmv<<<nblocks(A.nrows), kBlock, 0, s>>>(A, x, y);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                                double* partials, unsigned* counter, double* out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v[1] = {i < n ? a[i] * b[i] : 0.0};
+  double tot[1];
+  if (grid_reduce<1>(v, partials, counter, tot) && threadIdx.x == 0) *out = tot[0];
+}
+
+void launch_dot(int n, const double* a, const double* b, double* partials, unsigned* counter, double* out,
+                hipStream_t s) {
+  k_dot<<<nblocks(n > 0 ? n : 1), kBlock, 0, s>>>(n, a, b, partials, counter, out);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// vector kernels of the step (time_step.hpp:499-569)
+
+__global__ void k_prepare(int n, const double* __restrict__ u, const double* __restrict__ vel,
+                          const double* __restrict__ accel, const unsigned char* __restrict__ dof,
+                          const double* __restrict__ dbc, double dt, double* __restrict__ uhat,
+                          double* __restrict__ unext) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double uh = u[i];
+  uh += dt * vel[i];
+  uh += dt * dt * accel[i];
+  if (dof[i]) uh = dbc[i];
+  uhat[i] = uh;
+  unext[i] = uh;
+}
+
+void launch_prepare(int n, const double* u, const double* vel, const double* accel, const unsigned char* dof,
+                    const double* dbc, double dt, double* uhat, double* unext, hipStream_t s) {
+  if (n <= 0) return;
+  k_prepare<<<nblocks(n), kBlock, 0, s>>>(n, u, vel, accel, dof, dbc, dt, uhat, unext);
+}
+
+__global__ void k_step_next(int n, const double* __restrict__ unext, double* __restrict__ u,
+                            double* __restrict__ uprev, double* __restrict__ vel, double dt, double damping) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double cur = u[i], nxt = unext[i];
+  uprev[i] = cur;
+  u[i] = nxt;
+  double v = nxt;
+  v += -1.0 * cur;
+  v *= 1.0 / dt;
+  if (damping != 0.0) v *= (1.0 - damping);
+  vel[i] = v;
+}
+
+void launch_step_next(int n, const double* unext, double* u, double* uprev, double* vel, double dt, double damping,
+                      hipStream_t s) {
+  if (n <= 0) return;
+  k_step_next<<<nblocks(n), kBlock, 0, s>>>(n, unext, u, uprev, vel, dt, damping);
+}
+
+__global__ void k_restore(int n, const unsigned char* __restrict__ dof, const double* __restrict__ dbc,
+                          double* __restrict__ u, double* __restrict__ uprev, double* __restrict__ vel, double dt) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double up = uprev[i], cu = u[i];
+  if (dof[i]) {
+    up = dbc[i];
+    cu = dbc[i];
+  }
+  uprev[i] = up;
+  u[i] = cu;
+  double v = cu;
+  v += -1.0 * up;
+  v *= 1.0 / dt;
+  vel[i] = v;
+}
+
+void launch_restore(int n, const unsigned char* dof, const double* dbc, double* u, double* uprev, double* vel,
+                    double dt, hipStream_t s) {
+  if (n <= 0) return;
+  k_restore<<<nblocks(n), kBlock, 0, s>>>(n, dof, dbc, u, uprev, vel, dt);
+}
+
+__global__ void k_axpy_to(int n, const double* __restrict__ x, double alpha, const double* __restrict__ d,
+                          double* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = x[i] + alpha * d[i];
+}
+
+void launch_axpy_to(int n, const double* x, double alpha, const double* d, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  k_axpy_to<<<nblocks(n), kBlock, 0, s>>>(n, x, alpha, d, out);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Jacobi-PCG (Hestenes–Stiefel), x0 = 0, stop when ‖r‖₂² ≤ rtol²·‖b‖₂² or after maxit iterations.
+// Scalars never leave the device; a solved system turns every later kernel of the chunk into an early exit.
+
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const double* __restrict__ b, double sign,
+                                                     const double* __restrict__ dinv, double* __restrict__ x,
+                                                     double* __restrict__ r, double* __restrict__ z,
+                                                     double* __restrict__ p, double rtol, int maxit, PcgScalars* sc,
+                                                     double* partials) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    const double bi = sign * b[i];
+    const double zi = dinv[i] * bi;
+    x[i] = 0.0;
+    r[i] = bi;
+    z[i] = zi;
+    p[i] = zi;
+    v[0] = bi * zi;
+    v[1] = bi * bi;
+  }
+  double tot[2];
+  if (grid_reduce<2>(v, partials, &sc->cnt[0], tot) && threadIdx.x == 0) {
+    sc->rz = tot[0];
+    sc->bb = tot[1];
+    sc->rr = tot[1];
+    sc->tol2 = rtol * rtol * tot[1];
+    sc->iters = 0;
+    sc->maxit = maxit;
+    sc->breakdown = 0;
+    sc->beta = 0.0;
+    sc->done = (tot[1] <= sc->tol2 || maxit <= 0 || !(tot[1] == tot[1])) ? 1 : 0;
+  }
+}
+
+void launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
+                     double* p, double rtol, int maxit, PcgScalars* sc, double* partials, hipStream_t s) {
+  k_pcg_init<<<nblocks(n > 0 ? n : 1), kBlock, 0, s>>>(n, b, sign, dinv, x, r, z, p, rtol, maxit, sc, partials);
+}
+
+// q = A p, pq = p·q, α = rz/pq
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(BellView A, const double* __restrict__ p, double* __restrict__ q,
+                                                     PcgScalars* sc, double* partials) {
+  if (sc->done) return;
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  double acc[1] = {0.0};
+  if (v < A.nrows) {
+    double y[3];
+    bell_row(A, v, p, y);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      q[3 * (size_t)v + a] = y[a];
+      acc[0] += p[3 * (size_t)v + a] * y[a];
+    }
+  }
+  double tot[1];
+  if (grid_reduce<1>(acc, partials, &sc->cnt[1], tot) && threadIdx.x == 0) {
+    sc->pq = tot[0];
+    if (!(tot[0] > 0.0)) {
+      sc->breakdown = 1;
+      sc->done = 1;
+      sc->alpha = 0.0;
+    } else {
+      sc->alpha = sc->rz / tot[0];
+    }
+  }
+}
+
+// x += α p, r −= α q, z = D⁻¹ r, rz' = r·z, rr = r·r, β = rz'/rz
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __restrict__ dinv, double* __restrict__ x,
+                                                       double* __restrict__ r, double* __restrict__ z,
+                                                       const double* __restrict__ p, const double* __restrict__ q,
+                                                       PcgScalars* sc, double* partials) {
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < n) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    const double zi = dinv[i] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    v[0] = ri * zi;
+    v[1] = ri * ri;
+  }
+  double tot[2];
+  if (grid_reduce<2>(v, partials, &sc->cnt[2], tot) && threadIdx.x == 0) {
+    sc->beta = tot[0] / sc->rz;
+    sc->rz = tot[0];
+    sc->rr = tot[1];
+    sc->iters += 1;
+    if (tot[1] <= sc->tol2 || sc->iters >= sc->maxit) sc->done = 1;
+    if (!(tot[1] == tot[1])) {
+      sc->breakdown = 2;
+      sc->done = 1;
+    }
+  }
+}
+
+// p = z + β p
+__global__ __launch_bounds__(kBlock) void k_pcg_p(int n, const double* __restrict__ z, double* __restrict__ p,
+                                                  const PcgScalars* sc) {
+  if (sc->done) return;
+  const double beta = sc->beta;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+void launch_pcg_spmv(const BellView& A, const double* p, double* q, PcgScalars* sc, double* partials, hipStream_t s) {
+  k_pcg_spmv<<<nblocks(A.nrows > 0 ? A.nrows : 1), kBlock, 0, s>>>(A, p, q, sc, partials);
+}
+
+void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
+                       PcgScalars* sc, double* partials, hipStream_t s) {
+  k_pcg_update<<<nblocks(n > 0 ? n : 1), kBlock, 0, s>>>(n, dinv, x, r, z, p, q, sc, partials);
+}
+
+void launch_pcg_p(int n, const double* z, double* p, const PcgScalars* sc, hipStream_t s) {
+  k_pcg_p<<<nblocks(n > 0 ? n : 1), kBlock, 0, s>>>(n, z, p, sc);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// L-BFGS helpers
+
+__global__ void k_scale_div(int n, const double* __restrict__ q, const double* __restrict__ diag,
+                            double* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = q[i] / diag[i];
+}
+
+void launch_scale_div(int n, const double* q, const double* diag, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  k_scale_div<<<nblocks(n), kBlock, 0, s>>>(n, q, diag, out);
+}
+
+// y += (c1·(*p1) − c2·(*p2))·x with the coefficients read from device memory (p2 may be null)
+__global__ void k_axpy_coef(int n, double c1, const double* __restrict__ p1, double c2, const double* __restrict__ p2,
+                            const double* __restrict__ x, double* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const double a = c1 * (*p1);
+  const double coef = p2 ? a - c2 * (*p2) : a;
+  if (i < n) y[i] += x[i] * coef;
+}
+
+void launch_axpy_coef(int n, double c1, const double* p1, double c2, const double* p2, const double* x, double* y,
+                      hipStream_t s) {
+  if (n <= 0) return;
+  k_axpy_coef<<<nblocks(n), kBlock, 0, s>>>(n, c1, p1, c2, p2, x, y);
+}
+
+__global__ void k_sub(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = a[i] - b[i];
+}
+
+void launch_sub(int n, const double* a, const double* b, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  k_sub<<<nblocks(n), kBlock, 0, s>>>(n, a, b, out);
+}
+
+}  // namespace ssim_hip

@@ -1,0 +1,209 @@
+"""Parity of the HIP path (through the C ABI) with the CPU FP64 oracle on identical seeded inputs.
+
+Bars: element values (Ψ, P, H, H⁺, K) ≤ 1e-10 relative per element; energy, gradient, Hessian and mass
+≤ 1e-10 relative (norm-wise); PCG solutions ≤ 1e-8; positions after N Newton steps ≤ 1e-8 relative to the
+oracle's positions (‖x_gpu − x_orc‖∞ / ‖x_orc‖∞ with x = X + u).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import per_item_rel_err, rel_err
+from ssim_amd import fem, mesh
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "snh_autograd.json")
+
+
+def _pair(case, method="newton", rtol=1e-3, **solver):
+    g = fem.TetFiniteElementSolver_Hip.from_case(case, method=method, rtol=rtol)
+    o = oracle.TimeStep.from_case(case)
+    o.set_threshold(rtol)
+    if solver:
+        g.set_solver_params(**solver)
+        o.set_solver(**{k: v for k, v in solver.items() if k in ("max_newton", "pcg_rtol", "pcg_maxit", "ls_c",
+                                                                "ls_beta", "ls_max")})
+    return g, o
+
+
+@pytest.fixture(scope="module")
+def bar():
+    return mesh.cantilever_bar(6, 3, 2, seed=11)
+
+
+def _strained(case, seed, amp):
+    rng = np.random.default_rng(seed)
+    u = case.u0 + amp * 0.01 * rng.standard_normal(3 * case.nv)
+    return u
+
+
+@pytest.mark.parametrize("amp", [0.05, 0.4, 1.5])
+def test_element_values_match_oracle(bar, amp):
+    print(f"[gpu] elements amp={amp} nt={bar.nt}", flush=True)
+    g, o = _pair(bar)
+    u = _strained(bar, 1, amp)
+    ref = o.elements(u, make_spsd=True, want_K=True)
+    for spsd in (True, False):
+        got = g.eval_elements(u, make_spsd=spsd, want_H=True, want_K=True)
+        assert per_item_rel_err(got["psi"][:, None], ref["psi"][:, None]) <= 1e-10 or \
+            np.abs(got["psi"] - ref["psi"]).max() <= 1e-10 * np.abs(ref["psi"]).max()
+        assert per_item_rel_err(got["P"], ref["P"]) <= 1e-10
+        assert per_item_rel_err(got["H"], ref["Hp"] if spsd else ref["H"]) <= 1e-10
+        if spsd:
+            assert per_item_rel_err(got["K"], ref["K"]) <= 1e-10
+
+
+def test_projection_on_golden_deformation_gradients():
+    """Single tets deformed to the fixture F's (rest, large strain, compressed, inverted, rotated): H⁺ vs
+    numpy eigh of the torch-autograd Hessian."""
+    with open(GOLDEN) as f:
+        gold = json.load(f)
+    X, C = mesh.tetra()
+    for k, c in enumerate(gold["cases"]):
+        F = np.array(c["F"]).reshape(3, 3)
+        u = (X @ F.T - X).reshape(-1)
+        nu = c["lam"] / (2 * (c["lam"] + c["mu"]))
+        E = c["mu"] * 2 * (1 + nu)
+        s = fem.TetFiniteElementSolver_Hip(X, C, 1e-2, E, nu, 1.0)
+        s.set_material_per_tet([c["lam"]], [c["mu"]])
+        got = s.eval_elements(u, make_spsd=True)
+        assert rel_err(got["H"][0].reshape(-1), c["Hp"]) <= 1e-10, k
+        raw = s.eval_elements(u, make_spsd=False)
+        assert rel_err(raw["H"][0].reshape(-1), c["H"]) <= 1e-10, k
+        assert rel_err(raw["P"][0].reshape(-1), c["P"]) <= 1e-10, k
+
+
+def test_energy_gradient_hessian_mass_match_oracle(bar):
+    print("[gpu] assembly", flush=True)
+    g, o = _pair(bar)
+    rng = np.random.default_rng(2)
+    u = _strained(bar, 3, 0.3)
+    uhat = u + 1e-3 * rng.standard_normal(3 * bar.nv)
+    uhat[bar.dof_type == 1] = 0
+    g.set_array("next_deform", u)
+    g.set_array("inertia", uhat)
+    o.set_vectors(u_hat=uhat)
+    E = g.update_energy_and_gradients()
+    Eo, go = o.energy_grad(u)
+    assert abs(E - Eo) <= 1e-10 * abs(Eo)
+    assert rel_err(g.forces().T.reshape(-1), go) <= 1e-10
+    for spsd in (True, False):
+        for filt in (True, False):
+            if filt:
+                g.update_hessian(spsd)
+            else:
+                g.update_hessian_unfiltered(spsd)
+            Hg = g.hessian()
+            Ho = o.hessian_csr(u, spsd, filt)
+            assert np.array_equal(Hg.indptr, Ho.indptr) and np.array_equal(Hg.indices, Ho.indices)
+            assert rel_err(Hg.data, Ho.data) <= 1e-10
+            assert np.array_equal(g.hessian_nonzeros(), Hg.data)
+    M = g.mass_matrix()
+    assert np.array_equal(M.indptr, Ho.indptr) and np.array_equal(M.indices, Ho.indices)
+    assert rel_err(M.data, o.mass_values()) <= 1e-12
+    assert abs(g.total_mass() - o.total_mass()) <= 1e-12 * o.total_mass()
+
+
+def test_spmv_and_pcg_match_oracle(bar):
+    print("[gpu] pcg", flush=True)
+    g, o = _pair(bar)
+    u = _strained(bar, 5, 0.2)
+    g.set_array("next_deform", u)
+    g.update_hessian(True)
+    H = o.hessian_csr(u, True, True)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(3 * bar.nv)
+    assert rel_err(g.spmv(x), H @ x) <= 1e-13
+    b = rng.standard_normal(3 * bar.nv)
+    xg, itg = g.pcg_solve(b, rtol=1e-8)
+    xo, ito, rel = oracle.pcg(H.indptr, H.indices, H.data, b, rtol=1e-8)
+    assert abs(itg - ito) <= 1
+    assert rel_err(xg, xo) <= 1e-8
+    assert np.linalg.norm(H @ xg - b) <= 1.01e-8 * np.linalg.norm(b)
+    xz, itz = g.pcg_solve(np.zeros_like(b))
+    assert itz == 0 and not xz.any()
+
+
+def _positions(case, u):
+    return (case.X.reshape(-1) + u)
+
+
+def test_newton_trajectory_config1_60_steps():
+    """Configuration 1 of the benchmark (8x8x32 bar, 60 steps, E=1e5, ν=0.45, ρ=1000, dt=1/60): positions after
+    every step within 1e-8 of the oracle, and identical Newton counts."""
+    case = mesh.cantilever_bar(32, 8, 8, seed=0)
+    g, o = _pair(case)
+    worst = 0.0
+    for step in range(60):
+        g.step()
+        _, st = o.step()
+        sg = g.last_stats()
+        assert sg["newton_iterations"] == st["newton"], (step, sg, st)
+        xo = _positions(case, o.get_state()[0])
+        xg = _positions(case, g.deformation().T.reshape(-1))
+        worst = max(worst, rel_err(xg, xo))
+        assert worst <= 1e-8, (step, worst)
+    print(f"[gpu] config1 60 steps: max position rel err {worst:.3e}", flush=True)
+
+
+def test_lbfgs_trajectory_matches_oracle():
+    case = mesh.twisted_cube(n=4, seed=3)
+    g, o = _pair(case, method="lbfgs", rtol=1e-6)
+    for step in range(3):
+        g.step()
+        o.step("lbfgs")
+        xo = _positions(case, o.get_state()[0])
+        xg = _positions(case, g.deformation().T.reshape(-1))
+        assert rel_err(xg, xo) <= 1e-8, step
+
+
+def test_repeat_runs_are_bitwise_identical(bar):
+    outs = []
+    for _ in range(2):
+        g, _o = _pair(bar)
+        for _s in range(3):
+            g.step()
+        outs.append((g.deformation().copy(), g.forces().copy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_dirichlet_marking_and_release(bar):
+    g, o = _pair(bar)
+    free_end = np.nonzero(bar.X[:, 0] == bar.X[:, 0].max())[0].astype(np.int32)
+    targ = np.zeros((3, free_end.size))
+    targ[1] = 0.002
+    g.mark_dirichlet_batched(free_end, targ)
+    g.reset()
+    g.step()
+    d = g.deformation()
+    assert np.allclose(d[:, free_end], targ, rtol=0, atol=0)
+    g.mark_general_batched(free_end)
+    g.step()
+    assert not np.allclose(g.deformation()[:, free_end], targ)
+
+
+def test_setters_validate_like_the_reference(bar):
+    g, _ = _pair(bar)
+    with pytest.raises(ValueError):
+        g.set_rtol(0.0)
+    with pytest.raises(ValueError):
+        g.set_solver_params(ls_c=2.0)
+    with pytest.raises(ValueError):
+        g.set_material_per_tet(np.zeros(bar.nt), np.ones(bar.nt))
+
+
+def test_fp32_element_variant_error_is_bounded(bar):
+    """FP32 element kernels (FP64 reductions and PCG): per-element stress error vs the FP64 oracle."""
+    g, o = _pair(bar)
+    g.set_solver_params(fp32_elements=1)
+    u = _strained(bar, 9, 0.3)
+    got = g.eval_elements(u, make_spsd=True, want_H=True)
+    ref = o.elements(u, make_spsd=True)
+    errP = per_item_rel_err(got["P"], ref["P"])
+    errH = per_item_rel_err(got["H"], ref["Hp"])
+    print(f"[gpu] fp32 element variant: P rel err {errP:.2e}, H+ rel err {errH:.2e}", flush=True)
+    assert errP <= 1e-4 and errH <= 1e-3
