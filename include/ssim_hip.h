@@ -162,9 +162,9 @@ typedef enum {
   SSIM_KERNEL_GRAD_REDUCE = 1, /* per-vertex segmented reduce of the gradient        */
   SSIM_KERNEL_ELEM_HESS = 2,   /* per-tet SNH Hessian, SPD projection, 3x3 blocks    */
   SSIM_KERNEL_HESS_REDUCE = 3, /* per-block segmented reduce into the blocked ELL    */
-  SSIM_KERNEL_PCG_SPMV = 4,    /* q = A p with p·q (one PCG iteration's SpMV)        */
-  SSIM_KERNEL_PCG_UPDATE = 5,  /* x, r, z updates with r·z and r·r                   */
-  SSIM_KERNEL_PCG_P = 6,       /* p = z + β p                                        */
+  SSIM_KERNEL_PCG_SPMV = 4,    /* p = z + βp fused with q = A p and p·q (PCG kernel A) */
+  SSIM_KERNEL_PCG_UPDATE = 5,  /* x, r, z updates with r·z and r·r (PCG kernel B)    */
+  SSIM_KERNEL_PCG_P = 6,       /* reserved (the p update is fused into kernel A)     */
   SSIM_KERNEL_COUNT = 7
 } ssim_kernel_t;
 

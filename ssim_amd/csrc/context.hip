@@ -94,7 +94,7 @@ struct ssim_ctx {
   // device: vertex arrays
   DBuf<double> arr[SSIM_ARRAY_COUNT];
   DBuf<unsigned char> dof;
-  DBuf<double> utry, gtry, dir, r, z, p, q, tmp, dinv, hrest;
+  DBuf<double> utry, gtry, dir, r, z, p, p2, q, tmp, dinv, hrest;
   // device: assembly maps
   DBuf<int> gperm, gseg, hperm, hseg, width, coloff, col;
   DBuf<int2> ub_rc, ub_pos;
@@ -103,6 +103,13 @@ struct ssim_ctx {
   DBuf<PcgScalars> pcg;
   DBuf<EvalScalars> ev;
   DBuf<double> partials, lb_S, lb_Y, lb_scal;
+  size_t pmax = 0;  // partials per slot
+  // partial-sum slots: pcg p·q, pcg (r·z, r·r) ping-pong, energy, ‖g‖², dot
+  double* part_pa() { return partials.p; }
+  double* part_pb(int k) { return partials.p + pmax + 2 * pmax * (size_t)(k & 1); }
+  double* part_e() { return partials.p + 5 * pmax; }
+  double* part_g() { return partials.p + 6 * pmax; }
+  double* part_d() { return partials.p + 7 * pmax; }
   double* pin = nullptr;  // pinned host mirror for scalar read-backs
 
   // sampled kernel timing: an event pair per kernel, armed at one launch and harvested at the next sync
@@ -368,7 +375,7 @@ void do_reset(ssim_ctx* c) {
     c->dof.alloc(n);
     HIPCHK(hipMemsetAsync(c->dof.p, 0, std::max(n, 1), s));
   }
-  for (DBuf<double>* b : {&c->utry, &c->gtry, &c->dir, &c->r, &c->z, &c->p, &c->q, &c->tmp, &c->dinv}) {
+  for (DBuf<double>* b : {&c->utry, &c->gtry, &c->dir, &c->r, &c->z, &c->p, &c->p2, &c->q, &c->tmp, &c->dinv}) {
     b->alloc(n);
     HIPCHK(hipMemsetAsync(b->p, 0, sizeof(double) * std::max(n, 1), s));
   }
@@ -379,8 +386,8 @@ void do_reset(ssim_ctx* c) {
   launch_rest_shape(c->X.p, c->cells.p, c->nt, c->B.p, c->V.p, s);
   HIPCHK(hipGetLastError());
   build_topology(c);
-  const int np = max_partials(std::max({c->nt, c->n, c->nub, 1}));
-  c->partials.alloc(2 * (size_t)np + 16);
+  c->pmax = (size_t)max_partials(std::max({c->nt, c->n, c->nub, 1}));
+  c->partials.alloc(8 * c->pmax);
   c->pcg.alloc(1);
   c->ev.alloc(1);
   HIPCHK(hipMemsetAsync(c->pcg.p, 0, sizeof(PcgScalars), s));
@@ -404,14 +411,15 @@ struct EvalOut {
 EvalOut eval_energy_grad(ssim_ctx* c, const double* u, double* g) {
   hipStream_t s = c->stream;
   const double dt2 = c->dt * c->dt;
+  int ne = 0, ng = 0;
   timed(c, SSIM_KERNEL_ELEM_GRAD, [&] {
-    launch_elem_grad(c->mesh(), u, c->a(SSIM_ARRAY_INERTIA), dt2, c->gperm.p, c->gcontrib.p, c->partials.p,
-                     &c->ev.p->cnt[0], &c->ev.p->energy, c->prm.fp32_elements != 0, s);
+    ne = launch_elem_grad(c->mesh(), u, c->a(SSIM_ARRAY_INERTIA), dt2, c->gperm.p, c->gcontrib.p, c->part_e(),
+                          c->prm.fp32_elements != 0, s);
   });
   timed(c, SSIM_KERNEL_GRAD_REDUCE, [&] {
-    launch_grad_reduce(c->nv, c->gseg.p, c->gcontrib.p, c->dof.p, g, c->partials.p, &c->ev.p->cnt[1], &c->ev.p->gg,
-                       s);
+    ng = launch_grad_reduce(c->nv, c->gseg.p, c->gcontrib.p, c->dof.p, g, c->part_g(), s);
   });
+  launch_finalize(c->part_e(), ne, 1, c->part_g(), ng, &c->ev.p->energy, s);  // energy, gg
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->pin, c->ev.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   sync(c);
@@ -420,7 +428,7 @@ EvalOut eval_energy_grad(ssim_ctx* c, const double* u, double* g) {
 }
 
 double device_dot(ssim_ctx* c, const double* a, const double* b) {
-  launch_dot(c->n, a, b, c->partials.p, &c->ev.p->cnt[2], &c->ev.p->dot, c->stream);
+  launch_dot(c->n, a, b, c->part_d(), &c->ev.p->dot, c->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->pin, &c->ev.p->dot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   sync(c);
@@ -445,23 +453,32 @@ void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_db
 int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit) {
   hipStream_t s = c->stream;
   const BellView A = c->bell(c->val.p);
-  launch_pcg_init(c->n, b, sign, c->dinv.p, x, c->r.p, c->z.p, c->p.p, rtol, maxit, c->pcg.p, c->partials.p, s);
+  const int nb = launch_pcg_init(c->n, b, sign, c->dinv.p, x, c->r.p, c->z.p, rtol, maxit, c->pcg.p,
+                                 c->part_pb(0), s);
   HIPCHK(hipGetLastError());
   int chunk = 8;
   int launched = 0;
   for (;;) {
-    for (int k = 0; k < chunk; ++k) {
+    for (int j = 0; j < chunk; ++j) {
+      const int k = launched + j;
+      double* p_new = (k & 1) ? c->p2.p : c->p.p;
+      const double* p_old = (k & 1) ? c->p.p : c->p2.p;
+      int na = 0;
+      auto spmv = [&] {
+        na = launch_pcg_spmv(A, c->z.p, p_old, p_new, c->q.p, c->part_pb(k), c->part_pb(k + 1), nb, k, c->pcg.p,
+                             c->part_pa(), s);
+      };
+      auto update = [&] {
+        launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, p_new, c->q.p, c->part_pa(), na, c->part_pb(k),
+                          c->part_pb(k + 1), c->pcg.p, s);
+      };
       // the first iteration of a chunk always does work (the previous chunk ended unconverged): sample it
-      if (k == 0) {
-        timed(c, SSIM_KERNEL_PCG_SPMV, [&] { launch_pcg_spmv(A, c->p.p, c->q.p, c->pcg.p, c->partials.p, s); });
-        timed(c, SSIM_KERNEL_PCG_UPDATE, [&] {
-          launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->pcg.p, c->partials.p, s);
-        });
-        timed(c, SSIM_KERNEL_PCG_P, [&] { launch_pcg_p(c->n, c->z.p, c->p.p, c->pcg.p, s); });
+      if (j == 0) {
+        timed(c, SSIM_KERNEL_PCG_SPMV, spmv);
+        timed(c, SSIM_KERNEL_PCG_UPDATE, update);
       } else {
-        launch_pcg_spmv(A, c->p.p, c->q.p, c->pcg.p, c->partials.p, s);
-        launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->pcg.p, c->partials.p, s);
-        launch_pcg_p(c->n, c->z.p, c->p.p, c->pcg.p, s);
+        spmv();
+        update();
       }
     }
     launched += chunk;
@@ -554,14 +571,14 @@ void lbfgs_solve(ssim_ctx* c) {
     for (int k = 0; k < hist; ++k) {
       const int idx = (head - 1 - k + m) % m;
       // a_i = ρ_i s_iᵀq ; q −= a_i y_i
-      launch_dot(n, c->lb_S.p + (size_t)idx * n, c->q.p, c->partials.p, &c->ev.p->cnt[3], av + idx, s);
+      launch_dot(n, c->lb_S.p + (size_t)idx * n, c->q.p, c->part_d(), av + idx, s);
       launch_axpy_coef(n, -rho_h[idx], av + idx, 0.0, nullptr, c->lb_Y.p + (size_t)idx * n, c->q.p, s);
     }
     launch_scale_div(n, c->q.p, c->hrest.p, d, s);  // r = H0 q
     for (int k = hist - 1; k >= 0; --k) {
       const int idx = (head - 1 - k + m) % m;
       // b = ρ_i y_iᵀr ; r += s_i (a_i − b)
-      launch_dot(n, c->lb_Y.p + (size_t)idx * n, d, c->partials.p, &c->ev.p->cnt[3], bv + idx, s);
+      launch_dot(n, c->lb_Y.p + (size_t)idx * n, d, c->part_d(), bv + idx, s);
       launch_axpy_coef(n, rho_h[idx], av + idx, rho_h[idx], bv + idx, c->lb_S.p + (size_t)idx * n, d, s);
     }
     HIPCHK(hipGetLastError());
@@ -928,8 +945,8 @@ ssim_status_t ssim_check_convergence(ssim_ctx* ctx, int* converged_out) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
   return run([&] {
     require_reset(ctx);
-    launch_dot(ctx->n, ctx->a(SSIM_ARRAY_FORCES), ctx->a(SSIM_ARRAY_FORCES), ctx->partials.p, &ctx->ev.p->cnt[2],
-               &ctx->ev.p->dot, ctx->stream);
+    launch_dot(ctx->n, ctx->a(SSIM_ARRAY_FORCES), ctx->a(SSIM_ARRAY_FORCES), ctx->part_d(), &ctx->ev.p->dot,
+               ctx->stream);
     HIPCHK(hipMemcpyAsync(ctx->pin, &ctx->ev.p->dot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     sync(ctx);
     if (converged_out) *converged_out = std::sqrt(ctx->pin[0]) < ctx->tol_abs;

@@ -34,28 +34,28 @@ struct BellView {
   int nrows, nslices;
 };
 
-// Scalars of one PCG solve, resident in device memory.  Counters implement last-workgroup reductions.
+// Scalars of one PCG solve, resident in device memory.
 struct PcgScalars {
   double rz, pq, alpha, beta, rr, bb, tol2, pad;
-  int done, iters, maxit, breakdown;
-  unsigned cnt[8];
+  int done, iters, maxit, breakdown;  // read back together by the host
 };
 
 // Reduction results of element / vector kernels.
 struct EvalScalars {
   double energy, gg, gd, dot;
-  unsigned cnt[8];
 };
 
 // ---- launchers (kernels.hip) ----
 void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, double* V, hipStream_t s);
-// energy + gradient contributions at deformation u (predictor uhat); energy written to *E_out
-void launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
-                      double* gcontrib, double* partials, unsigned* counter, double* E_out, bool fp32,
-                      hipStream_t s);
-// g[v] = Σ contributions of v, zeroed on Dirichlet dofs; ‖g‖² written to *gg_out
-void launch_grad_reduce(int nv, const int* gseg, const double* gcontrib, const unsigned char* dof, double* g,
-                        double* partials, unsigned* counter, double* gg_out, hipStream_t s);
+// Reductions: a kernel stores one partial per workgroup; launch_* return the number of partials written.
+// energy + gradient contributions at deformation u (predictor uhat); energy partials to part_e
+int launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
+                     double* gcontrib, double* part_e, bool fp32, hipStream_t s);
+// g[v] = Σ contributions of v, zeroed on Dirichlet dofs; ‖g‖² partials to part_g
+int launch_grad_reduce(int nv, const int* gseg, const double* gcontrib, const unsigned char* dof, double* g,
+                       double* part_g, hipStream_t s);
+// one workgroup: out[0..ka) = Σ pa (width ka ∈ {1,2}); if pb: out[ka] = Σ pb (width 1)
+void launch_finalize(const double* pa, int na, int ka, const double* pb, int nb, double* out, hipStream_t s);
 // Hessian contributions (10 oriented 3x3 blocks per tet, dt²V-scaled, consistent mass on the diagonal blocks)
 void launch_elem_hess(const MeshView& m, const double* u, double dt2, int make_spsd, const int* hperm,
                       double* hcontrib, bool fp32, hipStream_t s);
@@ -66,8 +66,8 @@ void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double*
                       double* K, bool fp32, hipStream_t s);
 
 void launch_spmv(const BellView& A, const double* x, double* y, hipStream_t s);
-void launch_dot(int n, const double* a, const double* b, double* partials, unsigned* counter, double* out,
-                hipStream_t s);
+// *out = a·b (partials in part, then a one-workgroup finalize)
+void launch_dot(int n, const double* a, const double* b, double* part, double* out, hipStream_t s);
 void launch_prepare(int n, const double* u, const double* vel, const double* accel, const unsigned char* dof,
                     const double* dbc, double dt, double* uhat, double* unext, hipStream_t s);
 void launch_step_next(int n, const double* unext, double* u, double* uprev, double* vel, double dt,
@@ -76,13 +76,17 @@ void launch_restore(int n, const unsigned char* dof, const double* dbc, double* 
                     double dt, hipStream_t s);
 void launch_axpy_to(int n, const double* x, double alpha, const double* d, double* out, hipStream_t s);
 
-// PCG on H x = sign·b with Jacobi preconditioner dinv
-void launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
-                     double* p, double rtol, int maxit, PcgScalars* sc, double* partials, hipStream_t s);
-void launch_pcg_spmv(const BellView& A, const double* p, double* q, PcgScalars* sc, double* partials, hipStream_t s);
+// PCG on H x = sign·b with Jacobi preconditioner dinv.  init writes (ρ0, ‖r0‖²) partials to pb[0] and
+// returns their count.  Iteration k: spmv (p_k from z_k, p_{k−1}; convergence test on pb[k%2]; partials pa),
+// update (reads pa and pb[k%2], writes pb[(k+1)%2]).
+int launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
+                    double rtol, int maxit, PcgScalars* sc, double* pb0, hipStream_t s);
+int launch_pcg_spmv(const BellView& A, const double* z, const double* p_old, double* p_new, double* q,
+                    const double* pb_cur, const double* pb_prev, int nb, int k, PcgScalars* sc, double* pa,
+                    hipStream_t s);
 void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
-                       PcgScalars* sc, double* partials, hipStream_t s);
-void launch_pcg_p(int n, const double* z, double* p, const PcgScalars* sc, hipStream_t s);
+                       const double* pa, int na, const double* pb_cur, double* pb_next, PcgScalars* sc, hipStream_t s);
+int pcg_vector_blocks(int n);
 
 // L-BFGS helpers
 void launch_scale_div(int n, const double* q, const double* diag, double* out, hipStream_t s);
@@ -91,6 +95,6 @@ void launch_axpy_coef(int n, double c1, const double* p1, double c2, const doubl
                       hipStream_t s);
 void launch_sub(int n, const double* a, const double* b, double* out, hipStream_t s);
 
-int max_partials(int n);
+int max_partials(int n);  // upper bound on partials any kernel writes for n items
 
 }  // namespace ssim_hip

@@ -21,14 +21,15 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sums K values over the grid.  Every workgroup stores its partial; the workgroup that arrives last sums all
-// partials in workgroup order (a fixed order, so the result is bitwise reproducible) and returns true with the
-// totals in tot[] (valid in every thread of that workgroup).
+// Reductions never synchronise workgroups inside a kernel.  A producing kernel stores one partial per
+// workgroup (plain stores); the consuming kernel -- the next launch on the stream, so the kernel boundary makes
+// the partials visible -- sums all partials in a fixed order in every workgroup.  Every workgroup therefore
+// sees the same, bitwise reproducible total.
+
+// block-wide sum of K values; the result is returned to every thread
 template <int K>
-__device__ __forceinline__ bool grid_reduce(double (&v)[K], double* partials, unsigned* counter, double (&tot)[K]) {
+__device__ __forceinline__ void block_sum(double (&v)[K]) {
   __shared__ double sm[K][kBlock / 64];
-  __shared__ double res[K];
-  __shared__ int is_last;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -36,48 +37,36 @@ __device__ __forceinline__ bool grid_reduce(double (&v)[K], double* partials, un
     if (lane == 0) sm[k][wid] = w;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = 0;
-#pragma unroll
-      for (int w = 0; w < kBlock / 64; ++w) s += sm[k][w];
-      __hip_atomic_store(&partials[(size_t)blockIdx.x * K + k], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  double acc[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = 0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock) {
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      acc[k] += __hip_atomic_load(&partials[(size_t)b * K + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const double w = wave_sum(acc[k]);
-    if (lane == 0) sm[k][wid] = w;
+    double s = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += sm[k][w];
+    v[k] = s;
   }
   __syncthreads();
+}
+
+// stores this workgroup's partial sums: part[blockIdx.x*K + k]
+template <int K>
+__device__ __forceinline__ void store_partials(double (&v)[K], double* part) {
+  block_sum<K>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double s = 0;
-#pragma unroll
-      for (int w = 0; w < kBlock / 64; ++w) s += sm[k][w];
-      res[k] = s;
-    }
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 0; k < K; ++k) part[(size_t)blockIdx.x * K + k] = v[k];
   }
-  __syncthreads();
+}
+
+// total of `count` partials of width K (written by an earlier kernel), in a fixed order
+template <int K>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int count, double (&tot)[K]) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) tot[k] = res[k];
-  return true;
+  for (int k = 0; k < K; ++k) tot[k] = 0;
+  for (int b = threadIdx.x; b < count; b += kBlock) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) tot[k] += part[(size_t)b * K + k];
+  }
+  block_sum<K>(tot);
 }
 
 int max_partials(int n) { return (n + kBlock - 1) / kBlock + 1; }
@@ -176,7 +165,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_elem_grad(MeshView m, const double* __restrict__ u,
                                                       const double* __restrict__ uhat, double dt2,
                                                       const int* __restrict__ gperm, double* __restrict__ gcontrib,
-                                                      double* partials, unsigned* counter, double* E_out) {
+                                                      double* __restrict__ part_e) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   double energy[1] = {0.0};
   if (e < m.nt) {
@@ -222,25 +211,24 @@ __global__ __launch_bounds__(kBlock) void k_elem_grad(MeshView m, const double* 
     for (int a = 0; a < 3; ++a) ss += S[a] * S[a];
     energy[0] = 0.5 * mm * (dd + ss) + sc * (double)s.psi;
   }
-  double tot[1];
-  if (grid_reduce<1>(energy, partials, counter, tot) && threadIdx.x == 0) *E_out = tot[0];
+  store_partials<1>(energy, part_e);
 }
 
-void launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
-                      double* gcontrib, double* partials, unsigned* counter, double* E_out, bool fp32,
-                      hipStream_t s) {
+int launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
+                     double* gcontrib, double* part_e, bool fp32, hipStream_t s) {
   const int nb = nblocks(m.nt > 0 ? m.nt : 1);
   if (fp32)
-    k_elem_grad<float><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, partials, counter, E_out);
+    k_elem_grad<float><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, part_e);
   else
-    k_elem_grad<double><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, partials, counter, E_out);
+    k_elem_grad<double><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, part_e);
+  return nb;
 }
 
 // segmented reduce of the vertex-sorted gradient contributions; Dirichlet dofs zeroed (boundary.hpp:45-51)
 __global__ __launch_bounds__(kBlock) void k_grad_reduce(int nv, const int* __restrict__ gseg,
                                                         const double* __restrict__ gcontrib,
                                                         const unsigned char* __restrict__ dof, double* __restrict__ g,
-                                                        double* partials, unsigned* counter, double* gg_out) {
+                                                        double* __restrict__ part_g) {
   const int v = blockIdx.x * kBlock + threadIdx.x;
   double gg[1] = {0.0};
   if (v < nv) {
@@ -257,13 +245,39 @@ __global__ __launch_bounds__(kBlock) void k_grad_reduce(int nv, const int* __res
       gg[0] += val * val;
     }
   }
-  double tot[1];
-  if (grid_reduce<1>(gg, partials, counter, tot) && threadIdx.x == 0) *gg_out = tot[0];
+  store_partials<1>(gg, part_g);
 }
 
-void launch_grad_reduce(int nv, const int* gseg, const double* gcontrib, const unsigned char* dof, double* g,
-                        double* partials, unsigned* counter, double* gg_out, hipStream_t s) {
-  k_grad_reduce<<<nblocks(nv > 0 ? nv : 1), kBlock, 0, s>>>(nv, gseg, gcontrib, dof, g, partials, counter, gg_out);
+int launch_grad_reduce(int nv, const int* gseg, const double* gcontrib, const unsigned char* dof, double* g,
+                       double* part_g, hipStream_t s) {
+  const int nb = nblocks(nv > 0 ? nv : 1);
+  k_grad_reduce<<<nb, kBlock, 0, s>>>(nv, gseg, gcontrib, dof, g, part_g);
+  return nb;
+}
+
+// one workgroup: out[k] = Σ partials (width K) for up to two partial arrays
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ pa, int na, int ka,
+                                                     const double* __restrict__ pb, int nb, double* __restrict__ out) {
+  double t[2];
+  if (ka == 2) {
+    sum_partials<2>(pa, na, t);
+  } else {
+    double t1[1];
+    sum_partials<1>(pa, na, t1);
+    t[0] = t1[0];
+    t[1] = 0;
+  }
+  double u1[1] = {0.0};
+  if (pb) sum_partials<1>(pb, nb, u1);
+  if (threadIdx.x == 0) {
+    out[0] = t[0];
+    if (ka == 2) out[1] = t[1];
+    if (pb) out[ka] = u1[0];
+  }
+}
+
+void launch_finalize(const double* pa, int na, int ka, const double* pb, int nb, double* out, hipStream_t s) {
+  k_finalize<<<1, kBlock, 0, s>>>(pa, na, ka, pb, nb, out);
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -514,17 +528,24 @@ void launch_spmv(const BellView& A, const double* x, double* y, hipStream_t s) {
   k_spmv<<<nblocks(A.nrows), kBlock, 0, s>>>(A, x, y);
 }
 
+constexpr int kVec = 4;  // elements per thread of the vector kernels
+static inline int vblocks(int n) { return (int)(((long long)(n > 0 ? n : 1) + kBlock * kVec - 1) / (kBlock * kVec)); }
+
 __global__ __launch_bounds__(kBlock) void k_dot(int n, const double* __restrict__ a, const double* __restrict__ b,
-                                                double* partials, unsigned* counter, double* out) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  double v[1] = {i < n ? a[i] * b[i] : 0.0};
-  double tot[1];
-  if (grid_reduce<1>(v, partials, counter, tot) && threadIdx.x == 0) *out = tot[0];
+                                                double* __restrict__ part) {
+  double v[1] = {0.0};
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
+    if (i < n) v[0] += a[i] * b[i];
+  }
+  store_partials<1>(v, part);
 }
 
-void launch_dot(int n, const double* a, const double* b, double* partials, unsigned* counter, double* out,
-                hipStream_t s) {
-  k_dot<<<nblocks(n > 0 ? n : 1), kBlock, 0, s>>>(n, a, b, partials, counter, out);
+void launch_dot(int n, const double* a, const double* b, double* part, double* out, hipStream_t s) {
+  const int nb = vblocks(n);
+  k_dot<<<nb, kBlock, 0, s>>>(n, a, b, part);
+  launch_finalize(part, nb, 1, nullptr, 0, out, s);
 }
 
 // ------------------------------------------------------------------------------------------------------
@@ -605,126 +626,184 @@ void launch_axpy_to(int n, const double* x, double alpha, const double* d, doubl
 }
 
 // ------------------------------------------------------------------------------------------------------
-// Jacobi-PCG (Hestenes–Stiefel), x0 = 0, stop when ‖r‖₂² ≤ rtol²·‖b‖₂² or after maxit iterations.
-// Scalars never leave the device; a solved system turns every later kernel of the chunk into an early exit.
+// Jacobi-PCG (Hestenes–Stiefel), x0 = 0, stop when ‖r‖₂² ≤ rtol²·‖b‖₂² or after maxit iterations, in two
+// kernels per iteration.  Scalars never leave the device: the (ρ = r·z, r·r) partials of residual k live in
+// pb[k%2] (the host alternates the buffers), p alternates between two vectors, and once the system is solved
+// every later kernel of the chunk returns at entry.
 
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const double* __restrict__ b, double sign,
                                                      const double* __restrict__ dinv, double* __restrict__ x,
                                                      double* __restrict__ r, double* __restrict__ z,
-                                                     double* __restrict__ p, double rtol, int maxit, PcgScalars* sc,
-                                                     double* partials) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+                                                     double* __restrict__ pb) {
   double v[2] = {0.0, 0.0};
-  if (i < n) {
-    const double bi = sign * b[i];
-    const double zi = dinv[i] * bi;
-    x[i] = 0.0;
-    r[i] = bi;
-    z[i] = zi;
-    p[i] = zi;
-    v[0] = bi * zi;
-    v[1] = bi * bi;
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
+    if (i < n) {
+      const double bi = sign * b[i];
+      const double zi = dinv[i] * bi;
+      x[i] = 0.0;
+      r[i] = bi;
+      z[i] = zi;
+      v[0] += bi * zi;
+      v[1] += bi * bi;
+    }
   }
-  double tot[2];
-  if (grid_reduce<2>(v, partials, &sc->cnt[0], tot) && threadIdx.x == 0) {
-    sc->rz = tot[0];
-    sc->bb = tot[1];
-    sc->rr = tot[1];
-    sc->tol2 = rtol * rtol * tot[1];
+  store_partials<2>(v, pb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_init_fin(const double* __restrict__ pb, int nb, double rtol,
+                                                         int maxit, PcgScalars* sc) {
+  double t[2];
+  sum_partials<2>(pb, nb, t);
+  if (threadIdx.x == 0) {
+    sc->rz = t[0];
+    sc->bb = t[1];
+    sc->rr = t[1];
+    sc->tol2 = rtol * rtol * t[1];
     sc->iters = 0;
     sc->maxit = maxit;
     sc->breakdown = 0;
-    sc->beta = 0.0;
-    sc->done = (tot[1] <= sc->tol2 || maxit <= 0 || !(tot[1] == tot[1])) ? 1 : 0;
+    sc->done = (t[1] <= sc->tol2 || maxit <= 0 || !(t[1] == t[1])) ? 1 : 0;
   }
 }
 
-void launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
-                     double* p, double rtol, int maxit, PcgScalars* sc, double* partials, hipStream_t s) {
-  k_pcg_init<<<nblocks(n > 0 ? n : 1), kBlock, 0, s>>>(n, b, sign, dinv, x, r, z, p, rtol, maxit, sc, partials);
+int launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
+                    double rtol, int maxit, PcgScalars* sc, double* pb, hipStream_t s) {
+  const int nb = vblocks(n);
+  k_pcg_init<<<nb, kBlock, 0, s>>>(n, b, sign, dinv, x, r, z, pb);
+  k_pcg_init_fin<<<1, kBlock, 0, s>>>(pb, nb, rtol, maxit, sc);
+  return nb;
 }
 
-// q = A p, pq = p·q, α = rz/pq
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(BellView A, const double* __restrict__ p, double* __restrict__ q,
-                                                     PcgScalars* sc, double* partials) {
+// Iteration k, kernel A: p_k = z_k + β p_{k−1} (β = ρ_k/ρ_{k−1}; p_0 = z_0) formed on the fly for every
+// gathered neighbour and stored for the lane's own row, q = A p_k, partials of p_k·q.  It also performs the
+// convergence test on ‖r_k‖² from the previous update's partials, so the solve ends here.
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(BellView A, const double* __restrict__ z,
+                                                     const double* __restrict__ p_old, double* __restrict__ p_new,
+                                                     double* __restrict__ q, const double* __restrict__ pb_cur,
+                                                     const double* __restrict__ pb_prev, int nb, int k,
+                                                     PcgScalars* sc, double* __restrict__ pa) {
   if (sc->done) return;
+  double beta = 0.0;
+  if (k > 0) {
+    double t_cur[2], t_prev[2];
+    sum_partials<2>(pb_cur, nb, t_cur);
+    sum_partials<2>(pb_prev, nb, t_prev);
+    const bool nan = !(t_cur[1] == t_cur[1]);
+    const bool done = t_cur[1] <= sc->tol2 || k >= sc->maxit || nan;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->iters = k;
+      sc->rr = t_cur[1];
+      sc->rz = t_cur[0];
+      if (nan) sc->breakdown = 2;
+      if (done) sc->done = 1;  // only ever 0 → 1: a workgroup that reads it early skips work that is not needed
+    }
+    if (done) return;
+    beta = t_cur[0] / t_prev[0];
+  }
   const int v = blockIdx.x * kBlock + threadIdx.x;
   double acc[1] = {0.0};
   if (v < A.nrows) {
-    double y[3];
-    bell_row(A, v, p, y);
+    const int s = v >> 6, lane = v & 63;
+    const int W = A.width[s];
+    const int off = A.col_off[s];
+    const int* col = A.col + off + lane;
+    const double* val = A.val + 9 * (size_t)off + lane;
+    double y0 = 0, y1 = 0, y2 = 0;
+    for (int kk = 0; kk < W; ++kk) {
+      const int w = col[kk * kSlice];
+      if (w >= 0) {
+        const size_t b3 = 3 * (size_t)w;
+        double x0 = z[b3], x1 = z[b3 + 1], x2 = z[b3 + 2];
+        if (k > 0) {
+          x0 += beta * p_old[b3];
+          x1 += beta * p_old[b3 + 1];
+          x2 += beta * p_old[b3 + 2];
+        }
+        const double* vk = val + (size_t)kk * 9 * kSlice;
+        y0 += vk[0 * kSlice] * x0 + vk[1 * kSlice] * x1 + vk[2 * kSlice] * x2;
+        y1 += vk[3 * kSlice] * x0 + vk[4 * kSlice] * x1 + vk[5 * kSlice] * x2;
+        y2 += vk[6 * kSlice] * x0 + vk[7 * kSlice] * x1 + vk[8 * kSlice] * x2;
+      }
+    }
+    const double y[3] = {y0, y1, y2};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      q[3 * (size_t)v + a] = y[a];
-      acc[0] += p[3 * (size_t)v + a] * y[a];
+      const size_t i = 3 * (size_t)v + a;
+      const double pv = (k > 0) ? z[i] + beta * p_old[i] : z[i];
+      p_new[i] = pv;
+      q[i] = y[a];
+      acc[0] += pv * y[a];
     }
   }
-  double tot[1];
-  if (grid_reduce<1>(acc, partials, &sc->cnt[1], tot) && threadIdx.x == 0) {
-    sc->pq = tot[0];
-    if (!(tot[0] > 0.0)) {
-      sc->breakdown = 1;
-      sc->done = 1;
-      sc->alpha = 0.0;
-    } else {
-      sc->alpha = sc->rz / tot[0];
-    }
-  }
+  store_partials<1>(acc, pa);
 }
 
-// x += α p, r −= α q, z = D⁻¹ r, rz' = r·z, rr = r·r, β = rz'/rz
+// Iteration k, kernel B: α = ρ_k/(p_k·q); x += α p_k, r −= α q, z = D⁻¹ r; partials of (r·z, r·r).
+// The streaming loads are issued before the partial sums so that their latency overlaps.
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __restrict__ dinv, double* __restrict__ x,
                                                        double* __restrict__ r, double* __restrict__ z,
                                                        const double* __restrict__ p, const double* __restrict__ q,
-                                                       PcgScalars* sc, double* partials) {
+                                                       const double* __restrict__ pa, int na,
+                                                       const double* __restrict__ pb_cur, double* __restrict__ pb_next,
+                                                       int nb, PcgScalars* sc) {
   if (sc->done) return;
-  const double alpha = sc->alpha;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  double v[2] = {0.0, 0.0};
-  if (i < n) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i];
-    const double zi = dinv[i] * ri;
-    r[i] = ri;
-    z[i] = zi;
-    v[0] = ri * zi;
-    v[1] = ri * ri;
-  }
-  double tot[2];
-  if (grid_reduce<2>(v, partials, &sc->cnt[2], tot) && threadIdx.x == 0) {
-    sc->beta = tot[0] / sc->rz;
-    sc->rz = tot[0];
-    sc->rr = tot[1];
-    sc->iters += 1;
-    if (tot[1] <= sc->tol2 || sc->iters >= sc->maxit) sc->done = 1;
-    if (!(tot[1] == tot[1])) {
-      sc->breakdown = 2;
-      sc->done = 1;
+  double xv[kVec], rv[kVec], pv[kVec], qv[kVec], dv[kVec];
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
+    if (i < n) {
+      xv[j] = x[i];
+      rv[j] = r[i];
+      pv[j] = p[i];
+      qv[j] = q[i];
+      dv[j] = dinv[i];
     }
   }
+  double pq[1], rz[2];
+  sum_partials<1>(pa, na, pq);
+  sum_partials<2>(pb_cur, nb, rz);
+  if (!(pq[0] > 0.0)) {  // not positive definite (or NaN): every workgroup sees the same pq
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->breakdown = 1;
+      sc->done = 1;
+    }
+    return;
+  }
+  const double alpha = rz[0] / pq[0];
+  double v[2] = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
+    if (i < n) {
+      x[i] = xv[j] + alpha * pv[j];
+      const double ri = rv[j] - alpha * qv[j];
+      const double zi = dv[j] * ri;
+      r[i] = ri;
+      z[i] = zi;
+      v[0] += ri * zi;
+      v[1] += ri * ri;
+    }
+  }
+  store_partials<2>(v, pb_next);
 }
 
-// p = z + β p
-__global__ __launch_bounds__(kBlock) void k_pcg_p(int n, const double* __restrict__ z, double* __restrict__ p,
-                                                  const PcgScalars* sc) {
-  if (sc->done) return;
-  const double beta = sc->beta;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = z[i] + beta * p[i];
-}
-
-void launch_pcg_spmv(const BellView& A, const double* p, double* q, PcgScalars* sc, double* partials, hipStream_t s) {
-  k_pcg_spmv<<<nblocks(A.nrows > 0 ? A.nrows : 1), kBlock, 0, s>>>(A, p, q, sc, partials);
+int launch_pcg_spmv(const BellView& A, const double* z, const double* p_old, double* p_new, double* q,
+                    const double* pb_cur, const double* pb_prev, int nb, int k, PcgScalars* sc, double* pa,
+                    hipStream_t s) {
+  const int nbk = nblocks(A.nrows > 0 ? A.nrows : 1);
+  k_pcg_spmv<<<nbk, kBlock, 0, s>>>(A, z, p_old, p_new, q, pb_cur, pb_prev, nb, k, sc, pa);
+  return nbk;
 }
 
 void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
-                       PcgScalars* sc, double* partials, hipStream_t s) {
-  k_pcg_update<<<nblocks(n > 0 ? n : 1), kBlock, 0, s>>>(n, dinv, x, r, z, p, q, sc, partials);
+                       const double* pa, int na, const double* pb_cur, double* pb_next, PcgScalars* sc, hipStream_t s) {
+  const int nb = vblocks(n);
+  k_pcg_update<<<nb, kBlock, 0, s>>>(n, dinv, x, r, z, p, q, pa, na, pb_cur, pb_next, nb, sc);
 }
 
-void launch_pcg_p(int n, const double* z, double* p, const PcgScalars* sc, hipStream_t s) {
-  k_pcg_p<<<nblocks(n > 0 ? n : 1), kBlock, 0, s>>>(n, z, p, sc);
-}
+int pcg_vector_blocks(int n) { return vblocks(n); }
 
 // ------------------------------------------------------------------------------------------------------
 // L-BFGS helpers

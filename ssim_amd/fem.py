@@ -64,8 +64,8 @@ class TetFiniteElementSolver_Hip:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            _lib.load().ssim_destroy(h)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.ssim_destroy(h)
             self._h = None
 
     # ---- lifecycle (fem_host.cpp:83-111) ----
