@@ -178,11 +178,23 @@ ssim_status_t ssim_get_kernel_timing(const ssim_ctx* ctx, ssim_kernel_t kernel, 
 typedef struct {
   int nv, nt;
   long long nonzero_blocks;  /* 3x3 blocks of the sparsity pattern (nnz / 9) */
-  long long padded_slots;    /* blocked-ELL slots including padding */
+  long long padded_slots;    /* stored (upper, row ≤ column) blocked-ELL slots including padding */
+  long long lower_slots;     /* lower-part reference slots including padding */
   int upper_blocks;          /* blocks with row ≤ column */
   int slices;                /* 64-row slices */
 } ssim_layout_info;
 ssim_status_t ssim_get_layout_info(const ssim_ctx* ctx, ssim_layout_info* info);
+
+/* ---- launch tuning (no reference counterpart: the reference's launch shapes live inside mathprim) ---- */
+typedef enum {
+  SSIM_TUNE_XCD_REMAP = 0,     /* 1 (default): each of the 8 XCDs takes a contiguous range of SpMV row slices */
+  SSIM_TUNE_PCG_CHUNK_MAX = 1, /* PCG iterations enqueued per done-flag poll (default 64) */
+  SSIM_TUNE_SPMV_LANES = 2     /* SpMV variant: 0 = one lane per row, fully unrolled over a layout padded to 8
+                                  upper / 7 lower blocks (default when every row fits), 1 = one lane per row with
+                                  loops (default otherwise), 4 = four lanes per row */
+} ssim_tune_t;
+ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value);
+ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value);
 
 #ifdef __cplusplus
 }

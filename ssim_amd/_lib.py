@@ -45,7 +45,8 @@ class StepStats(ctypes.Structure):
 
 class LayoutInfo(ctypes.Structure):
     _fields_ = [("nv", ctypes.c_int), ("nt", ctypes.c_int), ("nonzero_blocks", ctypes.c_longlong),
-                ("padded_slots", ctypes.c_longlong), ("upper_blocks", ctypes.c_int), ("slices", ctypes.c_int)]
+                ("padded_slots", ctypes.c_longlong), ("lower_slots", ctypes.c_longlong),
+                ("upper_blocks", ctypes.c_int), ("slices", ctypes.c_int)]
 
 
 KERNELS = {"elem_grad": 0, "grad_reduce": 1, "elem_hess": 2, "hess_reduce": 3, "pcg_spmv": 4, "pcg_update": 5,
@@ -97,7 +98,11 @@ SIGNATURES = {
     "ssim_set_kernel_timing": [_V, ctypes.c_int],
     "ssim_get_kernel_timing": [_V, ctypes.c_int, _D, ctypes.POINTER(ctypes.c_longlong)],
     "ssim_get_layout_info": [_V, ctypes.POINTER(LayoutInfo)],
+    "ssim_set_tuning": [_V, ctypes.c_int, ctypes.c_int],
+    "ssim_get_tuning": [_V, ctypes.c_int, _I],
 }
+
+TUNING = {"xcd_remap": 0, "pcg_chunk_max": 1, "spmv_lanes": 2}
 
 _lib = None
 

@@ -84,9 +84,16 @@ struct ssim_ctx {
 
   // host topology (block adjacency, rows and columns ascending)
   std::vector<int> brow, bcol, kdiag;
-  std::vector<int> hwidth, hcoloff;
-  long long ncolslots = 0;
+  std::vector<int> hwu, hoffu;  // symmetric blocked ELL: upper widths and slot offsets per slice
+  long long ncolslots = 0;      // upper slots (with padding)
+  long long nlowslots = 0;      // lower reference slots (with padding)
   int nslices = 0, nub = 0;
+  int xcd_remap = 1;
+  int pcg_chunk_max = 64;
+  int spmv_lanes = 1;
+  bool spmv_lanes_default = true;  // pick the variant from the layout at reset
+  bool fixed_layout = false;       // every slice padded to kFixedWU / kFixedWL slots
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
 
   // device: mesh and rest data
   DBuf<double> X, B, V, lam, mu;
@@ -94,10 +101,10 @@ struct ssim_ctx {
   // device: vertex arrays
   DBuf<double> arr[SSIM_ARRAY_COUNT];
   DBuf<unsigned char> dof;
-  DBuf<double> utry, gtry, dir, r, z, p, p2, q, tmp, dinv, hrest;
+  DBuf<double> utry, gtry, dir, r, z, p, q, tmp, dinv, hrest;
   // device: assembly maps
-  DBuf<int> gperm, gseg, hperm, hseg, width, coloff, col;
-  DBuf<int2> ub_rc, ub_pos;
+  DBuf<int> gperm, gseg, hperm, hseg, wu, offu, ucol, wl, offl, ub_pos;
+  DBuf<int2> ub_rc, lref;
   DBuf<double> gcontrib, hcontrib, val, massval;
   // device: scalars
   DBuf<PcgScalars> pcg;
@@ -124,6 +131,8 @@ struct ssim_ctx {
 
   ~ssim_ctx() {
     if (pin) (void)hipHostFree(pin);
+    for (auto& e : poll_ev)
+      if (e) (void)hipEventDestroy(e);
     for (auto& t : kt)
       for (auto& e : t.ev)
         if (e) (void)hipEventDestroy(e);
@@ -144,12 +153,17 @@ struct ssim_ctx {
   }
   BellView bell(const double* values) const {
     BellView A;
-    A.width = width.p;
-    A.col_off = coloff.p;
-    A.col = col.p;
+    A.wu = wu.p;
+    A.offu = offu.p;
+    A.ucol = ucol.p;
     A.val = values;
+    A.wl = wl.p;
+    A.offl = offl.p;
+    A.lref = lref.p;
     A.nrows = nv;
     A.nslices = nslices;
+    A.xcd_remap = xcd_remap;
+    A.lanes_per_row = spmv_lanes;
     return A;
   }
   double* a(ssim_array_t w) { return arr[w].p; }
@@ -182,6 +196,7 @@ void check_material(double youngs, double poisson, double density) {
 void harvest_timers(ssim_ctx* c) {
   for (auto& t : c->kt) {
     if (!t.pending) continue;
+    if (hipEventQuery(t.ev[1]) != hipSuccess) continue;  // not finished yet: harvested at a later sync
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, t.ev[0], t.ev[1]));
     t.sum_ms += ms;
@@ -253,41 +268,79 @@ void build_topology(ssim_ctx* c) {
   c->kdiag.resize(nv);
   for (int v = 0; v < nv; ++v) c->kdiag[v] = slot(v, v);
 
-  // blocked ELL: slices of 64 rows, width = longest row in the slice
+  // symmetric blocked ELL: slices of 64 rows; the upper part (w ≥ v, diagonal first) is stored, the lower
+  // part refers to the stored transposes
   c->nslices = (nv + kSlice - 1) / kSlice;
-  c->hwidth.assign(std::max(c->nslices, 1), 0);
-  c->hcoloff.assign(std::max(c->nslices, 1) + 1, 0);
-  long long off = 0;
+  const int ns = std::max(c->nslices, 1);
+  c->hwu.assign(ns, 0);
+  c->hoffu.assign(ns + 1, 0);
+  std::vector<int> hwl(ns, 0), hoffl(ns + 1, 0);
+  int max_wu = 0, max_wl = 0;
   for (int s = 0; s < c->nslices; ++s) {
-    int w = 0;
-    for (int l = 0; l < kSlice && s * kSlice + l < nv; ++l) w = std::max(w, len[s * kSlice + l]);
-    c->hwidth[s] = w;
-    c->hcoloff[s] = (int)off;
-    off += (long long)w * kSlice;
+    for (int l = 0; l < kSlice && s * kSlice + l < nv; ++l) {
+      const int v = s * kSlice + l;
+      c->hwu[s] = std::max(c->hwu[s], len[v] - c->kdiag[v]);
+      hwl[s] = std::max(hwl[s], c->kdiag[v]);
+    }
+    max_wu = std::max(max_wu, c->hwu[s]);
+    max_wl = std::max(max_wl, hwl[s]);
   }
-  c->ncolslots = off;
-  if (9 * off >= (1LL << 31)) fail(SSIM_ERR_INVALID_ARGUMENT, "mesh too large for 32-bit blocked-ELL offsets");
-  std::vector<int> hcol(std::max<long long>(off, 1), -1);
+  // rows narrow enough for the fixed-width kernel: pad every slice to the same widths
+  c->fixed_layout = max_wu <= kFixedWU && max_wl <= kFixedWL;
+  if (c->fixed_layout)
+    for (int s = 0; s < c->nslices; ++s) {
+      c->hwu[s] = kFixedWU;
+      hwl[s] = kFixedWL;
+    }
+  if (c->spmv_lanes == 0 && !c->fixed_layout) c->spmv_lanes = 1;
+  if (c->spmv_lanes_default) c->spmv_lanes = c->fixed_layout ? 0 : 1;
+  long long offu = 0, offl = 0;
+  for (int s = 0; s < c->nslices; ++s) {
+    c->hoffu[s] = (int)offu;
+    hoffl[s] = (int)offl;
+    offu += (long long)c->hwu[s] * kSlice;
+    offl += (long long)hwl[s] * kSlice;
+  }
+  c->ncolslots = offu;
+  c->nlowslots = offl;
+
+  if (9 * offu + 9 * kSlice >= (1LL << 31) || offl >= (1LL << 31))
+    fail(SSIM_ERR_INVALID_ARGUMENT, "mesh too large for 32-bit blocked-ELL offsets");
+  auto valpos = [&](int v, int ku) -> int {  // value base of upper slot ku of row v
+    const int s = v / kSlice, lane = v % kSlice;
+    return 9 * c->hoffu[s] + ku * 9 * kSlice + lane;
+  };
+  // padding: upper slots → (v, zero values); lower references → (v, a zero block after the last slice)
+  std::vector<int> hucol(std::max<long long>(offu, 1), 0);
+  std::vector<int2> hlref(std::max<long long>(offl, 1), make_int2(0, 0));
+  const long long zero_block = 9 * offu;
+  for (int s = 0; s < c->nslices; ++s)
+    for (int l = 0; l < kSlice; ++l) {
+      const int v = std::min(s * kSlice + l, nv - 1);
+      for (int k = 0; k < c->hwu[s]; ++k) hucol[c->hoffu[s] + k * kSlice + l] = v;
+      for (int k = 0; k < hwl[s]; ++k) hlref[hoffl[s] + k * kSlice + l] = make_int2(v, (int)(zero_block + l));
+    }
   for (int v = 0; v < nv; ++v) {
     const int s = v / kSlice, lane = v % kSlice;
-    for (int k = 0; k < len[v]; ++k) hcol[c->hcoloff[s] + k * kSlice + lane] = c->bcol[c->brow[v] + k];
+    const int kd = c->kdiag[v];
+    for (int k = kd; k < len[v]; ++k) hucol[c->hoffu[s] + (k - kd) * kSlice + lane] = c->bcol[c->brow[v] + k];
+    for (int k = 0; k < kd; ++k) {
+      const int w = c->bcol[c->brow[v] + k];
+      hlref[hoffl[s] + k * kSlice + lane] = make_int2(w, valpos(w, slot(w, v) - c->kdiag[w]));
+    }
   }
-  auto valpos = [&](int v, int k) -> int {
-    const int s = v / kSlice, lane = v % kSlice;
-    return 9 * c->hcoloff[s] + k * 9 * kSlice + lane;
-  };
 
-  // upper blocks (r ≤ c) in row-major order, with the value positions of (r,c) and (c,r)
+  // upper blocks (r ≤ c) in row-major order, with the value base of (r,c)
   std::vector<int> ubptr(nv + 1, 0);
   for (int v = 0; v < nv; ++v) ubptr[v + 1] = ubptr[v] + (len[v] - c->kdiag[v]);
   c->nub = ubptr[nv];
-  std::vector<int2> hub_rc(std::max(c->nub, 1)), hub_pos(std::max(c->nub, 1));
+  std::vector<int2> hub_rc(std::max(c->nub, 1));
+  std::vector<int> hub_pos(std::max(c->nub, 1));
   for (int v = 0; v < nv; ++v)
     for (int k = c->kdiag[v]; k < len[v]; ++k) {
-      const int w = c->bcol[c->brow[v] + k];
       const int ub = ubptr[v] + (k - c->kdiag[v]);
-      hub_rc[ub] = make_int2(v, w);
-      hub_pos[ub] = make_int2(valpos(v, k), valpos(w, slot(w, v)));
+      hub_rc[ub] = make_int2(v, c->bcol[c->brow[v] + k]);
+      hub_pos[ub] = valpos(v, k - c->kdiag[v]);
     }
 
   // Hessian contributions: 10 local pairs (v ≤ w) per tet, sorted by (upper block, tet, pair)
@@ -320,16 +373,19 @@ void build_topology(ssim_ctx* c) {
       for (int i = 0; i < 4; ++i) gperm[4 * e + i] = pos[cells[4 * e + i]]++;
   }
   hipStream_t s = c->stream;
-  c->width.upload(c->hwidth.data(), std::max(c->nslices, 1), s);
-  c->coloff.upload(c->hcoloff.data(), std::max(c->nslices, 1), s);
-  c->col.upload(hcol, s);
+  c->wu.upload(c->hwu.data(), ns, s);
+  c->offu.upload(c->hoffu.data(), ns, s);
+  c->ucol.upload(hucol, s);
+  c->wl.upload(hwl.data(), ns, s);
+  c->offl.upload(hoffl.data(), ns, s);
+  c->lref.upload(hlref, s);
   c->ub_rc.upload(hub_rc, s);
   c->ub_pos.upload(hub_pos, s);
   c->hseg.upload(hcount, s);
   c->hperm.upload(hperm, s);
   c->gseg.upload(gseg, s);
   c->gperm.upload(gperm, s);
-  c->val.alloc(9 * (size_t)std::max<long long>(off, 1));
+  c->val.alloc(9 * (size_t)offu + 9 * kSlice);  // + the zero block of the padded lower references
   HIPCHK(hipMemsetAsync(c->val.p, 0, c->val.count * sizeof(double), s));
   c->gcontrib.alloc(12 * (size_t)std::max(nt, 1));
   c->hcontrib.alloc(90 * (size_t)std::max(nt, 1));
@@ -375,7 +431,7 @@ void do_reset(ssim_ctx* c) {
     c->dof.alloc(n);
     HIPCHK(hipMemsetAsync(c->dof.p, 0, std::max(n, 1), s));
   }
-  for (DBuf<double>* b : {&c->utry, &c->gtry, &c->dir, &c->r, &c->z, &c->p, &c->p2, &c->q, &c->tmp, &c->dinv}) {
+  for (DBuf<double>* b : {&c->utry, &c->gtry, &c->dir, &c->r, &c->z, &c->p, &c->q, &c->tmp, &c->dinv}) {
     b->alloc(n);
     HIPCHK(hipMemsetAsync(b->p, 0, sizeof(double) * std::max(n, 1), s));
   }
@@ -386,7 +442,9 @@ void do_reset(ssim_ctx* c) {
   launch_rest_shape(c->X.p, c->cells.p, c->nt, c->B.p, c->V.p, s);
   HIPCHK(hipGetLastError());
   build_topology(c);
-  c->pmax = (size_t)max_partials(std::max({c->nt, c->n, c->nub, 1}));
+  // the SpMV grid has one workgroup per 64-row slice (or per 16 rows with 4 lanes per row), rounded up to a
+  // multiple of 8 (XCD remapping)
+  c->pmax = std::max<size_t>((size_t)max_partials(std::max({c->nt, c->n, c->nub, 1})), 4 * (size_t)c->nslices) + 8;
   c->partials.alloc(8 * c->pmax);
   c->pcg.alloc(1);
   c->ev.alloc(1);
@@ -456,24 +514,24 @@ int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol,
   const int nb = launch_pcg_init(c->n, b, sign, c->dinv.p, x, c->r.p, c->z.p, rtol, maxit, c->pcg.p,
                                  c->part_pb(0), s);
   HIPCHK(hipGetLastError());
-  int chunk = 8;
-  int launched = 0;
-  for (;;) {
-    for (int j = 0; j < chunk; ++j) {
-      const int k = launched + j;
-      double* p_new = (k & 1) ? c->p2.p : c->p.p;
-      const double* p_old = (k & 1) ? c->p.p : c->p2.p;
+  for (auto& e : c->poll_ev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // Chunks of iterations are enqueued one ahead of the done-flag poll, so the GPU never drains while the host
+  // reads a flag; kernels enqueued after convergence return at entry.
+  int* flags = reinterpret_cast<int*>(c->pin);  // two slots of 4 ints: done, iters, maxit, breakdown
+  auto enqueue = [&](int first, int count, int slot) {
+    for (int j = 0; j < count; ++j) {
+      const int k = first + j;
       int na = 0;
       auto spmv = [&] {
-        na = launch_pcg_spmv(A, c->z.p, p_old, p_new, c->q.p, c->part_pb(k), c->part_pb(k + 1), nb, k, c->pcg.p,
+        na = launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(k), c->part_pb(k + 1), nb, k, c->pcg.p,
                              c->part_pa(), s);
       };
       auto update = [&] {
-        launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, p_new, c->q.p, c->part_pa(), na, c->part_pb(k),
+        launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(k),
                           c->part_pb(k + 1), c->pcg.p, s);
       };
-      // the first iteration of a chunk always does work (the previous chunk ended unconverged): sample it
-      if (j == 0) {
+      if (j == 0) {  // the first iteration of a chunk does work unless the solve ended a chunk earlier
         timed(c, SSIM_KERNEL_PCG_SPMV, spmv);
         timed(c, SSIM_KERNEL_PCG_UPDATE, update);
       } else {
@@ -481,18 +539,28 @@ int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol,
         update();
       }
     }
-    launched += chunk;
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c->pin, &c->pcg.p->done, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-    sync(c);
-    const int* flags = reinterpret_cast<const int*>(c->pin);
-    if (flags[0]) {
-      if (flags[3] == 1) fail(SSIM_ERR_NUMERIC, "PCG breakdown: pᵀAp ≤ 0 (matrix not positive definite)");
-      if (flags[3] == 2) fail(SSIM_ERR_NUMERIC, "PCG breakdown: NaN residual");
-      return flags[1];
+    HIPCHK(hipMemcpyAsync(flags + 4 * slot, &c->pcg.p->done, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(c->poll_ev[slot], s));
+  };
+  int chunk = 8, launched = 0, slot = 0;
+  enqueue(launched, chunk, slot);
+  launched += chunk;
+  for (;;) {
+    const int next = std::min(chunk * 2, std::max(8, c->pcg_chunk_max));
+    enqueue(launched, next, slot ^ 1);
+    launched += next;
+    HIPCHK(hipEventSynchronize(c->poll_ev[slot]));
+    if (c->timing) harvest_timers(c);
+    const int* f = flags + 4 * slot;
+    if (f[0]) {
+      if (f[3] == 1) fail(SSIM_ERR_NUMERIC, "PCG breakdown: pᵀAp ≤ 0 (matrix not positive definite)");
+      if (f[3] == 2) fail(SSIM_ERR_NUMERIC, "PCG breakdown: NaN residual");
+      return f[1];
     }
-    if (launched > maxit + 8) fail(SSIM_ERR_STATE, "internal: PCG did not stop");
-    chunk = std::min(chunk * 2, 64);
+    if (launched > maxit + 2 * std::max(8, c->pcg_chunk_max) + 16) fail(SSIM_ERR_STATE, "internal: PCG did not stop");
+    chunk = next;
+    slot ^= 1;
   }
 }
 
@@ -1040,14 +1108,27 @@ static void export_csr(ssim_ctx* c, const double* dval, int* rowptr, int* colidx
     std::vector<double> hv(9 * (size_t)std::max<long long>(c->ncolslots, 1));
     HIPCHK(hipMemcpyAsync(hv.data(), dval, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, c->stream));
     sync(c);
+    auto base = [&](int v, int ku) -> size_t {
+      const int s = v / kSlice, lane = v % kSlice;
+      return 9 * (size_t)c->hoffu[s] + (size_t)ku * 9 * kSlice + lane;
+    };
     long long p = 0;
     for (int v = 0; v < nv; ++v) {
-      const int s = v / kSlice, lane = v % kSlice;
-      const size_t base = 9 * (size_t)c->hcoloff[s] + lane;
       const int len = c->brow[v + 1] - c->brow[v];
+      const int kd = c->kdiag[v];
       for (int b = 0; b < 3; ++b)
-        for (int k = 0; k < len; ++k)
-          for (int d = 0; d < 3; ++d) values[p++] = hv[base + ((size_t)k * 9 + 3 * b + d) * kSlice];
+        for (int k = 0; k < len; ++k) {
+          const int w = c->bcol[c->brow[v] + k];
+          if (k >= kd) {
+            const size_t bs = base(v, k - kd);
+            for (int d = 0; d < 3; ++d) values[p++] = hv[bs + (size_t)(3 * b + d) * kSlice];
+          } else {  // lower block (v, w) = transpose of the stored (w, v)
+            const int* bw = c->bcol.data() + c->brow[w];
+            const int kw = (int)(std::lower_bound(bw, bw + (c->brow[w + 1] - c->brow[w]), v) - bw);
+            const size_t bs = base(w, kw - c->kdiag[w]);
+            for (int d = 0; d < 3; ++d) values[p++] = hv[bs + (size_t)(3 * d + b) * kSlice];
+          }
+        }
     }
   }
 }
@@ -1161,9 +1242,56 @@ ssim_status_t ssim_get_layout_info(const ssim_ctx* ctx, ssim_layout_info* info) 
   info->nt = ctx->nt;
   info->nonzero_blocks = ctx->brow.empty() ? 0 : ctx->brow[ctx->nv];
   info->padded_slots = ctx->ncolslots;
+  info->lower_slots = ctx->nlowslots;
   info->upper_blocks = ctx->nub;
   info->slices = ctx->nslices;
   return SSIM_OK;
+}
+
+ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    sync(ctx);
+    switch (key) {
+      case SSIM_TUNE_XCD_REMAP:
+        ctx->xcd_remap = value != 0;
+        break;
+      case SSIM_TUNE_PCG_CHUNK_MAX:
+        if (value < 1 || value > 4096) fail(SSIM_ERR_INVALID_ARGUMENT, "PCG chunk must be in [1, 4096]");
+        ctx->pcg_chunk_max = value;
+        break;
+      case SSIM_TUNE_SPMV_LANES:
+        if (value != 0 && value != 1 && value != 4) fail(SSIM_ERR_INVALID_ARGUMENT, "SpMV variant must be 0, 1 or 4");
+        if (value == 0 && !ctx->fixed_layout)
+          fail(SSIM_ERR_INVALID_ARGUMENT, "the fixed-width SpMV needs at most 8 upper and 7 lower blocks per row");
+        ctx->spmv_lanes = value;
+        ctx->spmv_lanes_default = false;
+        break;
+      default:
+        fail(SSIM_ERR_INVALID_ARGUMENT, "unknown tuning key");
+    }
+  });
+}
+
+ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value) {
+  if (!ctx || !value) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  switch (key) {
+    case SSIM_TUNE_XCD_REMAP:
+      *value = ctx->xcd_remap;
+      return SSIM_OK;
+    case SSIM_TUNE_PCG_CHUNK_MAX:
+      *value = ctx->pcg_chunk_max;
+      return SSIM_OK;
+    case SSIM_TUNE_SPMV_LANES:
+      *value = ctx->spmv_lanes;
+      return SSIM_OK;
+    default:
+      g_err = "unknown tuning key";
+      return SSIM_ERR_INVALID_ARGUMENT;
+  }
 }
 
 }  // extern "C"

@@ -9,6 +9,8 @@ namespace ssim_hip {
 constexpr int kBlock = 256;      // threads per workgroup for every kernel (4 wave64)
 constexpr int kSlice = 64;       // rows per blocked-ELL slice (one wavefront)
 constexpr double kSpdFloor = 1e-3;  // eigenvalue floor of the SPD projection (time_step.hpp:52)
+constexpr int kFixedWU = 8;  // uniform upper / lower slot counts of the fixed-width SpMV layout
+constexpr int kFixedWL = 7;
 
 // Per-tet rest data in HBM.  cells are int4 (one 16-byte load per tet); B = Dm⁻¹ is structure-of-arrays
 // B[k·nt + e] for k = 0..8 so that a wavefront's loads of one component are contiguous.
@@ -23,15 +25,25 @@ struct MeshView {
   double rho;
 };
 
-// Blocked ELLPACK of 3x3 blocks.  Rows (vertices) are grouped in slices of 64; slice s has width[s] slots.
-// col[col_off[s] + k*64 + lane] = column vertex of slot k of row 64 s + lane (−1 = padding);
-// val[9*col_off[s] + (k*9 + 3b + d)*64 + lane] = entry (b, d) of that block.
+// Symmetric blocked ELLPACK of 3x3 blocks.  Rows (vertices) are grouped in slices of 64 lanes.
+// Upper part (columns w ≥ v, diagonal first): slice s has wu[s] slots,
+//   ucol[offu[s] + k*64 + lane] = column vertex (−1 = padding),
+//   val[9*offu[s] + (k*9 + 3b + d)*64 + lane] = entry (b, d) of block (v, w).
+// Lower part (columns w < v) is not stored: slot k of slice s holds lref[offl[s] + k*64 + lane] =
+//   (w, base) with val[base + (3b + d)*64] = entry (b, d) of the stored block (w, v), used transposed.
+// Padding: upper slots hold column v with zero values; lower references hold (v, base of a zero block that
+// follows the last slice), so the SpMV loops need no branch.
 struct BellView {
-  const int* width;
-  const int* col_off;
-  const int* col;
+  const int* wu;
+  const int* offu;
+  const int* ucol;
   const double* val;
+  const int* wl;
+  const int* offl;
+  const int2* lref;
   int nrows, nslices;
+  int xcd_remap;      // 1: give each of the 8 XCDs a contiguous range of rows
+  int lanes_per_row;  // 1 or 4
 };
 
 // Scalars of one PCG solve, resident in device memory.
@@ -59,14 +71,13 @@ void launch_finalize(const double* pa, int na, int ka, const double* pb, int nb,
 // Hessian contributions (10 oriented 3x3 blocks per tet, dt²V-scaled, consistent mass on the diagonal blocks)
 void launch_elem_hess(const MeshView& m, const double* u, double dt2, int make_spsd, const int* hperm,
                       double* hcontrib, bool fp32, hipStream_t s);
-void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const int2* ub_rc, const int2* ub_pos,
+void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const int2* ub_rc, const int* ub_pos,
                         const unsigned char* dof, int filter_dbc, double* bell_val, double* dinv, double* diag,
                         hipStream_t s);
 void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double* psi, double* P, double* H,
                       double* K, bool fp32, hipStream_t s);
 
 void launch_spmv(const BellView& A, const double* x, double* y, hipStream_t s);
-// *out = a·b (partials in part, then a one-workgroup finalize)
 void launch_dot(int n, const double* a, const double* b, double* part, double* out, hipStream_t s);
 void launch_prepare(int n, const double* u, const double* vel, const double* accel, const unsigned char* dof,
                     const double* dbc, double dt, double* uhat, double* unext, hipStream_t s);
@@ -77,13 +88,12 @@ void launch_restore(int n, const unsigned char* dof, const double* dbc, double* 
 void launch_axpy_to(int n, const double* x, double alpha, const double* d, double* out, hipStream_t s);
 
 // PCG on H x = sign·b with Jacobi preconditioner dinv.  init writes (ρ0, ‖r0‖²) partials to pb[0] and
-// returns their count.  Iteration k: spmv (p_k from z_k, p_{k−1}; convergence test on pb[k%2]; partials pa),
-// update (reads pa and pb[k%2], writes pb[(k+1)%2]).
+// returns their count.  Iteration k: kernel A (s = A z_k; convergence test on pb[k%2]; p_k = z_k + β p_{k−1},
+// q_k = s + β q_{k−1} in place; partials of p_k·q_k to pa), kernel B (reads pa and pb[k%2], writes pb[(k+1)%2]).
 int launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
                     double rtol, int maxit, PcgScalars* sc, double* pb0, hipStream_t s);
-int launch_pcg_spmv(const BellView& A, const double* z, const double* p_old, double* p_new, double* q,
-                    const double* pb_cur, const double* pb_prev, int nb, int k, PcgScalars* sc, double* pa,
-                    hipStream_t s);
+int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb_cur,
+                    const double* pb_prev, int nb, int k, PcgScalars* sc, double* pa, hipStream_t s);
 void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
                        const double* pa, int na, const double* pb_cur, double* pb_next, PcgScalars* sc, hipStream_t s);
 int pcg_vector_blocks(int n);

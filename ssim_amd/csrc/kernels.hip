@@ -26,47 +26,65 @@ __device__ __forceinline__ double wave_sum(double v) {
 // the partials visible -- sums all partials in a fixed order in every workgroup.  Every workgroup therefore
 // sees the same, bitwise reproducible total.
 
-// block-wide sum of K values; the result is returned to every thread
-template <int K>
+// block-wide sum of K values over NT threads; the result is returned to every thread
+template <int K, int NT = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[K]) {
-  __shared__ double sm[K][kBlock / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  static_assert(NT % 64 == 0, "workgroups are whole waves");
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const double w = wave_sum(v[k]);
-    if (lane == 0) sm[k][wid] = w;
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if (NT == 64) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = __shfl(v[k], 0, 64);
+    return;
+  }
+  __shared__ double sm[K][NT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sm[k][wid] = v[k];
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     double s = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) s += sm[k][w];
+    for (int w = 0; w < NT / 64; ++w) s += sm[k][w];
     v[k] = s;
   }
   __syncthreads();
 }
 
 // stores this workgroup's partial sums: part[blockIdx.x*K + k]
-template <int K>
+template <int K, int NT = kBlock>
 __device__ __forceinline__ void store_partials(double (&v)[K], double* part) {
-  block_sum<K>(v);
+  block_sum<K, NT>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) part[(size_t)blockIdx.x * K + k] = v[k];
   }
 }
 
-// total of `count` partials of width K (written by an earlier kernel), in a fixed order
-template <int K>
+// total of `count` partials of width K (written by an earlier kernel), in a fixed order.  Each thread loads its
+// partials U at a time (independent loads in flight together) before adding them in index order.
+template <int K, int NT = kBlock>
 __device__ __forceinline__ void sum_partials(const double* __restrict__ part, int count, double (&tot)[K]) {
+  constexpr int U = 8;
 #pragma unroll
   for (int k = 0; k < K; ++k) tot[k] = 0;
-  for (int b = threadIdx.x; b < count; b += kBlock) {
+  for (int b0 = threadIdx.x; b0 < count; b0 += U * NT) {
+    double buf[U][K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) tot[k] += part[(size_t)b * K + k];
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * NT;
+#pragma unroll
+      for (int k = 0; k < K; ++k) buf[u][k] = (b < count) ? part[(size_t)b * K + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < K; ++k) tot[k] += buf[u][k];
   }
-  block_sum<K>(tot);
+  block_sum<K, NT>(tot);
 }
 
 int max_partials(int n) { return (n + kBlock - 1) / kBlock + 1; }
@@ -377,11 +395,11 @@ void launch_elem_hess(const MeshView& m, const double* u, double dt2, int make_s
 }
 
 // One thread per upper block (r ≤ c): sum its contiguous contribution run, apply the Dirichlet identity
-// filter (identity_hessian_impl, boundary.hpp:53-67), write block (r,c) and its transpose (c,r) into the
-// blocked-ELL values, and the Jacobi inverse diagonal for r == c.
+// filter (identity_hessian_impl, boundary.hpp:53-67), write block (r,c) into the symmetric blocked-ELL values
+// (the lower block (c,r) is its transpose and is not stored), and the Jacobi inverse diagonal for r == c.
 __global__ __launch_bounds__(kBlock) void k_hess_reduce(int nub, const int* __restrict__ hseg,
                                                         const double* __restrict__ hcontrib,
-                                                        const int2* __restrict__ ub_rc, const int2* __restrict__ ub_pos,
+                                                        const int2* __restrict__ ub_rc, const int* __restrict__ ub_pos,
                                                         const unsigned char* __restrict__ dof, int filter_dbc,
                                                         double* __restrict__ val, double* __restrict__ dinv,
                                                         double* __restrict__ diag) {
@@ -404,15 +422,10 @@ __global__ __launch_bounds__(kBlock) void k_hess_reduce(int nub, const int* __re
       for (int dd = 0; dd < 3; ++dd)
         if (dof[3 * (size_t)rc.x + b] || dof[3 * (size_t)rc.y + dd]) S[b * 3 + dd] = (rc.x == rc.y && b == dd) ? 1.0 : 0.0;
   }
-  const int2 pos = ub_pos[ub];
+  const size_t pos = (size_t)ub_pos[ub];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) val[(size_t)pos.x + (size_t)k * kSlice] = S[k];
-  if (rc.x != rc.y) {
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-      for (int dd = 0; dd < 3; ++dd) val[(size_t)pos.y + (size_t)(dd * 3 + b) * kSlice] = S[b * 3 + dd];
-  } else {
+  for (int k = 0; k < 9; ++k) val[pos + (size_t)k * kSlice] = S[k];
+  if (rc.x == rc.y) {
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
       dinv[3 * (size_t)rc.x + b] = 1.0 / S[b * 4];
@@ -421,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_hess_reduce(int nub, const int* __re
   }
 }
 
-void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const int2* ub_rc, const int2* ub_pos,
+void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const int2* ub_rc, const int* ub_pos,
                         const unsigned char* dof, int filter_dbc, double* bell_val, double* dinv, double* diag,
                         hipStream_t s) {
   if (nub <= 0) return;
@@ -489,23 +502,80 @@ void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double*
 }
 
 // ------------------------------------------------------------------------------------------------------
-// blocked-ELL SpMV: one lane per block row, coalesced 64-wide loads of every value component
+// Symmetric blocked-ELL SpMV.  Each block row is handled by LPR lanes (LPR = 1 or 4; lane j takes slots j,
+// j+LPR, ...).  Padded slots hold a valid column (the row itself) and zero values, and padded lower references
+// point at a zero block, so the slot loops are branch-free with a trip count uniform over the slice and can be
+// software-pipelined.  With LPR = 4 a wave covers 16 rows: lane l takes row l & 15 and slot phase j = l >> 4, so
+// every 16-lane group reads 16 consecutive doubles; the 4 partial results of a row are combined with two
+// shuffles in a fixed order.  Upper blocks are read from the row's own slots, lower blocks as transposes of the
+// stored (w, v) blocks.
 
-__device__ __forceinline__ void bell_row(const BellView& A, int v, const double* __restrict__ x, double y[3]) {
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X.  With remapping, physical workgroup i
+// handles logical block (i mod 8)·per + i/8, so each XCD (and its L2) sees a contiguous range of rows and the
+// transposed reads of a row's lower blocks mostly hit lines its own XCD has just read.  Returns −1 for the idle
+// tail of the padded grid.
+__device__ __forceinline__ int logical_block(int nb, int remap) {
+  if (!remap) return blockIdx.x;
+  const int per = (nb + 7) >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  return lb < nb ? lb : -1;
+}
+
+static inline int grid_for(int nb, int remap) { return remap ? ((nb + 7) / 8) * 8 : nb; }
+
+// SpMV workgroups are single waves: one 64-row slice per workgroup (LPR ≤ 1) spreads the ~2,200 waves of the
+// 786k-tet bar evenly over the 1,024 SIMDs in one dispatch round.
+constexpr int kSpmvBlock = 64;
+
+template <int LPR>
+struct RowMap {
+  static constexpr int rows_per_block = kSpmvBlock / (LPR > 0 ? LPR : 1);
+  // row and slot phase of this thread within logical block lb
+  __device__ static __forceinline__ void map(int lb, int& v, int& j) {
+    if (LPR <= 1) {
+      v = lb * kSpmvBlock + threadIdx.x;
+      j = 0;
+    } else {
+      const int lane = threadIdx.x & 63;
+      v = lb * rows_per_block + (threadIdx.x >> 6) * 16 + (lane & 15);
+      j = lane >> 4;
+    }
+  }
+  __host__ __device__ static int blocks(int nrows) { return (nrows + rows_per_block - 1) / rows_per_block; }
+};
+
+template <int LPR>
+__device__ __forceinline__ void bell_row_part(const BellView& A, int v, int j, const double* __restrict__ x,
+                                              double y[3]) {
   const int s = v >> 6, lane = v & 63;
-  const int W = A.width[s];
-  const int off = A.col_off[s];
-  const int* col = A.col + off + lane;
-  const double* val = A.val + 9 * (size_t)off + lane;
   double y0 = 0, y1 = 0, y2 = 0;
-  for (int k = 0; k < W; ++k) {
-    const int w = col[k * kSlice];
-    if (w >= 0) {
-      const double x0 = x[3 * (size_t)w], x1 = x[3 * (size_t)w + 1], x2 = x[3 * (size_t)w + 2];
+  {
+    const int W = A.wu[s];
+    const int off = A.offu[s];
+    const int* col = A.ucol + off + lane;
+    const double* val = A.val + 9 * (size_t)off + lane;
+#pragma unroll 4
+    for (int k = j; k < W; k += LPR) {
+      const size_t w3 = 3 * (size_t)col[k * kSlice];
+      const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
       const double* vk = val + (size_t)k * 9 * kSlice;
       y0 += vk[0 * kSlice] * x0 + vk[1 * kSlice] * x1 + vk[2 * kSlice] * x2;
       y1 += vk[3 * kSlice] * x0 + vk[4 * kSlice] * x1 + vk[5 * kSlice] * x2;
       y2 += vk[6 * kSlice] * x0 + vk[7 * kSlice] * x1 + vk[8 * kSlice] * x2;
+    }
+  }
+  {
+    const int W = A.wl[s];
+    const int2* ref = A.lref + A.offl[s] + lane;
+#pragma unroll 4
+    for (int k = j; k < W; k += LPR) {
+      const int2 rw = ref[k * kSlice];
+      const size_t w3 = 3 * (size_t)rw.x;
+      const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
+      const double* b = A.val + (size_t)rw.y;
+      y0 += b[0 * kSlice] * x0 + b[3 * kSlice] * x1 + b[6 * kSlice] * x2;
+      y1 += b[1 * kSlice] * x0 + b[4 * kSlice] * x1 + b[7 * kSlice] * x2;
+      y2 += b[2 * kSlice] * x0 + b[5 * kSlice] * x1 + b[8 * kSlice] * x2;
     }
   }
   y[0] = y0;
@@ -513,19 +583,95 @@ __device__ __forceinline__ void bell_row(const BellView& A, int v, const double*
   y[2] = y2;
 }
 
-__global__ __launch_bounds__(kBlock) void k_spmv(BellView A, const double* __restrict__ x, double* __restrict__ y) {
-  const int v = blockIdx.x * kBlock + threadIdx.x;
-  if (v >= A.nrows) return;
-  double r[3];
-  bell_row(A, v, x, r);
-  y[3 * (size_t)v] = r[0];
-  y[3 * (size_t)v + 1] = r[1];
-  y[3 * (size_t)v + 2] = r[2];
+// Fixed-width variant for layouts padded to exactly WU upper and WL lower slots in every slice (used when the
+// mesh allows it; the Freudenthal bar has 8 and 7): straight-line code, so all column / reference loads of the
+// row go out together, then all gathers and value loads -- a few dependent memory round trips per row instead
+// of two per slot.
+template <int WU, int WL>
+__device__ __forceinline__ void bell_row_fixed(const BellView& A, int v, const double* __restrict__ x, double y[3]) {
+  const int s = v >> 6, lane = v & 63;
+  const int offu = s * WU * kSlice;  // uniform widths: the offsets are implied
+  const int* col = A.ucol + offu + lane;
+  const double* val = A.val + 9 * (size_t)offu + lane;
+  const int2* ref = A.lref + s * WL * kSlice + lane;
+  int cu[WU];
+  int2 cl[WL];
+#pragma unroll
+  for (int k = 0; k < WU; ++k) cu[k] = col[k * kSlice];
+#pragma unroll
+  for (int k = 0; k < WL; ++k) cl[k] = ref[k * kSlice];
+  double y0 = 0, y1 = 0, y2 = 0;
+#pragma unroll
+  for (int k = 0; k < WU; ++k) {
+    const size_t w3 = 3 * (size_t)cu[k];
+    const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
+    const double* vk = val + (size_t)k * 9 * kSlice;
+    y0 += vk[0 * kSlice] * x0 + vk[1 * kSlice] * x1 + vk[2 * kSlice] * x2;
+    y1 += vk[3 * kSlice] * x0 + vk[4 * kSlice] * x1 + vk[5 * kSlice] * x2;
+    y2 += vk[6 * kSlice] * x0 + vk[7 * kSlice] * x1 + vk[8 * kSlice] * x2;
+  }
+#pragma unroll
+  for (int k = 0; k < WL; ++k) {
+    const size_t w3 = 3 * (size_t)cl[k].x;
+    const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
+    const double* b = A.val + (size_t)cl[k].y;
+    y0 += b[0 * kSlice] * x0 + b[3 * kSlice] * x1 + b[6 * kSlice] * x2;
+    y1 += b[1 * kSlice] * x0 + b[4 * kSlice] * x1 + b[7 * kSlice] * x2;
+    y2 += b[2 * kSlice] * x0 + b[5 * kSlice] * x1 + b[8 * kSlice] * x2;
+  }
+  y[0] = y0;
+  y[1] = y1;
+  y[2] = y2;
+}
+
+// LPR = 0 selects the fixed-width single-lane variant (layout padded to 8 upper / 7 lower slots)
+template <int LPR>
+__device__ __forceinline__ void row_part(const BellView& A, int v, int j, const double* __restrict__ x, double y[3]) {
+  if (LPR == 0)
+    bell_row_fixed<kFixedWU, kFixedWL>(A, v, x, y);
+  else
+    bell_row_part<LPR>(A, v, j, x, y);
+}
+
+// sum over the LPR lanes of a row (lanes l, l+16, l+32, l+48 for LPR = 4); every lane gets the total
+template <int LPR>
+__device__ __forceinline__ void row_combine(double y[3]) {
+  if (LPR <= 1) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    y[a] += __shfl_xor(y[a], 16, 64);
+    y[a] += __shfl_xor(y[a], 32, 64);
+  }
+}
+
+template <int LPR>
+__global__ __launch_bounds__(kSpmvBlock) void k_spmv(BellView A, const double* __restrict__ x, double* __restrict__ y) {
+  const int lb = logical_block(RowMap<LPR>::blocks(A.nrows), A.xcd_remap);
+  if (lb < 0) return;
+  int v, j;
+  RowMap<LPR>::map(lb, v, j);
+  double r[3] = {0.0, 0.0, 0.0};
+  if (v < A.nrows) row_part<LPR>(A, v, j, x, r);
+  row_combine<LPR>(r);
+  if (v < A.nrows) {
+    if (LPR <= 1) {
+      y[3 * (size_t)v] = r[0];
+      y[3 * (size_t)v + 1] = r[1];
+      y[3 * (size_t)v + 2] = r[2];
+    } else if (j < 3) {
+      y[3 * (size_t)v + j] = r[j];
+    }
+  }
 }
 
 void launch_spmv(const BellView& A, const double* x, double* y, hipStream_t s) {
   if (A.nrows <= 0) return;
-  k_spmv<<<nblocks(A.nrows), kBlock, 0, s>>>(A, x, y);
+  if (A.lanes_per_row == 4)
+    k_spmv<4><<<grid_for(RowMap<4>::blocks(A.nrows), A.xcd_remap), kSpmvBlock, 0, s>>>(A, x, y);
+  else if (A.lanes_per_row == 0)
+    k_spmv<0><<<grid_for(RowMap<0>::blocks(A.nrows), A.xcd_remap), kSpmvBlock, 0, s>>>(A, x, y);
+  else
+    k_spmv<1><<<grid_for(RowMap<1>::blocks(A.nrows), A.xcd_remap), kSpmvBlock, 0, s>>>(A, x, y);
 }
 
 constexpr int kVec = 4;  // elements per thread of the vector kernels
@@ -676,20 +822,31 @@ int launch_pcg_init(int n, const double* b, double sign, const double* dinv, dou
   return nb;
 }
 
-// Iteration k, kernel A: p_k = z_k + β p_{k−1} (β = ρ_k/ρ_{k−1}; p_0 = z_0) formed on the fly for every
-// gathered neighbour and stored for the lane's own row, q = A p_k, partials of p_k·q.  It also performs the
-// convergence test on ‖r_k‖² from the previous update's partials, so the solve ends here.
-__global__ __launch_bounds__(kBlock) void k_pcg_spmv(BellView A, const double* __restrict__ z,
-                                                     const double* __restrict__ p_old, double* __restrict__ p_new,
+// Iteration k, kernel A: s = A z_k, then (after the loop, so the partial-sum latency overlaps other
+// workgroups' SpMV work) the convergence test on ‖r_k‖² from the previous update's partials, β = ρ_k/ρ_{k−1},
+// p_k = z_k + β p_{k−1} and q_k = A p_k = s + β q_{k−1} for the lane's own row (in place), and the partials of
+// p_k·q_k.  For k = 0: p_0 = z_0, q_0 = s.
+// At 786k tets the SpMV has about 2.2 waves per SIMD: a register budget for 3 waves keeps the whole grid in
+// one dispatch round while leaving the compiler room to keep many of a row's loads in flight.
+template <int LPR>
+__global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_pcg_spmv(
+    BellView A, const double* __restrict__ z, double* __restrict__ p,
                                                      double* __restrict__ q, const double* __restrict__ pb_cur,
                                                      const double* __restrict__ pb_prev, int nb, int k,
                                                      PcgScalars* sc, double* __restrict__ pa) {
   if (sc->done) return;
+  const int lb = logical_block(RowMap<LPR>::blocks(A.nrows), A.xcd_remap);
+  int v = 0, j = 0;
+  if (lb >= 0) RowMap<LPR>::map(lb, v, j);
+  const bool active = lb >= 0 && v < A.nrows;
+  double sv[3] = {0.0, 0.0, 0.0};
+  if (active) row_part<LPR>(A, v, j, z, sv);
+  row_combine<LPR>(sv);
   double beta = 0.0;
   if (k > 0) {
     double t_cur[2], t_prev[2];
-    sum_partials<2>(pb_cur, nb, t_cur);
-    sum_partials<2>(pb_prev, nb, t_prev);
+    sum_partials<2, kSpmvBlock>(pb_cur, nb, t_cur);
+    sum_partials<2, kSpmvBlock>(pb_prev, nb, t_prev);
     const bool nan = !(t_cur[1] == t_cur[1]);
     const bool done = t_cur[1] <= sc->tol2 || k >= sc->maxit || nan;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -702,42 +859,24 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(BellView A, const double* _
     if (done) return;
     beta = t_cur[0] / t_prev[0];
   }
-  const int v = blockIdx.x * kBlock + threadIdx.x;
   double acc[1] = {0.0};
-  if (v < A.nrows) {
-    const int s = v >> 6, lane = v & 63;
-    const int W = A.width[s];
-    const int off = A.col_off[s];
-    const int* col = A.col + off + lane;
-    const double* val = A.val + 9 * (size_t)off + lane;
-    double y0 = 0, y1 = 0, y2 = 0;
-    for (int kk = 0; kk < W; ++kk) {
-      const int w = col[kk * kSlice];
-      if (w >= 0) {
-        const size_t b3 = 3 * (size_t)w;
-        double x0 = z[b3], x1 = z[b3 + 1], x2 = z[b3 + 2];
-        if (k > 0) {
-          x0 += beta * p_old[b3];
-          x1 += beta * p_old[b3 + 1];
-          x2 += beta * p_old[b3 + 2];
-        }
-        const double* vk = val + (size_t)kk * 9 * kSlice;
-        y0 += vk[0 * kSlice] * x0 + vk[1 * kSlice] * x1 + vk[2 * kSlice] * x2;
-        y1 += vk[3 * kSlice] * x0 + vk[4 * kSlice] * x1 + vk[5 * kSlice] * x2;
-        y2 += vk[6 * kSlice] * x0 + vk[7 * kSlice] * x1 + vk[8 * kSlice] * x2;
-      }
-    }
-    const double y[3] = {y0, y1, y2};
+  if (active) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const size_t i = 3 * (size_t)v + a;
-      const double pv = (k > 0) ? z[i] + beta * p_old[i] : z[i];
-      p_new[i] = pv;
-      q[i] = y[a];
-      acc[0] += pv * y[a];
+      if (LPR <= 1 || a == j) {  // with LPR = 4, lane j of the row owns component j
+        const size_t i = 3 * (size_t)v + a;
+        double pv = z[i], qv = sv[a];
+        if (k > 0) {
+          pv += beta * p[i];
+          qv += beta * q[i];
+        }
+        p[i] = pv;
+        q[i] = qv;
+        acc[0] += pv * qv;
+      }
     }
   }
-  store_partials<1>(acc, pa);
+  store_partials<1, kSpmvBlock>(acc, pa);
 }
 
 // Iteration k, kernel B: α = ρ_k/(p_k·q); x += α p_k, r −= α q, z = D⁻¹ r; partials of (r·z, r·r).
@@ -789,12 +928,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
   store_partials<2>(v, pb_next);
 }
 
-int launch_pcg_spmv(const BellView& A, const double* z, const double* p_old, double* p_new, double* q,
-                    const double* pb_cur, const double* pb_prev, int nb, int k, PcgScalars* sc, double* pa,
-                    hipStream_t s) {
-  const int nbk = nblocks(A.nrows > 0 ? A.nrows : 1);
-  k_pcg_spmv<<<nbk, kBlock, 0, s>>>(A, z, p_old, p_new, q, pb_cur, pb_prev, nb, k, sc, pa);
-  return nbk;
+int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb_cur,
+                    const double* pb_prev, int nb, int k, PcgScalars* sc, double* pa, hipStream_t s) {
+  if (A.lanes_per_row == 4) {
+    const int grid = grid_for(RowMap<4>::blocks(A.nrows > 0 ? A.nrows : 1), A.xcd_remap);
+    k_pcg_spmv<4><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb_cur, pb_prev, nb, k, sc, pa);
+    return grid;
+  }
+  if (A.lanes_per_row == 0) {
+    const int grid = grid_for(RowMap<0>::blocks(A.nrows > 0 ? A.nrows : 1), A.xcd_remap);
+    k_pcg_spmv<0><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb_cur, pb_prev, nb, k, sc, pa);
+    return grid;
+  }
+  const int grid = grid_for(RowMap<1>::blocks(A.nrows > 0 ? A.nrows : 1), A.xcd_remap);
+  k_pcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb_cur, pb_prev, nb, k, sc, pa);
+  return grid;
 }
 
 void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,

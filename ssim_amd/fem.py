@@ -262,6 +262,20 @@ class TetFiniteElementSolver_Hip:
             out[name] = (ms.value, cnt.value)
         return out
 
+    def set_tuning(self, **kw):
+        for k, v in kw.items():
+            if k not in _lib.TUNING:
+                raise ValueError(f"unknown tuning key {k}")
+            _lib.call("ssim_set_tuning", self._h, _lib.TUNING[k], int(v))
+
+    def tuning(self) -> dict:
+        out = {}
+        for k, key in _lib.TUNING.items():
+            v = ctypes.c_int()
+            _lib.call("ssim_get_tuning", self._h, key, ctypes.byref(v))
+            out[k] = v.value
+        return out
+
     def layout_info(self) -> dict:
         info = _lib.LayoutInfo()
         _lib.call("ssim_get_layout_info", self._h, ctypes.byref(info))
