@@ -36,14 +36,18 @@ def build_case(nx, ny, nz, seed):
 
 
 def spmv_algorithmic_bytes(info):
-    """K3 (BSR3 SpMV): 76 B per nonzero 3x3 block (72 B values + 4 B column index), 4 B per row pointer,
-    and one read of x plus one write of y (2·n·8 B)."""
+    """Bytes one launch of the PCG SpMV kernel must move, for the format it reads (symmetric BSR3: only blocks
+    with row <= column are stored).  76 B per stored block (72 B of values + a 4 B column index), 8 B per
+    lower-block reference (column + value offset), the gathered input z once (n·8 B), and the lane's own rows
+    of p and q read and written (4·n·8 B).  Padding slots are not counted."""
     n = 3 * info["nv"]
-    return 76 * info["nonzero_blocks"] + 4 * (info["nv"] + 1) + 2 * n * 8
+    upper = info["upper_blocks"]
+    lower = info["nonzero_blocks"] - upper
+    return 76 * upper + 8 * lower + 5 * n * 8
 
 
 def load_traffic():
-    path = os.path.join(ROOT, "profiles", "pcg_spmv_traffic.json")
+    path = os.path.join(ROOT, "profiles", "k_pcg_spmv_786k_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -144,7 +148,7 @@ def main():
         "unit": "GB/s",
         "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
         "traffic": traffic,
-        "kernel": "k_pcg_spmv (blocked-ELL 3x3 SpMV + p·q)",
+        "kernel": "k_pcg_spmv (symmetric blocked-ELL 3x3 SpMV, p/q update, p·q partials)",
         "bytes_per_launch": bytes_per_launch,
         "avg_launch_ms": spmv_ms,
         "sampled_launches": spmv_n,
