@@ -151,8 +151,9 @@ def test_newton_trajectory_config1_60_steps():
 
 
 def test_lbfgs_trajectory_matches_oracle():
+    # tightly converged steps: L-BFGS iterates are rounding-sensitive inside the tolerance ball
     case = mesh.twisted_cube(n=4, seed=3)
-    g, o = _pair(case, method="lbfgs", rtol=1e-6)
+    g, o = _pair(case, method="lbfgs", rtol=1e-10)
     for step in range(3):
         g.step()
         o.step("lbfgs")
