@@ -196,6 +196,51 @@ typedef enum {
 ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value);
 ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value);
 
+/* ---- batches: many independent meshes stepped by one set of launches ----
+ * No reference counterpart: the reference steps one mesh per finite_elements/time_step.hpp object.  Each mesh
+ * of a batch follows that object's Newton step exactly (backward_euler.hpp:15-33, same threshold, line search
+ * and PCG limits, per-mesh threshold·mass tolerance); the batch advances the meshes' Newton iterations in lock
+ * step and every PCG iteration or line-search trial runs as one launch pair / evaluation over all meshes that
+ * still take part.  Meshes have per-tet Lamé parameters; density and time step are shared.  A mesh index of −1
+ * in the array calls means "all meshes, concatenated in batch order". */
+typedef struct ssim_batch ssim_batch;
+/* vertices: Σnv[m] rows of 3 doubles; cells: Σnt[m] rows of 4 mesh-local vertex indices; lambda, mu: Σnt[m]
+ * (replaces one time_step constructor + reset() per mesh, time_step.hpp:260-265, 368-494) */
+ssim_status_t ssim_batch_create(ssim_batch** out, int nmesh, const int* nv, const int* nt, const double* vertices,
+                                const int* cells, const double* lambda, const double* mu, double density,
+                                double time_step);
+ssim_status_t ssim_batch_destroy(ssim_batch* batch);
+ssim_status_t ssim_batch_set_stream(ssim_batch* batch, void* hip_stream);
+ssim_status_t ssim_batch_num_meshes(const ssim_batch* batch, int* nmesh);
+ssim_status_t ssim_batch_set_threshold(ssim_batch* batch, double rtol);  /* set_threshold, per mesh ×mass */
+ssim_status_t ssim_batch_set_solver_params(ssim_batch* batch, const ssim_solver_params* params);
+ssim_status_t ssim_batch_get_solver_params(const ssim_batch* batch, ssim_solver_params* params);
+/* mark_dirichlet_batched / mark_general_batched of one mesh (bindings/src/fem_host.cpp:123-144), mesh-local
+ * indices */
+ssim_status_t ssim_batch_mark_dirichlet(ssim_batch* batch, int mesh, const int* verts, int count,
+                                        const double* targets);
+ssim_status_t ssim_batch_mark_general(ssim_batch* batch, int mesh, const int* verts, int count);
+ssim_status_t ssim_batch_reset(ssim_batch* batch);
+/* prepare_step / compute_step / step_next / step of every mesh (time_step.hpp:499-569) */
+ssim_status_t ssim_batch_prepare_step(ssim_batch* batch);
+ssim_status_t ssim_batch_compute_step(ssim_batch* batch);
+ssim_status_t ssim_batch_step_next(ssim_batch* batch);
+ssim_status_t ssim_batch_step(ssim_batch* batch);
+/* statistics of the last compute_step: per mesh, or (mesh = −1) for the batch: lock-step Newton iterations,
+ * PCG iterations summed over Newton iterations of the slowest mesh, batched evaluations, all converged */
+ssim_status_t ssim_batch_get_step_stats(const ssim_batch* batch, int mesh, ssim_step_stats* stats);
+ssim_status_t ssim_batch_get_array(ssim_batch* batch, ssim_array_t which, int mesh, double* host_out);
+ssim_status_t ssim_batch_set_array(ssim_batch* batch, ssim_array_t which, int mesh, const double* host_in);
+ssim_status_t ssim_batch_total_mass(const ssim_batch* batch, int mesh, double* mass);
+/* layout of the concatenated, padded mesh */
+ssim_status_t ssim_batch_get_layout_info(const ssim_batch* batch, ssim_layout_info* info);
+/* launch tuning (same keys; SpMV variant 0 or 1) and sampled kernel timing of the batch's launches */
+ssim_status_t ssim_batch_set_tuning(ssim_batch* batch, ssim_tune_t key, int value);
+ssim_status_t ssim_batch_get_tuning(const ssim_batch* batch, ssim_tune_t key, int* value);
+ssim_status_t ssim_batch_set_kernel_timing(ssim_batch* batch, int enabled);
+ssim_status_t ssim_batch_get_kernel_timing(const ssim_batch* batch, ssim_kernel_t kernel, double* avg_ms,
+                                           long long* samples);
+
 #ifdef __cplusplus
 }
 #endif

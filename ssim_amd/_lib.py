@@ -100,6 +100,31 @@ SIGNATURES = {
     "ssim_get_layout_info": [_V, ctypes.POINTER(LayoutInfo)],
     "ssim_set_tuning": [_V, ctypes.c_int, ctypes.c_int],
     "ssim_get_tuning": [_V, ctypes.c_int, _I],
+    # batches
+    "ssim_batch_create": [ctypes.POINTER(_V), ctypes.c_int, _I, _I, _D, _I, _D, _D, ctypes.c_double,
+                          ctypes.c_double],
+    "ssim_batch_destroy": [_V],
+    "ssim_batch_set_stream": [_V, _V],
+    "ssim_batch_num_meshes": [_V, _I],
+    "ssim_batch_set_threshold": [_V, ctypes.c_double],
+    "ssim_batch_set_solver_params": [_V, ctypes.POINTER(SolverParams)],
+    "ssim_batch_get_solver_params": [_V, ctypes.POINTER(SolverParams)],
+    "ssim_batch_mark_dirichlet": [_V, ctypes.c_int, _I, ctypes.c_int, _D],
+    "ssim_batch_mark_general": [_V, ctypes.c_int, _I, ctypes.c_int],
+    "ssim_batch_reset": [_V],
+    "ssim_batch_prepare_step": [_V],
+    "ssim_batch_compute_step": [_V],
+    "ssim_batch_step_next": [_V],
+    "ssim_batch_step": [_V],
+    "ssim_batch_get_step_stats": [_V, ctypes.c_int, ctypes.POINTER(StepStats)],
+    "ssim_batch_get_array": [_V, ctypes.c_int, ctypes.c_int, _D],
+    "ssim_batch_set_array": [_V, ctypes.c_int, ctypes.c_int, _D],
+    "ssim_batch_total_mass": [_V, ctypes.c_int, _D],
+    "ssim_batch_get_layout_info": [_V, ctypes.POINTER(LayoutInfo)],
+    "ssim_batch_set_tuning": [_V, ctypes.c_int, ctypes.c_int],
+    "ssim_batch_get_tuning": [_V, ctypes.c_int, _I],
+    "ssim_batch_set_kernel_timing": [_V, ctypes.c_int],
+    "ssim_batch_get_kernel_timing": [_V, ctypes.c_int, _D, ctypes.POINTER(ctypes.c_longlong)],
 }
 
 TUNING = {"xcd_remap": 0, "pcg_chunk_max": 1, "spmv_lanes": 2}

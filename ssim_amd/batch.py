@@ -1,0 +1,198 @@
+"""Batched stepping: many independent tetrahedral meshes of uneven size advanced by one set of GPU launches.
+
+``TetFiniteElementBatch_Hip`` holds a list of meshes, each with the state and Newton step of one
+``TetFiniteElementSolver_Hip`` (reference: one ``TetFiniteElementSolver_*`` object per mesh,
+bindings/src/fem_host.cpp).  Per-mesh methods take a mesh index; vertex outputs are (3, N) like the reference's.
+Meshes carry per-tet Lamé parameters; time step and density are shared by the batch.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+class TetFiniteElementBatch_Hip:
+    """Backward-Euler FEM time step of a batch of meshes on one MI355X (FP64, Newton + Jacobi-PCG)."""
+
+    def __init__(self, vertices: Sequence[np.ndarray], elements: Sequence[np.ndarray], lam: Sequence[np.ndarray],
+                 mu: Sequence[np.ndarray], time_step: float = 1e-2, density: float = 1e2):
+        if not (len(vertices) == len(elements) == len(lam) == len(mu)) or len(vertices) == 0:
+            raise ValueError("vertices, elements, lam and mu must be non-empty lists of equal length")
+        Xs = [np.ascontiguousarray(x, dtype=np.float64) for x in vertices]
+        Cs = [np.ascontiguousarray(c, dtype=np.int32) for c in elements]
+        for X, C in zip(Xs, Cs):
+            if X.ndim != 2 or X.shape[1] != 3 or C.ndim != 2 or C.shape[1] != 4:
+                raise ValueError("each mesh needs vertices (N, 3) and elements (M, 4)")
+        self._nv = np.array([x.shape[0] for x in Xs], dtype=np.int32)
+        self._nt = np.array([c.shape[0] for c in Cs], dtype=np.int32)
+        lam = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64).reshape(-1) for v in lam]))
+        mu = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64).reshape(-1) for v in mu]))
+        if lam.size != self._nt.sum() or mu.size != self._nt.sum():
+            raise ValueError("lam and mu need one entry per tet of every mesh")
+        X = np.ascontiguousarray(np.concatenate(Xs))
+        C = np.ascontiguousarray(np.concatenate(Cs))
+        self._off = np.concatenate([[0], np.cumsum(3 * self._nv.astype(np.int64))])
+        h = ctypes.c_void_p()
+        _lib.call("ssim_batch_create", ctypes.byref(h), len(Xs), _ip(self._nv), _ip(self._nt), _dp(X), _ip(C),
+                  _dp(lam), _dp(mu), float(density), float(time_step))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.ssim_batch_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def num_meshes(self) -> int:
+        return int(self._nv.size)
+
+    def num_vertices(self, mesh: int) -> int:
+        return int(self._nv[mesh])
+
+    # ---- lifecycle ----
+    def reset(self):
+        _lib.call("ssim_batch_reset", self._h)
+
+    def prepare_step(self):
+        _lib.call("ssim_batch_prepare_step", self._h)
+
+    def compute_step(self):
+        _lib.call("ssim_batch_compute_step", self._h)
+
+    def step_next(self):
+        _lib.call("ssim_batch_step_next", self._h)
+
+    def step(self):
+        _lib.call("ssim_batch_step", self._h)
+
+    # ---- boundary conditions and parameters ----
+    def mark_dirichlet_batched(self, mesh: int, verts: np.ndarray, targ_deform: np.ndarray):
+        v = np.ascontiguousarray(verts, dtype=np.int32).reshape(-1)
+        t = np.asarray(targ_deform, dtype=np.float64)
+        if t.shape != (3, v.size):
+            raise ValueError(f"targ_deform must have shape (3, {v.size}), got {t.shape}")
+        t = np.ascontiguousarray(t.T)
+        _lib.call("ssim_batch_mark_dirichlet", self._h, int(mesh), _ip(v), v.size, _dp(t))
+
+    def mark_general_batched(self, mesh: int, verts: np.ndarray):
+        v = np.ascontiguousarray(verts, dtype=np.int32).reshape(-1)
+        _lib.call("ssim_batch_mark_general", self._h, int(mesh), _ip(v), v.size)
+
+    def set_rtol(self, rtol: float):
+        _lib.call("ssim_batch_set_threshold", self._h, float(rtol))
+
+    def solver_params(self) -> dict:
+        p = _lib.SolverParams()
+        _lib.call("ssim_batch_get_solver_params", self._h, ctypes.byref(p))
+        return {k: getattr(p, k) for k, _ in p._fields_}
+
+    def set_solver_params(self, **kw):
+        p = _lib.SolverParams()
+        _lib.call("ssim_batch_get_solver_params", self._h, ctypes.byref(p))
+        for k, v in kw.items():
+            if not hasattr(p, k):
+                raise ValueError(f"unknown solver parameter {k}")
+            setattr(p, k, v)
+        _lib.call("ssim_batch_set_solver_params", self._h, ctypes.byref(p))
+
+    # ---- state ----
+    def get_array(self, name: str, mesh: int = -1) -> np.ndarray:
+        n = int(self._off[-1]) if mesh < 0 else 3 * int(self._nv[mesh])
+        out = np.empty(n)
+        _lib.call("ssim_batch_get_array", self._h, _lib.ARRAYS[name], int(mesh), _dp(out))
+        return out
+
+    def set_array(self, name: str, values: np.ndarray, mesh: int = -1):
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        n = int(self._off[-1]) if mesh < 0 else 3 * int(self._nv[mesh])
+        if a.size != n:
+            raise ValueError(f"{name} must hold {n} values")
+        _lib.call("ssim_batch_set_array", self._h, _lib.ARRAYS[name], int(mesh), _dp(a))
+
+    def deformation(self, mesh: int) -> np.ndarray:
+        return np.ascontiguousarray(self.get_array("deformation", mesh).reshape(-1, 3).T)
+
+    def forces(self, mesh: int) -> np.ndarray:
+        return np.ascontiguousarray(self.get_array("forces", mesh).reshape(-1, 3).T)
+
+    def last_stats(self, mesh: int = -1) -> dict:
+        st = _lib.StepStats()
+        _lib.call("ssim_batch_get_step_stats", self._h, int(mesh), ctypes.byref(st))
+        return st.as_dict()
+
+    def total_mass(self, mesh: int) -> float:
+        m = ctypes.c_double()
+        _lib.call("ssim_batch_total_mass", self._h, int(mesh), ctypes.byref(m))
+        return m.value
+
+    # ---- launch tuning and instrumentation ----
+    def set_tuning(self, **kw):
+        for k, v in kw.items():
+            if k not in _lib.TUNING:
+                raise ValueError(f"unknown tuning key {k}")
+            _lib.call("ssim_batch_set_tuning", self._h, _lib.TUNING[k], int(v))
+
+    def tuning(self) -> dict:
+        out = {}
+        for k, key in _lib.TUNING.items():
+            v = ctypes.c_int()
+            _lib.call("ssim_batch_get_tuning", self._h, key, ctypes.byref(v))
+            out[k] = v.value
+        return out
+
+    def set_kernel_timing(self, enabled: bool = True):
+        _lib.call("ssim_batch_set_kernel_timing", self._h, int(bool(enabled)))
+
+    def kernel_timing(self) -> dict:
+        out = {}
+        for name, k in _lib.KERNELS.items():
+            ms = ctypes.c_double()
+            cnt = ctypes.c_longlong()
+            _lib.call("ssim_batch_get_kernel_timing", self._h, k, ctypes.byref(ms), ctypes.byref(cnt))
+            out[name] = (ms.value, cnt.value)
+        return out
+
+    def layout_info(self) -> dict:
+        info = _lib.LayoutInfo()
+        _lib.call("ssim_batch_get_layout_info", self._h, ctypes.byref(info))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    @classmethod
+    def from_cases(cls, cases, rtol=None):
+        """Batch of seeded ``ssim_amd.mesh.BarCase`` problems (all with the same dt and density)."""
+        dt, rho = cases[0].dt, cases[0].rho
+        if any(c.dt != dt or c.rho != rho for c in cases):
+            raise ValueError("a batch shares one time step and one density")
+        b = cls([c.X for c in cases], [c.cells for c in cases], [c.lam for c in cases], [c.mu for c in cases],
+                dt, rho)
+        for m, c in enumerate(cases):
+            clamped = np.nonzero(c.dof_type.reshape(-1, 3).any(axis=1))[0].astype(np.int32)
+            if clamped.size:
+                b.mark_dirichlet_batched(m, clamped, c.dbc.reshape(-1, 3)[clamped].T)
+        b.set_array("ext_accel", np.concatenate([c.accel for c in cases]))
+        b.set_array("deformation", np.concatenate([c.u0 for c in cases]))
+        b.set_array("velocity", np.concatenate([c.v0 for c in cases]))
+        if rtol is not None:
+            b.set_rtol(rtol)
+        b.reset()
+        return b

@@ -1,0 +1,628 @@
+// batch.hip -- host side of batched stepping: many independent meshes of uneven size advanced together.
+//
+// The reference steps one mesh per time-step object (finite_elements/time_step.hpp); a batch here holds many
+// such meshes in one context and runs each phase of their Newton solves -- energy and gradient, Hessian
+// assembly, PCG iterations, line-search trials -- as one set of launches over all meshes that still take part.
+// Every mesh follows exactly the single-mesh algorithm (backward_euler.hpp:15-33 with the same threshold,
+// Armijo constant, line-search and PCG limits); only the scheduling differs: Newton iterations advance in
+// lock step, a converged mesh sits out (its workgroups return at entry), and each line-search trial evaluates
+// only the meshes still searching.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "context.h"
+
+using namespace ssim_hip;
+
+struct ssim_batch {
+  ssim_ctx c;  // the concatenated, padded mesh
+  int nmesh = 0;
+  std::vector<int> nv_m, nt_m;      // real sizes
+  std::vector<int> vbase, tbase;    // first vertex / tet of each mesh in the concatenation
+  std::vector<int> hvwg_off, htwg_off;
+  int nvwg = 0, ntwg = 0;
+  DBuf<int> vwg_mesh, twg_mesh, vwg_off, twg_off, tend;
+  DBuf<int> mask_all, mask_newton, mask_ls;
+  DBuf<PcgScalars> pcg;
+  DBuf<EvalScalars> ev;
+  DBuf<PcgIter> iter;
+  DBuf<double> alpha;
+  std::vector<double> mass, tol;
+  std::vector<ssim_step_stats> stats;  // per mesh, last compute_step
+  ssim_step_stats total{};             // batch: lock-step iteration counts
+  unsigned char* pin = nullptr;        // pinned staging for per-mesh scalars
+  size_t pin_bytes = 0;
+
+  ~ssim_batch() {
+    if (pin) (void)hipHostFree(pin);
+  }
+  BatchView view(const DBuf<int>& mask) const {
+    BatchView b;
+    b.nmesh = nmesh;
+    b.vwg_mesh = vwg_mesh.p;
+    b.twg_mesh = twg_mesh.p;
+    b.vwg_off = vwg_off.p;
+    b.twg_off = twg_off.p;
+    b.tend = tend.p;
+    b.mask = mask.p;
+    return b;
+  }
+  EvalScalars* pin_ev() { return reinterpret_cast<EvalScalars*>(pin); }
+  PcgScalars* pin_pcg(int slot) {
+    return reinterpret_cast<PcgScalars*>(pin + sizeof(EvalScalars) * (size_t)nmesh) + (size_t)slot * nmesh;
+  }
+};
+
+namespace {
+
+ssim_status_t bguard(const ssim_batch* b) {
+  if (!b) {
+    g_err = "null batch";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  return SSIM_OK;
+}
+
+void check_mesh(const ssim_batch* b, int mesh, bool allow_all) {
+  if (mesh < (allow_all ? -1 : 0) || mesh >= b->nmesh)
+    fail(SSIM_ERR_INVALID_ARGUMENT, "mesh index " + std::to_string(mesh) + " outside [0, " + std::to_string(b->nmesh) +
+                                        ")");
+}
+
+void upload_mask(ssim_batch* b, DBuf<int>& dst, const std::vector<int>& h) {
+  HIPCHK(hipMemcpyAsync(dst.p, h.data(), sizeof(int) * b->nmesh, hipMemcpyHostToDevice, b->c.stream));
+}
+
+// per-mesh masses and thresholds; padding vertices Dirichlet-fixed (after do_reset)
+void batch_reset(ssim_batch* b) {
+  ssim_ctx* c = &b->c;
+  do_reset(c);
+  std::vector<double> hV(c->nt);
+  HIPCHK(hipMemcpyAsync(hV.data(), c->V.p, sizeof(double) * c->nt, hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned char> dof(c->n);
+  HIPCHK(hipMemcpyAsync(dof.data(), c->dof.p, c->n, hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  b->mass.assign(b->nmesh, 0.0);
+  b->tol.assign(b->nmesh, 0.0);
+  for (int m = 0; m < b->nmesh; ++m) {
+    double s = 0;
+    for (int e = b->tbase[m]; e < b->tbase[m] + b->nt_m[m]; ++e) s += c->rho_reset * hV[e];
+    b->mass[m] = s;
+    b->tol[m] = c->rtol * s;
+    const int pad0 = b->vbase[m] + b->nv_m[m], pad1 = kBatchRows * b->hvwg_off[m + 1];
+    for (int v = pad0; v < pad1; ++v)
+      for (int a = 0; a < 3; ++a) dof[3 * (size_t)v + a] = 1;
+  }
+  HIPCHK(hipMemcpyAsync(c->dof.p, dof.data(), c->n, hipMemcpyHostToDevice, c->stream));
+  b->pcg.alloc(b->nmesh);
+  b->ev.alloc(b->nmesh);
+  b->iter.alloc(1);
+  b->alpha.alloc(b->nmesh);
+  HIPCHK(hipMemsetAsync(b->pcg.p, 0, sizeof(PcgScalars) * b->nmesh, c->stream));
+  HIPCHK(hipMemsetAsync(b->ev.p, 0, sizeof(EvalScalars) * b->nmesh, c->stream));
+  HIPCHK(hipMemsetAsync(b->iter.p, 0, sizeof(PcgIter), c->stream));
+  sync(c);
+}
+
+// energy and gradient of the masked meshes at u → g; per-mesh (E, ‖g‖²) into the pinned mirror
+void beval(ssim_batch* b, const DBuf<int>& mask, const double* u, double* g) {
+  ssim_ctx* c = &b->c;
+  hipStream_t s = c->stream;
+  const BatchView bt = b->view(mask);
+  timed(c, SSIM_KERNEL_ELEM_GRAD, [&] {
+    launch_belem_grad(c->mesh(), bt, b->ntwg, u, c->a(SSIM_ARRAY_INERTIA), c->dt * c->dt, c->gperm.p,
+                      c->gcontrib.p, c->part_e(), c->prm.fp32_elements != 0, s);
+  });
+  timed(c, SSIM_KERNEL_GRAD_REDUCE, [&] {
+    launch_bgrad_reduce(bt, b->nvwg, c->gseg.p, c->gcontrib.p, c->dof.p, g, c->part_g(), s);
+  });
+  launch_bfinalize_eval(bt, c->part_e(), c->part_g(), b->ev.p, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->pin_ev(), b->ev.p, sizeof(EvalScalars) * b->nmesh, hipMemcpyDeviceToHost, s));
+  sync(c);
+}
+
+void bassemble(ssim_batch* b, const DBuf<int>& mask, const double* u) {
+  ssim_ctx* c = &b->c;
+  hipStream_t s = c->stream;
+  const BatchView bt = b->view(mask);
+  timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
+    launch_belem_hess(c->mesh(), bt, b->ntwg, u, c->dt * c->dt, 1, c->hperm.p, c->hcontrib.p,
+                      c->prm.fp32_elements != 0, s);
+  });
+  timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
+    launch_bhess_reduce(bt, c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->val.p,
+                        c->dinv.p, s);
+  });
+  HIPCHK(hipGetLastError());
+}
+
+// batched Jacobi-PCG of H x = −g for the masked meshes; per-mesh iteration counts into iters
+void bpcg(ssim_batch* b, const double* g, double* x, std::vector<int>& iters) {
+  ssim_ctx* c = &b->c;
+  hipStream_t s = c->stream;
+  const BatchView bt = b->view(b->mask_newton);
+  const BellView A = c->bell(c->val.p);
+  const int maxit = c->prm.pcg_maxit;
+  launch_bpcg_init(bt, b->nvwg, g, -1.0, c->dinv.p, x, c->r.p, c->z.p, c->prm.pcg_rtol, maxit, b->pcg.p, b->iter.p,
+                   c->part_pb(0), s);
+  HIPCHK(hipGetLastError());
+  for (auto& e : c->poll_ev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  auto enqueue = [&](int count, int slot) {
+    for (int j = 0; j < count; ++j) {
+      auto spmv = [&] {
+        launch_bpcg_spmv(A, bt, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), b->pcg.p, b->iter.p,
+                         c->part_pa(), s);
+      };
+      auto update = [&] {
+        launch_bpcg_update(bt, b->nvwg, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), c->part_pb(0),
+                           c->part_pb(1), b->pcg.p, b->iter.p, s);
+      };
+      if (j == 0) {
+        timed(c, SSIM_KERNEL_PCG_SPMV, spmv);
+        timed(c, SSIM_KERNEL_PCG_UPDATE, update);
+      } else {
+        spmv();
+        update();
+      }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->pin_pcg(slot), b->pcg.p, sizeof(PcgScalars) * b->nmesh, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(c->poll_ev[slot], s));
+  };
+  int chunk = 8, launched = 0, slot = 0;
+  enqueue(chunk, slot);
+  launched += chunk;
+  for (;;) {
+    const int next = std::min(chunk * 2, std::max(8, c->pcg_chunk_max));
+    enqueue(next, slot ^ 1);
+    launched += next;
+    HIPCHK(hipEventSynchronize(c->poll_ev[slot]));
+    if (c->timing) harvest_timers(c);
+    const PcgScalars* f = b->pin_pcg(slot);
+    bool all = true;
+    for (int m = 0; m < b->nmesh && all; ++m) all = f[m].done != 0;
+    if (all) {
+      for (int m = 0; m < b->nmesh; ++m) {
+        if (f[m].breakdown == 1)
+          fail(SSIM_ERR_NUMERIC, "mesh " + std::to_string(m) + ": PCG breakdown: pᵀAp ≤ 0 (matrix not positive definite)");
+        if (f[m].breakdown == 2) fail(SSIM_ERR_NUMERIC, "mesh " + std::to_string(m) + ": PCG breakdown: NaN residual");
+        iters[m] = f[m].iters;
+      }
+      return;
+    }
+    if (launched > maxit + 2 * std::max(8, c->pcg_chunk_max) + 16) fail(SSIM_ERR_STATE, "internal: PCG did not stop");
+    chunk = next;
+    slot ^= 1;
+  }
+}
+
+// Newton with backtracking Armijo line search, per mesh, in lock step (backward_euler.hpp:15-33)
+void bnewton(ssim_batch* b) {
+  ssim_ctx* c = &b->c;
+  hipStream_t s = c->stream;
+  const int M = b->nmesh;
+  const ssim_solver_params& P = c->prm;
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* g = c->a(SSIM_ARRAY_FORCES);
+  std::vector<double> E(M), gg(M), gd(M), alpha(M), Et(M), ggt(M);
+  std::vector<int> act(M, 1), search(M, 0), iters(M, 0);
+  for (auto& st : b->stats) st = ssim_step_stats{};
+  b->total = ssim_step_stats{};
+  beval(b, b->mask_all, u, g);
+  for (int m = 0; m < M; ++m) {
+    E[m] = b->pin_ev()[m].energy;
+    gg[m] = b->pin_ev()[m].gg;
+    b->stats[m].energy_evals++;
+  }
+  b->total.energy_evals++;
+  int it = 0;
+  for (; it < P.max_newton; ++it) {
+    int nact = 0;
+    for (int m = 0; m < M; ++m) {
+      if (!act[m]) continue;
+      if (!std::isfinite(E[m]) || !std::isfinite(gg[m]))
+        fail(SSIM_ERR_NUMERIC, "mesh " + std::to_string(m) + ": non-finite energy or gradient");
+      if (std::sqrt(gg[m]) < b->tol[m]) {
+        act[m] = 0;
+        b->stats[m].converged = 1;
+        b->stats[m].newton_iterations = it;
+      } else {
+        ++nact;
+      }
+    }
+    if (nact == 0) break;
+    upload_mask(b, b->mask_newton, act);
+    bassemble(b, b->mask_newton, u);
+    bpcg(b, g, c->dir.p, iters);
+    int maxk = 0;
+    for (int m = 0; m < M; ++m)
+      if (act[m]) {
+        b->stats[m].hessian_evals++;
+        b->stats[m].pcg_iterations += iters[m];
+        maxk = std::max(maxk, iters[m]);
+      }
+    b->total.hessian_evals++;
+    b->total.pcg_iterations += maxk;
+    launch_bdot(b->view(b->mask_newton), b->nvwg, g, c->dir.p, c->part_d(), b->ev.p, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->pin_ev(), b->ev.p, sizeof(EvalScalars) * M, hipMemcpyDeviceToHost, s));
+    sync(c);
+    for (int m = 0; m < M; ++m) {
+      gd[m] = act[m] ? b->pin_ev()[m].gd : 0.0;
+      alpha[m] = 1.0;
+      search[m] = act[m];
+    }
+    for (int ls = 0; ls < P.ls_max; ++ls) {
+      upload_mask(b, b->mask_ls, search);
+      HIPCHK(hipMemcpyAsync(b->alpha.p, alpha.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
+      launch_baxpy_to(b->view(b->mask_ls), b->nvwg, u, b->alpha.p, c->dir.p, c->utry.p, s);
+      beval(b, b->mask_ls, c->utry.p, c->gtry.p);
+      b->total.energy_evals++;
+      int left = 0;
+      for (int m = 0; m < M; ++m) {
+        if (!search[m]) continue;
+        Et[m] = b->pin_ev()[m].energy;
+        ggt[m] = b->pin_ev()[m].gg;
+        b->stats[m].energy_evals++;
+        if (Et[m] <= E[m] + P.ls_c * alpha[m] * gd[m]) {
+          search[m] = 0;
+        } else {
+          alpha[m] *= P.ls_beta;
+          ++left;
+        }
+      }
+      if (!left) break;
+    }
+    for (int m = 0; m < M; ++m) {
+      if (!act[m]) continue;
+      if (search[m]) b->stats[m].ls_failures++;  // the last trial is taken, as in the single-mesh solver
+      E[m] = Et[m];
+      gg[m] = ggt[m];
+    }
+    launch_bcopy2(b->view(b->mask_newton), b->nvwg, c->utry.p, u, c->gtry.p, g, s);
+    HIPCHK(hipGetLastError());
+  }
+  for (int m = 0; m < M; ++m)
+    if (act[m]) b->stats[m].newton_iterations = it;  // stopped at the iteration cap
+  b->total.newton_iterations = it;
+}
+
+}  // namespace
+
+// ======================================================================================================
+// extern "C" ABI
+
+extern "C" {
+
+ssim_status_t ssim_batch_create(ssim_batch** out, int nmesh, const int* nv, const int* nt, const double* vertices,
+                                const int* cells, const double* lambda, const double* mu, double density,
+                                double time_step) {
+  if (!out) {
+    g_err = "out is null";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  ssim_batch* b = nullptr;
+  const ssim_status_t st = run([&] {
+    if (nmesh <= 0) fail(SSIM_ERR_INVALID_ARGUMENT, "a batch needs at least one mesh");
+    if (!nv || !nt || !vertices || !cells || !lambda || !mu) fail(SSIM_ERR_INVALID_ARGUMENT, "null batch arrays");
+    if (!(time_step > 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Time step must be positive.");
+    if (!(density >= 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Density must be positive, got " + std::to_string(density));
+    long long pv = 0, pt = 0;
+    for (int m = 0; m < nmesh; ++m) {
+      if (nv[m] <= 0 || nt[m] <= 0) fail(SSIM_ERR_INVALID_ARGUMENT, "mesh " + std::to_string(m) + " is empty");
+      pv += (nv[m] + kBatchRows - 1) / kBatchRows * (long long)kBatchRows;
+      pt += (nt[m] + kBatchRows - 1) / kBatchRows * (long long)kBatchRows;
+    }
+    if (3 * pv >= (1LL << 31) || 12 * pt >= (1LL << 31)) fail(SSIM_ERR_INVALID_ARGUMENT, "batch too large for 32-bit indices");
+    {
+      long long off = 0, toff = 0;
+      for (int m = 0; m < nmesh; ++m) {
+        for (long long k = 0; k < 4LL * nt[m]; ++k) {
+          const int v = cells[toff + k];
+          if (v < 0 || v >= nv[m])
+            fail(SSIM_ERR_INVALID_ARGUMENT, "mesh " + std::to_string(m) + ": cell " + std::to_string(k / 4) +
+                                                " references vertex " + std::to_string(v) + " outside [0, " +
+                                                std::to_string(nv[m]) + ")");
+        }
+        for (long long e = 0; e < nt[m]; ++e)
+          if (!(lambda[off + e] > 0) || !(mu[off + e] > 0))
+            fail(SSIM_ERR_INVALID_ARGUMENT, "mesh " + std::to_string(m) + ": Lamé parameters must be positive");
+        off += nt[m];
+        toff += 4LL * nt[m];
+      }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) fail(SSIM_ERR_HIP, "no HIP device available");
+    b = new ssim_batch();
+    ssim_ctx* c = &b->c;
+    b->nmesh = nmesh;
+    b->nv_m.assign(nv, nv + nmesh);
+    b->nt_m.assign(nt, nt + nmesh);
+    b->vbase.resize(nmesh);
+    b->tbase.resize(nmesh);
+    b->hvwg_off.assign(nmesh + 1, 0);
+    b->htwg_off.assign(nmesh + 1, 0);
+    for (int m = 0; m < nmesh; ++m) {
+      b->hvwg_off[m + 1] = b->hvwg_off[m] + (nv[m] + kBatchRows - 1) / kBatchRows;
+      b->htwg_off[m + 1] = b->htwg_off[m] + (nt[m] + kBatchRows - 1) / kBatchRows;
+      b->vbase[m] = kBatchRows * b->hvwg_off[m];
+      b->tbase[m] = kBatchRows * b->htwg_off[m];
+    }
+    b->nvwg = b->hvwg_off[nmesh];
+    b->ntwg = b->htwg_off[nmesh];
+    c->nv = (int)pv;
+    c->nt = (int)pt;
+    c->n = 3 * c->nv;
+    c->dt = time_step;
+    c->density = density;
+    c->per_tet = true;
+    default_params(c->prm);
+    c->hX.assign(3 * (size_t)pv, 0.0);
+    c->hcells.assign(4 * (size_t)pt, -1);
+    c->hlam.assign(pt, 1.0);
+    c->hmu.assign(pt, 1.0);
+    std::vector<int> vwg_mesh(b->nvwg), twg_mesh(b->ntwg), tend(nmesh);
+    {
+      long long voff = 0, toff = 0;
+      for (int m = 0; m < nmesh; ++m) {
+        std::copy(vertices + 3 * voff, vertices + 3 * (voff + nv[m]), c->hX.begin() + 3 * (size_t)b->vbase[m]);
+        for (long long k = 0; k < 4LL * nt[m]; ++k) c->hcells[4 * (size_t)b->tbase[m] + k] = cells[4 * toff + k] + b->vbase[m];
+        std::copy(lambda + toff, lambda + toff + nt[m], c->hlam.begin() + b->tbase[m]);
+        std::copy(mu + toff, mu + toff + nt[m], c->hmu.begin() + b->tbase[m]);
+        for (int w = b->hvwg_off[m]; w < b->hvwg_off[m + 1]; ++w) vwg_mesh[w] = m;
+        for (int w = b->htwg_off[m]; w < b->htwg_off[m + 1]; ++w) twg_mesh[w] = m;
+        tend[m] = b->tbase[m] + nt[m];
+        voff += nv[m];
+        toff += nt[m];
+      }
+    }
+    hipStream_t s = c->stream;
+    b->pin_bytes = (sizeof(EvalScalars) + 2 * sizeof(PcgScalars)) * (size_t)nmesh;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->pin), b->pin_bytes, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->pin), 64 * sizeof(double), hipHostMallocDefault));
+    c->X.upload(c->hX, s);
+    c->cells.upload(reinterpret_cast<const int4*>(c->hcells.data()), pt, s);
+    b->vwg_mesh.upload(vwg_mesh, s);
+    b->twg_mesh.upload(twg_mesh, s);
+    b->vwg_off.upload(b->hvwg_off, s);
+    b->twg_off.upload(b->htwg_off, s);
+    b->tend.upload(tend, s);
+    std::vector<int> ones(nmesh, 1);
+    b->mask_all.upload(ones, s);
+    b->mask_newton.upload(ones, s);
+    b->mask_ls.upload(ones, s);
+    b->stats.assign(nmesh, ssim_step_stats{});
+    batch_reset(b);
+  });
+  if (st != SSIM_OK) {
+    delete b;
+    return st;
+  }
+  *out = b;
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_batch_destroy(ssim_batch* b) {
+  if (!b) return SSIM_OK;
+  (void)hipStreamSynchronize(b->c.stream);
+  delete b;
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_batch_set_stream(ssim_batch* b, void* hip_stream) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    sync(&b->c);
+    b->c.stream = reinterpret_cast<hipStream_t>(hip_stream);
+  });
+}
+
+ssim_status_t ssim_batch_num_meshes(const ssim_batch* b, int* nmesh) {
+  if (bguard(b) || !nmesh) return SSIM_ERR_INVALID_ARGUMENT;
+  *nmesh = b->nmesh;
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_batch_set_threshold(ssim_batch* b, double rtol) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!(rtol > 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "threshold must be positive");
+    b->c.rtol = rtol;
+    for (int m = 0; m < b->nmesh; ++m) b->tol[m] = rtol * b->mass[m];
+  });
+}
+
+ssim_status_t ssim_batch_set_solver_params(ssim_batch* b, const ssim_solver_params* p) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!p) fail(SSIM_ERR_INVALID_ARGUMENT, "params is null");
+    if (p->max_newton < 0 || !(p->pcg_rtol > 0) || p->pcg_maxit < 0 || !(p->ls_c > 0 && p->ls_c < 1) ||
+        !(p->ls_beta > 0 && p->ls_beta < 1) || p->ls_max < 1)
+      fail(SSIM_ERR_INVALID_ARGUMENT, "solver parameters out of range");
+    b->c.prm = *p;
+  });
+}
+
+ssim_status_t ssim_batch_get_solver_params(const ssim_batch* b, ssim_solver_params* p) {
+  if (bguard(b) || !p) return SSIM_ERR_INVALID_ARGUMENT;
+  *p = b->c.prm;
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_batch_mark_dirichlet(ssim_batch* b, int mesh, const int* verts, int count, const double* targets) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    check_mesh(b, mesh, false);
+    if (count < 0 || (count > 0 && !verts)) fail(SSIM_ERR_INVALID_ARGUMENT, "bad vertex list");
+    std::vector<int> gv(count);
+    for (int i = 0; i < count; ++i) {
+      if (verts[i] < 0 || verts[i] >= b->nv_m[mesh]) fail(SSIM_ERR_INVALID_ARGUMENT, "vertex index out of range");
+      gv[i] = verts[i] + b->vbase[mesh];
+    }
+    edit_dirichlet(&b->c, gv.data(), count, targets, true);
+  });
+}
+
+ssim_status_t ssim_batch_mark_general(ssim_batch* b, int mesh, const int* verts, int count) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    check_mesh(b, mesh, false);
+    if (count < 0 || (count > 0 && !verts)) fail(SSIM_ERR_INVALID_ARGUMENT, "bad vertex list");
+    std::vector<int> gv(count);
+    for (int i = 0; i < count; ++i) {
+      if (verts[i] < 0 || verts[i] >= b->nv_m[mesh]) fail(SSIM_ERR_INVALID_ARGUMENT, "vertex index out of range");
+      gv[i] = verts[i] + b->vbase[mesh];
+    }
+    edit_dirichlet(&b->c, gv.data(), count, nullptr, false);
+  });
+}
+
+ssim_status_t ssim_batch_reset(ssim_batch* b) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] { batch_reset(b); });
+}
+
+ssim_status_t ssim_batch_prepare_step(ssim_batch* b) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    ssim_ctx* c = &b->c;
+    launch_prepare(c->n, c->a(SSIM_ARRAY_DEFORMATION), c->a(SSIM_ARRAY_VELOCITY), c->a(SSIM_ARRAY_EXT_ACCEL), c->dof.p,
+                   c->a(SSIM_ARRAY_DBC_VALUES), c->dt, c->a(SSIM_ARRAY_INERTIA), c->a(SSIM_ARRAY_NEXT_DEFORM),
+                   c->stream);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ssim_status_t ssim_batch_compute_step(ssim_batch* b) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    ssim_ctx* c = &b->c;
+    bnewton(b);
+    // compute_step's final energy / gradient evaluation (time_step.hpp:520-528), for every mesh
+    beval(b, b->mask_all, c->a(SSIM_ARRAY_NEXT_DEFORM), c->a(SSIM_ARRAY_FORCES));
+    b->total.energy_evals++;
+    int conv = 1;
+    double gmax = 0, esum = 0;
+    for (int m = 0; m < b->nmesh; ++m) {
+      const EvalScalars& e = b->pin_ev()[m];
+      if (!std::isfinite(e.energy) || !std::isfinite(e.gg))
+        fail(SSIM_ERR_NUMERIC, "mesh " + std::to_string(m) + ": non-finite energy or gradient");
+      b->stats[m].grad_norm = std::sqrt(e.gg);
+      b->stats[m].energy = e.energy;
+      b->stats[m].energy_evals++;
+      conv &= b->stats[m].converged;
+      gmax = std::max(gmax, b->stats[m].grad_norm);
+      esum += e.energy;
+    }
+    b->total.converged = conv;
+    b->total.grad_norm = gmax;
+    b->total.energy = esum;
+  });
+}
+
+ssim_status_t ssim_batch_step_next(ssim_batch* b) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    ssim_ctx* c = &b->c;
+    launch_step_next(c->n, c->a(SSIM_ARRAY_NEXT_DEFORM), c->a(SSIM_ARRAY_DEFORMATION), c->a(SSIM_ARRAY_PREV_DEFORM),
+                     c->a(SSIM_ARRAY_VELOCITY), c->dt, c->damping, c->stream);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+ssim_status_t ssim_batch_step(ssim_batch* b) {
+  ssim_status_t st = ssim_batch_prepare_step(b);
+  if (st) return st;
+  st = ssim_batch_compute_step(b);
+  if (st) return st;
+  return ssim_batch_step_next(b);
+}
+
+ssim_status_t ssim_batch_get_step_stats(const ssim_batch* b, int mesh, ssim_step_stats* stats) {
+  if (bguard(b) || !stats) return SSIM_ERR_INVALID_ARGUMENT;
+  if (mesh < -1 || mesh >= b->nmesh) {
+    g_err = "mesh index out of range";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  *stats = mesh < 0 ? b->total : b->stats[mesh];
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_batch_get_array(ssim_batch* b, ssim_array_t which, int mesh, double* host_out) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (which < 0 || which >= SSIM_ARRAY_COUNT || !host_out) fail(SSIM_ERR_INVALID_ARGUMENT, "bad array request");
+    check_mesh(b, mesh, true);
+    ssim_ctx* c = &b->c;
+    size_t off = 0;
+    for (int m = 0; m < b->nmesh; ++m) {
+      if (mesh >= 0 && m != mesh) continue;
+      const size_t cnt = 3 * (size_t)b->nv_m[m];
+      HIPCHK(hipMemcpyAsync(host_out + off, c->a(which) + 3 * (size_t)b->vbase[m], sizeof(double) * cnt,
+                            hipMemcpyDeviceToHost, c->stream));
+      off += cnt;
+    }
+    sync(c);
+  });
+}
+
+ssim_status_t ssim_batch_set_array(ssim_batch* b, ssim_array_t which, int mesh, const double* host_in) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (which < 0 || which >= SSIM_ARRAY_COUNT || !host_in) fail(SSIM_ERR_INVALID_ARGUMENT, "bad array request");
+    check_mesh(b, mesh, true);
+    ssim_ctx* c = &b->c;
+    size_t off = 0;
+    for (int m = 0; m < b->nmesh; ++m) {
+      if (mesh >= 0 && m != mesh) continue;
+      const size_t cnt = 3 * (size_t)b->nv_m[m];
+      HIPCHK(hipMemcpyAsync(c->a(which) + 3 * (size_t)b->vbase[m], host_in + off, sizeof(double) * cnt,
+                            hipMemcpyHostToDevice, c->stream));
+      off += cnt;
+    }
+    sync(c);
+  });
+}
+
+ssim_status_t ssim_batch_total_mass(const ssim_batch* b, int mesh, double* mass) {
+  if (bguard(b) || !mass || mesh < 0 || mesh >= b->nmesh) return SSIM_ERR_INVALID_ARGUMENT;
+  *mass = b->mass[mesh];
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_batch_get_layout_info(const ssim_batch* b, ssim_layout_info* info) {
+  if (bguard(b) || !info) return SSIM_ERR_INVALID_ARGUMENT;
+  return ssim_get_layout_info(&b->c, info);
+}
+
+ssim_status_t ssim_batch_set_tuning(ssim_batch* b, ssim_tune_t key, int value) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  if (key == SSIM_TUNE_SPMV_LANES && value == 4) {
+    g_err = "batched PCG runs one row per lane (SpMV variant 0 or 1)";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  return ssim_set_tuning(&b->c, key, value);
+}
+
+ssim_status_t ssim_batch_get_tuning(const ssim_batch* b, ssim_tune_t key, int* value) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return ssim_get_tuning(&b->c, key, value);
+}
+
+ssim_status_t ssim_batch_set_kernel_timing(ssim_batch* b, int enabled) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return ssim_set_kernel_timing(&b->c, enabled);
+}
+
+ssim_status_t ssim_batch_get_kernel_timing(const ssim_batch* b, ssim_kernel_t kernel, double* avg_ms,
+                                           long long* samples) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return ssim_get_kernel_timing(&b->c, kernel, avg_ms, samples);
+}
+
+}  // extern "C"

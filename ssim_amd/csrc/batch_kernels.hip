@@ -1,0 +1,390 @@
+// batch_kernels.hip -- kernels of batched stepping: many meshes of uneven size advanced by one set of launches.
+//
+// The batch is one concatenated mesh whose per-mesh vertex and tet ranges are padded to whole workgroups, so
+// a workgroup reads its mesh from a small table and every per-workgroup partial sum belongs to one mesh.  The
+// element bodies, the segmented reduces and the blocked-ELL row product are the single-mesh ones; what changes
+// is that scalars (energies, dot products, PCG ρ, α, β, convergence) are per mesh, and a workgroup whose mesh
+// is masked out or has converged returns at entry.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace ssim_hip {
+
+namespace {
+
+// dofs of vertex workgroup w handled by thread t: 3·kBatchRows·w + t + kBatchRows·j, j = 0..2 (coalesced)
+constexpr int kBVec = 3;
+
+__device__ __forceinline__ size_t bdof(int j) {
+  return (size_t)blockIdx.x * (kBVec * kBatchRows) + (size_t)j * kBatchRows + threadIdx.x;
+}
+
+// ------------------------------------------------------------------------------------------------------
+// energy + gradient
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_belem_grad(MeshView m, BatchView bt, const double* __restrict__ u,
+                                                       const double* __restrict__ uhat, double dt2,
+                                                       const int* __restrict__ gperm, double* __restrict__ gcontrib,
+                                                       double* __restrict__ part_e) {
+  const int mesh = bt.twg_mesh[blockIdx.x];
+  if (!bt.mask[mesh]) return;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  double energy[1] = {0.0};
+  if (e < bt.tend[mesh]) energy[0] = elem_grad_tet<T>(m, e, u, uhat, dt2, gperm, gcontrib);
+  store_partials<1>(energy, part_e);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bgrad_reduce(BatchView bt, const int* __restrict__ gseg,
+                                                         const double* __restrict__ gcontrib,
+                                                         const unsigned char* __restrict__ dof, double* __restrict__ g,
+                                                         double* __restrict__ part_g) {
+  if (!bt.mask[bt.vwg_mesh[blockIdx.x]]) return;
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  double gg[1] = {grad_reduce_vertex(v, gseg, gcontrib, dof, g)};
+  store_partials<1>(gg, part_g);
+}
+
+// one workgroup per mesh: energy over the mesh's tet workgroups, ‖g‖² over its vertex workgroups
+__global__ __launch_bounds__(kBlock) void k_bfinalize_eval(BatchView bt, const double* __restrict__ part_e,
+                                                           const double* __restrict__ part_g, EvalScalars* out) {
+  const int mesh = blockIdx.x;
+  if (!bt.mask[mesh]) return;
+  double te[1], tg[1] = {0.0};
+  const int t0 = bt.twg_off[mesh], v0 = bt.vwg_off[mesh];
+  sum_partials<1>(part_e + t0, bt.twg_off[mesh + 1] - t0, te);
+  if (part_g) sum_partials<1>(part_g + v0, bt.vwg_off[mesh + 1] - v0, tg);
+  if (threadIdx.x == 0) {
+    out[mesh].energy = te[0];
+    if (part_g) out[mesh].gg = tg[0];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bdot_part(BatchView bt, const double* __restrict__ a,
+                                                      const double* __restrict__ b, double* __restrict__ part) {
+  if (!bt.mask[bt.vwg_mesh[blockIdx.x]]) return;
+  double v[1] = {0.0};
+#pragma unroll
+  for (int j = 0; j < kBVec; ++j) {
+    const size_t i = bdof(j);
+    v[0] += a[i] * b[i];
+  }
+  store_partials<1>(v, part);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfinalize_dot(BatchView bt, const double* __restrict__ part,
+                                                          EvalScalars* out) {
+  const int mesh = blockIdx.x;
+  if (!bt.mask[mesh]) return;
+  double t[1];
+  const int v0 = bt.vwg_off[mesh];
+  sum_partials<1>(part + v0, bt.vwg_off[mesh + 1] - v0, t);
+  if (threadIdx.x == 0) out[mesh].gd = t[0];
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Hessian
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_belem_hess(MeshView m, BatchView bt, const double* __restrict__ u,
+                                                       double dt2, int make_spsd, const int* __restrict__ hperm,
+                                                       double* __restrict__ hcontrib) {
+  const int mesh = bt.twg_mesh[blockIdx.x];
+  if (!bt.mask[mesh]) return;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= bt.tend[mesh]) return;
+  elem_hess_tet<T>(m, e, u, dt2, make_spsd, hperm, hcontrib);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bhess_reduce(BatchView bt, int nub, const int* __restrict__ hseg,
+                                                         const double* __restrict__ hcontrib,
+                                                         const int2* __restrict__ ub_rc,
+                                                         const int* __restrict__ ub_pos,
+                                                         const unsigned char* __restrict__ dof, int filter_dbc,
+                                                         double* __restrict__ val, double* __restrict__ dinv) {
+  const int ub = blockIdx.x * kBlock + threadIdx.x;
+  if (ub >= nub) return;
+  const int2 rc = ub_rc[ub];
+  if (!bt.mask[bt.vwg_mesh[rc.x / kBatchRows]]) return;
+  hess_reduce_block(ub, rc, hseg, hcontrib, ub_pos, dof, filter_dbc, val, dinv, nullptr);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// line-search vector kernels
+
+__global__ __launch_bounds__(kBlock) void k_baxpy_to(BatchView bt, const double* __restrict__ x,
+                                                     const double* __restrict__ alpha, const double* __restrict__ d,
+                                                     double* __restrict__ out) {
+  const int mesh = bt.vwg_mesh[blockIdx.x];
+  if (!bt.mask[mesh]) return;
+  const double a = alpha[mesh];
+#pragma unroll
+  for (int j = 0; j < kBVec; ++j) {
+    const size_t i = bdof(j);
+    out[i] = x[i] + a * d[i];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bcopy2(BatchView bt, const double* __restrict__ s1,
+                                                   double* __restrict__ d1, const double* __restrict__ s2,
+                                                   double* __restrict__ d2) {
+  if (!bt.mask[bt.vwg_mesh[blockIdx.x]]) return;
+#pragma unroll
+  for (int j = 0; j < kBVec; ++j) {
+    const size_t i = bdof(j);
+    d1[i] = s1[i];
+    d2[i] = s2[i];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Batched Jacobi-PCG: the single-mesh two-kernel iteration with per-mesh scalars.  The partials of ρ = r·z
+// and r·r are per vertex workgroup (pb0 / pb1 alternate with the iteration), those of p·q per 64-row slice;
+// each mesh's workgroups sum only the mesh's own partials (a few dozen), in a fixed order.
+
+__global__ __launch_bounds__(kBlock) void k_bpcg_init(BatchView bt, const double* __restrict__ b, double sign,
+                                                      const double* __restrict__ dinv, double* __restrict__ x,
+                                                      double* __restrict__ r, double* __restrict__ z,
+                                                      double* __restrict__ pb) {
+  if (!bt.mask[bt.vwg_mesh[blockIdx.x]]) return;
+  double v[2] = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < kBVec; ++j) {
+    const size_t i = bdof(j);
+    const double bi = sign * b[i];
+    const double zi = dinv[i] * bi;
+    x[i] = 0.0;
+    r[i] = bi;
+    z[i] = zi;
+    v[0] += bi * zi;
+    v[1] += bi * bi;
+  }
+  store_partials<2>(v, pb);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bpcg_init_fin(BatchView bt, const double* __restrict__ pb, double rtol,
+                                                          int maxit, PcgScalars* sc, PcgIter* it) {
+  const int mesh = blockIdx.x;
+  if (mesh == 0 && threadIdx.x == 0) {
+    it->ka = 0;
+    it->kb = 0;
+  }
+  PcgScalars* S = sc + mesh;
+  if (!bt.mask[mesh]) {
+    if (threadIdx.x == 0) {
+      S->iters = 0;
+      S->breakdown = 0;
+      S->done = 1;
+    }
+    return;
+  }
+  double t[2];
+  const int v0 = bt.vwg_off[mesh];
+  sum_partials<2>(pb + 2 * (size_t)v0, bt.vwg_off[mesh + 1] - v0, t);
+  if (threadIdx.x == 0) {
+    S->rz = t[0];
+    S->bb = t[1];
+    S->rr = t[1];
+    S->tol2 = rtol * rtol * t[1];
+    S->iters = 0;
+    S->maxit = maxit;
+    S->breakdown = 0;
+    S->done = (t[1] <= S->tol2 || maxit <= 0 || !(t[1] == t[1])) ? 1 : 0;
+  }
+}
+
+// kernel A: convergence test on the mesh's ‖r_k‖² (a few dozen partials, so it goes first and a converged
+// mesh skips its SpMV), β, s = A z_k on one 64-row slice, p_k, q_k, p_k·q_k
+template <int LPR>
+__global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_bpcg_spmv(
+    BellView A, BatchView bt, const double* __restrict__ z, double* __restrict__ p, double* __restrict__ q,
+    const double* __restrict__ pb0, const double* __restrict__ pb1, PcgScalars* sc, PcgIter* it,
+    double* __restrict__ pa) {
+  const int k = it->ka;
+  if (blockIdx.x == 0 && threadIdx.x == 0) it->kb = k;
+  const int lb = logical_block(A.nslices, A.xcd_remap);
+  if (lb < 0) return;
+  const int vw = lb / kBatchSlices;
+  const int mesh = bt.vwg_mesh[vw];
+  PcgScalars* S = sc + mesh;
+  if (S->done) return;
+  double beta = 0.0;
+  if (k > 0) {
+    const double* pb_cur = (k & 1) ? pb1 : pb0;
+    const double* pb_prev = (k & 1) ? pb0 : pb1;
+    const int v0 = bt.vwg_off[mesh], nw = bt.vwg_off[mesh + 1] - v0;
+    double t_cur[2], t_prev[2];
+    sum_partials<2, kSpmvBlock, 1>(pb_cur + 2 * (size_t)v0, nw, t_cur);
+    sum_partials<2, kSpmvBlock, 1>(pb_prev + 2 * (size_t)v0, nw, t_prev);
+    const bool nan = !(t_cur[1] == t_cur[1]);
+    const bool done = t_cur[1] <= S->tol2 || k >= S->maxit || nan;
+    if (lb == v0 * kBatchSlices && threadIdx.x == 0) {  // the mesh's first slice reports
+      S->iters = k;
+      S->rr = t_cur[1];
+      S->rz = t_cur[0];
+      if (nan) S->breakdown = 2;
+      if (done) S->done = 1;
+    }
+    if (done) return;
+    beta = t_cur[0] / t_prev[0];
+  }
+  const int v = lb * kSlice + threadIdx.x;
+  double sv[3] = {0.0, 0.0, 0.0};
+  // v < nrows always holds here; keeping the row product under a branch, as in the single-mesh kernel, gives it a
+  // scheduling region of its own -- without it the compiler spills the fixed-width row at this register budget
+  if (v < A.nrows) row_part<LPR>(A, v, 0, z, sv);
+  double acc[1] = {0.0};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const size_t i = 3 * (size_t)v + a;
+    double pv = z[i], qv = sv[a];
+    if (k > 0) {
+      pv += beta * p[i];
+      qv += beta * q[i];
+    }
+    p[i] = pv;
+    q[i] = qv;
+    acc[0] += pv * qv;
+  }
+  store_partials<1, kSpmvBlock>(acc, pa, lb);
+}
+
+// kernel B: α = ρ_k / (p_k·q_k); x += α p, r −= α q, z = D⁻¹ r; partials of (ρ_{k+1}, ‖r_{k+1}‖²)
+__global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const double* __restrict__ dinv,
+                                                        double* __restrict__ x, double* __restrict__ r,
+                                                        double* __restrict__ z, const double* __restrict__ p,
+                                                        const double* __restrict__ q, const double* __restrict__ pa,
+                                                        double* __restrict__ pb0, double* __restrict__ pb1,
+                                                        PcgScalars* sc, PcgIter* it) {
+  const int k = it->kb;
+  if (blockIdx.x == 0 && threadIdx.x == 0) it->ka = k + 1;
+  const int mesh = bt.vwg_mesh[blockIdx.x];
+  PcgScalars* S = sc + mesh;
+  if (S->done) return;
+  double xv[kBVec], rv[kBVec], pv[kBVec], qv[kBVec], dv[kBVec];
+#pragma unroll
+  for (int j = 0; j < kBVec; ++j) {
+    const size_t i = bdof(j);
+    xv[j] = x[i];
+    rv[j] = r[i];
+    pv[j] = p[i];
+    qv[j] = q[i];
+    dv[j] = dinv[i];
+  }
+  const double* pb_cur = (k & 1) ? pb1 : pb0;
+  double* pb_next = (k & 1) ? pb0 : pb1;
+  const int v0 = bt.vwg_off[mesh], nw = bt.vwg_off[mesh + 1] - v0;
+  double pq[1], rz[2];
+  sum_partials<1, kBlock, 1>(pa + (size_t)v0 * kBatchSlices, nw * kBatchSlices, pq);
+  sum_partials<2, kBlock, 1>(pb_cur + 2 * (size_t)v0, nw, rz);
+  if (!(pq[0] > 0.0)) {
+    if (blockIdx.x == v0 && threadIdx.x == 0) {
+      S->breakdown = 1;
+      S->done = 1;
+    }
+    return;
+  }
+  const double alpha = rz[0] / pq[0];
+  double v[2] = {0.0, 0.0};
+#pragma unroll
+  for (int j = 0; j < kBVec; ++j) {
+    const size_t i = bdof(j);
+    x[i] = xv[j] + alpha * pv[j];
+    const double ri = rv[j] - alpha * qv[j];
+    const double zi = dv[j] * ri;
+    r[i] = ri;
+    z[i] = zi;
+    v[0] += ri * zi;
+    v[1] += ri * ri;
+  }
+  store_partials<2>(v, pb_next);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------------
+// launchers
+
+void launch_belem_grad(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* uhat,
+                       double dt2, const int* gperm, double* gcontrib, double* part_e, bool fp32, hipStream_t s) {
+  if (ntwg <= 0) return;
+  if (fp32)
+    k_belem_grad<float><<<ntwg, kBlock, 0, s>>>(m, bt, u, uhat, dt2, gperm, gcontrib, part_e);
+  else
+    k_belem_grad<double><<<ntwg, kBlock, 0, s>>>(m, bt, u, uhat, dt2, gperm, gcontrib, part_e);
+}
+
+void launch_bgrad_reduce(const BatchView& bt, int nvwg, const int* gseg, const double* gcontrib,
+                         const unsigned char* dof, double* g, double* part_g, hipStream_t s) {
+  if (nvwg <= 0) return;
+  k_bgrad_reduce<<<nvwg, kBlock, 0, s>>>(bt, gseg, gcontrib, dof, g, part_g);
+}
+
+void launch_bfinalize_eval(const BatchView& bt, const double* part_e, const double* part_g, EvalScalars* out,
+                           hipStream_t s) {
+  if (bt.nmesh <= 0) return;
+  k_bfinalize_eval<<<bt.nmesh, kBlock, 0, s>>>(bt, part_e, part_g, out);
+}
+
+void launch_bdot(const BatchView& bt, int nvwg, const double* a, const double* b, double* part, EvalScalars* out,
+                 hipStream_t s) {
+  if (nvwg <= 0) return;
+  k_bdot_part<<<nvwg, kBlock, 0, s>>>(bt, a, b, part);
+  k_bfinalize_dot<<<bt.nmesh, kBlock, 0, s>>>(bt, part, out);
+}
+
+void launch_belem_hess(const MeshView& m, const BatchView& bt, int ntwg, const double* u, double dt2, int make_spsd,
+                       const int* hperm, double* hcontrib, bool fp32, hipStream_t s) {
+  if (ntwg <= 0) return;
+  if (fp32)
+    k_belem_hess<float><<<ntwg, kBlock, 0, s>>>(m, bt, u, dt2, make_spsd, hperm, hcontrib);
+  else
+    k_belem_hess<double><<<ntwg, kBlock, 0, s>>>(m, bt, u, dt2, make_spsd, hperm, hcontrib);
+}
+
+void launch_bhess_reduce(const BatchView& bt, int nub, const int* hseg, const double* hcontrib, const int2* ub_rc,
+                         const int* ub_pos, const unsigned char* dof, int filter_dbc, double* bell_val, double* dinv,
+                         hipStream_t s) {
+  if (nub <= 0) return;
+  k_bhess_reduce<<<(nub + kBlock - 1) / kBlock, kBlock, 0, s>>>(bt, nub, hseg, hcontrib, ub_rc, ub_pos, dof,
+                                                                 filter_dbc, bell_val, dinv);
+}
+
+void launch_baxpy_to(const BatchView& bt, int nvwg, const double* x, const double* alpha, const double* d,
+                     double* out, hipStream_t s) {
+  if (nvwg <= 0) return;
+  k_baxpy_to<<<nvwg, kBlock, 0, s>>>(bt, x, alpha, d, out);
+}
+
+void launch_bcopy2(const BatchView& bt, int nvwg, const double* src1, double* dst1, const double* src2, double* dst2,
+                   hipStream_t s) {
+  if (nvwg <= 0) return;
+  k_bcopy2<<<nvwg, kBlock, 0, s>>>(bt, src1, dst1, src2, dst2);
+}
+
+void launch_bpcg_init(const BatchView& bt, int nvwg, const double* b, double sign, const double* dinv, double* x,
+                      double* r, double* z, double rtol, int maxit, PcgScalars* sc, PcgIter* it, double* pb0,
+                      hipStream_t s) {
+  if (nvwg > 0) k_bpcg_init<<<nvwg, kBlock, 0, s>>>(bt, b, sign, dinv, x, r, z, pb0);
+  if (bt.nmesh > 0) k_bpcg_init_fin<<<bt.nmesh, kBlock, 0, s>>>(bt, pb0, rtol, maxit, sc, it);
+}
+
+void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const double* z, double* p, double* q,
+                      const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa, hipStream_t s) {
+  if (A.nslices <= 0) return;
+  const int grid = grid_for(A.nslices, A.xcd_remap);
+  if (A.lanes_per_row == 0)
+    k_bpcg_spmv<0><<<grid, kSpmvBlock, 0, s>>>(A, bt, z, p, q, pb0, pb1, sc, it, pa);
+  else
+    k_bpcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, bt, z, p, q, pb0, pb1, sc, it, pa);
+}
+
+void launch_bpcg_update(const BatchView& bt, int nvwg, const double* dinv, double* x, double* r, double* z,
+                        const double* p, const double* q, const double* pa, double* pb0, double* pb1,
+                        PcgScalars* sc, PcgIter* it, hipStream_t s) {
+  if (nvwg <= 0) return;
+  k_bpcg_update<<<nvwg, kBlock, 0, s>>>(bt, dinv, x, r, z, p, q, pa, pb0, pb1, sc, it);
+}
+
+}  // namespace ssim_hip

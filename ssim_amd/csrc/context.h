@@ -1,0 +1,209 @@
+// context.h -- state of a libssim_hip.so context (one mesh, or a padded concatenation of meshes for batching),
+// shared by context.hip (single-mesh ABI) and batch.hip (batched ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ssim_hip.h"
+#include "internal.h"
+
+
+namespace ssim_hip {
+
+extern thread_local std::string g_err;
+
+struct Fail {
+  ssim_status_t code;
+};
+
+[[noreturn]] void fail(ssim_status_t code, const std::string& msg);
+
+#define HIPCHK(x)                                                                               \
+  do {                                                                                          \
+    hipError_t e_ = (x);                                                                        \
+    if (e_ != hipSuccess) ssim_hip::fail(SSIM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t count = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    count = 0;
+  }
+  void alloc(size_t n) {
+    if (n == count && p) return;
+    release();
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    HIPCHK(hipMalloc(&p, bytes));
+    count = n;
+  }
+  void upload(const T* h, size_t n, hipStream_t s) {
+    alloc(n);
+    if (n) HIPCHK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+};
+
+}  // namespace ssim_hip
+
+struct ssim_ctx {
+  int nv = 0, nt = 0, n = 0;
+  double dt = 1e-3, youngs = 1e6, poisson = 0.33, density = 1e3, damping = 0.0;
+  double rtol = 1e-2, tol_abs = 1e-2, total_mass = 0.0;
+  double rho_reset = 0.0;  // density the mass terms use: captured at reset, as the reference's mass matrix
+  bool per_tet = false;
+  bool has_reset = false;
+  ssim_stepper_t stepper = SSIM_STEPPER_NEWTON;
+  ssim_solver_params prm{};
+  ssim_step_stats stats{};
+  hipStream_t stream = nullptr;
+
+  std::vector<double> hX;
+  std::vector<int> hcells;
+  std::vector<double> hlam, hmu;  // per-tet material when per_tet
+
+  // host topology (block adjacency, rows and columns ascending)
+  std::vector<int> brow, bcol, kdiag;
+  std::vector<int> hwu, hoffu;  // symmetric blocked ELL: upper widths and slot offsets per slice
+  long long ncolslots = 0;      // upper slots (with padding)
+  long long nlowslots = 0;      // lower reference slots (with padding)
+  int nslices = 0, nub = 0;
+  int xcd_remap = 1;
+  int pcg_chunk_max = 64;
+  int spmv_lanes = 1;
+  bool spmv_lanes_default = true;  // pick the variant from the layout at reset
+  bool fixed_layout = false;       // every slice padded to kFixedWU / kFixedWL slots
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
+
+  // device: mesh and rest data
+  ssim_hip::DBuf<double> X, B, V, lam, mu;
+  ssim_hip::DBuf<int4> cells;
+  // device: vertex arrays
+  ssim_hip::DBuf<double> arr[SSIM_ARRAY_COUNT];
+  ssim_hip::DBuf<unsigned char> dof;
+  ssim_hip::DBuf<double> utry, gtry, dir, r, z, p, q, tmp, dinv, hrest;
+  // device: assembly maps
+  ssim_hip::DBuf<int> gperm, gseg, hperm, hseg, wu, offu, ucol, wl, offl, ub_pos;
+  ssim_hip::DBuf<int2> ub_rc, lref;
+  ssim_hip::DBuf<double> gcontrib, hcontrib, val, massval;
+  // device: scalars
+  ssim_hip::DBuf<ssim_hip::PcgScalars> pcg;
+  ssim_hip::DBuf<ssim_hip::EvalScalars> ev;
+  ssim_hip::DBuf<double> partials, lb_S, lb_Y, lb_scal;
+  size_t pmax = 0;  // partials per slot
+  // partial-sum slots: pcg p·q, pcg (r·z, r·r) ping-pong, energy, ‖g‖², dot
+  double* part_pa() { return partials.p; }
+  double* part_pb(int k) { return partials.p + pmax + 2 * pmax * (size_t)(k & 1); }
+  double* part_e() { return partials.p + 5 * pmax; }
+  double* part_g() { return partials.p + 6 * pmax; }
+  double* part_d() { return partials.p + 7 * pmax; }
+  double* pin = nullptr;  // pinned host mirror for scalar read-backs
+
+  // sampled kernel timing: an event pair per kernel, armed at one launch and harvested at the next sync
+  struct KTimer {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double sum_ms = 0;
+    long long cnt = 0;
+    bool pending = false;
+  };
+  bool timing = false;
+  KTimer kt[SSIM_KERNEL_COUNT];
+
+  ~ssim_ctx() {
+    if (pin) (void)hipHostFree(pin);
+    for (auto& e : poll_ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& t : kt)
+      for (auto& e : t.ev)
+        if (e) (void)hipEventDestroy(e);
+  }
+
+  ssim_hip::MeshView mesh() const {
+    ssim_hip::MeshView m;
+    m.X = X.p;
+    m.cells = cells.p;
+    m.B = B.p;
+    m.V = V.p;
+    m.lam = lam.p;
+    m.mu = mu.p;
+    m.nv = nv;
+    m.nt = nt;
+    m.rho = rho_reset;
+    return m;
+  }
+  ssim_hip::BellView bell(const double* values) const {
+    ssim_hip::BellView A;
+    A.wu = wu.p;
+    A.offu = offu.p;
+    A.ucol = ucol.p;
+    A.val = values;
+    A.wl = wl.p;
+    A.offl = offl.p;
+    A.lref = lref.p;
+    A.nrows = nv;
+    A.nslices = nslices;
+    A.xcd_remap = xcd_remap;
+    A.lanes_per_row = spmv_lanes;
+    return A;
+  }
+  double* a(ssim_array_t w) { return arr[w].p; }
+};
+
+
+namespace ssim_hip {
+// helpers defined in context.hip, used by batch.hip
+void sync(ssim_ctx* c);
+void default_params(ssim_solver_params& p);
+// marks (mark) or releases vertices as Dirichlet; targets (3 per vertex) may be null for zero targets
+void edit_dirichlet(ssim_ctx* c, const int* verts, int count, const double* targets, bool mark);
+void harvest_timers(ssim_ctx* c);
+// Runs launch(); when timing is on and this kernel's timer is free, brackets it with the timer's events.
+template <typename F>
+void timed(ssim_ctx* c, ssim_kernel_t k, F&& launch) {
+  auto& t = c->kt[k];
+  const bool arm = c->timing && !t.pending;
+  if (arm) {
+    if (!t.ev[0]) {
+      HIPCHK(hipEventCreate(&t.ev[0]));
+      HIPCHK(hipEventCreate(&t.ev[1]));
+    }
+    HIPCHK(hipEventRecord(t.ev[0], c->stream));
+  }
+  launch();
+  if (arm) {
+    HIPCHK(hipEventRecord(t.ev[1], c->stream));
+    t.pending = true;
+  }
+}
+
+void do_reset(ssim_ctx* c);
+void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_dbc, double dt2, double* values);
+template <typename F>
+ssim_status_t run(F&& f) {
+  try {
+    f();
+    return SSIM_OK;
+  } catch (const Fail& e) {
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host out of memory";
+    return SSIM_ERR_HIP;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return SSIM_ERR_STATE;
+  }
+}
+}  // namespace ssim_hip

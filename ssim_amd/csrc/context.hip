@@ -4,7 +4,7 @@
 // The host drives control flow only (Newton iterations, line-search decisions, PCG chunking); every
 // floating-point operation of a step runs in the HIP kernels of kernels.hip.  PCG scalars stay on the device
 // and the host polls a done-flag once per chunk of iterations.
-#include <hip/hip_runtime.h>
+#include "context.h"
 
 #include <algorithm>
 #include <cmath>
@@ -13,165 +13,17 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ssim_hip.h"
-#include "internal.h"
-
 using namespace ssim_hip;
 
-namespace {
-
+namespace ssim_hip {
 thread_local std::string g_err = "";
-
-struct Fail {
-  ssim_status_t code;
-};
-
 [[noreturn]] void fail(ssim_status_t code, const std::string& msg) {
   g_err = msg;
   throw Fail{code};
 }
+}  // namespace ssim_hip
 
-#define HIPCHK(x)                                                                               \
-  do {                                                                                          \
-    hipError_t e_ = (x);                                                                        \
-    if (e_ != hipSuccess) fail(SSIM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename T>
-struct DBuf {
-  T* p = nullptr;
-  size_t count = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    count = 0;
-  }
-  void alloc(size_t n) {
-    if (n == count && p) return;
-    release();
-    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-    HIPCHK(hipMalloc(&p, bytes));
-    count = n;
-  }
-  void upload(const T* h, size_t n, hipStream_t s) {
-    alloc(n);
-    if (n) HIPCHK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
-  }
-  void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
-};
-
-}  // namespace
-
-struct ssim_ctx {
-  int nv = 0, nt = 0, n = 0;
-  double dt = 1e-3, youngs = 1e6, poisson = 0.33, density = 1e3, damping = 0.0;
-  double rtol = 1e-2, tol_abs = 1e-2, total_mass = 0.0;
-  double rho_reset = 0.0;  // density the mass terms use: captured at reset, as the reference's mass matrix
-  bool per_tet = false;
-  bool has_reset = false;
-  ssim_stepper_t stepper = SSIM_STEPPER_NEWTON;
-  ssim_solver_params prm{};
-  ssim_step_stats stats{};
-  hipStream_t stream = nullptr;
-
-  std::vector<double> hX;
-  std::vector<int> hcells;
-  std::vector<double> hlam, hmu;  // per-tet material when per_tet
-
-  // host topology (block adjacency, rows and columns ascending)
-  std::vector<int> brow, bcol, kdiag;
-  std::vector<int> hwu, hoffu;  // symmetric blocked ELL: upper widths and slot offsets per slice
-  long long ncolslots = 0;      // upper slots (with padding)
-  long long nlowslots = 0;      // lower reference slots (with padding)
-  int nslices = 0, nub = 0;
-  int xcd_remap = 1;
-  int pcg_chunk_max = 64;
-  int spmv_lanes = 1;
-  bool spmv_lanes_default = true;  // pick the variant from the layout at reset
-  bool fixed_layout = false;       // every slice padded to kFixedWU / kFixedWL slots
-  hipEvent_t poll_ev[2] = {nullptr, nullptr};
-
-  // device: mesh and rest data
-  DBuf<double> X, B, V, lam, mu;
-  DBuf<int4> cells;
-  // device: vertex arrays
-  DBuf<double> arr[SSIM_ARRAY_COUNT];
-  DBuf<unsigned char> dof;
-  DBuf<double> utry, gtry, dir, r, z, p, q, tmp, dinv, hrest;
-  // device: assembly maps
-  DBuf<int> gperm, gseg, hperm, hseg, wu, offu, ucol, wl, offl, ub_pos;
-  DBuf<int2> ub_rc, lref;
-  DBuf<double> gcontrib, hcontrib, val, massval;
-  // device: scalars
-  DBuf<PcgScalars> pcg;
-  DBuf<EvalScalars> ev;
-  DBuf<double> partials, lb_S, lb_Y, lb_scal;
-  size_t pmax = 0;  // partials per slot
-  // partial-sum slots: pcg p·q, pcg (r·z, r·r) ping-pong, energy, ‖g‖², dot
-  double* part_pa() { return partials.p; }
-  double* part_pb(int k) { return partials.p + pmax + 2 * pmax * (size_t)(k & 1); }
-  double* part_e() { return partials.p + 5 * pmax; }
-  double* part_g() { return partials.p + 6 * pmax; }
-  double* part_d() { return partials.p + 7 * pmax; }
-  double* pin = nullptr;  // pinned host mirror for scalar read-backs
-
-  // sampled kernel timing: an event pair per kernel, armed at one launch and harvested at the next sync
-  struct KTimer {
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    double sum_ms = 0;
-    long long cnt = 0;
-    bool pending = false;
-  };
-  bool timing = false;
-  KTimer kt[SSIM_KERNEL_COUNT];
-
-  ~ssim_ctx() {
-    if (pin) (void)hipHostFree(pin);
-    for (auto& e : poll_ev)
-      if (e) (void)hipEventDestroy(e);
-    for (auto& t : kt)
-      for (auto& e : t.ev)
-        if (e) (void)hipEventDestroy(e);
-  }
-
-  MeshView mesh() const {
-    MeshView m;
-    m.X = X.p;
-    m.cells = cells.p;
-    m.B = B.p;
-    m.V = V.p;
-    m.lam = lam.p;
-    m.mu = mu.p;
-    m.nv = nv;
-    m.nt = nt;
-    m.rho = rho_reset;
-    return m;
-  }
-  BellView bell(const double* values) const {
-    BellView A;
-    A.wu = wu.p;
-    A.offu = offu.p;
-    A.ucol = ucol.p;
-    A.val = values;
-    A.wl = wl.p;
-    A.offl = offl.p;
-    A.lref = lref.p;
-    A.nrows = nv;
-    A.nslices = nslices;
-    A.xcd_remap = xcd_remap;
-    A.lanes_per_row = spmv_lanes;
-    return A;
-  }
-  double* a(ssim_array_t w) { return arr[w].p; }
-};
-
-namespace {
-
-void default_params(ssim_solver_params& p) {
+void ssim_hip::default_params(ssim_solver_params& p) {
   p.max_newton = 10;
   p.pcg_rtol = 1e-8;
   p.pcg_maxit = 10000;
@@ -185,6 +37,8 @@ void default_params(ssim_solver_params& p) {
   p.fp32_elements = 0;
 }
 
+namespace {
+
 void check_material(double youngs, double poisson, double density) {
   // time_step.hpp:369-374
   if (!(youngs >= 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Young's modulus must be positive, got " + std::to_string(youngs));
@@ -193,7 +47,9 @@ void check_material(double youngs, double poisson, double density) {
   if (!(density >= 0)) fail(SSIM_ERR_INVALID_ARGUMENT, "Density must be positive, got " + std::to_string(density));
 }
 
-void harvest_timers(ssim_ctx* c) {
+}  // namespace
+
+void ssim_hip::harvest_timers(ssim_ctx* c) {
   for (auto& t : c->kt) {
     if (!t.pending) continue;
     if (hipEventQuery(t.ev[1]) != hipSuccess) continue;  // not finished yet: harvested at a later sync
@@ -205,46 +61,34 @@ void harvest_timers(ssim_ctx* c) {
   }
 }
 
-void sync(ssim_ctx* c) {
+void ssim_hip::sync(ssim_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   if (c->timing) harvest_timers(c);
 }
 
-// Runs launch(); when timing is on and this kernel's timer is free, brackets it with the timer's events.
-template <typename F>
-void timed(ssim_ctx* c, ssim_kernel_t k, F&& launch) {
-  auto& t = c->kt[k];
-  const bool arm = c->timing && !t.pending;
-  if (arm) {
-    if (!t.ev[0]) {
-      HIPCHK(hipEventCreate(&t.ev[0]));
-      HIPCHK(hipEventCreate(&t.ev[1]));
-    }
-    HIPCHK(hipEventRecord(t.ev[0], c->stream));
-  }
-  launch();
-  if (arm) {
-    HIPCHK(hipEventRecord(t.ev[1], c->stream));
-    t.pending = true;
-  }
-}
+namespace {
 
 // Sparsity, blocked-ELL layout and the two assembly permutations.
 void build_topology(ssim_ctx* c) {
   const int nv = c->nv, nt = c->nt;
   const int* cells = c->hcells.data();
-  // block adjacency: neighbours (self included) of every vertex, ascending
-  std::vector<int> cnt(nv + 1, 0);
+  // block adjacency: neighbours (self included, also for a vertex in no tet) of every vertex, ascending.
+  // Tets with a negative first index are absent (padding of a batched layout) and take no part.
+  std::vector<int> cnt(nv + 1, 1);
+  cnt[0] = 0;
   for (int e = 0; e < nt; ++e)
-    for (int i = 0; i < 4; ++i) cnt[cells[4 * e + i] + 1] += 4;
+    if (cells[4 * e] >= 0)
+      for (int i = 0; i < 4; ++i) cnt[cells[4 * e + i] + 1] += 4;
   for (int v = 0; v < nv; ++v) cnt[v + 1] += cnt[v];
   std::vector<int> cand(std::max(cnt[nv], 1));
   std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+  for (int v = 0; v < nv; ++v) cand[fillp[v]++] = v;
   for (int e = 0; e < nt; ++e)
-    for (int i = 0; i < 4; ++i) {
-      const int v = cells[4 * e + i];
-      for (int j = 0; j < 4; ++j) cand[fillp[v]++] = cells[4 * e + j];
-    }
+    if (cells[4 * e] >= 0)
+      for (int i = 0; i < 4; ++i) {
+        const int v = cells[4 * e + i];
+        for (int j = 0; j < 4; ++j) cand[fillp[v]++] = cells[4 * e + j];
+      }
   c->brow.assign(nv + 1, 0);
   std::vector<int> len(nv, 0);
   for (int v = 0; v < nv; ++v) {
@@ -345,32 +189,36 @@ void build_topology(ssim_ctx* c) {
 
   // Hessian contributions: 10 local pairs (v ≤ w) per tet, sorted by (upper block, tet, pair)
   std::vector<int> hcount(c->nub + 1, 0);
-  std::vector<int> hub_of(10 * (size_t)nt);
+  std::vector<int> hub_of(10 * (size_t)nt, -1);
   for (int e = 0; e < nt; ++e)
-    for (int w = 0; w < 4; ++w)
-      for (int v = 0; v <= w; ++v) {
-        const int a = cells[4 * e + v], b = cells[4 * e + w];
-        const int r = std::min(a, b), cc = std::max(a, b);
-        const int ub = ubptr[r] + (slot(r, cc) - c->kdiag[r]);
-        hub_of[10 * (size_t)e + w * (w + 1) / 2 + v] = ub;
-        hcount[ub + 1]++;
-      }
+    if (cells[4 * e] >= 0)
+      for (int w = 0; w < 4; ++w)
+        for (int v = 0; v <= w; ++v) {
+          const int a = cells[4 * e + v], b = cells[4 * e + w];
+          const int r = std::min(a, b), cc = std::max(a, b);
+          const int ub = ubptr[r] + (slot(r, cc) - c->kdiag[r]);
+          hub_of[10 * (size_t)e + w * (w + 1) / 2 + v] = ub;
+          hcount[ub + 1]++;
+        }
   for (int u = 0; u < c->nub; ++u) hcount[u + 1] += hcount[u];
-  std::vector<int> hperm(10 * (size_t)nt);
+  std::vector<int> hperm(10 * (size_t)nt, -1);
   {
     std::vector<int> pos(hcount.begin(), hcount.end() - 1);
-    for (size_t k = 0; k < 10 * (size_t)nt; ++k) hperm[k] = pos[hub_of[k]]++;
+    for (size_t k = 0; k < 10 * (size_t)nt; ++k)
+      if (hub_of[k] >= 0) hperm[k] = pos[hub_of[k]]++;
   }
   // gradient contributions: 4 per tet, sorted by (vertex, tet, corner)
   std::vector<int> gseg(nv + 1, 0);
   for (int e = 0; e < nt; ++e)
-    for (int i = 0; i < 4; ++i) gseg[cells[4 * e + i] + 1]++;
+    if (cells[4 * e] >= 0)
+      for (int i = 0; i < 4; ++i) gseg[cells[4 * e + i] + 1]++;
   for (int v = 0; v < nv; ++v) gseg[v + 1] += gseg[v];
-  std::vector<int> gperm(4 * (size_t)nt);
+  std::vector<int> gperm(4 * (size_t)nt, -1);
   {
     std::vector<int> pos(gseg.begin(), gseg.end() - 1);
     for (int e = 0; e < nt; ++e)
-      for (int i = 0; i < 4; ++i) gperm[4 * e + i] = pos[cells[4 * e + i]]++;
+      if (cells[4 * e] >= 0)
+        for (int i = 0; i < 4; ++i) gperm[4 * e + i] = pos[cells[4 * e + i]]++;
   }
   hipStream_t s = c->stream;
   c->wu.upload(c->hwu.data(), ns, s);
@@ -409,8 +257,10 @@ void upload_material(ssim_ctx* c) {
   sync(c);
 }
 
+}  // namespace
+
 // reset(): finite_elements/time_step.hpp:368-494
-void do_reset(ssim_ctx* c) {
+void ssim_hip::do_reset(ssim_ctx* c) {
   check_material(c->youngs, c->poisson, c->density);
   if (0.5 - c->poisson < 1e-4)
     fprintf(stderr, "Warning: got extremely high Poisson's ratio, may cause numerical instability. %lf\n", c->poisson);
@@ -461,6 +311,8 @@ void do_reset(ssim_ctx* c) {
   c->has_reset = true;
 }
 
+namespace {
+
 struct EvalOut {
   double E, gg;
 };
@@ -493,8 +345,11 @@ double device_dot(ssim_ctx* c, const double* a, const double* b) {
   return c->pin[0];
 }
 
+}  // namespace
+
 // update_hessian at deformation u into the blocked-ELL values and the Jacobi diagonal
-void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_dbc, double dt2, double* values) {
+void ssim_hip::assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_dbc, double dt2,
+                                double* values) {
   hipStream_t s = c->stream;
   timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
     launch_elem_hess(c->mesh(), u, dt2, make_spsd, c->hperm.p, c->hcontrib.p, c->prm.fp32_elements != 0, s);
@@ -506,6 +361,8 @@ void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_db
   HIPCHK(hipGetLastError());
   c->stats.hessian_evals++;
 }
+
+namespace {
 
 // Jacobi-PCG on the current matrix: x = H⁻¹ (sign·b).  Returns iterations.
 int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit) {
@@ -689,27 +546,32 @@ ssim_status_t guard(ssim_ctx* c) {
   return SSIM_OK;
 }
 
-template <typename F>
-ssim_status_t run(F&& f) {
-  try {
-    f();
-    return SSIM_OK;
-  } catch (const Fail& e) {
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_err = "host out of memory";
-    return SSIM_ERR_HIP;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return SSIM_ERR_STATE;
-  }
-}
-
 void require_reset(ssim_ctx* c) {
   if (!c->has_reset) fail(SSIM_ERR_STATE, "context not reset");
 }
 
 }  // namespace
+
+void ssim_hip::edit_dirichlet(ssim_ctx* c, const int* verts, int count, const double* targets, bool mark) {
+  if (count < 0 || (count > 0 && !verts)) fail(SSIM_ERR_INVALID_ARGUMENT, "bad vertex list");
+  for (int i = 0; i < count; ++i)
+    if (verts[i] < 0 || verts[i] >= c->nv) fail(SSIM_ERR_INVALID_ARGUMENT, "vertex index out of range");
+  std::vector<unsigned char> dof(c->n);
+  std::vector<double> dbc(c->n);
+  hipStream_t s = c->stream;
+  HIPCHK(hipMemcpyAsync(dof.data(), c->dof.p, c->n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(dbc.data(), c->a(SSIM_ARRAY_DBC_VALUES), sizeof(double) * c->n, hipMemcpyDeviceToHost, s));
+  sync(c);
+  for (int i = 0; i < count; ++i)
+    for (int j = 0; j < 3; ++j) {
+      dof[3 * (size_t)verts[i] + j] = mark ? 1 : 0;
+      dbc[3 * (size_t)verts[i] + j] = mark ? (targets ? targets[3 * (size_t)i + j] : 0.0) : 0.0;
+    }
+  HIPCHK(hipMemcpyAsync(c->dof.p, dof.data(), c->n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->a(SSIM_ARRAY_DBC_VALUES), dbc.data(), sizeof(double) * c->n, hipMemcpyHostToDevice, s));
+  sync(c);
+  c->hrest.release();
+}
 
 // ======================================================================================================
 // extern "C" ABI
@@ -870,27 +732,6 @@ ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper) {
   }
   ctx->stepper = stepper;
   return SSIM_OK;
-}
-
-static void edit_dirichlet(ssim_ctx* c, const int* verts, int count, const double* targets, bool mark) {
-  if (count < 0 || (count > 0 && !verts)) fail(SSIM_ERR_INVALID_ARGUMENT, "bad vertex list");
-  for (int i = 0; i < count; ++i)
-    if (verts[i] < 0 || verts[i] >= c->nv) fail(SSIM_ERR_INVALID_ARGUMENT, "vertex index out of range");
-  std::vector<unsigned char> dof(c->n);
-  std::vector<double> dbc(c->n);
-  hipStream_t s = c->stream;
-  HIPCHK(hipMemcpyAsync(dof.data(), c->dof.p, c->n, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(dbc.data(), c->a(SSIM_ARRAY_DBC_VALUES), sizeof(double) * c->n, hipMemcpyDeviceToHost, s));
-  sync(c);
-  for (int i = 0; i < count; ++i)
-    for (int j = 0; j < 3; ++j) {
-      dof[3 * (size_t)verts[i] + j] = mark ? 1 : 0;
-      dbc[3 * (size_t)verts[i] + j] = mark ? (targets ? targets[3 * (size_t)i + j] : 0.0) : 0.0;
-    }
-  HIPCHK(hipMemcpyAsync(c->dof.p, dof.data(), c->n, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(c->a(SSIM_ARRAY_DBC_VALUES), dbc.data(), sizeof(double) * c->n, hipMemcpyHostToDevice, s));
-  sync(c);
-  c->hrest.release();
 }
 
 ssim_status_t ssim_mark_dirichlet(ssim_ctx* ctx, const int* verts, int count, const double* targets) {

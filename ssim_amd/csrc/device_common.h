@@ -1,0 +1,475 @@
+// device_common.h -- device-side building blocks shared by the kernels of libssim_hip.so: deterministic
+// reductions, per-tet gathers and element bodies, and the blocked-ELL row product.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "elem_math.h"
+#include "internal.h"
+
+namespace ssim_hip {
+
+// ------------------------------------------------------------------------------------------------------
+// reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Reductions never synchronise workgroups inside a kernel.  A producing kernel stores one partial per
+// workgroup (plain stores); the consuming kernel -- the next launch on the stream, so the kernel boundary makes
+// the partials visible -- sums all partials in a fixed order in every workgroup.  Every workgroup therefore
+// sees the same, bitwise reproducible total.
+
+// block-wide sum of K values over NT threads; the result is returned to every thread
+template <int K, int NT = kBlock>
+__device__ __forceinline__ void block_sum(double (&v)[K]) {
+  static_assert(NT % 64 == 0, "workgroups are whole waves");
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if (NT == 64) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = __shfl(v[k], 0, 64);
+    return;
+  }
+  __shared__ double sm[K][NT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sm[k][wid] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) s += sm[k][w];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+// stores this workgroup's partial sums: part[slot*K + k], slot = blockIdx.x unless given
+template <int K, int NT = kBlock>
+__device__ __forceinline__ void store_partials(double (&v)[K], double* part, int slot = -1) {
+  block_sum<K, NT>(v);
+  if (slot < 0) slot = blockIdx.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[(size_t)slot * K + k] = v[k];
+  }
+}
+
+// total of `count` partials of width K (written by an earlier kernel), in a fixed order.  Each thread loads its
+// partials U at a time (independent loads in flight together) before adding them in index order; U = 1 suits
+// short runs (at most NT partials), where a deeper buffer only costs registers.
+template <int K, int NT = kBlock, int U = 8>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int count, double (&tot)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) tot[k] = 0;
+  for (int b0 = threadIdx.x; b0 < count; b0 += U * NT) {
+    double buf[U][K];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * NT;
+#pragma unroll
+      for (int k = 0; k < K; ++k) buf[u][k] = (b < count) ? part[(size_t)b * K + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < K; ++k) tot[k] += buf[u][k];
+  }
+  block_sum<K, NT>(tot);
+}
+
+// ------------------------------------------------------------------------------------------------------
+// element gathers
+
+struct TetLocal {
+  int id[4];
+  double B[9];
+  double V, lam, mu;
+};
+
+__device__ __forceinline__ TetLocal load_tet(const MeshView& m, int e) {
+  TetLocal t;
+  const int4 c = m.cells[e];
+  t.id[0] = c.x;
+  t.id[1] = c.y;
+  t.id[2] = c.z;
+  t.id[3] = c.w;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) t.B[k] = m.B[(size_t)k * m.nt + e];
+  t.V = m.V[e];
+  t.lam = m.lam[e];
+  t.mu = m.mu[e];
+  return t;
+}
+
+// F = Ds·B with Ds from positions X + u (def_grad.hpp:270-288); differences formed in FP64
+__device__ __forceinline__ void def_grad(const double x[4][3], const double B[9], double F[9]) {
+  double Ds[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    Ds[a * 3 + 0] = x[1][a] - x[0][a];
+    Ds[a * 3 + 1] = x[2][a] - x[0][a];
+    Ds[a * 3 + 2] = x[3][a] - x[0][a];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      F[a * 3 + c] = Ds[a * 3 + 0] * B[0 * 3 + c] + Ds[a * 3 + 1] * B[1 * 3 + c] + Ds[a * 3 + 2] * B[2 * 3 + c];
+}
+
+// local force f[v][b] = Σ_c P[b][c]·B[v−1][c], f[0] = −Σ f[1..3] (time_step.hpp:73-89)
+template <typename T>
+__device__ __forceinline__ void local_force(const T P[9], const T B[9], T f[4][3]) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    T tot = 0;
+#pragma unroll
+    for (int i = 1; i < 4; ++i) {
+      const T val = P[j * 3 + 0] * B[(i - 1) * 3 + 0] + P[j * 3 + 1] * B[(i - 1) * 3 + 1] + P[j * 3 + 2] * B[(i - 1) * 3 + 2];
+      f[i][j] = val;
+      tot += val;
+    }
+    f[0][j] = -tot;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Hessian pullback
+
+// Block (v, w), v ≤ w, of K = pfpxᵀ·H·pfpx: K_vw[b][d] = Σ_{c,e} D[v][c]·H[3b+c][3d+e]·D[w][e].
+template <typename T, typename Emit>
+__device__ __forceinline__ void pullback_blocks(const T H[45], const T D[4][3], Emit&& emit) {
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    T G[9][3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd)
+        G[i][dd] = H[sym9(i, 3 * dd + 0)] * D[w][0] + H[sym9(i, 3 * dd + 1)] * D[w][1] + H[sym9(i, 3 * dd + 2)] * D[w][2];
+#pragma unroll
+    for (int v = 0; v <= w; ++v) {
+      T K[9];
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd)
+          K[b * 3 + dd] = D[v][0] * G[3 * b + 0][dd] + D[v][1] * G[3 * b + 1][dd] + D[v][2] * G[3 * b + 2][dd];
+      emit(v, w, K);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void shape_grads(const T B[9], T D[4][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    D[0][c] = -B[0 * 3 + c] - B[1 * 3 + c] - B[2 * 3 + c];
+    D[1][c] = B[0 * 3 + c];
+    D[2][c] = B[1 * 3 + c];
+    D[3][c] = B[2 * 3 + c];
+  }
+}
+
+// pair index of local blocks (v ≤ w): w(w+1)/2 + v
+__host__ __device__ constexpr int pair_index(int v, int w) { return w * (w + 1) / 2 + v; }
+
+// ------------------------------------------------------------------------------------------------------
+// element bodies: energy + gradient (update_energy_and_gradients, time_step.hpp:577-624), element-wise:
+//   e = ½·(ρV/20)·(Σ|Δ_i|² + |ΣΔ_i|²) + dt²·V·Ψ,   g_i = (ρV/20)·(Δ_i + ΣΔ_j) + dt²·V·f_i,   Δ = u − û;
+// Hessian blocks (update_hessian, time_step.hpp:626-658; wrapped_pfpx :41-59).
+
+// energy of tet e at deformation u (predictor uhat); its 4 gradient contributions go to the slots gperm[4e+i]
+template <typename T>
+__device__ __forceinline__ double elem_grad_tet(const MeshView& m, int e, const double* __restrict__ u,
+                                                const double* __restrict__ uhat, double dt2,
+                                                const int* __restrict__ gperm, double* __restrict__ gcontrib) {
+  const TetLocal t = load_tet(m, e);
+  double x[4][3], d[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const size_t k = 3 * (size_t)t.id[i] + a;
+      const double uu = u[k];
+      x[i][a] = m.X[k] + uu;
+      d[i][a] = uu - uhat[k];
+    }
+  double Fd[9];
+  def_grad(x, t.B, Fd);
+  T F[9], Bt[9], C[9], P[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    F[k] = (T)Fd[k];
+    Bt[k] = (T)t.B[k];
+  }
+  const SnhScalars<T> s = snh_scalars(F, (T)t.lam, (T)t.mu);
+  cofactor(F, C);
+  snh_stress(F, C, s, P);
+  T f[4][3];
+  local_force(P, Bt, f);
+  const double mm = m.rho * t.V / 20.0;
+  const double sc = dt2 * t.V;
+  double S[3], dd = 0, ss = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) S[a] = d[0][a] + d[1][a] + d[2][a] + d[3][a];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const size_t pos = 3 * (size_t)gperm[4 * (size_t)e + i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      dd += d[i][a] * d[i][a];
+      gcontrib[pos + a] = mm * (d[i][a] + S[a]) + sc * (double)f[i][a];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) ss += S[a] * S[a];
+  return 0.5 * mm * (dd + ss) + sc * (double)s.psi;
+}
+
+// the 10 oriented upper blocks of tet e (dt²V-scaled stiffness plus consistent mass) to the slots hperm[10e+k]
+template <typename T>
+__device__ __forceinline__ void elem_hess_tet(const MeshView& m, int e, const double* __restrict__ u, double dt2,
+                                              int make_spsd, const int* __restrict__ hperm,
+                                              double* __restrict__ hcontrib) {
+  const TetLocal t = load_tet(m, e);
+  double x[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const size_t k = 3 * (size_t)t.id[i] + a;
+      x[i][a] = m.X[k] + u[k];
+    }
+  double Fd[9];
+  def_grad(x, t.B, Fd);
+  T F[9], Bt[9], C[9], H[45], D[4][3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    F[k] = (T)Fd[k];
+    Bt[k] = (T)t.B[k];
+  }
+  const T lam = (T)t.lam;
+  const SnhScalars<T> s = snh_scalars(F, lam, (T)t.mu);
+  cofactor(F, C);
+  snh_hessian_packed(F, C, lam, s, H);
+  if (make_spsd) snh_project_spd(F, lam, s, (T)kSpdFloor, H);
+  shape_grads(Bt, D);
+  const double sc = dt2 * t.V;
+  const double mm = m.rho * t.V / 20.0;
+  const int* pp = hperm + 10 * (size_t)e;
+  pullback_blocks(H, D, [&](int v, int w, const T K[9]) {
+    const size_t pos = 9 * (size_t)pp[pair_index(v, w)];
+    const double mvw = (v == w) ? 2.0 * mm : mm;
+    const bool tr = t.id[v] > t.id[w];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd) {
+        const double kv = sc * (double)(tr ? K[dd * 3 + b] : K[b * 3 + dd]) + (b == dd ? mvw : 0.0);
+        hcontrib[pos + b * 3 + dd] = kv;
+      }
+  });
+}
+
+// segmented reduce of vertex v's gradient contributions, zeroed on Dirichlet dofs (boundary.hpp:45-51);
+// returns the vertex's share of ‖g‖²
+__device__ __forceinline__ double grad_reduce_vertex(int v, const int* __restrict__ gseg,
+                                                     const double* __restrict__ gcontrib,
+                                                     const unsigned char* __restrict__ dof, double* __restrict__ g) {
+  double gg = 0.0;
+  double acc[3] = {0, 0, 0};
+  const int lo = gseg[v], hi = gseg[v + 1];
+  for (int q = lo; q < hi; ++q) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] += gcontrib[3 * (size_t)q + a];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double val = dof[3 * (size_t)v + a] ? 0.0 : acc[a];
+    g[3 * (size_t)v + a] = val;
+    gg += val * val;
+  }
+  return gg;
+}
+
+// upper block ub = (rc.x, rc.y): sum of its contribution run, Dirichlet identity filter, store into the
+// blocked-ELL values; Jacobi inverse diagonal (and optionally the diagonal) for a diagonal block
+__device__ __forceinline__ void hess_reduce_block(int ub, int2 rc, const int* __restrict__ hseg,
+                                                  const double* __restrict__ hcontrib, const int* __restrict__ ub_pos,
+                                                  const unsigned char* __restrict__ dof, int filter_dbc,
+                                                  double* __restrict__ val, double* __restrict__ dinv,
+                                                  double* __restrict__ diag) {
+  double S[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) S[k] = 0;
+  const int lo = hseg[ub], hi = hseg[ub + 1];
+  for (int q = lo; q < hi; ++q) {
+    const double* src = hcontrib + 9 * (size_t)q;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) S[k] += src[k];
+  }
+  if (filter_dbc) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd)
+        if (dof[3 * (size_t)rc.x + b] || dof[3 * (size_t)rc.y + dd]) S[b * 3 + dd] = (rc.x == rc.y && b == dd) ? 1.0 : 0.0;
+  }
+  const size_t pos = (size_t)ub_pos[ub];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) val[pos + (size_t)k * kSlice] = S[k];
+  if (rc.x == rc.y) {
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      dinv[3 * (size_t)rc.x + b] = 1.0 / S[b * 4];
+      if (diag) diag[3 * (size_t)rc.x + b] = S[b * 4];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// blocked-ELL row product
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X.  With remapping, physical workgroup i
+// handles logical block (i mod 8)·per + i/8, so each XCD (and its L2) sees a contiguous range of rows and the
+// transposed reads of a row's lower blocks mostly hit lines its own XCD has just read.  Returns −1 for the idle
+// tail of the padded grid.
+__device__ __forceinline__ int logical_block(int nb, int remap) {
+  if (!remap) return blockIdx.x;
+  const int per = (nb + 7) >> 3;
+  const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  return lb < nb ? lb : -1;
+}
+
+inline int grid_for(int nb, int remap) { return remap ? ((nb + 7) / 8) * 8 : nb; }
+
+// SpMV workgroups are single waves: one 64-row slice per workgroup (LPR ≤ 1) spreads the ~2,200 waves of the
+// 786k-tet bar evenly over the 1,024 SIMDs in one dispatch round.
+constexpr int kSpmvBlock = 64;
+
+template <int LPR>
+struct RowMap {
+  static constexpr int rows_per_block = kSpmvBlock / (LPR > 0 ? LPR : 1);
+  // row and slot phase of this thread within logical block lb
+  __device__ static __forceinline__ void map(int lb, int& v, int& j) {
+    if (LPR <= 1) {
+      v = lb * kSpmvBlock + threadIdx.x;
+      j = 0;
+    } else {
+      const int lane = threadIdx.x & 63;
+      v = lb * rows_per_block + (threadIdx.x >> 6) * 16 + (lane & 15);
+      j = lane >> 4;
+    }
+  }
+  __host__ __device__ static int blocks(int nrows) { return (nrows + rows_per_block - 1) / rows_per_block; }
+};
+
+template <int LPR>
+__device__ __forceinline__ void bell_row_part(const BellView& A, int v, int j, const double* __restrict__ x,
+                                              double y[3]) {
+  const int s = v >> 6, lane = v & 63;
+  double y0 = 0, y1 = 0, y2 = 0;
+  {
+    const int W = A.wu[s];
+    const int off = A.offu[s];
+    const int* col = A.ucol + off + lane;
+    const double* val = A.val + 9 * (size_t)off + lane;
+#pragma unroll 4
+    for (int k = j; k < W; k += LPR) {
+      const size_t w3 = 3 * (size_t)col[k * kSlice];
+      const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
+      const double* vk = val + (size_t)k * 9 * kSlice;
+      y0 += vk[0 * kSlice] * x0 + vk[1 * kSlice] * x1 + vk[2 * kSlice] * x2;
+      y1 += vk[3 * kSlice] * x0 + vk[4 * kSlice] * x1 + vk[5 * kSlice] * x2;
+      y2 += vk[6 * kSlice] * x0 + vk[7 * kSlice] * x1 + vk[8 * kSlice] * x2;
+    }
+  }
+  {
+    const int W = A.wl[s];
+    const int2* ref = A.lref + A.offl[s] + lane;
+#pragma unroll 4
+    for (int k = j; k < W; k += LPR) {
+      const int2 rw = ref[k * kSlice];
+      const size_t w3 = 3 * (size_t)rw.x;
+      const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
+      const double* b = A.val + (size_t)rw.y;
+      y0 += b[0 * kSlice] * x0 + b[3 * kSlice] * x1 + b[6 * kSlice] * x2;
+      y1 += b[1 * kSlice] * x0 + b[4 * kSlice] * x1 + b[7 * kSlice] * x2;
+      y2 += b[2 * kSlice] * x0 + b[5 * kSlice] * x1 + b[8 * kSlice] * x2;
+    }
+  }
+  y[0] = y0;
+  y[1] = y1;
+  y[2] = y2;
+}
+
+// Fixed-width variant for layouts padded to exactly WU upper and WL lower slots in every slice (used when the
+// mesh allows it; the Freudenthal bar has 8 and 7): straight-line code, so all column / reference loads of the
+// row go out together, then all gathers and value loads -- a few dependent memory round trips per row instead
+// of two per slot.
+template <int WU, int WL>
+__device__ __forceinline__ void bell_row_fixed(const BellView& A, int v, const double* __restrict__ x, double y[3]) {
+  const int s = v >> 6, lane = v & 63;
+  const int offu = s * WU * kSlice;  // uniform widths: the offsets are implied
+  const int* col = A.ucol + offu + lane;
+  const double* val = A.val + 9 * (size_t)offu + lane;
+  const int2* ref = A.lref + s * WL * kSlice + lane;
+  int cu[WU];
+  int2 cl[WL];
+#pragma unroll
+  for (int k = 0; k < WU; ++k) cu[k] = col[k * kSlice];
+#pragma unroll
+  for (int k = 0; k < WL; ++k) cl[k] = ref[k * kSlice];
+  double y0 = 0, y1 = 0, y2 = 0;
+#pragma unroll
+  for (int k = 0; k < WU; ++k) {
+    const size_t w3 = 3 * (size_t)cu[k];
+    const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
+    const double* vk = val + (size_t)k * 9 * kSlice;
+    y0 += vk[0 * kSlice] * x0 + vk[1 * kSlice] * x1 + vk[2 * kSlice] * x2;
+    y1 += vk[3 * kSlice] * x0 + vk[4 * kSlice] * x1 + vk[5 * kSlice] * x2;
+    y2 += vk[6 * kSlice] * x0 + vk[7 * kSlice] * x1 + vk[8 * kSlice] * x2;
+  }
+#pragma unroll
+  for (int k = 0; k < WL; ++k) {
+    const size_t w3 = 3 * (size_t)cl[k].x;
+    const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
+    const double* b = A.val + (size_t)cl[k].y;
+    y0 += b[0 * kSlice] * x0 + b[3 * kSlice] * x1 + b[6 * kSlice] * x2;
+    y1 += b[1 * kSlice] * x0 + b[4 * kSlice] * x1 + b[7 * kSlice] * x2;
+    y2 += b[2 * kSlice] * x0 + b[5 * kSlice] * x1 + b[8 * kSlice] * x2;
+  }
+  y[0] = y0;
+  y[1] = y1;
+  y[2] = y2;
+}
+
+// LPR = 0 selects the fixed-width single-lane variant (layout padded to 8 upper / 7 lower slots)
+template <int LPR>
+__device__ __forceinline__ void row_part(const BellView& A, int v, int j, const double* __restrict__ x, double y[3]) {
+  if (LPR == 0)
+    bell_row_fixed<kFixedWU, kFixedWL>(A, v, x, y);
+  else
+    bell_row_part<LPR>(A, v, j, x, y);
+}
+
+// sum over the LPR lanes of a row (lanes l, l+16, l+32, l+48 for LPR = 4); every lane gets the total
+template <int LPR>
+__device__ __forceinline__ void row_combine(double y[3]) {
+  if (LPR <= 1) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    y[a] += __shfl_xor(y[a], 16, 64);
+    y[a] += __shfl_xor(y[a], 32, 64);
+  }
+}
+
+}  // namespace ssim_hip

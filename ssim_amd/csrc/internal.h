@@ -57,6 +57,30 @@ struct EvalScalars {
   double energy, gg, gd, dot;
 };
 
+// ---- batches of meshes ----
+// A batch concatenates its meshes into one mesh whose per-mesh vertex and tet ranges are padded to whole
+// workgroups of kBatchRows: every vector / row-slice / element workgroup then belongs to exactly one mesh, so
+// per-workgroup partial sums are per-mesh partial sums, and a mesh that sits out a launch (converged, or done
+// with its line search) skips it at workgroup granularity.  Padding vertices are Dirichlet-fixed and in no tet;
+// padding tets are absent (first vertex index −1).
+constexpr int kBatchRows = 256;         // vertex and tet granule of a mesh (= kBlock)
+constexpr int kBatchSlices = kBatchRows / kSlice;  // blocked-ELL slices per vertex workgroup
+struct BatchView {
+  int nmesh;
+  const int* vwg_mesh;  // [vertex workgroups] mesh of each block of kBatchRows vertices
+  const int* twg_mesh;  // [tet workgroups] mesh of each block of kBatchRows tets
+  const int* vwg_off;   // [nmesh+1] first vertex workgroup of each mesh
+  const int* twg_off;   // [nmesh+1] first tet workgroup of each mesh
+  const int* tend;      // [nmesh] one past the last real tet of each mesh
+  const int* mask;      // [nmesh] nonzero: the mesh takes part in the launch
+};
+// Iteration counter of a batched PCG, kept on the device so that every iteration's launches are identical
+// (replayable from a captured graph): kernel A reads ka and publishes kb = ka, kernel B reads kb and
+// publishes ka = kb + 1; neither kernel writes the field it reads.
+struct PcgIter {
+  int ka, kb;
+};
+
 // ---- launchers (kernels.hip) ----
 void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, double* V, hipStream_t s);
 // Reductions: a kernel stores one partial per workgroup; launch_* return the number of partials written.
@@ -106,5 +130,38 @@ void launch_axpy_coef(int n, double c1, const double* p1, double c2, const doubl
 void launch_sub(int n, const double* a, const double* b, double* out, hipStream_t s);
 
 int max_partials(int n);  // upper bound on partials any kernel writes for n items
+
+// ---- batched launchers (batch_kernels.hip); nvwg / ntwg = vertex / tet workgroups of the batch ----
+void launch_belem_grad(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* uhat,
+                       double dt2, const int* gperm, double* gcontrib, double* part_e, bool fp32, hipStream_t s);
+void launch_bgrad_reduce(const BatchView& bt, int nvwg, const int* gseg, const double* gcontrib,
+                         const unsigned char* dof, double* g, double* part_g, hipStream_t s);
+// per mesh (masked): out[m].energy = Σ part_e over its tet workgroups, out[m].gg = Σ part_g over its vertex
+// workgroups (part_g may be null)
+void launch_bfinalize_eval(const BatchView& bt, const double* part_e, const double* part_g, EvalScalars* out,
+                           hipStream_t s);
+// per mesh (masked): out[m].gd = a·b over the mesh's dofs
+void launch_bdot(const BatchView& bt, int nvwg, const double* a, const double* b, double* part, EvalScalars* out,
+                 hipStream_t s);
+void launch_belem_hess(const MeshView& m, const BatchView& bt, int ntwg, const double* u, double dt2, int make_spsd,
+                       const int* hperm, double* hcontrib, bool fp32, hipStream_t s);
+void launch_bhess_reduce(const BatchView& bt, int nub, const int* hseg, const double* hcontrib, const int2* ub_rc,
+                         const int* ub_pos, const unsigned char* dof, int filter_dbc, double* bell_val, double* dinv,
+                         hipStream_t s);
+// out = x + alpha[m]·d on the dofs of masked meshes
+void launch_baxpy_to(const BatchView& bt, int nvwg, const double* x, const double* alpha, const double* d,
+                     double* out, hipStream_t s);
+// dst1 = src1, dst2 = src2 on the dofs of masked meshes
+void launch_bcopy2(const BatchView& bt, int nvwg, const double* src1, double* dst1, const double* src2, double* dst2,
+                   hipStream_t s);
+// batched PCG: per-mesh scalars sc[m]; unmasked meshes are marked done at init
+void launch_bpcg_init(const BatchView& bt, int nvwg, const double* b, double sign, const double* dinv, double* x,
+                      double* r, double* z, double rtol, int maxit, PcgScalars* sc, PcgIter* it, double* pb0,
+                      hipStream_t s);
+void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const double* z, double* p, double* q,
+                      const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa, hipStream_t s);
+void launch_bpcg_update(const BatchView& bt, int nvwg, const double* dinv, double* x, double* r, double* z,
+                        const double* p, const double* q, const double* pa, double* pb0, double* pb1,
+                        PcgScalars* sc, PcgIter* it, hipStream_t s);
 
 }  // namespace ssim_hip

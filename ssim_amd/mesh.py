@@ -202,3 +202,32 @@ def twisted_cube(n=64, h=0.01, rho=1000.0, dt=1.0 / 60, seed=0, max_twist_deg=90
     u[dof == 1] = 0.0
     return BarCase(X, cells, lam, mu, rho, dt, dof, np.zeros(3 * nv), np.zeros(3 * nv), u, np.zeros(3 * nv),
                    name=f"twist{n}", meta=dict(n=n, h=h, seed=seed, theta_max=float(theta_max)))
+
+
+def bar_batch(nmesh=256, seed=0, lo=4, hi=24, h=0.01, rho=1000.0, dt=1.0 / 60, jitter=0.05, gravity=-9.8):
+    """Batch of independent cantilever bars of uneven size.
+
+    Bar b: grid dims (nx, ny, nz) uniform integers in [lo, hi] (stream (seed, 100), three draws per bar), one
+    E log-uniform in [1e4, 1e6] Pa (stream 101) and ν uniform in [0.3, 0.48] (stream 102) for all its tets,
+    x = 0 end clamped, gravity along y, free components jittered by U(±jitter·h) (stream 1000 + b).
+    """
+    dims = lo + np.floor(uniform(seed, 100, 3 * nmesh) * (hi - lo + 1)).astype(np.int64)
+    dims = np.minimum(dims, hi).reshape(nmesh, 3)
+    E = np.exp(np.log(1e4) + uniform(seed, 101, nmesh) * (np.log(1e6) - np.log(1e4)))
+    nu = 0.3 + uniform(seed, 102, nmesh) * 0.18
+    cases = []
+    for b in range(nmesh):
+        nx, ny, nz = (int(d) for d in dims[b])
+        X, cells = bar_freudenthal(nx, ny, nz, h)
+        nv, nt = X.shape[0], cells.shape[0]
+        lam, mu = lame(np.full(nt, E[b]), np.full(nt, nu[b]))
+        clamped = X[:, 0] == 0.0
+        dof = np.repeat(clamped.astype(np.uint8), 3)
+        u0 = (uniform(seed, 1000 + b, 3 * nv) * 2 - 1) * jitter * h
+        u0[dof == 1] = 0.0
+        accel = np.zeros(3 * nv)
+        accel[1::3] = gravity
+        cases.append(BarCase(X, cells, lam, mu, rho, dt, dof, np.zeros(3 * nv), accel, u0, np.zeros(3 * nv),
+                             name=f"batchbar{b}_{nx}x{ny}x{nz}",
+                             meta=dict(nx=nx, ny=ny, nz=nz, h=h, seed=seed, E=float(E[b]), nu=float(nu[b]))))
+    return cases

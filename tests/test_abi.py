@@ -21,7 +21,7 @@ def _declared():
 def test_library_exports_every_declared_symbol():
     L = _lib.load()
     names = _declared()
-    assert len(names) >= 40
+    assert len(names) >= 60
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     # and the ctypes table covers all of them
@@ -55,3 +55,26 @@ def test_status_codes_map_to_python_exceptions():
     with pytest.raises(_lib.SsimError):
         _lib.check(2)
     _lib.check(0)
+
+
+def test_batch_arguments_are_rejected_before_device_use():
+    from ssim_amd import batch
+
+    X, C = mesh.tetra()
+    lam, mu = np.ones(1), np.ones(1)
+    bad = C.copy()
+    bad[0, 0] = 7
+    with pytest.raises(ValueError, match="outside"):
+        batch.TetFiniteElementBatch_Hip([X, X], [C, bad], [lam, lam], [mu, mu])
+    with pytest.raises(ValueError, match="Lamé"):
+        batch.TetFiniteElementBatch_Hip([X], [C], [-lam], [mu])
+    with pytest.raises(ValueError, match="one entry per tet"):
+        batch.TetFiniteElementBatch_Hip([X], [C], [np.ones(2)], [mu])
+    with pytest.raises(ValueError, match="Time step"):
+        batch.TetFiniteElementBatch_Hip([X], [C], [lam], [mu], time_step=0.0)
+    with pytest.raises(ValueError, match="non-empty"):
+        batch.TetFiniteElementBatch_Hip([], [], [], [])
+    L = _lib.load()
+    assert L.ssim_batch_create(None, 1, None, None, None, None, None, None, 1.0, 1.0) == 1
+    assert L.ssim_batch_destroy(None) == 0
+    assert L.ssim_batch_step(None) == 1 and b"null batch" in L.ssim_last_error()

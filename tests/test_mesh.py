@@ -74,3 +74,28 @@ def test_twisted_cube_material_ranges():
     assert E.min() >= 1e4 * (1 - 1e-12) and E.max() <= 1e6 * (1 + 1e-12)
     assert nu.min() >= 0.3 - 1e-12 and nu.max() <= 0.48 + 1e-12
     assert np.all(case.u0.reshape(-1, 3)[case.X[:, 2] == 0] == 0)
+
+
+def test_bar_batch_is_seeded_and_within_ranges():
+    a = mesh.bar_batch(16, seed=4)
+    b = mesh.bar_batch(16, seed=4)
+    c = mesh.bar_batch(16, seed=5)
+    assert [x.name for x in a] == [x.name for x in b]
+    assert [x.name for x in a] != [x.name for x in c]
+    for x, y in zip(a, b):
+        assert np.array_equal(x.u0, y.u0) and np.array_equal(x.lam, y.lam)
+    for x in a:
+        nx, ny, nz = x.meta["nx"], x.meta["ny"], x.meta["nz"]
+        assert 4 <= min(nx, ny, nz) and max(nx, ny, nz) <= 24
+        assert x.nv == (nx + 1) * (ny + 1) * (nz + 1) and x.nt == 6 * nx * ny * nz
+        assert 1e4 <= x.meta["E"] <= 1e6 and 0.3 <= x.meta["nu"] <= 0.48
+        assert np.unique(x.lam).size == 1 and np.unique(x.mu).size == 1  # one material per bar
+        clamped = x.dof_type.reshape(-1, 3).any(axis=1)
+        assert np.array_equal(clamped, x.X[:, 0] == 0.0)
+        assert not x.u0[x.dof_type == 1].any()
+        assert np.all(np.abs(x.u0) <= 0.05 * 0.01 + 1e-18)
+    # the full 256-bar batch: dims cover the range, several million tets
+    full = mesh.bar_batch(256, seed=0)
+    dims = np.array([[x.meta["nx"], x.meta["ny"], x.meta["nz"]] for x in full])
+    assert dims.min() == 4 and dims.max() == 24
+    assert 3e6 <= sum(x.nt for x in full) <= 6e6
