@@ -56,6 +56,9 @@ typedef struct {
   double lbfgs_c;      /* Armijo constant for L-BFGS, default 1e-4 */
   int lbfgs_ls_max;    /* default 30 */
   int fp32_elements;   /* 1: element energy/stress/Hessian in FP32 (reductions and PCG stay FP64) */
+  int newton_polish;   /* full Newton steps (α = 1, no line search) after the Newton loop, default 0 (the
+                          reference's algorithm); 1-2 take the solution to roundoff where the Armijo test can
+                          no longer resolve energy decreases, as finite-difference checks of gradients need */
 } ssim_solver_params;
 
 /* Counters of the last compute_step (mathprim optim_result of stepper/backward_euler.hpp:28-32, plus
@@ -126,6 +129,9 @@ ssim_status_t ssim_check_convergence(ssim_ctx* ctx, int* converged_out);
 /* host copies of the vertex arrays (3·nv doubles) */
 ssim_status_t ssim_get_array(ssim_ctx* ctx, ssim_array_t which, double* host_out);
 ssim_status_t ssim_set_array(ssim_ctx* ctx, ssim_array_t which, const double* host_in);
+/* device-to-device copies of a vertex array (3·nv doubles) from / into caller-owned device memory */
+ssim_status_t ssim_get_array_device(ssim_ctx* ctx, ssim_array_t which, double* dst_device);
+ssim_status_t ssim_set_array_device(ssim_ctx* ctx, ssim_array_t which, const double* src_device);
 /* device pointer of a vertex array (3·nv doubles, owned by ctx, valid until destroy/reset) */
 ssim_status_t ssim_device_array(ssim_ctx* ctx, ssim_array_t which, double** device_ptr);
 /* Dirichlet flags per dof (dof_type_, 0 = general, 1 = dirichlet; boundary.hpp:8-11) */
@@ -154,6 +160,19 @@ ssim_status_t ssim_eval_elements(ssim_ctx* ctx, const double* u, int make_spsd, 
 ssim_status_t ssim_pcg_solve(ssim_ctx* ctx, const double* b, double* x, double rtol, int maxit, int* iters_out);
 /* y = H x with the current system matrix (host arrays) */
 ssim_status_t ssim_spmv(ssim_ctx* ctx, const double* x, double* y);
+
+/* ---- adjoint of one implicit step (no reference counterpart: the reference is forward-only) ----
+ * At the converged u* = next_deform of the last compute_step, the step solves g(u; û, λ, μ) = 0 on the free dofs
+ * with g = M(u − û) + dt²·Σ_e V_e f_e(u; λ_e, μ_e) (update_energy_and_gradients, time_step.hpp:577-624).  For
+ * a loss with ∂L/∂u* = grad_u (Dirichlet entries ignored) this solves H l = grad_u with the unprojected,
+ * Dirichlet-filtered Hessian (update_hessian(make_spsd = false), time_step.hpp:626-658) by Jacobi-PCG (the
+ * solver parameters' rtol / maxit) and writes
+ *   grad_uhat   = ∂L/∂û = M l (3·nv; zero on Dirichlet dofs),
+ *   grad_lambda = ∂L/∂λ_e, grad_mu = ∂L/∂μ_e per tet (nt each; may be NULL),
+ *   adjoint_out = l (3·nv; may be NULL).
+ * All arrays are DEVICE pointers.  PCG breakdown (H not positive definite) returns SSIM_ERR_NUMERIC. */
+ssim_status_t ssim_adjoint_step(ssim_ctx* ctx, const double* grad_u, double* grad_uhat, double* grad_lambda,
+                                double* grad_mu, double* adjoint_out, int* pcg_iters_out);
 
 /* ---- instrumentation (the reference times whole steps with std::chrono, bindings/src/fem_host.cpp:102-111;
  *      here kernels are timed on the GPU with hipEvents) ---- */
@@ -232,6 +251,15 @@ ssim_status_t ssim_batch_get_step_stats(const ssim_batch* batch, int mesh, ssim_
 ssim_status_t ssim_batch_get_array(ssim_batch* batch, ssim_array_t which, int mesh, double* host_out);
 ssim_status_t ssim_batch_set_array(ssim_batch* batch, ssim_array_t which, int mesh, const double* host_in);
 ssim_status_t ssim_batch_total_mass(const ssim_batch* batch, int mesh, double* mass);
+/* device-to-device copies of a vertex array of all meshes, compact layout (Σ 3·nv[m] doubles) */
+ssim_status_t ssim_batch_get_array_device(ssim_batch* batch, ssim_array_t which, double* dst_device);
+ssim_status_t ssim_batch_set_array_device(ssim_batch* batch, ssim_array_t which, const double* src_device);
+/* replace every mesh's per-tet Lamé parameters (host arrays of Σnt[m]); takes effect at once */
+ssim_status_t ssim_batch_set_material(ssim_batch* batch, const double* lambda, const double* mu);
+/* ssim_adjoint_step for every mesh of the batch; device arrays in the compact layout (grad_u, grad_uhat,
+ * adjoint_out: Σ 3·nv[m]; grad_lambda, grad_mu: Σ nt[m]); pcg_iters_out: host array of nmesh (may be NULL) */
+ssim_status_t ssim_batch_adjoint_step(ssim_batch* batch, const double* grad_u, double* grad_uhat, double* grad_lambda,
+                                      double* grad_mu, double* adjoint_out, int* pcg_iters_out);
 /* layout of the concatenated, padded mesh */
 ssim_status_t ssim_batch_get_layout_info(const ssim_batch* batch, ssim_layout_info* info);
 /* launch tuning (same keys; SpMV variant 0 or 1) and sampled kernel timing of the batch's launches */

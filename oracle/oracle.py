@@ -58,6 +58,7 @@ def lib():
             "orc_ts_get_vectors": (None, [_V, _D, _D, _D]),
             "orc_ts_set_solver": (None, [_V, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_int]),
+            "orc_ts_set_polish": (None, [_V, ctypes.c_int]),
             "orc_ts_set_lbfgs": (None, [_V, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int]),
             "orc_ts_elements": (None, [_V, _D, _D, _D, _D, _D, _D, ctypes.c_int]),
@@ -152,10 +153,86 @@ def pcg(rowptr, colidx, vals, b, rtol=1e-8, maxit=10000):
     return x, it, rel.value
 
 
+def cofactor(F):
+    """cof F = ∂det F/∂F for F (..., 3, 3): columns f1×f2, f2×f0, f0×f1 of F's columns f0, f1, f2."""
+    F = np.asarray(F, dtype=np.float64)
+    f0, f1, f2 = F[..., :, 0], F[..., :, 1], F[..., :, 2]
+    return np.stack([np.cross(f1, f2), np.cross(f2, f0), np.cross(f0, f1)], axis=-1)
+
+
+def snh_dP_dlame(F):
+    """(∂P/∂λ, ∂P/∂μ) of the stable Neo-Hookean first Piola stress (elast/stable_neohookean.hpp:23-62):
+    P = λ(J − α)·cof F + μ(1 − 1/(Ic+δ))·F with α = 1 + (1 − 1/(3+δ))·μ/λ, δ = 1, is linear in (λ, μ), so
+    ∂P/∂λ = (J − 1)·cof F and ∂P/∂μ = −(1 − 1/(3+δ))·cof F + (1 − 1/(Ic+δ))·F, independent of (λ, μ)."""
+    F = np.asarray(F, dtype=np.float64)
+    C = cofactor(F)
+    J = np.linalg.det(F)
+    Ic = np.sum(F * F, axis=(-2, -1))
+    dl = (J - 1.0)[..., None, None] * C
+    dm = -(1.0 - 1.0 / 4.0) * C + (1.0 - 1.0 / (Ic + 1.0))[..., None, None] * F
+    return dl, dm
+
+
+def _edge_matrix(x, cells):
+    """Ds per tet: columns x1 − x0, x2 − x0, x3 − x0 (def_grad.hpp:270-288), shape (nt, 3, 3)."""
+    x0 = x[cells[:, 0]]
+    return np.stack([x[cells[:, 1]] - x0, x[cells[:, 2]] - x0, x[cells[:, 3]] - x0], axis=2)
+
+
+def rollout_gradient(case, steps, loss_grad, lam=None, mu=None, rtol=1e-8, pcg_rtol=1e-12, polish=2):
+    """Forward rollout of `steps` Newton steps from case's initial state, then the adjoint recursion backwards.
+
+    loss_grad(u_N) -> (L, ∂L/∂u_N).  Step n: û = u_n + dt·v_n + dt²·a (free dofs), u_{n+1} = solution,
+    v_{n+1} = (u_{n+1} − u_n)/dt (time_step.hpp:499-543).  Backwards, with w = ū_{n+1} + v̄_{n+1}/dt:
+    (l, ∂L/∂û) = adjoint_step(u_{n+1}, w), ū_n = ∂L/∂û − v̄_{n+1}/dt, v̄_n = dt·∂L/∂û.
+    Returns (L, dict(u0, v0, lam, mu)) with per-tet Lamé gradients summed over steps."""
+    lam = case.lam if lam is None else lam
+    mu = case.mu if mu is None else mu
+    ts = TimeStep(case.X, case.cells, case.dt, lam, mu, case.rho)
+    ts.set_dirichlet(case.dof_type, case.dbc)
+    ts.set_ext_accel(case.accel)
+    ts.set_state(case.u0, np.zeros_like(case.u0), case.v0)
+    ts.set_threshold(rtol)
+    ts.set_solver(pcg_rtol=pcg_rtol)
+    ts.set_polish(polish)
+    us = [np.array(case.u0, dtype=np.float64)]
+    for _ in range(steps):
+        ts.step()
+        us.append(ts.get_state()[0])
+    L, gu = loss_grad(us[-1])
+    ubar, vbar = np.array(gu, dtype=np.float64), np.zeros(ts.n)
+    glam, gmu = np.zeros(ts.nt), np.zeros(ts.nt)
+    dt = ts.dt
+    for k in range(steps - 1, -1, -1):
+        _, guhat, gl, gm = ts.adjoint_step(us[k + 1], ubar + vbar / dt)
+        glam += gl
+        gmu += gm
+        ubar, vbar = guhat - vbar / dt, dt * guhat
+    return L, dict(u0=ubar, v0=vbar, lam=glam, mu=gmu)
+
+
+def rollout_loss(case, steps, loss_grad, lam=None, mu=None, v0=None, rtol=1e-8, pcg_rtol=1e-12, polish=2):
+    """Loss of the forward rollout only (finite-difference checks)."""
+    lam = case.lam if lam is None else lam
+    mu = case.mu if mu is None else mu
+    ts = TimeStep(case.X, case.cells, case.dt, lam, mu, case.rho)
+    ts.set_dirichlet(case.dof_type, case.dbc)
+    ts.set_ext_accel(case.accel)
+    ts.set_state(case.u0, np.zeros_like(case.u0), case.v0 if v0 is None else v0)
+    ts.set_threshold(rtol)
+    ts.set_solver(pcg_rtol=pcg_rtol)
+    ts.set_polish(polish)
+    for _ in range(steps):
+        ts.step()
+    return loss_grad(ts.get_state()[0])[0]
+
+
 class TimeStep:
     """Oracle mirror of ssim::fem::basic_time_step (finite_elements/time_step.hpp:171-715)."""
 
     def __init__(self, X, cells, dt, lam, mu, rho):
+        self.dt = float(dt)
+        self.dof = None
         self.X = _f64(X).reshape(-1, 3)
         self.cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
         self.nv, self.nt = self.X.shape[0], self.cells.shape[0]
@@ -200,6 +277,7 @@ class TimeStep:
 
     def set_dirichlet(self, dof_type, dbc):
         d = np.ascontiguousarray(dof_type, dtype=np.uint8).reshape(-1)
+        self.dof = d.copy()
         lib().orc_ts_set_dirichlet(self._h, d.ctypes.data_as(_U8), _dp(_f64(dbc).reshape(-1)))
 
     def set_ext_accel(self, a):
@@ -230,6 +308,9 @@ class TimeStep:
 
     def set_solver(self, max_newton=10, pcg_rtol=1e-8, pcg_maxit=10000, ls_c=0.3, ls_beta=0.5, ls_max=20):
         lib().orc_ts_set_solver(self._h, max_newton, pcg_rtol, pcg_maxit, ls_c, ls_beta, ls_max)
+
+    def set_polish(self, steps):
+        lib().orc_ts_set_polish(self._h, int(steps))
 
     def set_lbfgs(self, m=8, max_iter=1000, ls_c=1e-4, ls_beta=0.5, ls_max=30):
         lib().orc_ts_set_lbfgs(self._h, m, max_iter, ls_c, ls_beta, ls_max)
@@ -282,6 +363,39 @@ class TimeStep:
         out = self.compute_step(method)
         self.step_next()
         return out
+
+    def mass_csr(self):
+        import scipy.sparse as sp
+
+        rowptr, colidx = self.csr_pattern()
+        return sp.csr_matrix((self.mass_values(), colidx, rowptr), shape=(self.n, self.n))
+
+    def deformation_gradients(self, u):
+        """F = Ds(X + u)·B per tet (def_grad.hpp:93-132), shape (nt, 3, 3)."""
+        B, _ = self.rest_data()
+        x = self.X + _f64(u).reshape(-1, 3)
+        return _edge_matrix(x, self.cells) @ B
+
+    def adjoint_step(self, u_star, grad_u):
+        """Adjoint of one implicit step at its converged solution u_star, by a sparse direct solve.
+
+        The step solves g(u; û, λ, μ) = M(u − û) + dt²·Σ_e V_e f_e(u) = 0 on the free dofs
+        (time_step.hpp:577-624).  With H the unprojected, Dirichlet-filtered Hessian (time_step.hpp:626-658):
+        H l = grad_u (free dofs), ∂L/∂û = M l (free dofs), ∂L/∂λ_e = −dt²·V_e·(∂P/∂λ : Ds(l)·B_e), likewise μ.
+        Returns (l, grad_uhat, grad_lambda, grad_mu)."""
+        import scipy.sparse.linalg as spla
+
+        free = np.ones(self.n, dtype=bool) if self.dof is None else self.dof == 0
+        rhs = np.where(free, _f64(grad_u).reshape(-1), 0.0)
+        H = self.hessian_csr(u_star, make_spsd=False, filter_dbc=True)
+        lv = spla.spsolve(H.tocsc(), rhs)
+        g_uhat = np.where(free, self.mass_csr() @ lv, 0.0)
+        B, V = self.rest_data()
+        F = self.deformation_gradients(u_star)
+        G = _edge_matrix(lv.reshape(-1, 3), self.cells) @ B
+        dl, dm = snh_dP_dlame(F)
+        sc = -self.dt * self.dt * V
+        return lv, g_uhat, sc * np.sum(dl * G, axis=(1, 2)), sc * np.sum(dm * G, axis=(1, 2))
 
     def rest_hessian_diag(self):
         d = np.zeros(self.n)

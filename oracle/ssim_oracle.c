@@ -299,6 +299,7 @@ struct orc_ts {
   double *Hvals;
   double rtol, tol_abs;
   int max_newton, pcg_maxit, ls_max;
+  int polish; /* full Newton steps after the Newton loop (0: the reference's algorithm) */
   double pcg_rtol, ls_c, ls_beta;
   int lbfgs_m, lbfgs_maxit, lbfgs_ls_max;
   double lbfgs_c, lbfgs_beta;
@@ -458,6 +459,7 @@ orc_ts* orc_ts_create(const double* X, int nv, const int* cells, int nt, double 
   ts->ls_c = 0.3; /* stepper/backward_euler.hpp:22 */
   ts->ls_beta = 0.5;
   ts->ls_max = 20;
+  ts->polish = 0;
   ts->lbfgs_m = 8;
   ts->lbfgs_maxit = 1000;
   ts->lbfgs_c = 1e-4;
@@ -531,6 +533,8 @@ void orc_ts_set_solver(orc_ts* ts, int max_newton, double pcg_rtol, int pcg_maxi
   ts->ls_beta = ls_beta;
   ts->ls_max = ls_max;
 }
+void orc_ts_set_polish(orc_ts* ts, int polish) { ts->polish = polish; }
+
 void orc_ts_set_lbfgs(orc_ts* ts, int m, int max_iter, double ls_c, double ls_beta, int ls_max) {
   ts->lbfgs_m = m;
   ts->lbfgs_maxit = max_iter;
@@ -746,6 +750,16 @@ double orc_ts_compute_step(orc_ts* ts, double* stats) {
     memcpy(ts->g, gt, sizeof(double) * n);
     E = Et;
     newton++;
+  }
+  /* optional polish: full Newton steps (α = 1), which take the solution to roundoff accuracy where the Armijo
+   * test can no longer resolve energy decreases (extension for finite-difference checks; not in the reference) */
+  for (int k = 0; k < ts->polish; ++k) {
+    orc_ts_hessian(ts, u, 1, 1, ts->Hvals);
+    for (int i = 0; i < n; ++i) mg[i] = -ts->g[i];
+    pcg_total += orc_pcg(n, ts->rowptr, ts->colidx, ts->Hvals, mg, d, ts->pcg_rtol, ts->pcg_maxit, NULL);
+    for (int i = 0; i < n; ++i) u[i] += d[i];
+    orc_ts_energy_grad(ts, u, ts->g);
+    evals++;
   }
   /* compute_step: final update_energy_and_gradients + ‖forces‖ */
   orc_ts_energy_grad(ts, u, ts->g);

@@ -54,6 +54,8 @@ void orc_ts_set_vectors(orc_ts* ts, const double* u_next, const double* u_hat);
 void orc_ts_get_vectors(const orc_ts* ts, double* u_next, double* u_hat, double* forces);
 void orc_ts_set_solver(orc_ts* ts, int max_newton, double pcg_rtol, int pcg_maxit, double ls_c,
                        double ls_beta, int ls_max);
+/* full Newton steps (α = 1) after the Newton loop of compute_step (default 0) */
+void orc_ts_set_polish(orc_ts* ts, int polish);
 void orc_ts_set_lbfgs(orc_ts* ts, int m, int max_iter, double ls_c, double ls_beta, int ls_max);
 
 /* element outputs at deformation u (any NULL output is skipped) */

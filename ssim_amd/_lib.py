@@ -28,7 +28,7 @@ class SolverParams(ctypes.Structure):
         ("max_newton", ctypes.c_int), ("pcg_rtol", ctypes.c_double), ("pcg_maxit", ctypes.c_int),
         ("ls_c", ctypes.c_double), ("ls_beta", ctypes.c_double), ("ls_max", ctypes.c_int),
         ("lbfgs_m", ctypes.c_int), ("lbfgs_maxit", ctypes.c_int), ("lbfgs_c", ctypes.c_double),
-        ("lbfgs_ls_max", ctypes.c_int), ("fp32_elements", ctypes.c_int),
+        ("lbfgs_ls_max", ctypes.c_int), ("fp32_elements", ctypes.c_int), ("newton_polish", ctypes.c_int),
     ]
 
 
@@ -84,6 +84,9 @@ SIGNATURES = {
     "ssim_get_array": [_V, ctypes.c_int, _D],
     "ssim_set_array": [_V, ctypes.c_int, _D],
     "ssim_device_array": [_V, ctypes.c_int, ctypes.POINTER(_D)],
+    "ssim_get_array_device": [_V, ctypes.c_int, _V],
+    "ssim_set_array_device": [_V, ctypes.c_int, _V],
+    "ssim_adjoint_step": [_V, _V, _V, _V, _V, _V, _I],
     "ssim_get_dof_type": [_V, _U8],
     "ssim_num_vertices": [_V, _I],
     "ssim_num_cells": [_V, _I],
@@ -120,6 +123,10 @@ SIGNATURES = {
     "ssim_batch_get_array": [_V, ctypes.c_int, ctypes.c_int, _D],
     "ssim_batch_set_array": [_V, ctypes.c_int, ctypes.c_int, _D],
     "ssim_batch_total_mass": [_V, ctypes.c_int, _D],
+    "ssim_batch_get_array_device": [_V, ctypes.c_int, _V],
+    "ssim_batch_set_array_device": [_V, ctypes.c_int, _V],
+    "ssim_batch_set_material": [_V, _D, _D],
+    "ssim_batch_adjoint_step": [_V, _V, _V, _V, _V, _V, _I],
     "ssim_batch_get_layout_info": [_V, ctypes.POINTER(LayoutInfo)],
     "ssim_batch_set_tuning": [_V, ctypes.c_int, ctypes.c_int],
     "ssim_batch_get_tuning": [_V, ctypes.c_int, _I],

@@ -24,6 +24,8 @@ struct ssim_batch {
   std::vector<int> hvwg_off, htwg_off;
   int nvwg = 0, ntwg = 0;
   DBuf<int> vwg_mesh, twg_mesh, vwg_off, twg_off, tend;
+  DBuf<int> vcoff, nvm, tcoff;  // compact (caller's) vertex / tet offsets and vertex counts per mesh
+  long long nv_compact = 0, nt_compact = 0;
   DBuf<int> mask_all, mask_newton, mask_ls;
   DBuf<PcgScalars> pcg;
   DBuf<EvalScalars> ev;
@@ -124,12 +126,12 @@ void beval(ssim_batch* b, const DBuf<int>& mask, const double* u, double* g) {
   sync(c);
 }
 
-void bassemble(ssim_batch* b, const DBuf<int>& mask, const double* u) {
+void bassemble(ssim_batch* b, const DBuf<int>& mask, const double* u, int make_spsd) {
   ssim_ctx* c = &b->c;
   hipStream_t s = c->stream;
   const BatchView bt = b->view(mask);
   timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
-    launch_belem_hess(c->mesh(), bt, b->ntwg, u, c->dt * c->dt, 1, c->hperm.p, c->hcontrib.p,
+    launch_belem_hess(c->mesh(), bt, b->ntwg, u, c->dt * c->dt, make_spsd, c->hperm.p, c->hcontrib.p,
                       c->prm.fp32_elements != 0, s);
   });
   timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
@@ -139,14 +141,14 @@ void bassemble(ssim_batch* b, const DBuf<int>& mask, const double* u) {
   HIPCHK(hipGetLastError());
 }
 
-// batched Jacobi-PCG of H x = −g for the masked meshes; per-mesh iteration counts into iters
-void bpcg(ssim_batch* b, const double* g, double* x, std::vector<int>& iters) {
+// batched Jacobi-PCG of H x = sign·rhs for the masked meshes; per-mesh iteration counts into iters
+void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, double* x, std::vector<int>& iters) {
   ssim_ctx* c = &b->c;
   hipStream_t s = c->stream;
-  const BatchView bt = b->view(b->mask_newton);
+  const BatchView bt = b->view(mask);
   const BellView A = c->bell(c->val.p);
   const int maxit = c->prm.pcg_maxit;
-  launch_bpcg_init(bt, b->nvwg, g, -1.0, c->dinv.p, x, c->r.p, c->z.p, c->prm.pcg_rtol, maxit, b->pcg.p, b->iter.p,
+  launch_bpcg_init(bt, b->nvwg, rhs, sign, c->dinv.p, x, c->r.p, c->z.p, c->prm.pcg_rtol, maxit, b->pcg.p, b->iter.p,
                    c->part_pb(0), s);
   HIPCHK(hipGetLastError());
   for (auto& e : c->poll_ev)
@@ -236,8 +238,8 @@ void bnewton(ssim_batch* b) {
     }
     if (nact == 0) break;
     upload_mask(b, b->mask_newton, act);
-    bassemble(b, b->mask_newton, u);
-    bpcg(b, g, c->dir.p, iters);
+    bassemble(b, b->mask_newton, u, 1);
+    bpcg(b, b->mask_newton, g, -1.0, c->dir.p, iters);
     int maxk = 0;
     for (int m = 0; m < M; ++m)
       if (act[m]) {
@@ -289,6 +291,22 @@ void bnewton(ssim_batch* b) {
   for (int m = 0; m < M; ++m)
     if (act[m]) b->stats[m].newton_iterations = it;  // stopped at the iteration cap
   b->total.newton_iterations = it;
+  for (int k = 0; k < P.newton_polish; ++k) {  // full Newton steps for every mesh, no line search
+    bassemble(b, b->mask_all, u, 1);
+    bpcg(b, b->mask_all, g, -1.0, c->dir.p, iters);
+    int maxk = 0;
+    for (int m = 0; m < M; ++m) {
+      b->stats[m].pcg_iterations += iters[m];
+      maxk = std::max(maxk, iters[m]);
+    }
+    b->total.pcg_iterations += maxk;
+    for (int m = 0; m < M; ++m) alpha[m] = 1.0;
+    HIPCHK(hipMemcpyAsync(b->alpha.p, alpha.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
+    launch_baxpy_to(b->view(b->mask_all), b->nvwg, u, b->alpha.p, c->dir.p, c->utry.p, s);
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, s));
+    beval(b, b->mask_all, u, g);
+    b->total.energy_evals++;
+  }
 }
 
 }  // namespace
@@ -366,7 +384,7 @@ ssim_status_t ssim_batch_create(ssim_batch** out, int nmesh, const int* nv, cons
     c->hcells.assign(4 * (size_t)pt, -1);
     c->hlam.assign(pt, 1.0);
     c->hmu.assign(pt, 1.0);
-    std::vector<int> vwg_mesh(b->nvwg), twg_mesh(b->ntwg), tend(nmesh);
+    std::vector<int> vwg_mesh(b->nvwg), twg_mesh(b->ntwg), tend(nmesh), hvcoff(nmesh), htcoff(nmesh);
     {
       long long voff = 0, toff = 0;
       for (int m = 0; m < nmesh; ++m) {
@@ -377,9 +395,13 @@ ssim_status_t ssim_batch_create(ssim_batch** out, int nmesh, const int* nv, cons
         for (int w = b->hvwg_off[m]; w < b->hvwg_off[m + 1]; ++w) vwg_mesh[w] = m;
         for (int w = b->htwg_off[m]; w < b->htwg_off[m + 1]; ++w) twg_mesh[w] = m;
         tend[m] = b->tbase[m] + nt[m];
+        hvcoff[m] = (int)voff;
+        htcoff[m] = (int)toff;
         voff += nv[m];
         toff += nt[m];
       }
+      b->nv_compact = voff;
+      b->nt_compact = toff;
     }
     hipStream_t s = c->stream;
     b->pin_bytes = (sizeof(EvalScalars) + 2 * sizeof(PcgScalars)) * (size_t)nmesh;
@@ -392,6 +414,9 @@ ssim_status_t ssim_batch_create(ssim_batch** out, int nmesh, const int* nv, cons
     b->vwg_off.upload(b->hvwg_off, s);
     b->twg_off.upload(b->htwg_off, s);
     b->tend.upload(tend, s);
+    b->vcoff.upload(hvcoff, s);
+    b->nvm.upload(b->nv_m, s);
+    b->tcoff.upload(htcoff, s);
     std::vector<int> ones(nmesh, 1);
     b->mask_all.upload(ones, s);
     b->mask_newton.upload(ones, s);
@@ -442,7 +467,7 @@ ssim_status_t ssim_batch_set_solver_params(ssim_batch* b, const ssim_solver_para
   return run([&] {
     if (!p) fail(SSIM_ERR_INVALID_ARGUMENT, "params is null");
     if (p->max_newton < 0 || !(p->pcg_rtol > 0) || p->pcg_maxit < 0 || !(p->ls_c > 0 && p->ls_c < 1) ||
-        !(p->ls_beta > 0 && p->ls_beta < 1) || p->ls_max < 1)
+        !(p->ls_beta > 0 && p->ls_beta < 1) || p->ls_max < 1 || p->newton_polish < 0 || p->newton_polish > 16)
       fail(SSIM_ERR_INVALID_ARGUMENT, "solver parameters out of range");
     b->c.prm = *p;
   });
@@ -586,6 +611,83 @@ ssim_status_t ssim_batch_set_array(ssim_batch* b, ssim_array_t which, int mesh, 
       off += cnt;
     }
     sync(c);
+  });
+}
+
+ssim_status_t ssim_batch_get_array_device(ssim_batch* b, ssim_array_t which, double* dst) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (which < 0 || which >= SSIM_ARRAY_COUNT || !dst) fail(SSIM_ERR_INVALID_ARGUMENT, "bad array request");
+    ssim_ctx* c = &b->c;
+    launch_bunpad_vec(b->view(b->mask_all), b->nvwg, b->vcoff.p, b->nvm.p, c->a(which), dst, c->stream);
+    HIPCHK(hipGetLastError());
+    sync(c);
+  });
+}
+
+ssim_status_t ssim_batch_set_array_device(ssim_batch* b, ssim_array_t which, const double* src) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (which < 0 || which >= SSIM_ARRAY_COUNT || !src) fail(SSIM_ERR_INVALID_ARGUMENT, "bad array request");
+    ssim_ctx* c = &b->c;
+    launch_bpad_vec(b->view(b->mask_all), b->nvwg, b->vcoff.p, b->nvm.p, src, c->a(which), c->stream);
+    HIPCHK(hipGetLastError());
+    sync(c);
+  });
+}
+
+ssim_status_t ssim_batch_set_material(ssim_batch* b, const double* lambda, const double* mu) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!lambda || !mu) fail(SSIM_ERR_INVALID_ARGUMENT, "null material arrays");
+    ssim_ctx* c = &b->c;
+    long long off = 0;
+    for (int m = 0; m < b->nmesh; ++m) {
+      for (int e = 0; e < b->nt_m[m]; ++e) {
+        const double l = lambda[off + e], u = mu[off + e];
+        if (!(l > 0) || !(u > 0) || !std::isfinite(l) || !std::isfinite(u))
+          fail(SSIM_ERR_INVALID_ARGUMENT, "mesh " + std::to_string(m) + ": Lamé parameters must be positive and finite");
+        c->hlam[b->tbase[m] + e] = l;
+        c->hmu[b->tbase[m] + e] = u;
+      }
+      off += b->nt_m[m];
+    }
+    c->lam.upload(c->hlam, c->stream);
+    c->mu.upload(c->hmu, c->stream);
+    sync(c);
+  });
+}
+
+// Adjoint of one implicit step of every mesh at its converged u* = next_deform (see ssim_adjoint_step);
+// device arrays in the compact layout
+ssim_status_t ssim_batch_adjoint_step(ssim_batch* b, const double* grad_u, double* grad_uhat, double* grad_lambda,
+                                      double* grad_mu, double* adjoint_out, int* pcg_iters_out) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (!grad_u || !grad_uhat) fail(SSIM_ERR_INVALID_ARGUMENT, "grad_u and grad_uhat are required");
+    ssim_ctx* c = &b->c;
+    hipStream_t s = c->stream;
+    const double dt2 = c->dt * c->dt;
+    const double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+    const BatchView bt = b->view(b->mask_all);
+    bassemble(b, b->mask_all, u, 0);
+    launch_bpad_vec(bt, b->nvwg, b->vcoff.p, b->nvm.p, grad_u, c->utry.p, s);
+    launch_mask_dbc(c->n, c->dof.p, c->utry.p, c->tmp.p, s);
+    HIPCHK(hipGetLastError());
+    std::vector<int> iters(b->nmesh, 0);
+    bpcg(b, b->mask_all, c->tmp.p, 1.0, c->dir.p, iters);
+    c->adj_lam.alloc(std::max(c->nt, 1));
+    c->adj_mu.alloc(std::max(c->nt, 1));
+    launch_belem_adjoint(c->mesh(), bt, b->ntwg, u, c->dir.p, dt2, c->gperm.p, c->gcontrib.p, c->adj_lam.p,
+                         c->adj_mu.p, s);
+    launch_bgrad_reduce(bt, b->nvwg, c->gseg.p, c->gcontrib.p, c->dof.p, c->gtry.p, c->part_g(), s);
+    launch_bunpad_vec(bt, b->nvwg, b->vcoff.p, b->nvm.p, c->gtry.p, grad_uhat, s);
+    if (grad_lambda) launch_bunpad_tet(bt, b->ntwg, b->tcoff.p, c->adj_lam.p, grad_lambda, s);
+    if (grad_mu) launch_bunpad_tet(bt, b->ntwg, b->tcoff.p, c->adj_mu.p, grad_mu, s);
+    if (adjoint_out) launch_bunpad_vec(bt, b->nvwg, b->vcoff.p, b->nvm.p, c->dir.p, adjoint_out, s);
+    HIPCHK(hipGetLastError());
+    sync(c);
+    if (pcg_iters_out) std::copy(iters.begin(), iters.end(), pcg_iters_out);
   });
 }
 

@@ -111,6 +111,53 @@ __global__ __launch_bounds__(kBlock) void k_bhess_reduce(BatchView bt, int nub, 
 }
 
 // ------------------------------------------------------------------------------------------------------
+// adjoint (element part) and conversions between the padded and the compact (caller's) layouts
+
+__global__ __launch_bounds__(kBlock) void k_belem_adjoint(MeshView m, BatchView bt, const double* __restrict__ u,
+                                                          const double* __restrict__ lv, double dt2,
+                                                          const int* __restrict__ gperm, double* __restrict__ gcontrib,
+                                                          double* __restrict__ g_lam, double* __restrict__ g_mu) {
+  const int mesh = bt.twg_mesh[blockIdx.x];
+  if (!bt.mask[mesh]) return;
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e < bt.tend[mesh]) elem_adjoint_tet(m, e, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+}
+
+// compact vertex v' of mesh m sits at padded vertex kBatchRows·vwg_off[m] + (v' − vcoff[m]); padding gets 0
+__global__ __launch_bounds__(kBlock) void k_bpad_vec(BatchView bt, const int* __restrict__ vcoff,
+                                                     const int* __restrict__ nvm, const double* __restrict__ compact,
+                                                     double* __restrict__ padded) {
+  const int mesh = bt.vwg_mesh[blockIdx.x];
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  const int local = v - kBatchRows * bt.vwg_off[mesh];
+  const bool real = local < nvm[mesh];
+  const size_t src = 3 * ((size_t)vcoff[mesh] + local);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) padded[3 * (size_t)v + a] = real ? compact[src + a] : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bunpad_vec(BatchView bt, const int* __restrict__ vcoff,
+                                                       const int* __restrict__ nvm, const double* __restrict__ padded,
+                                                       double* __restrict__ compact) {
+  const int mesh = bt.vwg_mesh[blockIdx.x];
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  const int local = v - kBatchRows * bt.vwg_off[mesh];
+  if (local >= nvm[mesh]) return;
+  const size_t dst = 3 * ((size_t)vcoff[mesh] + local);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) compact[dst + a] = padded[3 * (size_t)v + a];
+}
+
+__global__ __launch_bounds__(kBlock) void k_bunpad_tet(BatchView bt, const int* __restrict__ tcoff,
+                                                       const double* __restrict__ padded,
+                                                       double* __restrict__ compact) {
+  const int mesh = bt.twg_mesh[blockIdx.x];
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= bt.tend[mesh]) return;
+  compact[(size_t)tcoff[mesh] + (e - kBatchRows * bt.twg_off[mesh])] = padded[e];
+}
+
+// ------------------------------------------------------------------------------------------------------
 // line-search vector kernels
 
 __global__ __launch_bounds__(kBlock) void k_baxpy_to(BatchView bt, const double* __restrict__ x,
@@ -349,6 +396,30 @@ void launch_bhess_reduce(const BatchView& bt, int nub, const int* hseg, const do
   if (nub <= 0) return;
   k_bhess_reduce<<<(nub + kBlock - 1) / kBlock, kBlock, 0, s>>>(bt, nub, hseg, hcontrib, ub_rc, ub_pos, dof,
                                                                  filter_dbc, bell_val, dinv);
+}
+
+void launch_belem_adjoint(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* lv,
+                          double dt2, const int* gperm, double* gcontrib, double* g_lam, double* g_mu, hipStream_t s) {
+  if (ntwg <= 0) return;
+  k_belem_adjoint<<<ntwg, kBlock, 0, s>>>(m, bt, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+}
+
+void launch_bpad_vec(const BatchView& bt, int nvwg, const int* vcoff, const int* nvm, const double* compact,
+                     double* padded, hipStream_t s) {
+  if (nvwg <= 0) return;
+  k_bpad_vec<<<nvwg, kBlock, 0, s>>>(bt, vcoff, nvm, compact, padded);
+}
+
+void launch_bunpad_vec(const BatchView& bt, int nvwg, const int* vcoff, const int* nvm, const double* padded,
+                       double* compact, hipStream_t s) {
+  if (nvwg <= 0) return;
+  k_bunpad_vec<<<nvwg, kBlock, 0, s>>>(bt, vcoff, nvm, padded, compact);
+}
+
+void launch_bunpad_tet(const BatchView& bt, int ntwg, const int* tcoff, const double* padded, double* compact,
+                       hipStream_t s) {
+  if (ntwg <= 0) return;
+  k_bunpad_tet<<<ntwg, kBlock, 0, s>>>(bt, tcoff, padded, compact);
 }
 
 void launch_baxpy_to(const BatchView& bt, int nvwg, const double* x, const double* alpha, const double* d,

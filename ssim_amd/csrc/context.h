@@ -99,6 +99,7 @@ struct ssim_ctx {
   ssim_hip::DBuf<int> gperm, gseg, hperm, hseg, wu, offu, ucol, wl, offl, ub_pos;
   ssim_hip::DBuf<int2> ub_rc, lref;
   ssim_hip::DBuf<double> gcontrib, hcontrib, val, massval;
+  ssim_hip::DBuf<double> adj_lam, adj_mu;  // per-tet material gradients when the caller passes no buffer
   // device: scalars
   ssim_hip::DBuf<ssim_hip::PcgScalars> pcg;
   ssim_hip::DBuf<ssim_hip::EvalScalars> ev;

@@ -35,6 +35,7 @@ void ssim_hip::default_params(ssim_solver_params& p) {
   p.lbfgs_c = 1e-4;
   p.lbfgs_ls_max = 30;
   p.fp32_elements = 0;
+  p.newton_polish = 0;
 }
 
 namespace {
@@ -457,6 +458,13 @@ void newton_solve(ssim_ctx* c) {
     cur = trial;
   }
   c->stats.newton_iterations = it;
+  for (int k = 0; k < c->prm.newton_polish; ++k) {  // full Newton steps, no line search
+    assemble_hessian(c, u, 1, 1, c->dt * c->dt, c->val.p);
+    c->stats.pcg_iterations += pcg_solve(c, g, -1.0, c->dir.p, c->prm.pcg_rtol, c->prm.pcg_maxit);
+    launch_axpy_to(n, u, 1.0, c->dir.p, c->utry.p, s);
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    eval_energy_grad(c, u, g);
+  }
 }
 
 // L-BFGS two-loop recursion (Nocedal & Wright, Alg. 7.4) with H0 = diag(H_rest)⁻¹ and backtracking Armijo;
@@ -709,7 +717,7 @@ ssim_status_t ssim_set_solver_params(ssim_ctx* ctx, const ssim_solver_params* p)
     if (!p) fail(SSIM_ERR_INVALID_ARGUMENT, "params is null");
     if (p->max_newton < 0 || p->pcg_maxit < 0 || !(p->pcg_rtol >= 0) || !(p->ls_c > 0 && p->ls_c < 1) ||
         !(p->ls_beta > 0 && p->ls_beta < 1) || p->ls_max < 1 || p->lbfgs_m < 1 || p->lbfgs_maxit < 0 ||
-        !(p->lbfgs_c > 0 && p->lbfgs_c < 1) || p->lbfgs_ls_max < 1)
+        !(p->lbfgs_c > 0 && p->lbfgs_c < 1) || p->lbfgs_ls_max < 1 || p->newton_polish < 0 || p->newton_polish > 16)
       fail(SSIM_ERR_INVALID_ARGUMENT, "solver parameter out of range");
     const bool fp32_changed = (p->fp32_elements != 0) != (ctx->prm.fp32_elements != 0);
     ctx->prm = *p;
@@ -877,6 +885,62 @@ ssim_status_t ssim_set_array(ssim_ctx* ctx, ssim_array_t which, const double* ho
     if (which < 0 || which >= SSIM_ARRAY_COUNT || !host_in) fail(SSIM_ERR_INVALID_ARGUMENT, "bad array request");
     HIPCHK(hipMemcpyAsync(ctx->a(which), host_in, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
     sync(ctx);
+  });
+}
+
+ssim_status_t ssim_get_array_device(ssim_ctx* ctx, ssim_array_t which, double* dst) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (which < 0 || which >= SSIM_ARRAY_COUNT || !dst) fail(SSIM_ERR_INVALID_ARGUMENT, "bad array request");
+    HIPCHK(hipMemcpyAsync(dst, ctx->a(which), sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
+    sync(ctx);
+  });
+}
+
+ssim_status_t ssim_set_array_device(ssim_ctx* ctx, ssim_array_t which, const double* src) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    if (which < 0 || which >= SSIM_ARRAY_COUNT || !src) fail(SSIM_ERR_INVALID_ARGUMENT, "bad array request");
+    HIPCHK(hipMemcpyAsync(ctx->a(which), src, sizeof(double) * ctx->n, hipMemcpyDeviceToDevice, ctx->stream));
+    sync(ctx);
+  });
+}
+
+// Adjoint of one implicit step at the converged u* = next_deform.  The step solves g(u*; û, λ, μ) = 0 on the
+// free dofs, g = M(u − û) + dt²·Σ_e V_e f_e(u; λ_e, μ_e) (time_step.hpp:577-624), so for a loss L(u*):
+//   H l = ∂L/∂u*  (H = ∂g/∂u, the unprojected Hessian with the Dirichlet identity filter; PCG),
+//   ∂L/∂û = M l (zero on Dirichlet dofs),   ∂L/∂θ = −(∂g/∂θ)ᵀ l for the per-tet Lamé parameters θ.
+ssim_status_t ssim_adjoint_step(ssim_ctx* ctx, const double* grad_u, double* grad_uhat, double* grad_lambda,
+                                double* grad_mu, double* adjoint_out, int* pcg_iters_out) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    if (!grad_u || !grad_uhat) fail(SSIM_ERR_INVALID_ARGUMENT, "grad_u and grad_uhat are required");
+    ssim_ctx* c = ctx;
+    hipStream_t s = c->stream;
+    const double dt2 = c->dt * c->dt;
+    const double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+    assemble_hessian(c, u, 0, 1, dt2, c->val.p);
+    launch_mask_dbc(c->n, c->dof.p, grad_u, c->tmp.p, s);
+    HIPCHK(hipGetLastError());
+    const int it = pcg_solve(c, c->tmp.p, 1.0, c->dir.p, c->prm.pcg_rtol, c->prm.pcg_maxit);
+    double* gl = grad_lambda;
+    double* gm = grad_mu;
+    if (!gl) {
+      c->adj_lam.alloc(std::max(c->nt, 1));
+      gl = c->adj_lam.p;
+    }
+    if (!gm) {
+      c->adj_mu.alloc(std::max(c->nt, 1));
+      gm = c->adj_mu.p;
+    }
+    launch_elem_adjoint(c->mesh(), u, c->dir.p, dt2, c->gperm.p, c->gcontrib.p, gl, gm, s);
+    launch_grad_reduce(c->nv, c->gseg.p, c->gcontrib.p, c->dof.p, grad_uhat, c->part_g(), s);
+    HIPCHK(hipGetLastError());
+    if (adjoint_out)
+      HIPCHK(hipMemcpyAsync(adjoint_out, c->dir.p, sizeof(double) * c->n, hipMemcpyDeviceToDevice, s));
+    sync(c);
+    if (pcg_iters_out) *pcg_iters_out = it;
   });
 }
 

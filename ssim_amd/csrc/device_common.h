@@ -279,6 +279,55 @@ __device__ __forceinline__ void elem_hess_tet(const MeshView& m, int e, const do
   });
 }
 
+// Element part of the adjoint of one implicit step, for the adjoint vector l (solution of H l = ∂L/∂u at the
+// converged u):
+//   mass contributions (ρV/20)(l_i + Σ_j l_j) to the gradient slots gperm[4e+i] (reduced per vertex: M l),
+//   ∂L/∂λ_e = −dt²·V·(∂P/∂λ : G),  ∂L/∂μ_e = −dt²·V·(∂P/∂μ : G),  G = Ds(l)·B = Σ_i l_i ⊗ ∇N_i,
+// because the step's residual depends on (λ, μ) only through the elastic forces V·P·∇N_i.  With
+// P = λ(J − α)·cof F + μ(1 − 1/(Ic+δ))·F and α = 1 + (1 − 1/(3+δ))·μ/λ (elast/stable_neohookean.hpp:23-62):
+//   ∂P/∂λ = (J − 1)·cof F,   ∂P/∂μ = −(1 − 1/(3+δ))·cof F + (1 − 1/(Ic+δ))·F.
+__device__ __forceinline__ void elem_adjoint_tet(const MeshView& m, int e, const double* __restrict__ u,
+                                                 const double* __restrict__ lv, double dt2,
+                                                 const int* __restrict__ gperm, double* __restrict__ gcontrib,
+                                                 double* __restrict__ g_lam, double* __restrict__ g_mu) {
+  const TetLocal t = load_tet(m, e);
+  double x[4][3], l[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const size_t k = 3 * (size_t)t.id[i] + a;
+      x[i][a] = m.X[k] + u[k];
+      l[i][a] = lv[k];
+    }
+  double F[9], G[9], C[9];
+  def_grad(x, t.B, F);
+  def_grad(l, t.B, G);
+  cofactor(F, C);
+  double Ic = 0, cG = 0, fG = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    Ic += F[k] * F[k];
+    cG += C[k] * G[k];
+    fG += F[k] * G[k];
+  }
+  const double J = det3(F);
+  constexpr double del = 1.0;
+  const double sc = -dt2 * t.V;
+  g_lam[e] = sc * (J - 1.0) * cG;
+  g_mu[e] = sc * (-(1.0 - 1.0 / (3.0 + del)) * cG + (1.0 - 1.0 / (Ic + del)) * fG);
+  const double mm = m.rho * t.V / 20.0;
+  double S[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) S[a] = l[0][a] + l[1][a] + l[2][a] + l[3][a];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const size_t pos = 3 * (size_t)gperm[4 * (size_t)e + i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) gcontrib[pos + a] = mm * (l[i][a] + S[a]);
+  }
+}
+
 // segmented reduce of vertex v's gradient contributions, zeroed on Dirichlet dofs (boundary.hpp:45-51);
 // returns the vertex's share of ‖g‖²
 __device__ __forceinline__ double grad_reduce_vertex(int v, const int* __restrict__ gseg,

@@ -122,6 +122,13 @@ void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* 
                        const double* pa, int na, const double* pb_cur, double* pb_next, PcgScalars* sc, hipStream_t s);
 int pcg_vector_blocks(int n);
 
+// adjoint of one step: mass contributions of the adjoint vector lv (reduce with launch_grad_reduce to get M·lv,
+// zero on Dirichlet dofs) and per-tet ∂L/∂λ, ∂L/∂μ at deformation u
+void launch_elem_adjoint(const MeshView& m, const double* u, const double* lv, double dt2, const int* gperm,
+                         double* gcontrib, double* g_lam, double* g_mu, hipStream_t s);
+// out = in with Dirichlet dofs zeroed
+void launch_mask_dbc(int n, const unsigned char* dof, const double* in, double* out, hipStream_t s);
+
 // L-BFGS helpers
 void launch_scale_div(int n, const double* q, const double* diag, double* out, hipStream_t s);
 // y += (c1·(*p1) − c2·(*p2))·x, coefficients read from device memory (p2 may be null)
@@ -148,6 +155,16 @@ void launch_belem_hess(const MeshView& m, const BatchView& bt, int ntwg, const d
 void launch_bhess_reduce(const BatchView& bt, int nub, const int* hseg, const double* hcontrib, const int2* ub_rc,
                          const int* ub_pos, const unsigned char* dof, int filter_dbc, double* bell_val, double* dinv,
                          hipStream_t s);
+void launch_belem_adjoint(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* lv,
+                          double dt2, const int* gperm, double* gcontrib, double* g_lam, double* g_mu, hipStream_t s);
+// compact (caller's, unpadded, meshes in order) ↔ padded layouts; vcoff / tcoff = compact vertex / tet offset of
+// each mesh, nvm = its vertex count; padding entries of padded vectors are written as 0
+void launch_bpad_vec(const BatchView& bt, int nvwg, const int* vcoff, const int* nvm, const double* compact,
+                     double* padded, hipStream_t s);
+void launch_bunpad_vec(const BatchView& bt, int nvwg, const int* vcoff, const int* nvm, const double* padded,
+                       double* compact, hipStream_t s);
+void launch_bunpad_tet(const BatchView& bt, int ntwg, const int* tcoff, const double* padded, double* compact,
+                       hipStream_t s);
 // out = x + alpha[m]·d on the dofs of masked meshes
 void launch_baxpy_to(const BatchView& bt, int nvwg, const double* x, const double* alpha, const double* d,
                      double* out, hipStream_t s);

@@ -220,6 +220,34 @@ void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double*
 }
 
 // ------------------------------------------------------------------------------------------------------
+// adjoint of one step (element part) and the Dirichlet mask of its right-hand side
+
+__global__ __launch_bounds__(kBlock) void k_elem_adjoint(MeshView m, const double* __restrict__ u,
+                                                         const double* __restrict__ lv, double dt2,
+                                                         const int* __restrict__ gperm, double* __restrict__ gcontrib,
+                                                         double* __restrict__ g_lam, double* __restrict__ g_mu) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e < m.nt) elem_adjoint_tet(m, e, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+}
+
+void launch_elem_adjoint(const MeshView& m, const double* u, const double* lv, double dt2, const int* gperm,
+                         double* gcontrib, double* g_lam, double* g_mu, hipStream_t s) {
+  if (m.nt <= 0) return;
+  k_elem_adjoint<<<nblocks(m.nt), kBlock, 0, s>>>(m, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+}
+
+__global__ void k_mask_dbc(int n, const unsigned char* __restrict__ dof, const double* __restrict__ in,
+                           double* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = dof[i] ? 0.0 : in[i];
+}
+
+void launch_mask_dbc(int n, const unsigned char* dof, const double* in, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  k_mask_dbc<<<nblocks(n), kBlock, 0, s>>>(n, dof, in, out);
+}
+
+// ------------------------------------------------------------------------------------------------------
 // Symmetric blocked-ELL SpMV.  Each block row is handled by LPR lanes (LPR = 1 or 4; lane j takes slots j,
 // j+LPR, ...).  Padded slots hold a valid column (the row itself) and zero values, and padded lower references
 // point at a zero block, so the slot loops are branch-free with a trip count uniform over the slice and can be

@@ -372,3 +372,131 @@ def test_planted_mistake_is_caught():
         wrong = oracle.snh_energy(F, c["lam"], c["mu"]) + 0.5 * c["mu"] * np.log(Ic + 1)
         bad += abs(wrong - c["psi"]) > 1e-12 * max(abs(c["psi"]), c["mu"])
     assert bad >= len(cases) - 2
+
+
+# ---- adjoint of the implicit step (extension; pinned by finite differences and closed forms) ----
+
+def test_snh_stress_lame_derivatives_match_finite_differences(golden):
+    for c in golden["cases"][:12]:
+        F = np.array(c["F"]).reshape(3, 3)
+        lam, mu = c["lam"], c["mu"]
+        dl, dm = oracle.snh_dP_dlame(F)
+        h = 1e-6
+        fd_l = (oracle.snh_stress(F, lam * (1 + h), mu)[1] - oracle.snh_stress(F, lam * (1 - h), mu)[1]) / (2 * h * lam)
+        fd_m = (oracle.snh_stress(F, lam, mu * (1 + h))[1] - oracle.snh_stress(F, lam, mu * (1 - h))[1]) / (2 * h * mu)
+        scale = max(np.abs(F).max(), np.abs(oracle.cofactor(F[None])[0]).max())  # the derivatives' natural size
+        assert np.abs(dl - fd_l).max() <= 1e-6 * scale
+        assert np.abs(dm - fd_m).max() <= 1e-6 * scale
+        # P is homogeneous of degree 1 in (λ, μ): λ ∂P/∂λ + μ ∂P/∂μ = P
+        P = oracle.snh_stress(F, lam, mu)[1]
+        assert np.abs(lam * dl + mu * dm - P).max() <= 1e-12 * (lam + mu) * scale
+    Fr = np.eye(3)
+    C = oracle.cofactor(Fr[None])[0]
+    assert np.allclose(C, np.eye(3))
+    R = np.array([[0.0, -1, 0], [1, 0, 0], [0, 0, 1]]) @ np.diag([1.2, 0.9, 1.1])
+    assert np.allclose(oracle.cofactor(R[None])[0], np.linalg.det(R) * np.linalg.inv(R).T)
+
+
+def _adjoint_case():
+    case = mesh.cantilever_bar(4, 2, 2, seed=3, E=1e5, nu=0.45)
+    rng = np.random.default_rng(0)
+    case.v0 = 0.01 * rng.standard_normal(case.v0.shape)
+    case.v0[case.dof_type == 1] = 0
+    fe = np.nonzero(case.X[:, 0] == case.X[:, 0].max())[0]
+    t = np.array([0.0, 0.012, 0.016])
+
+    def loss_grad(u):
+        d = u.reshape(-1, 3)[fe] - t
+        g = np.zeros((case.nv, 3))
+        g[fe] = 2 * d
+        return float(np.sum(d * d)), g.reshape(-1)
+
+    return case, loss_grad
+
+
+def test_adjoint_step_solves_the_linearised_step():
+    case, _ = _adjoint_case()
+    ts = oracle.TimeStep.from_case(case)
+    ts.set_threshold(1e-8)
+    ts.set_solver(pcg_rtol=1e-12)
+    ts.set_polish(2)
+    ts.step()
+    u = ts.get_state()[0]
+    rng = np.random.default_rng(1)
+    gu = rng.standard_normal(ts.n)
+    lv, guhat, gl, gm = ts.adjoint_step(u, gu)
+    free = case.dof_type == 0
+    H = ts.hessian_csr(u, make_spsd=False, filter_dbc=True)
+    assert np.linalg.norm((H @ lv - np.where(free, gu, 0))) <= 1e-10 * np.linalg.norm(gu)
+    assert not lv[~free].any() and not guhat[~free].any()
+    # ∂L/∂û = M l for L = gu·u*: central differences of the step's solution in the predictor û
+    d_uhat = np.where(free, rng.standard_normal(ts.n), 0.0)
+    ts2 = oracle.TimeStep.from_case(case)
+    ts2.set_threshold(1e-8)
+    ts2.set_solver(pcg_rtol=1e-12)
+    ts2.set_polish(2)
+    ts2.prepare_step()
+    _, uhat, _ = ts2.get_vectors()
+    h = 1e-6
+    sol = []
+    for sgn in (1, -1):
+        ts2.set_vectors(u_next=uhat + sgn * h * d_uhat, u_hat=uhat + sgn * h * d_uhat)
+        ts2.compute_step()
+        sol.append(ts2.get_vectors()[0])
+    lv2, guhat2, _, _ = ts2.adjoint_step(u, gu)
+    # exact identity: gu·H⁻¹(M δû) = (M l)·δû (H symmetric), against the tangent solve
+    du_lin = scipy.sparse.linalg.spsolve(H.tocsc(), np.where(free, ts2.mass_csr() @ d_uhat, 0.0))
+    assert abs(gu @ du_lin - guhat2 @ d_uhat) <= 1e-12 * abs(gu @ du_lin)
+    # and the tangent is the step's: central differences of the converged solution (a random functional
+    # has cancellation, so the differences resolve it to a few 1e-6)
+    fd_du = (sol[0] - sol[1]) / (2 * h)
+    assert np.abs(fd_du - du_lin).max() <= 1e-6 * np.abs(du_lin).max()
+    assert abs(gu @ fd_du - guhat2 @ d_uhat) <= 1e-5 * abs(gu @ du_lin)
+
+
+def test_rollout_adjoint_matches_central_differences():
+    case, loss_grad = _adjoint_case()
+    steps = 4
+    L, g = oracle.rollout_gradient(case, steps, loss_grad)
+    g_log_e = case.lam * g["lam"] + case.mu * g["mu"]
+    rng = np.random.default_rng(2)
+    h = 1e-6
+    for _ in range(3):
+        d = rng.standard_normal(case.nt)
+        Lp = oracle.rollout_loss(case, steps, loss_grad, lam=case.lam * np.exp(h * d), mu=case.mu * np.exp(h * d))
+        Lm = oracle.rollout_loss(case, steps, loss_grad, lam=case.lam * np.exp(-h * d), mu=case.mu * np.exp(-h * d))
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(fd - g_log_e @ d) <= 1e-5 * abs(fd)
+        dv = rng.standard_normal(case.nv * 3)
+        dv[case.dof_type == 1] = 0
+        Lp = oracle.rollout_loss(case, steps, loss_grad, v0=case.v0 + h * dv)
+        Lm = oracle.rollout_loss(case, steps, loss_grad, v0=case.v0 - h * dv)
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(fd - g["v0"] @ dv) <= 1e-5 * abs(fd)
+
+
+def test_planted_adjoint_mistake_fails_the_difference_check():
+    """Dropping the velocity path of the recursion (ū_n = M l instead of M l − v̄/dt) must break agreement
+    with central differences: the check has the power to see a missing term."""
+    case, loss_grad = _adjoint_case()
+    steps = 4
+    ts = oracle.TimeStep.from_case(case)
+    ts.set_threshold(1e-8)
+    ts.set_solver(pcg_rtol=1e-12)
+    ts.set_polish(2)
+    us = [case.u0.copy()]
+    for _ in range(steps):
+        ts.step()
+        us.append(ts.get_state()[0])
+    _, gu = loss_grad(us[-1])
+    ubar, vbar = gu.copy(), np.zeros(ts.n)
+    for k in range(steps - 1, -1, -1):
+        _, guhat, _, _ = ts.adjoint_step(us[k + 1], ubar + vbar / ts.dt)
+        ubar, vbar = guhat, ts.dt * guhat  # planted: − v̄/dt dropped
+    rng = np.random.default_rng(3)
+    dv = rng.standard_normal(case.nv * 3)
+    dv[case.dof_type == 1] = 0
+    h = 1e-6
+    fd = (oracle.rollout_loss(case, steps, loss_grad, v0=case.v0 + h * dv)
+          - oracle.rollout_loss(case, steps, loss_grad, v0=case.v0 - h * dv)) / (2 * h)
+    assert abs(fd - vbar @ dv) > 1e-3 * abs(fd)
