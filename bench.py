@@ -11,6 +11,13 @@ region starts; the timed region is K whole steps (prepare + Newton solve + step_
 Prints one JSON line.  ``roofline`` is for the dominant kernel (the PCG SpMV), whose average duration is
 measured with hipEvents inside the timed region; ``cpu_baseline`` is the CPU FP64 oracle timed on a bounded
 sample of the same workload on this host (see ``cpu_sample`` below).
+
+Two more objects in the same line cover the batched and differentiable variants of the step:
+``batched``: 256 bars of uneven size (dims uniform in 4..24, per-bar E log-uniform in [1e4, 1e6] Pa, ν uniform in
+[0.3, 0.48], x = 0 end clamped, gravity) stepped by one set of launches, against stepping the same 256 meshes one
+after another through the single-mesh path on the GPU, with the per-mesh agreement of the two after one step;
+``adjoint``: a 20-step rollout of 32 bars of 16×16×64 cubes (per-tet log E uniform in [ln 1e4, ln 1e6], ν = 0.45)
+with the loss Σ‖u − t‖² over the free-end vertices, timed forward and backward (adjoint PCG per step).
 """
 from __future__ import annotations
 
@@ -91,6 +98,110 @@ def cpu_sample(case, counts, threads, rtol):
     }
 
 
+def bench_batched(rtol, steps=3, seed=0):
+    """Batched vs sequential per-mesh GPU stepping of 256 uneven bars; per-mesh agreement after one step."""
+    import torch
+
+    from ssim_amd import batch, fem, mesh
+
+    cases = mesh.bar_batch(256, seed=seed)
+    t0 = time.perf_counter()
+    B = batch.TetFiniteElementBatch_Hip.from_cases(cases, rtol=rtol)
+    setup_b = time.perf_counter() - t0
+    singles = [fem.TetFiniteElementSolver_Hip.from_case(c, rtol=rtol) for c in cases]
+    B.step()
+    for s in singles:
+        s.step()
+    worst, mism = 0.0, 0
+    for m, (c, s) in enumerate(zip(cases, singles)):
+        xb = c.X.reshape(-1) + B.get_array("deformation", m)
+        xs = c.X.reshape(-1) + s.get_array("deformation")
+        worst = max(worst, float(np.abs(xb - xs).max() / np.abs(xs).max()))
+        mism += int(B.last_stats(m)["newton_iterations"] != s.last_stats()["newton_iterations"])
+    print(f"[bench] batched: per-mesh max position rel err {worst:.3e}, Newton count mismatches {mism}",
+          file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tot = dict(newton_iterations=0, pcg_iterations=0)
+    for _ in range(steps):
+        B.step()
+        st = B.last_stats()
+        tot["newton_iterations"] += st["newton_iterations"]
+        tot["pcg_iterations"] += st["pcg_iterations"]
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for s in singles:
+        s.step()
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    print(f"[bench] batched {tb:.3f} s/step, sequential {ts:.3f} s/step", file=sys.stderr, flush=True)
+    return {
+        "workload": "256 bars, dims uniform in 4..24, per-bar E log-uniform [1e4,1e6], nu uniform [0.3,0.48]",
+        "meshes": len(cases),
+        "tets": int(sum(c.nt for c in cases)),
+        "vertices": int(sum(c.nv for c in cases)),
+        "newton_tol": f"{rtol:g}*mass per mesh",
+        "batched_steps_per_s": 1.0 / tb,
+        "sequential_steps_per_s": 1.0 / ts,
+        "speedup_vs_sequential": ts / tb,
+        "timed_batched_steps": steps,
+        "lockstep_newton_iterations": tot["newton_iterations"],
+        "lockstep_pcg_iterations": tot["pcg_iterations"],
+        "setup_s": setup_b,
+        "per_mesh_max_position_rel_err": worst,
+        "newton_count_mismatches": mism,
+    }
+
+
+def bench_adjoint(rtol, nbar=32, seed=0):
+    """Forward and adjoint (backward) time of a 20-step differentiable rollout of 32 bars of 16x16x64 cubes."""
+    import torch
+
+    from ssim_amd import adjoint, batch, mesh
+
+    rc = mesh.rollout_batch(nbar, seed=seed)
+    B = batch.TetFiniteElementBatch_Hip.from_cases(rc.cases, rtol=rtol)
+    runner = adjoint.Rollout(B, rc.nu, rc.steps, rc.cases[0].dt)
+    idx, tgt, off = [], [], 0
+    for c, fe, t in zip(rc.cases, rc.free_end, rc.targets):
+        idx.append(off + fe)
+        tgt.append(np.broadcast_to(t, (fe.size, 3)))
+        off += c.nv
+    tidx = torch.as_tensor(np.concatenate(idx), device="cuda")
+    ttgt = torch.as_tensor(np.concatenate(tgt), device="cuda")
+    le = torch.as_tensor(rc.log_e, device="cuda").requires_grad_(True)
+    v0 = torch.zeros(off * 3, dtype=torch.float64, device="cuda").requires_grad_(True)
+    u0 = torch.as_tensor(np.concatenate([c.u0 for c in rc.cases]), device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    uN = adjoint.rollout(runner, le, v0, u0)
+    d = uN.view(-1, 3)[tidx] - ttgt
+    L = (d * d).sum()
+    loss = L.item()
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    L.backward()
+    torch.cuda.synchronize()
+    bwd = time.perf_counter() - t0
+    print(f"[bench] adjoint: forward {fwd:.2f} s, backward {bwd:.2f} s", file=sys.stderr, flush=True)
+    return {
+        "workload": (f"{nbar} bars of 16x16x64 cubes ({rc.cases[0].nt} tets each), per-tet log E uniform in "
+                     "[ln 1e4, ln 1e6], nu=0.45, 20 steps, loss sum over free-end vertices of |u - t|^2, |t|=0.02 m"),
+        "tets": int(sum(c.nt for c in rc.cases)),
+        "steps": rc.steps,
+        "newton_tol": f"{rtol:g}*mass per mesh",
+        "loss": loss,
+        "forward_s": fwd,
+        "backward_s": bwd,
+        "backward_over_forward": bwd / fwd,
+        "adjoint_pcg_iterations_lockstep": runner.adjoint_pcg_iterations,
+        "gradient_norm_log_e": float(le.grad.norm().item()),
+        "gradient_norm_v0": float(v0.grad.norm().item()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +212,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build's time step runs on a single device "
@@ -157,6 +270,9 @@ def main():
     if not args.no_cpu:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
         cpu = cpu_sample(case, counts, args.cpu_threads, args.rtol)
+    del solver
+    batched = None if args.no_batched else bench_batched(args.rtol)
+    adj = None if args.no_adjoint else bench_adjoint(args.rtol)
     value = args.steps / elapsed
     line = {
         "metric": METRIC,
@@ -184,6 +300,8 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "adjoint": adj,
     }
     print(json.dumps(line), flush=True)
 

@@ -151,12 +151,24 @@ class InvalidArgument(SsimError, ValueError):
     pass
 
 
+def _share_torch_runtime():
+    """PyTorch-ROCm wheels carry their own HIP and HSA runtime libraries.  If libssim_hip.so were loaded first,
+    it would bring in the system's HIP runtime and a later ``import torch`` would start a second runtime in the
+    same process, which then finds no GPU.  Importing torch first (when it is installed) makes the library bind
+    to the runtime torch already loaded, so the process has exactly one."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(path: str = LIB_PATH):
     """Load libssim_hip.so.  Raises if it has not been built (no fallback exists)."""
     global _lib
     if _lib is None:
         if not os.path.exists(path):
             raise ImportError(f"libssim_hip.so not found at {path}: build it with `python -m ssim_amd.build`")
+        _share_torch_runtime()
         L = ctypes.CDLL(path)
         L.ssim_last_error.restype = ctypes.c_char_p
         L.ssim_last_error.argtypes = []

@@ -231,3 +231,43 @@ def bar_batch(nmesh=256, seed=0, lo=4, hi=24, h=0.01, rho=1000.0, dt=1.0 / 60, j
                              name=f"batchbar{b}_{nx}x{ny}x{nz}",
                              meta=dict(nx=nx, ny=ny, nz=nz, h=h, seed=seed, E=float(E[b]), nu=float(nu[b]))))
     return cases
+
+
+@dataclass
+class RolloutCase:
+    """A batch of bars for a differentiable rollout: per-tet log E, fixed ν, and a free-end target offset."""
+
+    cases: list  # BarCase per bar (its lam / mu are the initial material)
+    log_e: np.ndarray  # per tet, all bars concatenated
+    nu: np.ndarray  # per tet
+    targets: np.ndarray  # (nbar, 3) free-end offsets
+    free_end: list  # free-end vertex indices per bar (x = max face)
+    steps: int = 20
+
+
+def rollout_batch(nbar=32, nx=64, ny=16, nz=16, h=0.01, nu=0.45, rho=1000.0, dt=1.0 / 60, seed=0, jitter=0.05,
+                  gravity=-9.8, offset=0.02, steps=20):
+    """Differentiable-rollout batch: nbar copies of the nx×ny×nz bar (x = 0 end clamped, gravity along y), each
+    with per-tet log E uniform in [ln 1e4, ln 1e6] (stream (seed, 200 + b)), ν fixed, free components jittered by
+    U(±jitter·h) (stream 300 + b), v₀ = 0, and a free-end target offset of norm `offset` in a seeded direction
+    (stream 400, three normal draws per bar by Box-Muller from the uniform stream)."""
+    X, cells = bar_freudenthal(nx, ny, nz, h)
+    nv, nt = X.shape[0], cells.shape[0]
+    clamped = X[:, 0] == 0.0
+    dof = np.repeat(clamped.astype(np.uint8), 3)
+    accel = np.zeros(3 * nv)
+    accel[1::3] = gravity
+    fe = np.nonzero(X[:, 0] == X[:, 0].max())[0]
+    u1 = uniform(seed, 400, 6 * nbar).reshape(nbar, 6)
+    z = np.sqrt(-2 * np.log(1 - u1[:, :3])) * np.cos(2 * np.pi * u1[:, 3:])  # Box-Muller normals
+    targets = offset * z / np.linalg.norm(z, axis=1, keepdims=True)
+    cases, log_e = [], []
+    for b in range(nbar):
+        le = np.log(1e4) + uniform(seed, 200 + b, nt) * (np.log(1e6) - np.log(1e4))
+        lam, mu = lame(np.exp(le), np.full(nt, nu))
+        u0 = (uniform(seed, 300 + b, 3 * nv) * 2 - 1) * jitter * h
+        u0[dof == 1] = 0.0
+        cases.append(BarCase(X, cells, lam, mu, rho, dt, dof, np.zeros(3 * nv), accel, u0, np.zeros(3 * nv),
+                             name=f"rollout{b}_{nx}x{ny}x{nz}", meta=dict(nx=nx, ny=ny, nz=nz, h=h, seed=seed)))
+        log_e.append(le)
+    return RolloutCase(cases, np.concatenate(log_e), np.full(nbar * nt, nu), targets, [fe] * nbar, steps)

@@ -78,3 +78,17 @@ def test_batch_arguments_are_rejected_before_device_use():
     assert L.ssim_batch_create(None, 1, None, None, None, None, None, None, 1.0, 1.0) == 1
     assert L.ssim_batch_destroy(None) == 0
     assert L.ssim_batch_step(None) == 1 and b"null batch" in L.ssim_last_error()
+
+
+def test_one_hip_runtime_per_process():
+    """Loading the library and torch in either order leaves a single HIP runtime mapped (two runtimes in one
+    process leave the second without a GPU)."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from ssim_amd import _lib\n_lib.load()\nimport torch\n"
+            "maps = open('/proc/self/maps').read().splitlines()\n"
+            "print(len({l.split()[-1] for l in maps if 'libamdhip64' in l}))\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True).stdout
+    assert out.strip() == "1", out
