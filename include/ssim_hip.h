@@ -56,9 +56,11 @@ typedef struct {
   double lbfgs_c;      /* Armijo constant for L-BFGS, default 1e-4 */
   int lbfgs_ls_max;    /* default 30 */
   int fp32_elements;   /* 1: element energy/stress/Hessian in FP32 (reductions and PCG stay FP64) */
-  int newton_polish;   /* full Newton steps (α = 1, no line search) after the Newton loop, default 0 (the
-                          reference's algorithm); 1-2 take the solution to roundoff where the Armijo test can
-                          no longer resolve energy decreases, as finite-difference checks of gradients need */
+  int newton_polish;   /* full Newton steps (α = 1, no line search, unprojected Hessian) after the Newton loop,
+                          default 0 (the reference's algorithm); 1-2 take the solution to roundoff where the
+                          Armijo test can no longer resolve energy decreases, as finite-difference checks of
+                          gradients need.  A Hessian that is not positive definite there fails the PCG
+                          (SSIM_ERR_NUMERIC) */
 } ssim_solver_params;
 
 /* Counters of the last compute_step (mathprim optim_result of stepper/backward_euler.hpp:28-32, plus

@@ -751,10 +751,12 @@ double orc_ts_compute_step(orc_ts* ts, double* stats) {
     E = Et;
     newton++;
   }
-  /* optional polish: full Newton steps (α = 1), which take the solution to roundoff accuracy where the Armijo
-   * test can no longer resolve energy decreases (extension for finite-difference checks; not in the reference) */
+  /* optional polish: full Newton steps (α = 1) with the unprojected Hessian, positive definite at the minimum
+   * the loop has converged to, so they converge quadratically and take the solution to roundoff accuracy where
+   * the Armijo test can no longer resolve energy decreases (extension for finite-difference checks; not in the
+   * reference) */
   for (int k = 0; k < ts->polish; ++k) {
-    orc_ts_hessian(ts, u, 1, 1, ts->Hvals);
+    orc_ts_hessian(ts, u, 0, 1, ts->Hvals);
     for (int i = 0; i < n; ++i) mg[i] = -ts->g[i];
     pcg_total += orc_pcg(n, ts->rowptr, ts->colidx, ts->Hvals, mg, d, ts->pcg_rtol, ts->pcg_maxit, NULL);
     for (int i = 0; i < n; ++i) u[i] += d[i];

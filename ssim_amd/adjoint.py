@@ -37,7 +37,7 @@ class Rollout:
     """Forward rollout and adjoint of a batch of meshes (see the module docstring)."""
 
     def __init__(self, batch: TetFiniteElementBatch_Hip, nu: np.ndarray, steps: int, dt: float,
-                 checkpoint_every: int = 1):
+                 checkpoint_every: int = 1, progress=None):
         if steps < 1 or checkpoint_every < 1:
             raise ValueError("steps and checkpoint_every must be positive")
         self.b = batch
@@ -54,6 +54,7 @@ class Rollout:
         self.ckpt = {}
         self.lam = self.mu = None
         self.adjoint_pcg_iterations = 0
+        self.progress = progress  # optional callable(phase, step) after each forward / adjoint step
 
     # ---- device state ----
     def _set(self, name, t):
@@ -85,6 +86,8 @@ class Rollout:
             self.b.step()
             if (n + 1) % self.k == 0 or n + 1 == self.steps:
                 self.ckpt[n + 1] = (self._get("deformation"), self._get("velocity"))
+            if self.progress:
+                self.progress("forward", n)
         return self.ckpt[self.steps][0].clone()
 
     def backward(self, grad_uN: torch.Tensor):
@@ -118,6 +121,8 @@ class Rollout:
                 glam += gl
                 gmu += gm
                 ubar, vbar = guhat - vbar / self.dt, self.dt * guhat
+                if self.progress:
+                    self.progress("adjoint", m)
         lam = torch.from_numpy(self.lam).to(dev)
         mu = torch.from_numpy(self.mu).to(dev)
         return lam * glam + mu * gmu, vbar, ubar

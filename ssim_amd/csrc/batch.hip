@@ -291,8 +291,9 @@ void bnewton(ssim_batch* b) {
   for (int m = 0; m < M; ++m)
     if (act[m]) b->stats[m].newton_iterations = it;  // stopped at the iteration cap
   b->total.newton_iterations = it;
-  for (int k = 0; k < P.newton_polish; ++k) {  // full Newton steps for every mesh, no line search
-    bassemble(b, b->mask_all, u, 1);
+  // polish: full Newton steps with the unprojected Hessian for every mesh, no line search
+  for (int k = 0; k < P.newton_polish; ++k) {
+    bassemble(b, b->mask_all, u, 0);
     bpcg(b, b->mask_all, g, -1.0, c->dir.p, iters);
     int maxk = 0;
     for (int m = 0; m < M; ++m) {

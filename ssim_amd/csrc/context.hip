@@ -458,8 +458,10 @@ void newton_solve(ssim_ctx* c) {
     cur = trial;
   }
   c->stats.newton_iterations = it;
-  for (int k = 0; k < c->prm.newton_polish; ++k) {  // full Newton steps, no line search
-    assemble_hessian(c, u, 1, 1, c->dt * c->dt, c->val.p);
+  // polish: full Newton steps with the unprojected Hessian (positive definite at the converged minimum, so the
+  // steps converge quadratically), no line search
+  for (int k = 0; k < c->prm.newton_polish; ++k) {
+    assemble_hessian(c, u, 0, 1, c->dt * c->dt, c->val.p);
     c->stats.pcg_iterations += pcg_solve(c, g, -1.0, c->dir.p, c->prm.pcg_rtol, c->prm.pcg_maxit);
     launch_axpy_to(n, u, 1.0, c->dir.p, c->utry.p, s);
     HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));

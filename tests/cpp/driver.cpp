@@ -1,0 +1,88 @@
+// driver.cpp -- a C++ program stepping a bar through the header-only front end (include/ssim_hip.hpp) the way
+// the reference's drivers use basic_time_step (tests/fem_time_step/main.cpp): construct from a mesh, mark the
+// clamped vertices, add gravity, reset with a stepper, then prepare_step / compute_step / step_next.
+//
+// Input file (little-endian): int32 nv, nt, steps, stepper (0 Newton, 1 L-BFGS); float64 dt, E, nu, rho, rtol,
+// gravity_y; float64 X[nv*3]; int32 cells[nt*4]; uint8 clamped[nv]; float64 u0[nv*3].
+// Output file: float64 u[nv*3] after the last step, then per step int32 newton_iterations, pcg_iterations.
+#include <cstdio>
+#include <cstdint>
+#include <vector>
+
+#include "ssim_hip.hpp"
+
+template <typename T>
+static void rd(FILE* f, T* p, size_t n) {
+  if (fread(p, sizeof(T), n, f) != n) throw std::runtime_error("short input file");
+}
+
+int main(int argc, char** argv) {
+  if (argc != 3) {
+    std::fprintf(stderr, "usage: %s input output\n", argv[0]);
+    return 2;
+  }
+  try {
+    FILE* in = std::fopen(argv[1], "rb");
+    if (!in) throw std::runtime_error("cannot open input");
+    int32_t hdr[4];
+    double prm[6];
+    rd(in, hdr, 4);
+    rd(in, prm, 6);
+    const int nv = hdr[0], nt = hdr[1], steps = hdr[2], stepper = hdr[3];
+    std::vector<double> X(3 * (size_t)nv), u0(3 * (size_t)nv);
+    std::vector<int32_t> cells(4 * (size_t)nt);
+    std::vector<uint8_t> clamped(nv);
+    rd(in, X.data(), X.size());
+    rd(in, cells.data(), cells.size());
+    rd(in, clamped.data(), clamped.size());
+    rd(in, u0.data(), u0.size());
+    std::fclose(in);
+
+    ssim::hip::time_step step(X.data(), nv, cells.data(), nt, prm[0], prm[1], prm[2], prm[3]);
+    std::vector<int> fixed;
+    for (int v = 0; v < nv; ++v)
+      if (clamped[v]) fixed.push_back(v);
+    std::vector<double> zeros(3 * fixed.size(), 0.0);
+    step.mark_dirichlet(fixed.data(), (int)fixed.size(), zeros.data());
+    step.add_ext_force_dof(1, prm[5]);
+    step.set(SSIM_ARRAY_DEFORMATION, u0);
+    step.set_threshold(prm[4]);
+    ssim::hip::newton_solver newton;
+    ssim::hip::lbfgs_solver lbfgs;
+    if (stepper == 1)
+      step.reset(lbfgs);
+    else
+      step.reset(newton);
+    std::vector<int32_t> counts;
+    for (int s = 0; s < steps; ++s) {
+      step.prepare_step();
+      if (stepper == 1)
+        step.compute_step(lbfgs);
+      else
+        step.compute_step(newton);
+      step.step_next();
+      const ssim_step_stats st = step.last_stats();
+      counts.push_back(st.newton_iterations);
+      counts.push_back(st.pcg_iterations);
+    }
+    const std::vector<double> u = step.get(SSIM_ARRAY_DEFORMATION);
+    // the setters reject what the reference rejects
+    bool threw = false;
+    try {
+      step.set_poisson(0.5);
+    } catch (const std::invalid_argument&) {
+      threw = true;
+    }
+    if (!threw) throw std::runtime_error("set_poisson(0.5) did not throw");
+    FILE* out = std::fopen(argv[2], "wb");
+    if (!out) throw std::runtime_error("cannot open output");
+    std::fwrite(u.data(), sizeof(double), u.size(), out);
+    std::fwrite(counts.data(), sizeof(int32_t), counts.size(), out);
+    std::fclose(out);
+    std::printf("driver: %d vertices, %d tets, %d steps done\n", nv, nt, steps);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "driver failed: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

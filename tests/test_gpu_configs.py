@@ -77,7 +77,7 @@ def test_config4_full_batch_agrees_with_single_mesh_path():
     assert worst <= 1e-8
 
 
-def _rollout_fd_and_oracle(rc, ndir, oracle_bars, h=1e-6):
+def _rollout_fd_and_oracle(rc, ndir, oracle_bars, h=1e-6, verbose=False):
     """Rollout gradient of the batch (per-tet log E, v₀) against central differences along ndir random
     directions for bar 0 (2·ndir perturbed copies stepped as one batch) and against the oracle's gradient for
     the first oracle_bars bars.  Returns (worst FD relative difference, worst oracle relative error)."""
@@ -92,15 +92,18 @@ def _rollout_fd_and_oracle(rc, ndir, oracle_bars, h=1e-6):
     def run(cases, log_e, v0, nu, grad):
         B = batch.TetFiniteElementBatch_Hip.from_cases(cases, rtol=1e-8)
         B.set_solver_params(**tight)
-        r = adjoint.Rollout(B, nu, rc.steps, dt)
+        prog = (lambda phase, k: print(f"[gpu]   {phase} step {k} ({len(cases)} meshes)", flush=True)) if verbose else None
+        r = adjoint.Rollout(B, nu, rc.steps, dt, progress=prog)
         le = torch.as_tensor(log_e, device="cuda").requires_grad_(grad)
         vv = torch.as_tensor(v0, device="cuda").requires_grad_(grad)
         u0 = torch.as_tensor(np.concatenate([c.u0 for c in cases]), device="cuda")
         uN = adjoint.rollout(r, le, vv, u0).view(len(cases), nv, 3)
+        run.last_stats = [B.last_stats(m) for m in range(min(2, len(cases)))]
         fe = torch.as_tensor(rc.free_end[0], device="cuda")
         return uN[:, fe, :], le, vv
 
     U, le, vv = run(rc.cases, rc.log_e, np.zeros(3 * nv * nb), rc.nu, True)
+    print(f"[gpu] rollout last-step stats (batch): {run.last_stats}", flush=True)
     tg = torch.as_tensor(rc.targets, device="cuda")[:, None, :]
     L = ((U - tg) ** 2).sum()
     L.backward()
@@ -118,7 +121,10 @@ def _rollout_fd_and_oracle(rc, ndir, oracle_bars, h=1e-6):
         Ls = ((U2 - tg[0]) ** 2).sum(dim=(1, 2)).cpu().numpy()
     fd = (Ls[0::2] - Ls[1::2]) / (2 * h)
     ad = D_le @ g_le[0] + D_v @ g_v0[0]
-    fd_worst = float(np.max(np.abs(fd - ad) / np.abs(ad)))
+    rel = np.abs(fd - ad) / np.abs(ad)
+    fd_worst = float(np.max(rel))
+    print(f"[gpu] FD directions: |ad| {np.array2string(np.abs(ad), precision=3, max_line_width=10**4)}, "
+          f"rel {np.array2string(rel, precision=2, max_line_width=10**4)}, loss {float(Ls.mean()):.6e}", flush=True)
 
     orc_worst = 0.0
     for b in range(oracle_bars):
@@ -151,9 +157,10 @@ def test_config5_full_rollout_gradient():
     """Configuration 5 at full size: 32 bars of 16×16×64 cubes, 20 steps; 64 finite-difference directions on bar 0.
     The oracle's gradient (CPU, minutes per step at this size) is checked on bar 0 over the first 2 steps."""
     rc = mesh.rollout_batch(32)
-    fd, _ = _rollout_fd_and_oracle(rc, ndir=64, oracle_bars=0)
+    fd, _ = _rollout_fd_and_oracle(rc, ndir=64, oracle_bars=0, verbose=True)
     short = mesh.rollout_batch(1, steps=2)
-    _, orc = _rollout_fd_and_oracle(short, ndir=2, oracle_bars=1)
+    print("[gpu] oracle gradient of bar 0 over 2 steps (CPU)", flush=True)
+    _, orc = _rollout_fd_and_oracle(short, ndir=2, oracle_bars=1, verbose=True)
     print(f"[gpu] configuration 5 full size: FD (64 directions, 20 steps) {fd:.2e}; oracle (bar 0, 2 steps) "
           f"{orc:.2e}", flush=True)
     assert fd <= 1e-5 and orc <= 1e-6

@@ -444,9 +444,10 @@ def test_adjoint_step_solves_the_linearised_step():
         ts2.compute_step()
         sol.append(ts2.get_vectors()[0])
     lv2, guhat2, _, _ = ts2.adjoint_step(u, gu)
-    # exact identity: gu·H⁻¹(M δû) = (M l)·δû (H symmetric), against the tangent solve
+    # identity gu·H⁻¹(M δû) = (M l)·δû (H symmetric), against the tangent solve (two sparse solves: the bar is
+    # their roundoff times the condition number of H)
     du_lin = scipy.sparse.linalg.spsolve(H.tocsc(), np.where(free, ts2.mass_csr() @ d_uhat, 0.0))
-    assert abs(gu @ du_lin - guhat2 @ d_uhat) <= 1e-12 * abs(gu @ du_lin)
+    assert abs(gu @ du_lin - guhat2 @ d_uhat) <= 1e-10 * abs(gu @ du_lin)
     # and the tangent is the step's: central differences of the converged solution (a random functional
     # has cancellation, so the differences resolve it to a few 1e-6)
     fd_du = (sol[0] - sol[1]) / (2 * h)
