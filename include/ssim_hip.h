@@ -210,9 +210,14 @@ ssim_status_t ssim_get_layout_info(const ssim_ctx* ctx, ssim_layout_info* info);
 typedef enum {
   SSIM_TUNE_XCD_REMAP = 0,     /* 1 (default): each of the 8 XCDs takes a contiguous range of SpMV row slices */
   SSIM_TUNE_PCG_CHUNK_MAX = 1, /* PCG iterations enqueued per done-flag poll (default 64) */
-  SSIM_TUNE_SPMV_LANES = 2     /* SpMV variant: 0 = one lane per row, fully unrolled over a layout padded to 8
+  SSIM_TUNE_SPMV_LANES = 2,    /* SpMV variant: 0 = one lane per row, fully unrolled over a layout padded to 8
                                   upper / 7 lower blocks (default when every row fits), 1 = one lane per row with
                                   loops (default otherwise), 4 = four lanes per row */
+  SSIM_TUNE_PCG_GRAPH = 3,     /* 1: replay PCG chunks from captured hipGraphs; 0 (default): launch each kernel
+                                  (measured the same speed for one 786k-tet mesh and for 256 batched meshes: the gap
+                                  between dependent kernels is not the host's launch cost) */
+  SSIM_TUNE_COMPACT_PCT = 4    /* batches: compact the PCG grids to the meshes still iterating once those hold at
+                                  most this percentage of the current grid (default 70; 0 disables) */
 } ssim_tune_t;
 ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value);
 ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value);

@@ -148,13 +148,13 @@ class TetFiniteElementBatch_Hip:
     # ---- launch tuning and instrumentation ----
     def set_tuning(self, **kw):
         for k, v in kw.items():
-            if k not in _lib.TUNING:
+            if k not in _lib.BATCH_TUNING:
                 raise ValueError(f"unknown tuning key {k}")
-            _lib.call("ssim_batch_set_tuning", self._h, _lib.TUNING[k], int(v))
+            _lib.call("ssim_batch_set_tuning", self._h, _lib.BATCH_TUNING[k], int(v))
 
     def tuning(self) -> dict:
         out = {}
-        for k, key in _lib.TUNING.items():
+        for k, key in _lib.BATCH_TUNING.items():
             v = ctypes.c_int()
             _lib.call("ssim_batch_get_tuning", self._h, key, ctypes.byref(v))
             out[k] = v.value

@@ -36,9 +36,31 @@ struct ssim_batch {
   ssim_step_stats total{};             // batch: lock-step iteration counts
   unsigned char* pin = nullptr;        // pinned staging for per-mesh scalars
   size_t pin_bytes = 0;
+  // compacted PCG grids: slice and vertex-workgroup lists of the meshes still iterating, double-buffered so that
+  // a list is rebuilt while the chunk enqueued ahead of the poll still reads the other one
+  DBuf<int> slist[2], wlist[2];
+  int* pin_list[2] = {nullptr, nullptr};  // pinned staging: slices, then vertex workgroups
+  int ns_list[2] = {0, 0}, nw_list[2] = {0, 0};
+  int compact_pct = 70;  // rebuild when the active slices fall to this percentage of the current list
+  // captured chunks of batched PCG iterations, keyed by everything their launches embed
+  struct Graph {
+    int count, buf, ns, nw;
+    const double* x;
+    const int* mask;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  int graph_mode = 0;  // replay chunks from graphs (measured: the same speed as plain launches)
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
 
   ~ssim_batch() {
+    drop_graphs();
     if (pin) (void)hipHostFree(pin);
+    for (int* q : pin_list)
+      if (q) (void)hipHostFree(q);
   }
   BatchView view(const DBuf<int>& mask) const {
     BatchView b;
@@ -80,6 +102,7 @@ void upload_mask(ssim_batch* b, DBuf<int>& dst, const std::vector<int>& h) {
 // per-mesh masses and thresholds; padding vertices Dirichlet-fixed (after do_reset)
 void batch_reset(ssim_batch* b) {
   ssim_ctx* c = &b->c;
+  b->drop_graphs();
   do_reset(c);
   std::vector<double> hV(c->nt);
   HIPCHK(hipMemcpyAsync(hV.data(), c->V.p, sizeof(double) * c->nt, hipMemcpyDeviceToHost, c->stream));
@@ -141,6 +164,27 @@ void bassemble(ssim_batch* b, const DBuf<int>& mask, const double* u, int make_s
   HIPCHK(hipGetLastError());
 }
 
+// Writes the slice / vertex-workgroup lists of the meshes with active[m] into list buffer `buf` (copies on the
+// stream, from pinned staging that is not rewritten before the copy has run: the alternate buffer's staging is
+// only reused after a later poll).
+void build_lists(ssim_batch* b, const std::vector<int>& active, int buf) {
+  ssim_ctx* c = &b->c;
+  int* st = b->pin_list[buf];
+  int ns = 0, nw = 0;
+  for (int m = 0; m < b->nmesh; ++m)
+    if (active[m])
+      for (int w = b->hvwg_off[m]; w < b->hvwg_off[m + 1]; ++w)
+        for (int q = 0; q < kBatchSlices; ++q) st[ns++] = kBatchSlices * w + q;
+  int* wst = st + c->nslices;
+  for (int m = 0; m < b->nmesh; ++m)
+    if (active[m])
+      for (int w = b->hvwg_off[m]; w < b->hvwg_off[m + 1]; ++w) wst[nw++] = w;
+  if (ns) HIPCHK(hipMemcpyAsync(b->slist[buf].p, st, sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
+  if (nw) HIPCHK(hipMemcpyAsync(b->wlist[buf].p, wst, sizeof(int) * nw, hipMemcpyHostToDevice, c->stream));
+  b->ns_list[buf] = ns;
+  b->nw_list[buf] = nw;
+}
+
 // batched Jacobi-PCG of H x = sign·rhs for the masked meshes; per-mesh iteration counts into iters
 void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, double* x, std::vector<int>& iters) {
   ssim_ctx* c = &b->c;
@@ -153,28 +197,72 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
   HIPCHK(hipGetLastError());
   for (auto& e : c->poll_ev)
     if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  auto enqueue = [&](int count, int slot) {
+  // the grids start with every slice of the masked meshes
+  std::vector<int> active(b->nmesh);
+  HIPCHK(hipMemcpyAsync(active.data(), mask.p, sizeof(int) * b->nmesh, hipMemcpyDeviceToHost, s));
+  sync(c);
+  int cur = 0;
+  build_lists(b, active, cur);
+  auto launch_iters = [&](int count, const int* sl, int ns, const int* wl, int nw, hipStream_t st) {
     for (int j = 0; j < count; ++j) {
-      auto spmv = [&] {
-        launch_bpcg_spmv(A, bt, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), b->pcg.p, b->iter.p,
-                         c->part_pa(), s);
-      };
-      auto update = [&] {
-        launch_bpcg_update(bt, b->nvwg, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), c->part_pb(0),
-                           c->part_pb(1), b->pcg.p, b->iter.p, s);
-      };
-      if (j == 0) {
-        timed(c, SSIM_KERNEL_PCG_SPMV, spmv);
-        timed(c, SSIM_KERNEL_PCG_UPDATE, update);
-      } else {
-        spmv();
-        update();
-      }
+      launch_bpcg_spmv(A, bt, sl, ns, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), b->pcg.p, b->iter.p,
+                       c->part_pa(), st);
+      launch_bpcg_update(bt, wl, nw, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), c->part_pb(0),
+                         c->part_pb(1), b->pcg.p, b->iter.p, st);
+    }
+  };
+  // a chunk of `count` iterations over list buffer `buf` as a captured graph (captured on first use; lists are
+  // rewritten in place, so a graph stays valid while its list length is unchanged)
+  auto chunk_graph = [&](int count, int buf) -> hipGraphExec_t {
+    const int ns = b->ns_list[buf], nw = b->nw_list[buf];
+    for (const auto& g : b->graphs)
+      if (g.count == count && g.buf == buf && g.ns == ns && g.nw == nw && g.x == x && g.mask == mask.p) return g.exec;
+    if (b->graphs.size() >= 24) b->drop_graphs();
+    if (!c->cap_stream) HIPCHK(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+      launch_iters(count, b->slist[buf].p, ns, b->wlist[buf].p, nw, c->cap_stream);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->cap_stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(c->cap_stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) fail(SSIM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    b->graphs.push_back({count, buf, ns, nw, x, mask.p, exec});
+    return exec;
+  };
+  auto enqueue = [&](int count, int slot) {
+    const int* sl = b->slist[cur].p;
+    const int* wl = b->wlist[cur].p;
+    const int ns = b->ns_list[cur], nw = b->nw_list[cur];
+    // the chunk's first iteration is launched on its own (it carries the sampled kernel timers)
+    timed(c, SSIM_KERNEL_PCG_SPMV, [&] {
+      launch_bpcg_spmv(A, bt, sl, ns, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), b->pcg.p, b->iter.p,
+                       c->part_pa(), s);
+    });
+    timed(c, SSIM_KERNEL_PCG_UPDATE, [&] {
+      launch_bpcg_update(bt, wl, nw, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), c->part_pb(0),
+                         c->part_pb(1), b->pcg.p, b->iter.p, s);
+    });
+    if (count > 1) {
+      if (b->graph_mode)
+        HIPCHK(hipGraphLaunch(chunk_graph(count - 1, cur), s));
+      else
+        launch_iters(count - 1, sl, ns, wl, nw, s);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(b->pin_pcg(slot), b->pcg.p, sizeof(PcgScalars) * b->nmesh, hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(c->poll_ev[slot], s));
   };
+  if (b->ns_list[cur] == 0) {  // nothing to solve: every masked-out mesh was marked done by the init
+    for (int m = 0; m < b->nmesh; ++m) iters[m] = 0;
+    return;
+  }
   int chunk = 8, launched = 0, slot = 0;
   enqueue(chunk, slot);
   launched += chunk;
@@ -185,9 +273,12 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
     HIPCHK(hipEventSynchronize(c->poll_ev[slot]));
     if (c->timing) harvest_timers(c);
     const PcgScalars* f = b->pin_pcg(slot);
-    bool all = true;
-    for (int m = 0; m < b->nmesh && all; ++m) all = f[m].done != 0;
-    if (all) {
+    int left_slices = 0;
+    for (int m = 0; m < b->nmesh; ++m) {
+      active[m] = f[m].done ? 0 : 1;
+      if (active[m]) left_slices += kBatchSlices * (b->hvwg_off[m + 1] - b->hvwg_off[m]);
+    }
+    if (left_slices == 0) {
       for (int m = 0; m < b->nmesh; ++m) {
         if (f[m].breakdown == 1)
           fail(SSIM_ERR_NUMERIC, "mesh " + std::to_string(m) + ": PCG breakdown: pᵀAp ≤ 0 (matrix not positive definite)");
@@ -197,6 +288,11 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
       return;
     }
     if (launched > maxit + 2 * std::max(8, c->pcg_chunk_max) + 16) fail(SSIM_ERR_STATE, "internal: PCG did not stop");
+    // compact the grids once enough meshes have finished (the chunk already enqueued keeps the old list)
+    if (100LL * left_slices <= (long long)b->compact_pct * b->ns_list[cur]) {
+      cur ^= 1;
+      build_lists(b, active, cur);
+    }
     chunk = next;
     slot ^= 1;
   }
@@ -424,6 +520,12 @@ ssim_status_t ssim_batch_create(ssim_batch** out, int nmesh, const int* nv, cons
     b->mask_ls.upload(ones, s);
     b->stats.assign(nmesh, ssim_step_stats{});
     batch_reset(b);
+    for (int q = 0; q < 2; ++q) {
+      b->slist[q].alloc(std::max(c->nslices, 1));
+      b->wlist[q].alloc(std::max(b->nvwg, 1));
+      HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->pin_list[q]),
+                           sizeof(int) * ((size_t)c->nslices + b->nvwg + 1), hipHostMallocDefault));
+    }
   });
   if (st != SSIM_OK) {
     delete b;
@@ -709,11 +811,29 @@ ssim_status_t ssim_batch_set_tuning(ssim_batch* b, ssim_tune_t key, int value) {
     g_err = "batched PCG runs one row per lane (SpMV variant 0 or 1)";
     return SSIM_ERR_INVALID_ARGUMENT;
   }
+  if (key == SSIM_TUNE_COMPACT_PCT) {
+    if (value < 0 || value > 100) {
+      g_err = "compaction threshold must be a percentage in [0, 100]";
+      return SSIM_ERR_INVALID_ARGUMENT;
+    }
+    b->compact_pct = value;
+    return SSIM_OK;
+  }
+  if (key == SSIM_TUNE_PCG_GRAPH) {
+    b->graph_mode = value != 0;
+    return SSIM_OK;
+  }
+  b->drop_graphs();  // launch shapes may change
   return ssim_set_tuning(&b->c, key, value);
 }
 
 ssim_status_t ssim_batch_get_tuning(const ssim_batch* b, ssim_tune_t key, int* value) {
   if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  if (key == SSIM_TUNE_COMPACT_PCT || key == SSIM_TUNE_PCG_GRAPH) {
+    if (!value) return SSIM_ERR_INVALID_ARGUMENT;
+    *value = key == SSIM_TUNE_COMPACT_PCT ? b->compact_pct : b->graph_mode;
+    return SSIM_OK;
+  }
   return ssim_get_tuning(&b->c, key, value);
 }
 

@@ -20,6 +20,11 @@ __device__ __forceinline__ size_t bdof(int j) {
   return (size_t)blockIdx.x * (kBVec * kBatchRows) + (size_t)j * kBatchRows + threadIdx.x;
 }
 
+// the same for an explicit vertex workgroup w (compacted grids)
+__device__ __forceinline__ size_t bdof_w(int w, int j) {
+  return (size_t)w * (kBVec * kBatchRows) + (size_t)j * kBatchRows + threadIdx.x;
+}
+
 // ------------------------------------------------------------------------------------------------------
 // energy + gradient
 
@@ -242,16 +247,18 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_init_fin(BatchView bt, const do
 }
 
 // kernel A: convergence test on the mesh's ‖r_k‖² (a few dozen partials, so it goes first and a converged
-// mesh skips its SpMV), β, s = A z_k on one 64-row slice, p_k, q_k, p_k·q_k
+// mesh skips its SpMV), β, s = A z_k on one 64-row slice, p_k, q_k, p_k·q_k.  The grid covers the slices in
+// slist[0..nlist) (all slices of the meshes still iterating when the host last compacted the list).
 template <int LPR>
 __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_bpcg_spmv(
-    BellView A, BatchView bt, const double* __restrict__ z, double* __restrict__ p, double* __restrict__ q,
-    const double* __restrict__ pb0, const double* __restrict__ pb1, PcgScalars* sc, PcgIter* it,
-    double* __restrict__ pa) {
+    BellView A, BatchView bt, const int* __restrict__ slist, int nlist, const double* __restrict__ z,
+    double* __restrict__ p, double* __restrict__ q, const double* __restrict__ pb0, const double* __restrict__ pb1,
+    PcgScalars* sc, PcgIter* it, double* __restrict__ pa) {
   const int k = it->ka;
   if (blockIdx.x == 0 && threadIdx.x == 0) it->kb = k;
-  const int lb = logical_block(A.nslices, A.xcd_remap);
-  if (lb < 0) return;
+  const int li = logical_block(nlist, A.xcd_remap);
+  if (li < 0) return;
+  const int lb = slist[li];
   const int vw = lb / kBatchSlices;
   const int mesh = bt.vwg_mesh[vw];
   PcgScalars* S = sc + mesh;
@@ -297,22 +304,24 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
   store_partials<1, kSpmvBlock>(acc, pa, lb);
 }
 
-// kernel B: α = ρ_k / (p_k·q_k); x += α p, r −= α q, z = D⁻¹ r; partials of (ρ_{k+1}, ‖r_{k+1}‖²)
-__global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const double* __restrict__ dinv,
-                                                        double* __restrict__ x, double* __restrict__ r,
-                                                        double* __restrict__ z, const double* __restrict__ p,
-                                                        const double* __restrict__ q, const double* __restrict__ pa,
-                                                        double* __restrict__ pb0, double* __restrict__ pb1,
-                                                        PcgScalars* sc, PcgIter* it) {
+// kernel B: α = ρ_k / (p_k·q_k); x += α p, r −= α q, z = D⁻¹ r; partials of (ρ_{k+1}, ‖r_{k+1}‖²).  Workgroup
+// i handles vertex workgroup wlist[i].
+__global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const int* __restrict__ wlist,
+                                                        const double* __restrict__ dinv, double* __restrict__ x,
+                                                        double* __restrict__ r, double* __restrict__ z,
+                                                        const double* __restrict__ p, const double* __restrict__ q,
+                                                        const double* __restrict__ pa, double* __restrict__ pb0,
+                                                        double* __restrict__ pb1, PcgScalars* sc, PcgIter* it) {
   const int k = it->kb;
   if (blockIdx.x == 0 && threadIdx.x == 0) it->ka = k + 1;
-  const int mesh = bt.vwg_mesh[blockIdx.x];
+  const int w = wlist[blockIdx.x];
+  const int mesh = bt.vwg_mesh[w];
   PcgScalars* S = sc + mesh;
   if (S->done) return;
   double xv[kBVec], rv[kBVec], pv[kBVec], qv[kBVec], dv[kBVec];
 #pragma unroll
   for (int j = 0; j < kBVec; ++j) {
-    const size_t i = bdof(j);
+    const size_t i = bdof_w(w, j);
     xv[j] = x[i];
     rv[j] = r[i];
     pv[j] = p[i];
@@ -326,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const doub
   sum_partials<1, kBlock, 1>(pa + (size_t)v0 * kBatchSlices, nw * kBatchSlices, pq);
   sum_partials<2, kBlock, 1>(pb_cur + 2 * (size_t)v0, nw, rz);
   if (!(pq[0] > 0.0)) {
-    if (blockIdx.x == v0 && threadIdx.x == 0) {
+    if (w == v0 && threadIdx.x == 0) {
       S->breakdown = 1;
       S->done = 1;
     }
@@ -336,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const doub
   double v[2] = {0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < kBVec; ++j) {
-    const size_t i = bdof(j);
+    const size_t i = bdof_w(w, j);
     x[i] = xv[j] + alpha * pv[j];
     const double ri = rv[j] - alpha * qv[j];
     const double zi = dv[j] * ri;
@@ -345,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const doub
     v[0] += ri * zi;
     v[1] += ri * ri;
   }
-  store_partials<2>(v, pb_next);
+  store_partials<2>(v, pb_next, w);
 }
 
 }  // namespace
@@ -441,21 +450,22 @@ void launch_bpcg_init(const BatchView& bt, int nvwg, const double* b, double sig
   if (bt.nmesh > 0) k_bpcg_init_fin<<<bt.nmesh, kBlock, 0, s>>>(bt, pb0, rtol, maxit, sc, it);
 }
 
-void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const double* z, double* p, double* q,
-                      const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa, hipStream_t s) {
-  if (A.nslices <= 0) return;
-  const int grid = grid_for(A.nslices, A.xcd_remap);
+void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const int* slist, int nlist, const double* z, double* p,
+                      double* q, const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa,
+                      hipStream_t s) {
+  if (nlist <= 0) return;
+  const int grid = grid_for(nlist, A.xcd_remap);
   if (A.lanes_per_row == 0)
-    k_bpcg_spmv<0><<<grid, kSpmvBlock, 0, s>>>(A, bt, z, p, q, pb0, pb1, sc, it, pa);
+    k_bpcg_spmv<0><<<grid, kSpmvBlock, 0, s>>>(A, bt, slist, nlist, z, p, q, pb0, pb1, sc, it, pa);
   else
-    k_bpcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, bt, z, p, q, pb0, pb1, sc, it, pa);
+    k_bpcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, bt, slist, nlist, z, p, q, pb0, pb1, sc, it, pa);
 }
 
-void launch_bpcg_update(const BatchView& bt, int nvwg, const double* dinv, double* x, double* r, double* z,
-                        const double* p, const double* q, const double* pa, double* pb0, double* pb1,
+void launch_bpcg_update(const BatchView& bt, const int* wlist, int nwl, const double* dinv, double* x, double* r,
+                        double* z, const double* p, const double* q, const double* pa, double* pb0, double* pb1,
                         PcgScalars* sc, PcgIter* it, hipStream_t s) {
-  if (nvwg <= 0) return;
-  k_bpcg_update<<<nvwg, kBlock, 0, s>>>(bt, dinv, x, r, z, p, q, pa, pb0, pb1, sc, it);
+  if (nwl <= 0) return;
+  k_bpcg_update<<<nwl, kBlock, 0, s>>>(bt, wlist, dinv, x, r, z, p, q, pa, pb0, pb1, sc, it);
 }
 
 }  // namespace ssim_hip

@@ -102,6 +102,17 @@ struct ssim_ctx {
   ssim_hip::DBuf<double> adj_lam, adj_mu;  // per-tet material gradients when the caller passes no buffer
   // device: scalars
   ssim_hip::DBuf<ssim_hip::PcgScalars> pcg;
+  ssim_hip::DBuf<ssim_hip::PcgIter> pcg_it;
+  // captured PCG chunks: `count` iterations of kernels A and B for one output vector x, replayed by one graph
+  // launch (the iteration counter is on the device, so every chunk of a given length is the same graph)
+  struct PcgGraph {
+    int count;
+    const double* x;
+    hipGraphExec_t exec;
+  };
+  std::vector<PcgGraph> pcg_graphs;
+  hipStream_t cap_stream = nullptr;  // private stream the chunks are captured on
+  int pcg_graph = 0;                 // replay PCG chunks from captured graphs
   ssim_hip::DBuf<ssim_hip::EvalScalars> ev;
   ssim_hip::DBuf<double> partials, lb_S, lb_Y, lb_scal;
   size_t pmax = 0;  // partials per slot
@@ -123,7 +134,13 @@ struct ssim_ctx {
   bool timing = false;
   KTimer kt[SSIM_KERNEL_COUNT];
 
+  void drop_graphs() {
+    for (auto& g : pcg_graphs) (void)hipGraphExecDestroy(g.exec);
+    pcg_graphs.clear();
+  }
   ~ssim_ctx() {
+    drop_graphs();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     if (pin) (void)hipHostFree(pin);
     for (auto& e : poll_ev)
       if (e) (void)hipEventDestroy(e);

@@ -297,9 +297,12 @@ void ssim_hip::do_reset(ssim_ctx* c) {
   // multiple of 8 (XCD remapping)
   c->pmax = std::max<size_t>((size_t)max_partials(std::max({c->nt, c->n, c->nub, 1})), 4 * (size_t)c->nslices) + 8;
   c->partials.alloc(8 * c->pmax);
+  c->drop_graphs();  // buffers may have moved
   c->pcg.alloc(1);
+  c->pcg_it.alloc(1);
   c->ev.alloc(1);
   HIPCHK(hipMemsetAsync(c->pcg.p, 0, sizeof(PcgScalars), s));
+  HIPCHK(hipMemsetAsync(c->pcg_it.p, 0, sizeof(PcgIter), s));
   HIPCHK(hipMemsetAsync(c->ev.p, 0, sizeof(EvalScalars), s));
   // total mass Σ ρV (= asum(M.values)/3, time_step.hpp:490)
   std::vector<double> hV(c->nt);
@@ -365,48 +368,79 @@ void ssim_hip::assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int
 
 namespace {
 
+// Enqueues `count` PCG iterations (kernels A and B) on stream s.
+void enqueue_pcg_iterations(ssim_ctx* c, const BellView& A, double* x, int nb, int na, int count, hipStream_t s) {
+  for (int j = 0; j < count; ++j) {
+    launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
+                    c->part_pa(), s);
+    launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(0),
+                      c->part_pb(1), c->pcg_it.p, c->pcg.p, s);
+  }
+}
+
+// The graph of `count` iterations writing x, captured on first use (buffers are fixed until the next reset,
+// which drops the cache).
+hipGraphExec_t pcg_chunk_graph(ssim_ctx* c, const BellView& A, double* x, int nb, int na, int count) {
+  for (const auto& g : c->pcg_graphs)
+    if (g.count == count && g.x == x) return g.exec;
+  if (!c->cap_stream) HIPCHK(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    enqueue_pcg_iterations(c, A, x, nb, na, count, c->cap_stream);
+  } catch (...) {
+    (void)hipStreamEndCapture(c->cap_stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(c->cap_stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) fail(SSIM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  c->pcg_graphs.push_back({count, x, exec});
+  return exec;
+}
+
 // Jacobi-PCG on the current matrix: x = H⁻¹ (sign·b).  Returns iterations.
 int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit) {
   hipStream_t s = c->stream;
   const BellView A = c->bell(c->val.p);
-  const int nb = launch_pcg_init(c->n, b, sign, c->dinv.p, x, c->r.p, c->z.p, rtol, maxit, c->pcg.p,
+  const int nb = launch_pcg_init(c->n, b, sign, c->dinv.p, x, c->r.p, c->z.p, rtol, maxit, c->pcg.p, c->pcg_it.p,
                                  c->part_pb(0), s);
   HIPCHK(hipGetLastError());
   for (auto& e : c->poll_ev)
     if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const int na = pcg_spmv_grid(A);
   // Chunks of iterations are enqueued one ahead of the done-flag poll, so the GPU never drains while the host
-  // reads a flag; kernels enqueued after convergence return at entry.
+  // reads a flag; kernels enqueued after convergence return at entry.  The first iteration of a chunk is launched
+  // on its own (it carries the sampled kernel timers); the others are replayed from a captured graph.
   int* flags = reinterpret_cast<int*>(c->pin);  // two slots of 4 ints: done, iters, maxit, breakdown
-  auto enqueue = [&](int first, int count, int slot) {
-    for (int j = 0; j < count; ++j) {
-      const int k = first + j;
-      int na = 0;
-      auto spmv = [&] {
-        na = launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(k), c->part_pb(k + 1), nb, k, c->pcg.p,
-                             c->part_pa(), s);
-      };
-      auto update = [&] {
-        launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(k),
-                          c->part_pb(k + 1), c->pcg.p, s);
-      };
-      if (j == 0) {  // the first iteration of a chunk does work unless the solve ended a chunk earlier
-        timed(c, SSIM_KERNEL_PCG_SPMV, spmv);
-        timed(c, SSIM_KERNEL_PCG_UPDATE, update);
-      } else {
-        spmv();
-        update();
-      }
+  auto enqueue = [&](int count, int slot) {
+    timed(c, SSIM_KERNEL_PCG_SPMV, [&] {
+      launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
+                      c->part_pa(), s);
+    });
+    timed(c, SSIM_KERNEL_PCG_UPDATE, [&] {
+      launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(0),
+                        c->part_pb(1), c->pcg_it.p, c->pcg.p, s);
+    });
+    if (count > 1) {
+      if (c->pcg_graph)
+        HIPCHK(hipGraphLaunch(pcg_chunk_graph(c, A, x, nb, na, count - 1), s));
+      else
+        enqueue_pcg_iterations(c, A, x, nb, na, count - 1, s);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(flags + 4 * slot, &c->pcg.p->done, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(c->poll_ev[slot], s));
   };
   int chunk = 8, launched = 0, slot = 0;
-  enqueue(launched, chunk, slot);
+  enqueue(chunk, slot);
   launched += chunk;
   for (;;) {
     const int next = std::min(chunk * 2, std::max(8, c->pcg_chunk_max));
-    enqueue(launched, next, slot ^ 1);
+    enqueue(next, slot ^ 1);
     launched += next;
     HIPCHK(hipEventSynchronize(c->poll_ev[slot]));
     if (c->timing) harvest_timers(c);
@@ -1162,6 +1196,7 @@ ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value) {
     switch (key) {
       case SSIM_TUNE_XCD_REMAP:
         ctx->xcd_remap = value != 0;
+        ctx->drop_graphs();
         break;
       case SSIM_TUNE_PCG_CHUNK_MAX:
         if (value < 1 || value > 4096) fail(SSIM_ERR_INVALID_ARGUMENT, "PCG chunk must be in [1, 4096]");
@@ -1173,6 +1208,10 @@ ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value) {
           fail(SSIM_ERR_INVALID_ARGUMENT, "the fixed-width SpMV needs at most 8 upper and 7 lower blocks per row");
         ctx->spmv_lanes = value;
         ctx->spmv_lanes_default = false;
+        ctx->drop_graphs();
+        break;
+      case SSIM_TUNE_PCG_GRAPH:
+        ctx->pcg_graph = value != 0;
         break;
       default:
         fail(SSIM_ERR_INVALID_ARGUMENT, "unknown tuning key");
@@ -1194,6 +1233,9 @@ ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value) 
       return SSIM_OK;
     case SSIM_TUNE_SPMV_LANES:
       *value = ctx->spmv_lanes;
+      return SSIM_OK;
+    case SSIM_TUNE_PCG_GRAPH:
+      *value = ctx->pcg_graph;
       return SSIM_OK;
     default:
       g_err = "unknown tuning key";

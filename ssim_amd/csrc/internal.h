@@ -111,16 +111,19 @@ void launch_restore(int n, const unsigned char* dof, const double* dbc, double* 
                     double dt, hipStream_t s);
 void launch_axpy_to(int n, const double* x, double alpha, const double* d, double* out, hipStream_t s);
 
-// PCG on H x = sign·b with Jacobi preconditioner dinv.  init writes (ρ0, ‖r0‖²) partials to pb[0] and
-// returns their count.  Iteration k: kernel A (s = A z_k; convergence test on pb[k%2]; p_k = z_k + β p_{k−1},
-// q_k = s + β q_{k−1} in place; partials of p_k·q_k to pa), kernel B (reads pa and pb[k%2], writes pb[(k+1)%2]).
+// PCG on H x = sign·b with Jacobi preconditioner dinv.  init writes (ρ0, ‖r0‖²) partials to pb0, resets the
+// device iteration counter and returns the partial count.  Iteration k (k from the counter): kernel A (s = A z_k;
+// convergence test on pb[k%2]; p_k = z_k + β p_{k−1}, q_k = s + β q_{k−1} in place; partials of p_k·q_k to pa),
+// kernel B (reads pa and pb[k%2], writes pb[(k+1)%2]).  Every iteration's launches are identical.
 int launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
-                    double rtol, int maxit, PcgScalars* sc, double* pb0, hipStream_t s);
-int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb_cur,
-                    const double* pb_prev, int nb, int k, PcgScalars* sc, double* pa, hipStream_t s);
+                    double rtol, int maxit, PcgScalars* sc, PcgIter* it, double* pb0, hipStream_t s);
+int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb0, const double* pb1,
+                    int nb, PcgIter* it, PcgScalars* sc, double* pa, hipStream_t s);
 void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
-                       const double* pa, int na, const double* pb_cur, double* pb_next, PcgScalars* sc, hipStream_t s);
+                       const double* pa, int na, double* pb0, double* pb1, PcgIter* it, PcgScalars* sc,
+                       hipStream_t s);
 int pcg_vector_blocks(int n);
+int pcg_spmv_grid(const BellView& A);  // workgroups (= p·q partials) of one kernel-A launch
 
 // adjoint of one step: mass contributions of the adjoint vector lv (reduce with launch_grad_reduce to get M·lv,
 // zero on Dirichlet dofs) and per-tet ∂L/∂λ, ∂L/∂μ at deformation u
@@ -175,10 +178,13 @@ void launch_bcopy2(const BatchView& bt, int nvwg, const double* src1, double* ds
 void launch_bpcg_init(const BatchView& bt, int nvwg, const double* b, double sign, const double* dinv, double* x,
                       double* r, double* z, double rtol, int maxit, PcgScalars* sc, PcgIter* it, double* pb0,
                       hipStream_t s);
-void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const double* z, double* p, double* q,
-                      const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa, hipStream_t s);
-void launch_bpcg_update(const BatchView& bt, int nvwg, const double* dinv, double* x, double* r, double* z,
-                        const double* p, const double* q, const double* pa, double* pb0, double* pb1,
+// iteration kernels over compacted grids: slist[0..nlist) = the 64-row slices, wlist[0..nwl) = the vertex
+// workgroups of the meshes still iterating (a superset is correct: a done mesh's workgroups return at entry)
+void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const int* slist, int nlist, const double* z, double* p,
+                      double* q, const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa,
+                      hipStream_t s);
+void launch_bpcg_update(const BatchView& bt, const int* wlist, int nwl, const double* dinv, double* x, double* r,
+                        double* z, const double* p, const double* q, const double* pa, double* pb0, double* pb1,
                         PcgScalars* sc, PcgIter* it, hipStream_t s);
 
 }  // namespace ssim_hip

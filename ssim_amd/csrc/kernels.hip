@@ -411,10 +411,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const double* __rest
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_init_fin(const double* __restrict__ pb, int nb, double rtol,
-                                                         int maxit, PcgScalars* sc) {
+                                                         int maxit, PcgScalars* sc, PcgIter* it) {
   double t[2];
   sum_partials<2>(pb, nb, t);
   if (threadIdx.x == 0) {
+    it->ka = 0;
+    it->kb = 0;
     sc->rz = t[0];
     sc->bb = t[1];
     sc->rr = t[1];
@@ -427,26 +429,31 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fin(const double* __restric
 }
 
 int launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
-                    double rtol, int maxit, PcgScalars* sc, double* pb, hipStream_t s) {
+                    double rtol, int maxit, PcgScalars* sc, PcgIter* it, double* pb, hipStream_t s) {
   const int nb = vblocks(n);
   k_pcg_init<<<nb, kBlock, 0, s>>>(n, b, sign, dinv, x, r, z, pb);
-  k_pcg_init_fin<<<1, kBlock, 0, s>>>(pb, nb, rtol, maxit, sc);
+  k_pcg_init_fin<<<1, kBlock, 0, s>>>(pb, nb, rtol, maxit, sc, it);
   return nb;
 }
 
 // Iteration k, kernel A: s = A z_k, then (after the loop, so the partial-sum latency overlaps other
 // workgroups' SpMV work) the convergence test on ‖r_k‖² from the previous update's partials, β = ρ_k/ρ_{k−1},
 // p_k = z_k + β p_{k−1} and q_k = A p_k = s + β q_{k−1} for the lane's own row (in place), and the partials of
-// p_k·q_k.  For k = 0: p_0 = z_0, q_0 = s.
+// p_k·q_k.  For k = 0: p_0 = z_0, q_0 = s.  k is read from the device counter (kernel A reads ka and publishes
+// kb = ka, kernel B reads kb and publishes ka = kb + 1), so every iteration's launch is identical and a chunk of
+// iterations can be replayed from a captured graph; the (r·z, r·r) partials of residual k live in pb[k % 2].
 // At 786k tets the SpMV has about 2.2 waves per SIMD: a register budget for 3 waves keeps the whole grid in
 // one dispatch round while leaving the compiler room to keep many of a row's loads in flight.
 template <int LPR>
 __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_pcg_spmv(
-    BellView A, const double* __restrict__ z, double* __restrict__ p,
-                                                     double* __restrict__ q, const double* __restrict__ pb_cur,
-                                                     const double* __restrict__ pb_prev, int nb, int k,
-                                                     PcgScalars* sc, double* __restrict__ pa) {
+    BellView A, const double* __restrict__ z, double* __restrict__ p, double* __restrict__ q,
+    const double* __restrict__ pb0, const double* __restrict__ pb1, int nb, PcgIter* it, PcgScalars* sc,
+    double* __restrict__ pa) {
+  const int k = it->ka;
+  if (blockIdx.x == 0 && threadIdx.x == 0) it->kb = k;
   if (sc->done) return;
+  const double* __restrict__ pb_cur = (k & 1) ? pb1 : pb0;
+  const double* __restrict__ pb_prev = (k & 1) ? pb0 : pb1;
   const int lb = logical_block(RowMap<LPR>::blocks(A.nrows), A.xcd_remap);
   int v = 0, j = 0;
   if (lb >= 0) RowMap<LPR>::map(lb, v, j);
@@ -496,10 +503,14 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __restrict__ dinv, double* __restrict__ x,
                                                        double* __restrict__ r, double* __restrict__ z,
                                                        const double* __restrict__ p, const double* __restrict__ q,
-                                                       const double* __restrict__ pa, int na,
-                                                       const double* __restrict__ pb_cur, double* __restrict__ pb_next,
-                                                       int nb, PcgScalars* sc) {
+                                                       const double* __restrict__ pa, int na, double* __restrict__ pb0,
+                                                       double* __restrict__ pb1, int nb, PcgIter* it,
+                                                       PcgScalars* sc) {
+  const int k = it->kb;
+  if (blockIdx.x == 0 && threadIdx.x == 0) it->ka = k + 1;
   if (sc->done) return;
+  const double* __restrict__ pb_cur = (k & 1) ? pb1 : pb0;
+  double* __restrict__ pb_next = (k & 1) ? pb0 : pb1;
   double xv[kVec], rv[kVec], pv[kVec], qv[kVec], dv[kVec];
 #pragma unroll
   for (int j = 0; j < kVec; ++j) {
@@ -540,30 +551,38 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
   store_partials<2>(v, pb_next);
 }
 
-int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb_cur,
-                    const double* pb_prev, int nb, int k, PcgScalars* sc, double* pa, hipStream_t s) {
+int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb0, const double* pb1,
+                    int nb, PcgIter* it, PcgScalars* sc, double* pa, hipStream_t s) {
   if (A.lanes_per_row == 4) {
     const int grid = grid_for(RowMap<4>::blocks(A.nrows > 0 ? A.nrows : 1), A.xcd_remap);
-    k_pcg_spmv<4><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb_cur, pb_prev, nb, k, sc, pa);
+    k_pcg_spmv<4><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb0, pb1, nb, it, sc, pa);
     return grid;
   }
   if (A.lanes_per_row == 0) {
     const int grid = grid_for(RowMap<0>::blocks(A.nrows > 0 ? A.nrows : 1), A.xcd_remap);
-    k_pcg_spmv<0><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb_cur, pb_prev, nb, k, sc, pa);
+    k_pcg_spmv<0><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb0, pb1, nb, it, sc, pa);
     return grid;
   }
   const int grid = grid_for(RowMap<1>::blocks(A.nrows > 0 ? A.nrows : 1), A.xcd_remap);
-  k_pcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb_cur, pb_prev, nb, k, sc, pa);
+  k_pcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb0, pb1, nb, it, sc, pa);
   return grid;
 }
 
 void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
-                       const double* pa, int na, const double* pb_cur, double* pb_next, PcgScalars* sc, hipStream_t s) {
+                       const double* pa, int na, double* pb0, double* pb1, PcgIter* it, PcgScalars* sc,
+                       hipStream_t s) {
   const int nb = vblocks(n);
-  k_pcg_update<<<nb, kBlock, 0, s>>>(n, dinv, x, r, z, p, q, pa, na, pb_cur, pb_next, nb, sc);
+  k_pcg_update<<<nb, kBlock, 0, s>>>(n, dinv, x, r, z, p, q, pa, na, pb0, pb1, nb, it, sc);
 }
 
 int pcg_vector_blocks(int n) { return vblocks(n); }
+
+int pcg_spmv_grid(const BellView& A) {
+  const int rows = A.nrows > 0 ? A.nrows : 1;
+  if (A.lanes_per_row == 4) return grid_for(RowMap<4>::blocks(rows), A.xcd_remap);
+  if (A.lanes_per_row == 0) return grid_for(RowMap<0>::blocks(rows), A.xcd_remap);
+  return grid_for(RowMap<1>::blocks(rows), A.xcd_remap);
+}
 
 // ------------------------------------------------------------------------------------------------------
 // L-BFGS helpers

@@ -208,3 +208,14 @@ def test_fp32_element_variant_error_is_bounded(bar):
     errH = per_item_rel_err(got["H"], ref["Hp"])
     print(f"[gpu] fp32 element variant: P rel err {errP:.2e}, H+ rel err {errH:.2e}", flush=True)
     assert errP <= 1e-4 and errH <= 1e-3
+
+
+def test_graph_replayed_pcg_is_bitwise_identical(bar):
+    out = []
+    for graph in (0, 1, 1):
+        g, _ = _pair(bar)
+        g.set_tuning(pcg_graph=graph, pcg_chunk_max=16)
+        for _ in range(3):
+            g.step()
+        out.append(g.deformation().copy())
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
