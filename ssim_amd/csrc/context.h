@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ssim_hip.h"
+#include "ssim_hip.h"
 #include "internal.h"
 
 

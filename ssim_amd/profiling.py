@@ -20,7 +20,7 @@ def _dbs(path):
 
 
 def _short(name):
-    return name.split("(")[0].replace("void ", "")
+    return name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
 
 
 def trace(path):
