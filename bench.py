@@ -17,7 +17,8 @@ Two more objects in the same line cover the batched and differentiable variants 
 [0.3, 0.48], x = 0 end clamped, gravity) stepped by one set of launches, against stepping the same 256 meshes one
 after another through the single-mesh path on the GPU, with the per-mesh agreement of the two after one step;
 ``adjoint``: a 20-step rollout of 32 bars of 16×16×64 cubes (per-tet log E uniform in [ln 1e4, ln 1e6], ν = 0.45)
-with the loss Σ‖u − t‖² over the free-end vertices, timed forward and backward (adjoint PCG per step).
+with the loss Σ‖u − t‖² over the free-end vertices, timed forward and backward (adjoint PCG per step);
+``lbfgs``: L-BFGS steps of the twisted 64³-cube (1,572,864 tets, per-tet material).
 """
 from __future__ import annotations
 
@@ -154,6 +155,38 @@ def bench_batched(rtol, steps=3, seed=0):
     }
 
 
+def bench_lbfgs(rtol, n=64, steps=3, seed=0):
+    """L-BFGS steps/s on the twisted cube of n³ cubes (per-tet E log-uniform, ν uniform, z = 0 face clamped)."""
+    import torch
+
+    from ssim_amd import fem, mesh
+
+    case = mesh.twisted_cube(n=n, seed=seed)
+    s = fem.TetFiniteElementSolver_Hip.from_case(case, method="lbfgs", rtol=rtol)
+    s.step()
+    its = evals = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+        st = s.last_stats()
+        its += st["newton_iterations"]
+        evals += st["energy_evals"]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    print(f"[bench] lbfgs: {dt:.3f} s/step, {its} iterations", file=sys.stderr, flush=True)
+    return {
+        "workload": (f"twisted cube {n}^3 Freudenthal cubes ({case.nv} vertices, {case.nt} tets), per-tet E log-uniform "
+                     "[1e4,1e6], nu uniform [0.3,0.48], z=0 clamped, L-BFGS m=8 with H0 = Jacobi of the rest Hessian, "
+                     "Armijo c=1e-4"),
+        "newton_tol": f"{rtol:g}*mass",
+        "steps_per_s": 1.0 / dt,
+        "timed_steps": steps,
+        "lbfgs_iterations": its,
+        "energy_evals": evals,
+    }
+
+
 def bench_adjoint(rtol, nbar=32, seed=0):
     """Forward and adjoint (backward) time of a 20-step differentiable rollout of 32 bars of 16x16x64 cubes."""
     import torch
@@ -214,6 +247,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
+    ap.add_argument("--no-lbfgs", action="store_true", help="skip the L-BFGS twisted-cube leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build's time step runs on a single device "
@@ -273,6 +307,7 @@ def main():
     del solver
     batched = None if args.no_batched else bench_batched(args.rtol)
     adj = None if args.no_adjoint else bench_adjoint(args.rtol)
+    lbfgs = None if args.no_lbfgs else bench_lbfgs(args.rtol)
     value = args.steps / elapsed
     line = {
         "metric": METRIC,
@@ -302,6 +337,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adj,
+        "lbfgs": lbfgs,
     }
     print(json.dumps(line), flush=True)
 

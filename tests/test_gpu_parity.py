@@ -219,3 +219,44 @@ def test_graph_replayed_pcg_is_bitwise_identical(bar):
             g.step()
         out.append(g.deformation().copy())
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+def _unit_box_case(nx=6, ny=5, nz=9, seed=4):
+    """The reference's own 5-tet unit_box (mesh/common_shapes.hpp:67-132): irregular vertex degrees, so its
+    rows do not fit the fixed-width layout and the looped / four-lane SpMV variants run."""
+    X, cells = mesh.unit_box_rows(nx, ny, nz)
+    nv, nt = X.shape[0], cells.shape[0]
+    lam, mu = mesh.lame(np.full(nt, 1e5), np.full(nt, 0.4))
+    clamped = X[:, 0] == X[:, 0].min()
+    dof = np.repeat(clamped.astype(np.uint8), 3)
+    rng = np.random.default_rng(seed)
+    u0 = 0.002 * rng.standard_normal(3 * nv)
+    u0[dof == 1] = 0.0
+    accel = np.zeros(3 * nv)
+    accel[1::3] = -9.8
+    return mesh.BarCase(X, cells, lam, mu, 1000.0, 1.0 / 60, dof, np.zeros(3 * nv), accel, u0, np.zeros(3 * nv),
+                        name="unit_box")
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_irregular_mesh_spmv_variants_match_oracle(lanes):
+    case = _unit_box_case()
+    g, o = _pair(case)
+    assert g.tuning()["spmv_lanes"] == 1  # rows too wide for the fixed-width layout
+    with pytest.raises(ValueError):
+        g.set_tuning(spmv_lanes=0)
+    g.set_tuning(spmv_lanes=lanes)
+    print(f"[gpu] unit_box {case.nv} vertices, SpMV variant {lanes}", flush=True)
+    u = _strained(case, 3, 0.3)
+    g.set_array("next_deform", u)
+    g.update_hessian(True)
+    H = o.hessian_csr(u, True, True)
+    x = np.random.default_rng(1).standard_normal(3 * case.nv)
+    assert rel_err(g.spmv(x), H @ x) <= 1e-13
+    g2, o2 = _pair(case)
+    g2.set_tuning(spmv_lanes=lanes)
+    for step in range(5):
+        g2.step()
+        _, st = o2.step()
+        assert g2.last_stats()["newton_iterations"] == st["newton"], step
+        assert rel_err(_positions(case, g2.deformation().T.reshape(-1)), _positions(case, o2.get_state()[0])) <= 1e-8
