@@ -26,7 +26,9 @@ typedef enum {
   SSIM_ERR_NUMERIC = 4           /* NaN/Inf in the state, or a solver breakdown */
 } ssim_status_t;
 
-typedef enum { SSIM_STEPPER_NEWTON = 0, SSIM_STEPPER_LBFGS = 1 } ssim_stepper_t;
+/* Newton (time_step_solver_backward_euler), L-BFGS (time_step_solver_lbfgs), nonlinear CG (time_step_solver_ncg,
+ * stepper/nonlinear_cg.hpp:9-37; it shares the L-BFGS iteration limit and line-search constants) */
+typedef enum { SSIM_STEPPER_NEWTON = 0, SSIM_STEPPER_LBFGS = 1, SSIM_STEPPER_NCG = 2 } ssim_stepper_t;
 
 /* vertex-shaped arrays addressable through ssim_get_array / ssim_set_array / ssim_device_array
  * (accessors of finite_elements/time_step.hpp:275-311) */
@@ -100,7 +102,7 @@ ssim_status_t ssim_set_threshold(ssim_ctx* ctx, double rtol);
 ssim_status_t ssim_set_material_per_tet(ssim_ctx* ctx, const double* lambda, const double* mu);
 ssim_status_t ssim_set_solver_params(ssim_ctx* ctx, const ssim_solver_params* params);
 ssim_status_t ssim_get_solver_params(const ssim_ctx* ctx, ssim_solver_params* params);
-/* Newton (time_step_solver_backward_euler) or L-BFGS (time_step_solver_lbfgs): stepper/backward_euler.hpp */
+/* Newton or L-BFGS (stepper/backward_euler.hpp) or nonlinear CG (stepper/nonlinear_cg.hpp) */
 ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper);
 
 /* mark_dirichlet_batched / mark_general_batched: bindings/src/fem_host.cpp:123-144 (targets [count,3]) */

@@ -54,6 +54,9 @@ struct newton_solver {
 struct lbfgs_solver {
   static constexpr ssim_stepper_t kind = SSIM_STEPPER_LBFGS;
 };
+struct ncg_solver {  // time_step_solver_ncg (stepper/nonlinear_cg.hpp:9-37)
+  static constexpr ssim_stepper_t kind = SSIM_STEPPER_NCG;
+};
 
 class time_step {
  public:

@@ -68,6 +68,7 @@ def lib():
             "orc_ts_prepare_step": (None, [_V]),
             "orc_ts_compute_step": (ctypes.c_double, [_V, _D]),
             "orc_ts_compute_step_lbfgs": (ctypes.c_double, [_V, _D]),
+            "orc_ts_compute_step_ncg": (ctypes.c_double, [_V, _D]),
             "orc_ts_step_next": (None, [_V]),
             "orc_ts_rest_hessian_diag": (None, [_V, _D]),
             "orc_ts_spmv": (None, [_V, _D, _D, _D]),
@@ -350,7 +351,8 @@ class TimeStep:
 
     def compute_step(self, method="newton"):
         st = np.zeros(6)
-        fn = lib().orc_ts_compute_step if method == "newton" else lib().orc_ts_compute_step_lbfgs
+        fn = {"newton": lib().orc_ts_compute_step, "lbfgs": lib().orc_ts_compute_step_lbfgs,
+              "ncg": lib().orc_ts_compute_step_ncg}[method]
         gn = fn(self._h, _dp(st))
         return gn, dict(newton=int(st[0]), pcg=int(st[1]), evals=int(st[2]), converged=bool(st[3]),
                         grad_norm=float(st[4]), ls_fail=int(st[5]))

@@ -886,6 +886,92 @@ double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats) {
   return gn;
 }
 
+/* Nonlinear CG stepper (stepper/nonlinear_cg.hpp:9-37: mathprim ncg_optimizer with the Dai-Kou strategy,
+ * learning rate 1e-2, backtracking line search, at most 1000 iterations, tol = threshold·mass).  mathprim is not
+ * available, so this follows the published method: Dai & Kou, "A nonlinear conjugate gradient algorithm with an
+ * optimal property and an improved Wolfe line search", SIAM J. Optim. 23 (2013), the DK+ choice
+ *   β = max(gₖ₊₁ᵀyₖ/dₖᵀyₖ − ‖yₖ‖²·gₖ₊₁ᵀdₖ/(dₖᵀyₖ)², η·gₖ₊₁ᵀdₖ/‖dₖ‖²), η = 0.5, yₖ = gₖ₊₁ − gₖ,
+ * dₖ₊₁ = −gₖ₊₁ + β dₖ, restarting with d = −g when dᵀy ≤ 0 or d is not a descent direction.  Step lengths:
+ * backtracking Armijo (the L-BFGS constants: c = 1e-4, ×0.5, at most lbfgs_ls_max trials) from a first trial of
+ * the learning rate 1e-2 in the first iteration and αₖ₋₁·gₖ₋₁ᵀdₖ₋₁/gₖᵀdₖ after it (Nocedal & Wright, eq. 3.60),
+ * capped at 1. */
+double orc_ts_compute_step_ncg(orc_ts* ts, double* stats) {
+  const int n = ts->n;
+  double* u = ts->u_next;
+  double* d = (double*)malloc(sizeof(double) * n);
+  double* ut = (double*)malloc(sizeof(double) * n);
+  double* gt = (double*)malloc(sizeof(double) * n);
+  int it = 0, evals = 0, converged = 0, ls_fail = 0;
+  double E = orc_ts_energy_grad(ts, u, ts->g);
+  evals++;
+  for (int i = 0; i < n; ++i) d[i] = -ts->g[i];
+  double alpha_prev = 1e-2, gd_prev = 0.0;
+  for (; it < ts->lbfgs_maxit;) {
+    const double gg = dot_n(n, ts->g, ts->g);
+    if (sqrt(gg) < ts->tol_abs) {
+      converged = 1;
+      break;
+    }
+    double gd = dot_n(n, ts->g, d);
+    if (!(gd < 0)) { /* not a descent direction: restart */
+      for (int i = 0; i < n; ++i) d[i] = -ts->g[i];
+      gd = -gg;
+    }
+    double alpha = it == 0 ? 1e-2 : alpha_prev * gd_prev / gd;
+    if (!(alpha > 0) || alpha > 1.0) alpha = 1.0;
+    double Et = 0;
+    int ok = 0;
+    for (int ls = 0; ls < ts->lbfgs_ls_max; ++ls) {
+      for (int i = 0; i < n; ++i) ut[i] = u[i] + alpha * d[i];
+      Et = orc_ts_energy_grad(ts, ut, gt);
+      evals++;
+      if (Et <= E + ts->lbfgs_c * alpha * gd) {
+        ok = 1;
+        break;
+      }
+      alpha *= ts->lbfgs_beta;
+    }
+    if (!ok) ls_fail++;
+    /* β from y = g_new − g: dᵀy, g_newᵀy, yᵀy, g_newᵀd, dᵀd */
+    double dy = 0, gy = 0, yy = 0, gnd = 0, dd = 0;
+    for (int i = 0; i < n; ++i) {
+      const double y = gt[i] - ts->g[i];
+      dy += d[i] * y;
+      gy += gt[i] * y;
+      yy += y * y;
+      gnd += gt[i] * d[i];
+      dd += d[i] * d[i];
+    }
+    double beta = 0.0;
+    if (dy > 0) {
+      const double bdk = gy / dy - yy * gnd / (dy * dy);
+      const double floor_ = 0.5 * gnd / dd;
+      beta = bdk > floor_ ? bdk : floor_;
+    }
+    for (int i = 0; i < n; ++i) d[i] = -gt[i] + beta * d[i];
+    memcpy(u, ut, sizeof(double) * n);
+    memcpy(ts->g, gt, sizeof(double) * n);
+    E = Et;
+    alpha_prev = alpha;
+    gd_prev = gd;
+    it++;
+  }
+  orc_ts_energy_grad(ts, u, ts->g);
+  const double gn = sqrt(dot_n(n, ts->g, ts->g));
+  if (stats) {
+    stats[0] = it;
+    stats[1] = 0;
+    stats[2] = evals;
+    stats[3] = converged;
+    stats[4] = gn;
+    stats[5] = ls_fail;
+  }
+  free(d);
+  free(ut);
+  free(gt);
+  return gn;
+}
+
 /* step_next: u_prev ← u; u ← u_next; v = (u − u_prev)/dt·(1 − damping), damping 0
  * (finite_elements/time_step.hpp:530-543) */
 void orc_ts_step_next(orc_ts* ts) {

@@ -72,6 +72,8 @@ void orc_ts_prepare_step(orc_ts* ts);
  *        [4]=final grad norm, [5]=line-search failures */
 double orc_ts_compute_step(orc_ts* ts, double* stats);
 double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats);
+/* nonlinear CG (Dai-Kou DK+ β, backtracking Armijo with the L-BFGS constants); stats as compute_step */
+double orc_ts_compute_step_ncg(orc_ts* ts, double* stats);
 void orc_ts_step_next(orc_ts* ts);
 void orc_ts_rest_hessian_diag(orc_ts* ts, double* diag);
 

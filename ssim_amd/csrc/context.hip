@@ -582,6 +582,66 @@ void lbfgs_solve(ssim_ctx* c) {
   c->stats.newton_iterations = it;
 }
 
+// Nonlinear CG (stepper/nonlinear_cg.hpp:9-37: Dai-Kou strategy, learning rate 1e-2, backtracking line search,
+// tol = threshold·mass), the published DK+ method: β = max(gₖ₊₁ᵀy/dᵀy − ‖y‖²·gₖ₊₁ᵀd/(dᵀy)², ½·gₖ₊₁ᵀd/‖d‖²),
+// d = −gₖ₊₁ + βd, restarts with d = −g when dᵀy ≤ 0 or d is not a descent direction; backtracking Armijo with the
+// L-BFGS constants from a first trial of 1e-2, then αₖ₋₁·gₖ₋₁ᵀdₖ₋₁/gₖᵀdₖ capped at 1.
+void ncg_solve(ssim_ctx* c) {
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* g = c->a(SSIM_ARRAY_FORCES);
+  double* d = c->dir.p;
+  c->lb_scal.alloc(std::max<size_t>(c->lb_scal.count, 16));
+  double* dots = c->lb_scal.p;  // five reductions of one iteration
+  // their partials (5 per vector block) fit the dot-product slot: pmax ≥ 4·slices = n/48 > 5·(n/1024 + 1)
+  if (5 * (size_t)pcg_vector_blocks(n) > c->pmax) fail(SSIM_ERR_STATE, "internal: partial buffer too small");
+  EvalOut cur = eval_energy_grad(c, u, g);
+  launch_axpby(n, -1.0, g, 0.0, d, s);
+  double alpha_prev = 1e-2, gd_prev = 0.0;
+  int it = 0;
+  for (; it < c->prm.lbfgs_maxit; ++it) {
+    if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+    if (std::sqrt(cur.gg) < c->tol_abs) {
+      c->stats.converged = 1;
+      break;
+    }
+    double gd = device_dot(c, g, d);
+    if (!(gd < 0)) {  // not a descent direction: restart along −g
+      launch_axpby(n, -1.0, g, 0.0, d, s);
+      gd = -cur.gg;
+    }
+    double alpha = it == 0 ? 1e-2 : alpha_prev * gd_prev / gd;
+    if (!(alpha > 0) || alpha > 1.0) alpha = 1.0;
+    EvalOut trial{0, 0};
+    bool ok = false;
+    for (int ls = 0; ls < c->prm.lbfgs_ls_max; ++ls) {
+      launch_axpy_to(n, u, alpha, d, c->utry.p, s);
+      trial = eval_energy_grad(c, c->utry.p, c->gtry.p);
+      if (trial.E <= cur.E + c->prm.lbfgs_c * alpha * gd) {
+        ok = true;
+        break;
+      }
+      alpha *= c->prm.ls_beta;
+    }
+    if (!ok) c->stats.ls_failures++;
+    launch_ncg_dots(n, c->gtry.p, g, d, c->part_d(), dots, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->pin, dots, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    const double dy = c->pin[0], gy = c->pin[1], yy = c->pin[2], gnd = c->pin[3], dd = c->pin[4];
+    double beta = 0.0;
+    if (dy > 0) beta = std::max(gy / dy - yy * gnd / (dy * dy), 0.5 * gnd / dd);
+    launch_axpby(n, -1.0, c->gtry.p, beta, d, s);
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    cur = trial;
+    alpha_prev = alpha;
+    gd_prev = gd;
+  }
+  c->stats.newton_iterations = it;
+}
+
 ssim_status_t guard(ssim_ctx* c) {
   if (!c) {
     g_err = "null context";
@@ -770,7 +830,7 @@ ssim_status_t ssim_get_solver_params(const ssim_ctx* ctx, ssim_solver_params* p)
 }
 ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
-  if (stepper != SSIM_STEPPER_NEWTON && stepper != SSIM_STEPPER_LBFGS) {
+  if (stepper != SSIM_STEPPER_NEWTON && stepper != SSIM_STEPPER_LBFGS && stepper != SSIM_STEPPER_NCG) {
     g_err = "unknown stepper";
     return SSIM_ERR_INVALID_ARGUMENT;
   }
@@ -827,8 +887,10 @@ ssim_status_t ssim_compute_step(ssim_ctx* ctx, double* residual_out) {
     ctx->stats = ssim_step_stats{};
     if (ctx->stepper == SSIM_STEPPER_NEWTON)
       newton_solve(ctx);
-    else
+    else if (ctx->stepper == SSIM_STEPPER_LBFGS)
       lbfgs_solve(ctx);
+    else
+      ncg_solve(ctx);
     // compute_step: final update_energy_and_gradients and ‖forces‖ (time_step.hpp:520-528)
     const EvalOut fin = eval_energy_grad(ctx, ctx->a(SSIM_ARRAY_NEXT_DEFORM), ctx->a(SSIM_ARRAY_FORCES));
     ctx->stats.grad_norm = std::sqrt(fin.gg);

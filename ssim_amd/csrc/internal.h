@@ -132,6 +132,12 @@ void launch_elem_adjoint(const MeshView& m, const double* u, const double* lv, d
 // out = in with Dirichlet dofs zeroed
 void launch_mask_dbc(int n, const unsigned char* dof, const double* in, double* out, hipStream_t s);
 
+// nonlinear CG: out[0..5) = (dᵀy, gtᵀy, yᵀy, gtᵀd, dᵀd), y = gt − g (part holds vector-block partials of width 5)
+void launch_ncg_dots(int n, const double* gt, const double* g, const double* d, double* part, double* out,
+                     hipStream_t s);
+// y = a·x + b·y (y is not read when b = 0)
+void launch_axpby(int n, double a, const double* x, double b, double* y, hipStream_t s);
+
 // L-BFGS helpers
 void launch_scale_div(int n, const double* q, const double* diag, double* out, hipStream_t s);
 // y += (c1·(*p1) − c2·(*p2))·x, coefficients read from device memory (p2 may be null)

@@ -585,6 +585,52 @@ int pcg_spmv_grid(const BellView& A) {
 }
 
 // ------------------------------------------------------------------------------------------------------
+// nonlinear-CG helpers
+
+// partials of (dᵀy, gtᵀy, yᵀy, gtᵀd, dᵀd) with y = gt − g
+__global__ __launch_bounds__(kBlock) void k_ncg_dots(int n, const double* __restrict__ gt, const double* __restrict__ g,
+                                                     const double* __restrict__ d, double* __restrict__ part) {
+  double v[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {
+    const int i = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
+    if (i < n) {
+      const double gi = gt[i], di = d[i], y = gi - g[i];
+      v[0] += di * y;
+      v[1] += gi * y;
+      v[2] += y * y;
+      v[3] += gi * di;
+      v[4] += di * di;
+    }
+  }
+  store_partials<5>(v, part);
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize5(const double* __restrict__ part, int np, double* __restrict__ out) {
+  double t[5];
+  sum_partials<5>(part, np, t);
+  if (threadIdx.x < 5) out[threadIdx.x] = t[threadIdx.x];
+}
+
+void launch_ncg_dots(int n, const double* gt, const double* g, const double* d, double* part, double* out,
+                     hipStream_t s) {
+  const int nb = vblocks(n);
+  k_ncg_dots<<<nb, kBlock, 0, s>>>(n, gt, g, d, part);
+  k_finalize5<<<1, kBlock, 0, s>>>(part, nb, out);
+}
+
+// y = a·x + b·y (b = 0 does not read y)
+__global__ void k_axpby(int n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) y[i] = b == 0.0 ? a * x[i] : a * x[i] + b * y[i];
+}
+
+void launch_axpby(int n, double a, const double* x, double b, double* y, hipStream_t s) {
+  if (n <= 0) return;
+  k_axpby<<<nblocks(n), kBlock, 0, s>>>(n, a, x, b, y);
+}
+
+// ------------------------------------------------------------------------------------------------------
 // L-BFGS helpers
 
 __global__ void k_scale_div(int n, const double* __restrict__ q, const double* __restrict__ diag,

@@ -36,14 +36,14 @@ def unit_box(nx: int, ny: int, nz: int) -> Tuple[np.ndarray, np.ndarray]:
 class TetFiniteElementSolver_Hip:
     """Implicit (backward Euler) tetrahedral FEM time step on one MI355X (FP64).
 
-    method: "newton" (Newton + Jacobi-PCG + backtracking Armijo) or "lbfgs" (L-BFGS, m = 8, H0 = diagonal
-    of the rest Hessian).
+    method: "newton" (Newton + Jacobi-PCG + backtracking Armijo), "lbfgs" (L-BFGS, m = 8, H0 = diagonal
+    of the rest Hessian) or "ncg" (nonlinear CG, Dai-Kou β, the L-BFGS line-search constants).
     """
 
     def __init__(self, vertices: np.ndarray, elements: np.ndarray, time_step: float = 1e-2,
                  young_modulus: float = 1e6, poisson_ratio: float = 0.33, density: float = 1e2,
-                 method: Literal["newton", "lbfgs"] = "newton"):
-        if method not in ("newton", "lbfgs"):
+                 method: Literal["newton", "lbfgs", "ncg"] = "newton"):
+        if method not in ("newton", "lbfgs", "ncg"):
             raise ValueError(f"Unknown method: {method}")
         X = np.ascontiguousarray(vertices, dtype=np.float64)
         C = np.ascontiguousarray(elements, dtype=np.int32)
@@ -59,7 +59,7 @@ class TetFiniteElementSolver_Hip:
                   float(young_modulus), float(poisson_ratio), float(density))
         self._h = h
         self._has_reset = False
-        _lib.call("ssim_set_stepper", self._h, 0 if method == "newton" else 1)
+        _lib.call("ssim_set_stepper", self._h, {"newton": 0, "lbfgs": 1, "ncg": 2}[method])
         self.method = method
 
     def __del__(self):

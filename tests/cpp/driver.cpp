@@ -2,7 +2,7 @@
 // the reference's drivers use basic_time_step (tests/fem_time_step/main.cpp): construct from a mesh, mark the
 // clamped vertices, add gravity, reset with a stepper, then prepare_step / compute_step / step_next.
 //
-// Input file (little-endian): int32 nv, nt, steps, stepper (0 Newton, 1 L-BFGS); float64 dt, E, nu, rho, rtol,
+// Input file (little-endian): int32 nv, nt, steps, stepper (0 Newton, 1 L-BFGS, 2 NCG); float64 dt, E, nu, rho, rtol,
 // gravity_y; float64 X[nv*3]; int32 cells[nt*4]; uint8 clamped[nv]; float64 u0[nv*3].
 // Output file: float64 u[nv*3] after the last step, then per step int32 newton_iterations, pcg_iterations.
 #include <cstdio>
@@ -49,8 +49,11 @@ int main(int argc, char** argv) {
     step.set_threshold(prm[4]);
     ssim::hip::newton_solver newton;
     ssim::hip::lbfgs_solver lbfgs;
+    ssim::hip::ncg_solver ncg;
     if (stepper == 1)
       step.reset(lbfgs);
+    else if (stepper == 2)
+      step.reset(ncg);
     else
       step.reset(newton);
     std::vector<int32_t> counts;
@@ -58,6 +61,8 @@ int main(int argc, char** argv) {
       step.prepare_step();
       if (stepper == 1)
         step.compute_step(lbfgs);
+      else if (stepper == 2)
+        step.compute_step(ncg);
       else
         step.compute_step(newton);
       step.step_next();

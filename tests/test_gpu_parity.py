@@ -162,6 +162,25 @@ def test_lbfgs_trajectory_matches_oracle():
         assert rel_err(xg, xo) <= 1e-8, step
 
 
+def test_ncg_stepper_reaches_the_oracle_solution():
+    """Nonlinear CG (Dai-Kou): GPU and oracle run the same iteration; over thousands of iterations rounding
+    differences change the path, so both are compared where they converge (threshold 1e-8 × mass, the GPU
+    positions within 1e-6 of the oracle's; the oracle's NCG is pinned to the Newton minimiser)."""
+    case = mesh.cantilever_bar(3, 1, 1, h=0.05, E=1e4, nu=0.3, rho=100.0, dt=0.02, seed=2)
+    g = fem.TetFiniteElementSolver_Hip.from_case(case, method="ncg", rtol=1e-8)
+    g.set_solver_params(lbfgs_maxit=20000)
+    o = oracle.TimeStep.from_case(case)
+    o.set_threshold(1e-8)
+    o.set_lbfgs(max_iter=20000)
+    for step in range(2):
+        g.step()
+        _, st = o.step("ncg")
+        sg = g.last_stats()
+        assert sg["converged"] and st["converged"], (sg, st)
+        assert rel_err(_positions(case, g.deformation().T.reshape(-1)), _positions(case, o.get_state()[0])) <= 1e-6
+    print(f"[gpu] NCG: {sg['newton_iterations']} iterations (oracle {st['newton']})", flush=True)
+
+
 def test_repeat_runs_are_bitwise_identical(bar):
     outs = []
     for _ in range(2):

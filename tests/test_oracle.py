@@ -342,6 +342,28 @@ def test_lbfgs_step_agrees_with_newton():
     assert rel_err(X + ub, X + ua) <= 1e-8
 
 
+def test_ncg_step_agrees_with_newton():
+    """The nonlinear-CG stepper converges to the minimiser the Newton stepper finds, to the accuracy its gradient
+    threshold allows (unpreconditioned NCG stalls on Armijo roundoff below ~1e-10, so the threshold is 1e-8 × mass).
+    The Dai-Kou iteration path itself is parity unpinned: mathprim's ncg_optimizer is not available."""
+    case = _tiny_case()
+    a = oracle.TimeStep.from_case(case)
+    b = oracle.TimeStep.from_case(case)
+    for ts in (a, b):
+        ts.set_solver(max_newton=50, pcg_rtol=1e-12)
+        ts.set_lbfgs(m=8, max_iter=20000)
+        ts.set_threshold(1e-8)
+    a.prepare_step()
+    b.prepare_step()
+    _, sa = a.compute_step("newton")
+    _, sb = b.compute_step("ncg")
+    assert sa["converged"] and sb["converged"], sb
+    ua, _, _ = a.get_vectors()
+    ub, _, _ = b.get_vectors()
+    X = case.X.reshape(-1)
+    assert rel_err(X + ub, X + ua) <= 1e-6
+
+
 def test_step_next_and_prepare_step_formulas():
     case = _tiny_case()
     ts = oracle.TimeStep.from_case(case)
