@@ -27,8 +27,19 @@ typedef enum {
 } ssim_status_t;
 
 /* Newton (time_step_solver_backward_euler), L-BFGS (time_step_solver_lbfgs), nonlinear CG (time_step_solver_ncg,
- * stepper/nonlinear_cg.hpp:9-37; it shares the L-BFGS iteration limit and line-search constants) */
-typedef enum { SSIM_STEPPER_NEWTON = 0, SSIM_STEPPER_LBFGS = 1, SSIM_STEPPER_NCG = 2 } ssim_stepper_t;
+ * stepper/nonlinear_cg.hpp:9-37; it shares the L-BFGS iteration limit and line-search constants), forward Euler
+ * (time_step_solver_forward_euler, finite_elements/time_step.hpp:717-736: next_deform += ½·dt²·forces, with
+ * forces evaluated once at the predictor) */
+typedef enum {
+  SSIM_STEPPER_NEWTON = 0,
+  SSIM_STEPPER_LBFGS = 1,
+  SSIM_STEPPER_NCG = 2,
+  SSIM_STEPPER_FORWARD_EULER = 3
+} ssim_stepper_t;
+
+/* material model, the ElastModel template argument of basic_time_step (finite_elements/time_step.hpp:168-172):
+ * stable Neo-Hookean (elast/stable_neohookean.hpp, the default) or linear elasticity (elast/linear.hpp) */
+typedef enum { SSIM_MATERIAL_STABLE_NEOHOOKEAN = 0, SSIM_MATERIAL_LINEAR = 1 } ssim_material_t;
 
 /* vertex-shaped arrays addressable through ssim_get_array / ssim_set_array / ssim_device_array
  * (accessors of finite_elements/time_step.hpp:275-311) */
@@ -102,8 +113,12 @@ ssim_status_t ssim_set_threshold(ssim_ctx* ctx, double rtol);
 ssim_status_t ssim_set_material_per_tet(ssim_ctx* ctx, const double* lambda, const double* mu);
 ssim_status_t ssim_set_solver_params(ssim_ctx* ctx, const ssim_solver_params* params);
 ssim_status_t ssim_get_solver_params(const ssim_ctx* ctx, ssim_solver_params* params);
-/* Newton or L-BFGS (stepper/backward_euler.hpp) or nonlinear CG (stepper/nonlinear_cg.hpp) */
+/* Newton or L-BFGS (stepper/backward_euler.hpp), nonlinear CG (stepper/nonlinear_cg.hpp) or forward Euler
+ * (finite_elements/time_step.hpp:717-736) */
 ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper);
+/* selects the ElastModel of basic_time_step<..., ElastModel> at run time; takes effect at the next evaluation */
+ssim_status_t ssim_set_material_model(ssim_ctx* ctx, ssim_material_t model);
+ssim_status_t ssim_get_material_model(const ssim_ctx* ctx, ssim_material_t* model);
 
 /* mark_dirichlet_batched / mark_general_batched: bindings/src/fem_host.cpp:123-144 (targets [count,3]) */
 ssim_status_t ssim_mark_dirichlet(ssim_ctx* ctx, const int* verts, int count, const double* targets);
@@ -265,6 +280,8 @@ ssim_status_t ssim_batch_get_array_device(ssim_batch* batch, ssim_array_t which,
 ssim_status_t ssim_batch_set_array_device(ssim_batch* batch, ssim_array_t which, const double* src_device);
 /* replace every mesh's per-tet Lamé parameters (host arrays of Σnt[m]); takes effect at once */
 ssim_status_t ssim_batch_set_material(ssim_batch* batch, const double* lambda, const double* mu);
+/* material model of every mesh of the batch (see ssim_set_material_model) */
+ssim_status_t ssim_batch_set_material_model(ssim_batch* batch, ssim_material_t model);
 /* ssim_adjoint_step for every mesh of the batch; device arrays in the compact layout (grad_u, grad_uhat,
  * adjoint_out: Σ 3·nv[m]; grad_lambda, grad_mu: Σ nt[m]); pcg_iters_out: host array of nmesh (may be NULL) */
 ssim_status_t ssim_batch_adjoint_step(ssim_batch* batch, const double* grad_u, double* grad_uhat, double* grad_lambda,

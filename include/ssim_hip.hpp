@@ -6,8 +6,9 @@
 // compute_step / step_next, update_energy_and_gradients / update_hessian / check_convergence, add_ext_force_dof,
 // restore, the setters (which throw std::invalid_argument where the reference does, time_step.hpp:316-347) and
 // the vertex-array views (device pointers of [nv, 3] row-major doubles, time_step.hpp:275-311).  Steppers are the
-// tag types newton_solver (time_step_solver_backward_euler, stepper/backward_euler.hpp:9-41) and lbfgs_solver
-// (time_step_solver_lbfgs, backward_euler.hpp:43-67), passed to reset / compute_step as in the reference.
+// tag types newton_solver (time_step_solver_backward_euler, stepper/backward_euler.hpp:9-41), lbfgs_solver
+// (time_step_solver_lbfgs, backward_euler.hpp:43-67), ncg_solver and forward_euler_solver, passed to reset /
+// compute_step as in the reference.
 //
 // ssim::hip::batch is the batched extension (many meshes per set of launches) and has no reference counterpart.
 #ifndef SSIM_HIP_HPP
@@ -56,6 +57,9 @@ struct lbfgs_solver {
 };
 struct ncg_solver {  // time_step_solver_ncg (stepper/nonlinear_cg.hpp:9-37)
   static constexpr ssim_stepper_t kind = SSIM_STEPPER_NCG;
+};
+struct forward_euler_solver {  // time_step_solver_forward_euler (finite_elements/time_step.hpp:717-736)
+  static constexpr ssim_stepper_t kind = SSIM_STEPPER_FORWARD_EULER;
 };
 
 class time_step {
@@ -128,6 +132,8 @@ class time_step {
   time_step& set_damping(double v) { return chk(ssim_set_damping(ctx_, v)); }
   time_step& set_threshold(double v) { return chk(ssim_set_threshold(ctx_, v)); }
   time_step& set_solver_params(const ssim_solver_params& p) { return chk(ssim_set_solver_params(ctx_, &p)); }
+  // the ElastModel template argument of basic_time_step, chosen at run time
+  time_step& set_material(ssim_material_t model) { return chk(ssim_set_material_model(ctx_, model)); }
   ssim_solver_params solver_params() const {
     ssim_solver_params p{};
     check(ssim_get_solver_params(ctx_, &p));

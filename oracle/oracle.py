@@ -69,6 +69,11 @@ def lib():
             "orc_ts_compute_step": (ctypes.c_double, [_V, _D]),
             "orc_ts_compute_step_lbfgs": (ctypes.c_double, [_V, _D]),
             "orc_ts_compute_step_ncg": (ctypes.c_double, [_V, _D]),
+            "orc_ts_compute_step_fe": (ctypes.c_double, [_V, _D]),
+            "orc_ts_set_model": (None, [_V, ctypes.c_int]),
+            "orc_lin_energy": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double]),
+            "orc_lin_stress": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, _D]),
+            "orc_lin_hessian": (None, [ctypes.c_double, ctypes.c_double, _D]),
             "orc_ts_step_next": (None, [_V]),
             "orc_ts_rest_hessian_diag": (None, [_V, _D]),
             "orc_ts_spmv": (None, [_V, _D, _D, _D]),
@@ -105,6 +110,23 @@ def snh_stress(F, lam, mu):
     P = np.zeros(9)
     e = lib().orc_snh_stress(_dp(F), lam, mu, _dp(P))
     return e, P.reshape(3, 3)
+
+
+def lin_stress(F, lam, mu):
+    F = _f64(F).reshape(9)
+    P = np.zeros(9)
+    e = lib().orc_lin_stress(_dp(F), lam, mu, _dp(P))
+    return e, P.reshape(3, 3)
+
+
+def lin_energy(F, lam, mu):
+    return lib().orc_lin_energy(_dp(_f64(F).reshape(9)), lam, mu)
+
+
+def lin_hessian(lam, mu):
+    H = np.zeros(81)
+    lib().orc_lin_hessian(lam, mu, _dp(H))
+    return H.reshape(9, 9)
 
 
 def snh_hessian(F, lam, mu):
@@ -174,13 +196,23 @@ def snh_dP_dlame(F):
     return dl, dm
 
 
+def lin_dP_dlame(F):
+    """(∂P/∂λ, ∂P/∂μ) of linear elasticity, P = 2μ·ε + λ·tr(ε)·I with ε = ½(F + Fᵀ) − I (elast/linear.hpp:37-50):
+    ∂P/∂λ = tr(ε)·I, ∂P/∂μ = 2ε."""
+    F = np.asarray(F, dtype=np.float64)
+    eps = 0.5 * (F + np.swapaxes(F, -1, -2)) - np.eye(3)
+    tr = np.trace(eps, axis1=-2, axis2=-1)
+    return tr[..., None, None] * np.eye(3), 2.0 * eps
+
+
 def _edge_matrix(x, cells):
     """Ds per tet: columns x1 − x0, x2 − x0, x3 − x0 (def_grad.hpp:270-288), shape (nt, 3, 3)."""
     x0 = x[cells[:, 0]]
     return np.stack([x[cells[:, 1]] - x0, x[cells[:, 2]] - x0, x[cells[:, 3]] - x0], axis=2)
 
 
-def rollout_gradient(case, steps, loss_grad, lam=None, mu=None, rtol=1e-8, pcg_rtol=1e-12, polish=2):
+def rollout_gradient(case, steps, loss_grad, lam=None, mu=None, rtol=1e-8, pcg_rtol=1e-12, polish=2,
+                     model="stable_neohookean"):
     """Forward rollout of `steps` Newton steps from case's initial state, then the adjoint recursion backwards.
 
     loss_grad(u_N) -> (L, ∂L/∂u_N).  Step n: û = u_n + dt·v_n + dt²·a (free dofs), u_{n+1} = solution,
@@ -196,6 +228,7 @@ def rollout_gradient(case, steps, loss_grad, lam=None, mu=None, rtol=1e-8, pcg_r
     ts.set_threshold(rtol)
     ts.set_solver(pcg_rtol=pcg_rtol)
     ts.set_polish(polish)
+    ts.set_model(model)
     us = [np.array(case.u0, dtype=np.float64)]
     for _ in range(steps):
         ts.step()
@@ -212,7 +245,8 @@ def rollout_gradient(case, steps, loss_grad, lam=None, mu=None, rtol=1e-8, pcg_r
     return L, dict(u0=ubar, v0=vbar, lam=glam, mu=gmu)
 
 
-def rollout_loss(case, steps, loss_grad, lam=None, mu=None, v0=None, rtol=1e-8, pcg_rtol=1e-12, polish=2):
+def rollout_loss(case, steps, loss_grad, lam=None, mu=None, v0=None, rtol=1e-8, pcg_rtol=1e-12, polish=2,
+                 model="stable_neohookean"):
     """Loss of the forward rollout only (finite-difference checks)."""
     lam = case.lam if lam is None else lam
     mu = case.mu if mu is None else mu
@@ -223,6 +257,7 @@ def rollout_loss(case, steps, loss_grad, lam=None, mu=None, v0=None, rtol=1e-8, 
     ts.set_threshold(rtol)
     ts.set_solver(pcg_rtol=pcg_rtol)
     ts.set_polish(polish)
+    ts.set_model(model)
     for _ in range(steps):
         ts.step()
     return loss_grad(ts.get_state()[0])[0]
@@ -234,6 +269,7 @@ class TimeStep:
     def __init__(self, X, cells, dt, lam, mu, rho):
         self.dt = float(dt)
         self.dof = None
+        self.model = "stable_neohookean"
         self.X = _f64(X).reshape(-1, 3)
         self.cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
         self.nv, self.nt = self.X.shape[0], self.cells.shape[0]
@@ -310,6 +346,11 @@ class TimeStep:
     def set_solver(self, max_newton=10, pcg_rtol=1e-8, pcg_maxit=10000, ls_c=0.3, ls_beta=0.5, ls_max=20):
         lib().orc_ts_set_solver(self._h, max_newton, pcg_rtol, pcg_maxit, ls_c, ls_beta, ls_max)
 
+    def set_model(self, model):
+        """"stable_neohookean" or "linear"."""
+        lib().orc_ts_set_model(self._h, {"stable_neohookean": 0, "linear": 1}[model])
+        self.model = model
+
     def set_polish(self, steps):
         lib().orc_ts_set_polish(self._h, int(steps))
 
@@ -352,7 +393,7 @@ class TimeStep:
     def compute_step(self, method="newton"):
         st = np.zeros(6)
         fn = {"newton": lib().orc_ts_compute_step, "lbfgs": lib().orc_ts_compute_step_lbfgs,
-              "ncg": lib().orc_ts_compute_step_ncg}[method]
+              "ncg": lib().orc_ts_compute_step_ncg, "forward_euler": lib().orc_ts_compute_step_fe}[method]
         gn = fn(self._h, _dp(st))
         return gn, dict(newton=int(st[0]), pcg=int(st[1]), evals=int(st[2]), converged=bool(st[3]),
                         grad_norm=float(st[4]), ls_fail=int(st[5]))
@@ -395,7 +436,7 @@ class TimeStep:
         B, V = self.rest_data()
         F = self.deformation_gradients(u_star)
         G = _edge_matrix(lv.reshape(-1, 3), self.cells) @ B
-        dl, dm = snh_dP_dlame(F)
+        dl, dm = (lin_dP_dlame if self.model == "linear" else snh_dP_dlame)(F)
         sc = -self.dt * self.dt * V
         return lv, g_uhat, sc * np.sum(dl * G, axis=(1, 2)), sc * np.sum(dm * G, axis=(1, 2))
 

@@ -191,6 +191,40 @@ void orc_spd_project9(const double H[81], double floor_, double Hp[81]) {
     }
 }
 
+/* Linear elasticity (elast/linear.hpp:24-69, strain internal_physic.hpp:16-19): ε = ½(F + Fᵀ) − I,
+ * Ψ = μ‖ε‖² + ½λ·tr(ε)², P = 2μ·ε + λ·tr(ε)·I, and the constant Hessian H = μ(I + S) + λ·vec(I)vec(I)ᵀ, S the
+ * transposition of vec(F) (entry (3i+j, 3j+i) gets μ, entry (3i+i, 3j+j) gets λ, linear.hpp:55-68). */
+static void lin_strain(const double F[9], double eps[9], double* tr) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) eps[3 * i + j] = 0.5 * (F[3 * i + j] + F[3 * j + i]) - (i == j ? 1.0 : 0.0);
+  *tr = eps[0] + eps[4] + eps[8];
+}
+
+double orc_lin_energy(const double F[9], double lam, double mu) {
+  double eps[9], tr, n2 = 0;
+  lin_strain(F, eps, &tr);
+  for (int k = 0; k < 9; ++k) n2 += eps[k] * eps[k];
+  return mu * n2 + 0.5 * lam * tr * tr;
+}
+
+double orc_lin_stress(const double F[9], double lam, double mu, double P[9]) {
+  double eps[9], tr;
+  lin_strain(F, eps, &tr);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) P[3 * i + j] = 2 * mu * eps[3 * i + j] + (i == j ? lam * tr : 0.0);
+  return orc_lin_energy(F, lam, mu);
+}
+
+void orc_lin_hessian(double lam, double mu, double H[81]) {
+  for (int k = 0; k < 81; ++k) H[k] = 0.0;
+  for (int a = 0; a < 9; ++a) H[a * 9 + a] = mu;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      H[(3 * i + j) * 9 + 3 * j + i] += mu;
+      H[(3 * i + i) * 9 + 3 * j + j] += lam;
+    }
+}
+
 static void inv3(const double A[9], double Ai[9]) {
   const double d = det3(A);
   Ai[0] = (A[4] * A[8] - A[5] * A[7]) / d;
@@ -300,6 +334,7 @@ struct orc_ts {
   double rtol, tol_abs;
   int max_newton, pcg_maxit, ls_max;
   int polish; /* full Newton steps after the Newton loop (0: the reference's algorithm) */
+  int model;  /* 0: stable Neo-Hookean, 1: linear elasticity */
   double pcg_rtol, ls_c, ls_beta;
   int lbfgs_m, lbfgs_maxit, lbfgs_ls_max;
   double lbfgs_c, lbfgs_beta;
@@ -460,6 +495,7 @@ orc_ts* orc_ts_create(const double* X, int nv, const int* cells, int nt, double 
   ts->ls_beta = 0.5;
   ts->ls_max = 20;
   ts->polish = 0;
+  ts->model = 0;
   ts->lbfgs_m = 8;
   ts->lbfgs_maxit = 1000;
   ts->lbfgs_c = 1e-4;
@@ -534,6 +570,13 @@ void orc_ts_set_solver(orc_ts* ts, int max_newton, double pcg_rtol, int pcg_maxi
   ts->ls_max = ls_max;
 }
 void orc_ts_set_polish(orc_ts* ts, int polish) { ts->polish = polish; }
+void orc_ts_set_model(orc_ts* ts, int model) {
+  ts->model = model;
+  if (ts->hrest_diag) {
+    free(ts->hrest_diag);
+    ts->hrest_diag = NULL;
+  }
+}
 
 void orc_ts_set_lbfgs(orc_ts* ts, int m, int max_iter, double ls_c, double ls_beta, int ls_max) {
   ts->lbfgs_m = m;
@@ -557,11 +600,14 @@ void orc_ts_elements(const orc_ts* ts, const double* u, double* psi, double* P, 
     double x[4][3], F[9], Pl[9], Hl[81], Hpl[81];
     tet_positions(ts, u, e, x);
     orc_def_grad(x[0], x[1], x[2], x[3], ts->B + 9 * e, F);
-    const double ps = orc_snh_stress(F, ts->lam[e], ts->mu[e], Pl);
+    const double ps = ts->model ? orc_lin_stress(F, ts->lam[e], ts->mu[e], Pl) : orc_snh_stress(F, ts->lam[e], ts->mu[e], Pl);
     if (psi) psi[e] = ps;
     if (P) memcpy(P + 9 * e, Pl, sizeof(Pl));
     if (H || Hp || K) {
-      orc_snh_hessian(F, ts->lam[e], ts->mu[e], Hl);
+      if (ts->model)
+        orc_lin_hessian(ts->lam[e], ts->mu[e], Hl);
+      else
+        orc_snh_hessian(F, ts->lam[e], ts->mu[e], Hl);
       if (H) memcpy(H + 81 * e, Hl, sizeof(Hl));
       if (Hp || (K && make_spsd)) {
         orc_spd_project9(Hl, 1e-3, Hpl);
@@ -601,7 +647,7 @@ double orc_ts_energy_grad(orc_ts* ts, const double* u, double* g) {
     double x[4][3], F[9], P[9];
     tet_positions(ts, u, e, x);
     orc_def_grad(x[0], x[1], x[2], x[3], ts->B + 9 * e, F);
-    ts->psi[e] = orc_snh_stress(F, ts->lam[e], ts->mu[e], P);
+    ts->psi[e] = ts->model ? orc_lin_stress(F, ts->lam[e], ts->mu[e], P) : orc_snh_stress(F, ts->lam[e], ts->mu[e], P);
     orc_local_force(P, ts->B + 9 * e, ts->floc + 12 * e);
   }
 #pragma omp parallel for schedule(dynamic, 256)
@@ -969,6 +1015,26 @@ double orc_ts_compute_step_ncg(orc_ts* ts, double* stats) {
   free(d);
   free(ut);
   free(gt);
+  return gn;
+}
+
+/* forward Euler stepper (time_step_solver_forward_euler, time_step.hpp:717-736): one gradient evaluation at the
+ * predictor, then u_next += ½·dt²·forces, then compute_step's final evaluation. */
+double orc_ts_compute_step_fe(orc_ts* ts, double* stats) {
+  const int n = ts->n;
+  const double h = 0.5 * ts->dt * ts->dt;
+  orc_ts_energy_grad(ts, ts->u_next, ts->g);
+  for (int i = 0; i < n; ++i) ts->u_next[i] += h * ts->g[i];
+  orc_ts_energy_grad(ts, ts->u_next, ts->g);
+  const double gn = sqrt(dot_n(n, ts->g, ts->g));
+  if (stats) {
+    stats[0] = 1; /* one explicit update */
+    stats[1] = 0;
+    stats[2] = 1; /* the solver's own evaluation; the final one is not counted, as for the other steppers */
+    stats[3] = 1;
+    stats[4] = gn;
+    stats[5] = 0;
+  }
   return gn;
 }
 

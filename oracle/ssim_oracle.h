@@ -56,6 +56,13 @@ void orc_ts_set_solver(orc_ts* ts, int max_newton, double pcg_rtol, int pcg_maxi
                        double ls_beta, int ls_max);
 /* full Newton steps (α = 1) after the Newton loop of compute_step (default 0) */
 void orc_ts_set_polish(orc_ts* ts, int polish);
+/* material model: 0 stable Neo-Hookean (default), 1 linear elasticity */
+void orc_ts_set_model(orc_ts* ts, int model);
+double orc_lin_energy(const double F[9], double lam, double mu);
+double orc_lin_stress(const double F[9], double lam, double mu, double P[9]);
+void orc_lin_hessian(double lam, double mu, double H[81]);
+/* forward Euler: u_next = û + ½dt²·g(û) (time_step.hpp:717-736) */
+double orc_ts_compute_step_fe(orc_ts* ts, double* stats);
 void orc_ts_set_lbfgs(orc_ts* ts, int m, int max_iter, double ls_c, double ls_beta, int ls_max);
 
 /* element outputs at deformation u (any NULL output is skipped) */

@@ -68,6 +68,8 @@ SIGNATURES = {
     "ssim_set_solver_params": [_V, ctypes.POINTER(SolverParams)],
     "ssim_get_solver_params": [_V, ctypes.POINTER(SolverParams)],
     "ssim_set_stepper": [_V, ctypes.c_int],
+    "ssim_set_material_model": [_V, ctypes.c_int],
+    "ssim_get_material_model": [_V, ctypes.POINTER(ctypes.c_int)],
     "ssim_mark_dirichlet": [_V, _I, ctypes.c_int, _D],
     "ssim_mark_general": [_V, _I, ctypes.c_int],
     "ssim_add_ext_force_dof": [_V, ctypes.c_int, ctypes.c_double],
@@ -126,6 +128,7 @@ SIGNATURES = {
     "ssim_batch_get_array_device": [_V, ctypes.c_int, _V],
     "ssim_batch_set_array_device": [_V, ctypes.c_int, _V],
     "ssim_batch_set_material": [_V, _D, _D],
+    "ssim_batch_set_material_model": [_V, ctypes.c_int],
     "ssim_batch_adjoint_step": [_V, _V, _V, _V, _V, _V, _I],
     "ssim_batch_get_layout_info": [_V, ctypes.POINTER(LayoutInfo)],
     "ssim_batch_set_tuning": [_V, ctypes.c_int, ctypes.c_int],

@@ -30,7 +30,10 @@ class TetFiniteElementBatch_Hip:
     """Backward-Euler FEM time step of a batch of meshes on one MI355X (FP64, Newton + Jacobi-PCG)."""
 
     def __init__(self, vertices: Sequence[np.ndarray], elements: Sequence[np.ndarray], lam: Sequence[np.ndarray],
-                 mu: Sequence[np.ndarray], time_step: float = 1e-2, density: float = 1e2):
+                 mu: Sequence[np.ndarray], time_step: float = 1e-2, density: float = 1e2,
+                 material: str = "stable_neohookean"):
+        if material not in ("stable_neohookean", "linear"):
+            raise ValueError(f"Unknown material: {material}")
         if not (len(vertices) == len(elements) == len(lam) == len(mu)) or len(vertices) == 0:
             raise ValueError("vertices, elements, lam and mu must be non-empty lists of equal length")
         Xs = [np.ascontiguousarray(x, dtype=np.float64) for x in vertices]
@@ -51,6 +54,8 @@ class TetFiniteElementBatch_Hip:
         _lib.call("ssim_batch_create", ctypes.byref(h), len(Xs), _ip(self._nv), _ip(self._nt), _dp(X), _ip(C),
                   _dp(lam), _dp(mu), float(density), float(time_step))
         self._h = h
+        _lib.call("ssim_batch_set_material_model", self._h, {"stable_neohookean": 0, "linear": 1}[material])
+        self.material = material
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -178,13 +183,13 @@ class TetFiniteElementBatch_Hip:
         return {k: getattr(info, k) for k, _ in info._fields_}
 
     @classmethod
-    def from_cases(cls, cases, rtol=None):
+    def from_cases(cls, cases, rtol=None, material="stable_neohookean"):
         """Batch of seeded ``ssim_amd.mesh.BarCase`` problems (all with the same dt and density)."""
         dt, rho = cases[0].dt, cases[0].rho
         if any(c.dt != dt or c.rho != rho for c in cases):
             raise ValueError("a batch shares one time step and one density")
         b = cls([c.X for c in cases], [c.cells for c in cases], [c.lam for c in cases], [c.mu for c in cases],
-                dt, rho)
+                dt, rho, material)
         for m, c in enumerate(cases):
             clamped = np.nonzero(c.dof_type.reshape(-1, 3).any(axis=1))[0].astype(np.int32)
             if clamped.size:

@@ -761,6 +761,16 @@ ssim_status_t ssim_batch_set_material(ssim_batch* b, const double* lambda, const
   });
 }
 
+ssim_status_t ssim_batch_set_material_model(ssim_batch* b, ssim_material_t model) {
+  if (bguard(b)) return SSIM_ERR_INVALID_ARGUMENT;
+  if (model != SSIM_MATERIAL_STABLE_NEOHOOKEAN && model != SSIM_MATERIAL_LINEAR) {
+    g_err = "unknown material model";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  b->c.model = model;
+  return SSIM_OK;
+}
+
 // Adjoint of one implicit step of every mesh at its converged u* = next_deform (see ssim_adjoint_step);
 // device arrays in the compact layout
 ssim_status_t ssim_batch_adjoint_step(ssim_batch* b, const double* grad_u, double* grad_uhat, double* grad_lambda,

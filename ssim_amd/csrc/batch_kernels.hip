@@ -28,7 +28,7 @@ __device__ __forceinline__ size_t bdof_w(int w, int j) {
 // ------------------------------------------------------------------------------------------------------
 // energy + gradient
 
-template <typename T>
+template <typename T, int Model>
 __global__ __launch_bounds__(kBlock) void k_belem_grad(MeshView m, BatchView bt, const double* __restrict__ u,
                                                        const double* __restrict__ uhat, double dt2,
                                                        const int* __restrict__ gperm, double* __restrict__ gcontrib,
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kBlock) void k_belem_grad(MeshView m, BatchView bt,
   if (!bt.mask[mesh]) return;
   const int e = blockIdx.x * kBlock + threadIdx.x;
   double energy[1] = {0.0};
-  if (e < bt.tend[mesh]) energy[0] = elem_grad_tet<T>(m, e, u, uhat, dt2, gperm, gcontrib);
+  if (e < bt.tend[mesh]) energy[0] = elem_grad_tet<T, Model>(m, e, u, uhat, dt2, gperm, gcontrib);
   store_partials<1>(energy, part_e);
 }
 
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_bfinalize_dot(BatchView bt, const do
 // ------------------------------------------------------------------------------------------------------
 // Hessian
 
-template <typename T>
+template <typename T, int Model>
 __global__ __launch_bounds__(kBlock) void k_belem_hess(MeshView m, BatchView bt, const double* __restrict__ u,
                                                        double dt2, int make_spsd, const int* __restrict__ hperm,
                                                        double* __restrict__ hcontrib) {
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_belem_hess(MeshView m, BatchView bt,
   if (!bt.mask[mesh]) return;
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= bt.tend[mesh]) return;
-  elem_hess_tet<T>(m, e, u, dt2, make_spsd, hperm, hcontrib);
+  elem_hess_tet<T, Model>(m, e, u, dt2, make_spsd, hperm, hcontrib);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bhess_reduce(BatchView bt, int nub, const int* __restrict__ hseg,
@@ -118,6 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_bhess_reduce(BatchView bt, int nub, 
 // ------------------------------------------------------------------------------------------------------
 // adjoint (element part) and conversions between the padded and the compact (caller's) layouts
 
+template <int Model>
 __global__ __launch_bounds__(kBlock) void k_belem_adjoint(MeshView m, BatchView bt, const double* __restrict__ u,
                                                           const double* __restrict__ lv, double dt2,
                                                           const int* __restrict__ gperm, double* __restrict__ gcontrib,
@@ -125,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_belem_adjoint(MeshView m, BatchView 
   const int mesh = bt.twg_mesh[blockIdx.x];
   if (!bt.mask[mesh]) return;
   const int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e < bt.tend[mesh]) elem_adjoint_tet(m, e, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+  if (e < bt.tend[mesh]) elem_adjoint_tet<Model>(m, e, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
 }
 
 // compact vertex v' of mesh m sits at padded vertex kBatchRows·vwg_off[m] + (v' − vcoff[m]); padding gets 0
@@ -365,10 +366,10 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const int*
 void launch_belem_grad(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* uhat,
                        double dt2, const int* gperm, double* gcontrib, double* part_e, bool fp32, hipStream_t s) {
   if (ntwg <= 0) return;
-  if (fp32)
-    k_belem_grad<float><<<ntwg, kBlock, 0, s>>>(m, bt, u, uhat, dt2, gperm, gcontrib, part_e);
-  else
-    k_belem_grad<double><<<ntwg, kBlock, 0, s>>>(m, bt, u, uhat, dt2, gperm, gcontrib, part_e);
+  dispatch_elem(m.model, fp32, [&](auto k) {
+    using K = decltype(k);
+    k_belem_grad<typename K::type, K::model><<<ntwg, kBlock, 0, s>>>(m, bt, u, uhat, dt2, gperm, gcontrib, part_e);
+  });
 }
 
 void launch_bgrad_reduce(const BatchView& bt, int nvwg, const int* gseg, const double* gcontrib,
@@ -393,10 +394,10 @@ void launch_bdot(const BatchView& bt, int nvwg, const double* a, const double* b
 void launch_belem_hess(const MeshView& m, const BatchView& bt, int ntwg, const double* u, double dt2, int make_spsd,
                        const int* hperm, double* hcontrib, bool fp32, hipStream_t s) {
   if (ntwg <= 0) return;
-  if (fp32)
-    k_belem_hess<float><<<ntwg, kBlock, 0, s>>>(m, bt, u, dt2, make_spsd, hperm, hcontrib);
-  else
-    k_belem_hess<double><<<ntwg, kBlock, 0, s>>>(m, bt, u, dt2, make_spsd, hperm, hcontrib);
+  dispatch_elem(m.model, fp32, [&](auto k) {
+    using K = decltype(k);
+    k_belem_hess<typename K::type, K::model><<<ntwg, kBlock, 0, s>>>(m, bt, u, dt2, make_spsd, hperm, hcontrib);
+  });
 }
 
 void launch_bhess_reduce(const BatchView& bt, int nub, const int* hseg, const double* hcontrib, const int2* ub_rc,
@@ -410,7 +411,10 @@ void launch_bhess_reduce(const BatchView& bt, int nub, const int* hseg, const do
 void launch_belem_adjoint(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* lv,
                           double dt2, const int* gperm, double* gcontrib, double* g_lam, double* g_mu, hipStream_t s) {
   if (ntwg <= 0) return;
-  k_belem_adjoint<<<ntwg, kBlock, 0, s>>>(m, bt, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+  if (m.model == kLinearElastic)
+    k_belem_adjoint<kLinearElastic><<<ntwg, kBlock, 0, s>>>(m, bt, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+  else
+    k_belem_adjoint<kStableNeoHookean><<<ntwg, kBlock, 0, s>>>(m, bt, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
 }
 
 void launch_bpad_vec(const BatchView& bt, int nvwg, const int* vcoff, const int* nvm, const double* compact,

@@ -67,6 +67,7 @@ struct ssim_ctx {
   bool per_tet = false;
   bool has_reset = false;
   ssim_stepper_t stepper = SSIM_STEPPER_NEWTON;
+  ssim_material_t model = SSIM_MATERIAL_STABLE_NEOHOOKEAN;
   ssim_solver_params prm{};
   ssim_step_stats stats{};
   hipStream_t stream = nullptr;
@@ -160,6 +161,7 @@ struct ssim_ctx {
     m.nv = nv;
     m.nt = nt;
     m.rho = rho_reset;
+    m.model = static_cast<int>(model);
     return m;
   }
   ssim_hip::BellView bell(const double* values) const {

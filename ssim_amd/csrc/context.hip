@@ -642,6 +642,21 @@ void ncg_solve(ssim_ctx* c) {
   c->stats.newton_iterations = it;
 }
 
+// Forward Euler (time_step_solver_forward_euler, time_step.hpp:717-736): one evaluation of the forces at the
+// predictor held in next_deform, then next_deform += ½·dt²·forces.  The forces array is the DBC-filtered
+// gradient of the incremental potential (time_step.hpp:577-624) and the update adds it with a positive sign,
+// exactly as the reference does; Dirichlet dofs keep their targets because their gradient entries are zero.
+void forward_euler_solve(ssim_ctx* c) {
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* f = c->a(SSIM_ARRAY_FORCES);
+  const EvalOut cur = eval_energy_grad(c, u, f);
+  if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+  launch_axpby(c->n, 0.5 * c->dt * c->dt, f, 1.0, u, c->stream);
+  HIPCHK(hipGetLastError());
+  c->stats.newton_iterations = 1;
+  c->stats.converged = 1;
+}
+
 ssim_status_t guard(ssim_ctx* c) {
   if (!c) {
     g_err = "null context";
@@ -807,6 +822,25 @@ ssim_status_t ssim_set_material_per_tet(ssim_ctx* ctx, const double* lambda, con
     ctx->hrest.release();
   });
 }
+ssim_status_t ssim_set_material_model(ssim_ctx* ctx, ssim_material_t model) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  if (model != SSIM_MATERIAL_STABLE_NEOHOOKEAN && model != SSIM_MATERIAL_LINEAR) {
+    g_err = "unknown material model";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  return run([&] {
+    if (ctx->model != model) ctx->hrest.release();
+    ctx->model = model;
+  });
+}
+ssim_status_t ssim_get_material_model(const ssim_ctx* ctx, ssim_material_t* model) {
+  if (!ctx || !model) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  *model = ctx->model;
+  return SSIM_OK;
+}
 ssim_status_t ssim_set_solver_params(ssim_ctx* ctx, const ssim_solver_params* p) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
   return run([&] {
@@ -830,7 +864,8 @@ ssim_status_t ssim_get_solver_params(const ssim_ctx* ctx, ssim_solver_params* p)
 }
 ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
-  if (stepper != SSIM_STEPPER_NEWTON && stepper != SSIM_STEPPER_LBFGS && stepper != SSIM_STEPPER_NCG) {
+  if (stepper != SSIM_STEPPER_NEWTON && stepper != SSIM_STEPPER_LBFGS && stepper != SSIM_STEPPER_NCG &&
+      stepper != SSIM_STEPPER_FORWARD_EULER) {
     g_err = "unknown stepper";
     return SSIM_ERR_INVALID_ARGUMENT;
   }
@@ -889,8 +924,10 @@ ssim_status_t ssim_compute_step(ssim_ctx* ctx, double* residual_out) {
       newton_solve(ctx);
     else if (ctx->stepper == SSIM_STEPPER_LBFGS)
       lbfgs_solve(ctx);
-    else
+    else if (ctx->stepper == SSIM_STEPPER_NCG)
       ncg_solve(ctx);
+    else
+      forward_euler_solve(ctx);
     // compute_step: final update_energy_and_gradients and ‖forces‖ (time_step.hpp:520-528)
     const EvalOut fin = eval_energy_grad(ctx, ctx->a(SSIM_ARRAY_NEXT_DEFORM), ctx->a(SSIM_ARRAY_FORCES));
     ctx->stats.grad_norm = std::sqrt(fin.gg);

@@ -84,6 +84,28 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, in
   block_sum<K, NT>(tot);
 }
 
+// Host-side dispatch of the element kernels over (arithmetic type, material model): fn(ElemKind<T, Model>{}).
+template <typename T, int Model>
+struct ElemKind {
+  using type = T;
+  static constexpr int model = Model;
+};
+
+template <typename Fn>
+void dispatch_elem(int model, bool fp32, Fn&& fn) {
+  if (fp32) {
+    if (model == kLinearElastic)
+      fn(ElemKind<float, kLinearElastic>{});
+    else
+      fn(ElemKind<float, kStableNeoHookean>{});
+  } else {
+    if (model == kLinearElastic)
+      fn(ElemKind<double, kLinearElastic>{});
+    else
+      fn(ElemKind<double, kStableNeoHookean>{});
+  }
+}
+
 // ------------------------------------------------------------------------------------------------------
 // element gathers
 
@@ -187,7 +209,7 @@ __host__ __device__ constexpr int pair_index(int v, int w) { return w * (w + 1) 
 // Hessian blocks (update_hessian, time_step.hpp:626-658; wrapped_pfpx :41-59).
 
 // energy of tet e at deformation u (predictor uhat); its 4 gradient contributions go to the slots gperm[4e+i]
-template <typename T>
+template <typename T, int Model>
 __device__ __forceinline__ double elem_grad_tet(const MeshView& m, int e, const double* __restrict__ u,
                                                 const double* __restrict__ uhat, double dt2,
                                                 const int* __restrict__ gperm, double* __restrict__ gcontrib) {
@@ -204,15 +226,13 @@ __device__ __forceinline__ double elem_grad_tet(const MeshView& m, int e, const 
     }
   double Fd[9];
   def_grad(x, t.B, Fd);
-  T F[9], Bt[9], C[9], P[9];
+  T F[9], Bt[9], P[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     F[k] = (T)Fd[k];
     Bt[k] = (T)t.B[k];
   }
-  const SnhScalars<T> s = snh_scalars(F, (T)t.lam, (T)t.mu);
-  cofactor(F, C);
-  snh_stress(F, C, s, P);
+  const T psi = material_stress<Model>(F, (T)t.lam, (T)t.mu, P);
   T f[4][3];
   local_force(P, Bt, f);
   const double mm = m.rho * t.V / 20.0;
@@ -231,11 +251,11 @@ __device__ __forceinline__ double elem_grad_tet(const MeshView& m, int e, const 
   }
 #pragma unroll
   for (int a = 0; a < 3; ++a) ss += S[a] * S[a];
-  return 0.5 * mm * (dd + ss) + sc * (double)s.psi;
+  return 0.5 * mm * (dd + ss) + sc * (double)psi;
 }
 
 // the 10 oriented upper blocks of tet e (dt²V-scaled stiffness plus consistent mass) to the slots hperm[10e+k]
-template <typename T>
+template <typename T, int Model>
 __device__ __forceinline__ void elem_hess_tet(const MeshView& m, int e, const double* __restrict__ u, double dt2,
                                               int make_spsd, const int* __restrict__ hperm,
                                               double* __restrict__ hcontrib) {
@@ -250,17 +270,13 @@ __device__ __forceinline__ void elem_hess_tet(const MeshView& m, int e, const do
     }
   double Fd[9];
   def_grad(x, t.B, Fd);
-  T F[9], Bt[9], C[9], H[45], D[4][3];
+  T F[9], Bt[9], H[45], D[4][3];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     F[k] = (T)Fd[k];
     Bt[k] = (T)t.B[k];
   }
-  const T lam = (T)t.lam;
-  const SnhScalars<T> s = snh_scalars(F, lam, (T)t.mu);
-  cofactor(F, C);
-  snh_hessian_packed(F, C, lam, s, H);
-  if (make_spsd) snh_project_spd(F, lam, s, (T)kSpdFloor, H);
+  material_hessian<Model>(F, (T)t.lam, (T)t.mu, make_spsd, (T)kSpdFloor, H);
   shape_grads(Bt, D);
   const double sc = dt2 * t.V;
   const double mm = m.rho * t.V / 20.0;
@@ -286,6 +302,8 @@ __device__ __forceinline__ void elem_hess_tet(const MeshView& m, int e, const do
 // because the step's residual depends on (λ, μ) only through the elastic forces V·P·∇N_i.  With
 // P = λ(J − α)·cof F + μ(1 − 1/(Ic+δ))·F and α = 1 + (1 − 1/(3+δ))·μ/λ (elast/stable_neohookean.hpp:23-62):
 //   ∂P/∂λ = (J − 1)·cof F,   ∂P/∂μ = −(1 − 1/(3+δ))·cof F + (1 − 1/(Ic+δ))·F.
+// With the linear model's P = 2μ·ε + λ·tr(ε)·I (elast/linear.hpp:37-50): ∂P/∂λ = tr(ε)·I, ∂P/∂μ = 2ε.
+template <int Model>
 __device__ __forceinline__ void elem_adjoint_tet(const MeshView& m, int e, const double* __restrict__ u,
                                                  const double* __restrict__ lv, double dt2,
                                                  const int* __restrict__ gperm, double* __restrict__ gcontrib,
@@ -300,22 +318,35 @@ __device__ __forceinline__ void elem_adjoint_tet(const MeshView& m, int e, const
       x[i][a] = m.X[k] + u[k];
       l[i][a] = lv[k];
     }
-  double F[9], G[9], C[9];
+  double F[9], G[9];
   def_grad(x, t.B, F);
   def_grad(l, t.B, G);
-  cofactor(F, C);
-  double Ic = 0, cG = 0, fG = 0;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    Ic += F[k] * F[k];
-    cG += C[k] * G[k];
-    fG += F[k] * G[k];
-  }
-  const double J = det3(F);
-  constexpr double del = 1.0;
   const double sc = -dt2 * t.V;
-  g_lam[e] = sc * (J - 1.0) * cG;
-  g_mu[e] = sc * (-(1.0 - 1.0 / (3.0 + del)) * cG + (1.0 - 1.0 / (Ic + del)) * fG);
+  if constexpr (Model == kLinearElastic) {
+    const double tr = F[0] + F[4] + F[8] - 3.0;
+    double eG = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        eG += (0.5 * (F[3 * a + c] + F[3 * c + a]) - (a == c ? 1.0 : 0.0)) * G[3 * a + c];
+    g_lam[e] = sc * tr * (G[0] + G[4] + G[8]);
+    g_mu[e] = sc * 2.0 * eG;
+  } else {
+    double C[9];
+    cofactor(F, C);
+    double Ic = 0, cG = 0, fG = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      Ic += F[k] * F[k];
+      cG += C[k] * G[k];
+      fG += F[k] * G[k];
+    }
+    const double J = det3(F);
+    constexpr double del = 1.0;
+    g_lam[e] = sc * (J - 1.0) * cG;
+    g_mu[e] = sc * (-(1.0 - 1.0 / (3.0 + del)) * cG + (1.0 - 1.0 / (Ic + del)) * fG);
+  }
   const double mm = m.rho * t.V / 20.0;
   double S[3];
 #pragma unroll

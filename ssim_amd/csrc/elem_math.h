@@ -1,4 +1,5 @@
-// elem_math.h -- per-tetrahedron math of the stable Neo-Hookean FEM step (device + host inline).
+// elem_math.h -- per-tetrahedron math of the FEM step (device + host inline): stable Neo-Hookean and linear
+// elasticity.
 //
 // Conventions (as in the reference): row-major 3x3 matrices, F[a][c] = Σ_k Ds[a][k]·B[k][c] with
 // Ds = [x1−x0 | x2−x0 | x3−x0] and B = Dm⁻¹ (finite_elements/def_grad.hpp:229-288); the 9x9 F-space Hessian
@@ -308,6 +309,90 @@ SSIM_HD void snh_project_spd(const T F[9], T lam, const SnhScalars<T>& s, T floo
                          W[2][m] * U[a * 3 + 2] * V[c * 3 + 2];
       rank1(Q, floor_ - lm);
     }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Linear elasticity (elast/linear.hpp:24-69; strain elast/internal_physic.hpp:16-19):
+//   ε = ½(F + Fᵀ) − I,  Ψ = μ‖ε‖² + ½λ·tr(ε)²,  P = 2μ·ε + λ·tr(ε)·I,
+//   H = μ·(I + S) + λ·t tᵀ with S the transposition vec(F) → vec(Fᵀ) and t = vec(I).
+
+template <typename T>
+SSIM_HD T linear_stress(const T F[9], T lam, T mu, T P[9]) {
+  const T tr = F[0] + F[4] + F[8] - T(3);
+  T n2 = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T e = T(0.5) * (F[3 * a + c] + F[3 * c + a]) - (a == c ? T(1) : T(0));
+      n2 += e * e;
+      P[3 * a + c] = T(2) * mu * e + (a == c ? lam * tr : T(0));
+    }
+  return mu * n2 + T(0.5) * lam * tr * tr;
+}
+
+template <typename T>
+SSIM_HD void linear_hessian_packed(T lam, T mu, T H[45]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = i; j < 9; ++j) {
+      const int a = i / 3, b = i % 3, c = j / 3, d = j % 3;
+      H[sym9(i, j)] = (i == j ? mu : T(0)) + (a == d && b == c ? mu : T(0)) + (a == b && c == d ? lam : T(0));
+    }
+}
+
+// SPD projection in closed form.  H has three eigenspaces: the skew matrices (eigenvalue 0, projector ½(I − S)),
+// the trace-free symmetric matrices (2μ, projector ½(I + S) − t tᵀ/3) and the multiples of I (2μ + 3λ,
+// projector t tᵀ/3).  Eigenvalues under the floor are raised to it, as the 9x9 eigen-decomposition of the
+// reference does (finite_elements/time_step.hpp:50-56).
+template <typename T>
+SSIM_HD void linear_project_spd(T lam, T mu, T floor_, T H[45]) {
+  const T ka = floor_ > T(0) ? floor_ : T(0);
+  const T kd = floor_ > T(2) * mu ? floor_ - T(2) * mu : T(0);
+  const T kv = floor_ > T(2) * mu + T(3) * lam ? floor_ - T(2) * mu - T(3) * lam : T(0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = i; j < 9; ++j) {
+      const int a = i / 3, b = i % 3, c = j / 3, d = j % 3;
+      const T id = i == j ? T(0.5) : T(0);
+      const T sw = a == d && b == c ? T(0.5) : T(0);
+      const T tt = a == b && c == d ? T(1) / T(3) : T(0);
+      H[sym9(i, j)] += ka * (id - sw) + kd * (id + sw - tt) + kv * tt;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------------
+// Material dispatch: the element bodies call these two with the model as a template argument.
+
+enum MaterialModel : int { kStableNeoHookean = 0, kLinearElastic = 1 };
+
+template <int Model, typename T>
+SSIM_HD T material_stress(const T F[9], T lam, T mu, T P[9]) {
+  if constexpr (Model == kLinearElastic) {
+    return linear_stress(F, lam, mu, P);
+  } else {
+    const SnhScalars<T> s = snh_scalars(F, lam, mu);
+    T C[9];
+    cofactor(F, C);
+    snh_stress(F, C, s, P);
+    return s.psi;
+  }
+}
+
+template <int Model, typename T>
+SSIM_HD void material_hessian(const T F[9], T lam, T mu, int make_spsd, T floor_, T H[45]) {
+  if constexpr (Model == kLinearElastic) {
+    linear_hessian_packed(lam, mu, H);
+    if (make_spsd) linear_project_spd(lam, mu, floor_, H);
+  } else {
+    const SnhScalars<T> s = snh_scalars(F, lam, mu);
+    T C[9];
+    cofactor(F, C);
+    snh_hessian_packed(F, C, lam, s, H);
+    if (make_spsd) snh_project_spd(F, lam, s, floor_, H);
   }
 }
 

@@ -23,6 +23,7 @@ struct MeshView {
   const double* mu;     // [nt]
   int nv, nt;
   double rho;
+  int model = 0;  // MaterialModel (elem_math.h)
 };
 
 // Symmetric blocked ELLPACK of 3x3 blocks.  Rows (vertices) are grouped in slices of 64 lanes.

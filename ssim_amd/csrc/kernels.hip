@@ -53,24 +53,24 @@ void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, do
 // The consistent mass M_e = ρV(1+δ_ij)/20 (bilinears.hpp:140-148) is applied element by element instead of
 // through a CSR SpMV.
 
-template <typename T>
+template <typename T, int Model>
 __global__ __launch_bounds__(kBlock) void k_elem_grad(MeshView m, const double* __restrict__ u,
                                                       const double* __restrict__ uhat, double dt2,
                                                       const int* __restrict__ gperm, double* __restrict__ gcontrib,
                                                       double* __restrict__ part_e) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   double energy[1] = {0.0};
-  if (e < m.nt) energy[0] = elem_grad_tet<T>(m, e, u, uhat, dt2, gperm, gcontrib);
+  if (e < m.nt) energy[0] = elem_grad_tet<T, Model>(m, e, u, uhat, dt2, gperm, gcontrib);
   store_partials<1>(energy, part_e);
 }
 
 int launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
                      double* gcontrib, double* part_e, bool fp32, hipStream_t s) {
   const int nb = nblocks(m.nt > 0 ? m.nt : 1);
-  if (fp32)
-    k_elem_grad<float><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, part_e);
-  else
-    k_elem_grad<double><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, part_e);
+  dispatch_elem(m.model, fp32, [&](auto k) {
+    using K = decltype(k);
+    k_elem_grad<typename K::type, K::model><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, part_e);
+  });
   return nb;
 }
 
@@ -120,22 +120,22 @@ void launch_finalize(const double* pa, int na, int ka, const double* pb, int nb,
 // ------------------------------------------------------------------------------------------------------
 // Hessian (update_hessian, time_step.hpp:626-658; wrapped_pfpx :41-59)
 
-template <typename T>
+template <typename T, int Model>
 __global__ __launch_bounds__(kBlock) void k_elem_hess(MeshView m, const double* __restrict__ u, double dt2,
                                                       int make_spsd, const int* __restrict__ hperm,
                                                       double* __restrict__ hcontrib) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= m.nt) return;
-  elem_hess_tet<T>(m, e, u, dt2, make_spsd, hperm, hcontrib);
+  elem_hess_tet<T, Model>(m, e, u, dt2, make_spsd, hperm, hcontrib);
 }
 
 void launch_elem_hess(const MeshView& m, const double* u, double dt2, int make_spsd, const int* hperm,
                       double* hcontrib, bool fp32, hipStream_t s) {
   if (m.nt <= 0) return;
-  if (fp32)
-    k_elem_hess<float><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, dt2, make_spsd, hperm, hcontrib);
-  else
-    k_elem_hess<double><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, dt2, make_spsd, hperm, hcontrib);
+  dispatch_elem(m.model, fp32, [&](auto k) {
+    using K = decltype(k);
+    k_elem_hess<typename K::type, K::model><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, dt2, make_spsd, hperm, hcontrib);
+  });
 }
 
 // One thread per upper block (r ≤ c): sum its contiguous contribution run, apply the Dirichlet identity
@@ -161,7 +161,7 @@ void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const 
 }
 
 // per-element outputs for parity checks: Ψ, P, H (full 9x9, projected if make_spsd), K (full 12x12, unscaled)
-template <typename T>
+template <typename T, int Model>
 __global__ __launch_bounds__(kBlock) void k_elem_eval(MeshView m, const double* __restrict__ u, int make_spsd,
                                                       double* psi, double* Pout, double* Hout, double* Kout) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
@@ -177,22 +177,18 @@ __global__ __launch_bounds__(kBlock) void k_elem_eval(MeshView m, const double* 
     }
   double Fd[9];
   def_grad(x, t.B, Fd);
-  T F[9], Bt[9], C[9], P[9], H[45], D[4][3];
+  T F[9], Bt[9], P[9], H[45], D[4][3];
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     F[k] = (T)Fd[k];
     Bt[k] = (T)t.B[k];
   }
-  const T lam = (T)t.lam;
-  const SnhScalars<T> s = snh_scalars(F, lam, (T)t.mu);
-  cofactor(F, C);
-  snh_stress(F, C, s, P);
-  if (psi) psi[e] = (double)s.psi;
+  const T ps = material_stress<Model>(F, (T)t.lam, (T)t.mu, P);
+  if (psi) psi[e] = (double)ps;
   if (Pout)
     for (int k = 0; k < 9; ++k) Pout[9 * (size_t)e + k] = (double)P[k];
   if (Hout || Kout) {
-    snh_hessian_packed(F, C, lam, s, H);
-    if (make_spsd) snh_project_spd(F, lam, s, (T)kSpdFloor, H);
+    material_hessian<Model>(F, (T)t.lam, (T)t.mu, make_spsd, (T)kSpdFloor, H);
     if (Hout)
       for (int i = 0; i < 9; ++i)
         for (int j = 0; j < 9; ++j) Hout[81 * (size_t)e + i * 9 + j] = (double)H[sym9(i, j)];
@@ -213,27 +209,31 @@ __global__ __launch_bounds__(kBlock) void k_elem_eval(MeshView m, const double* 
 void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double* psi, double* P, double* H,
                       double* K, bool fp32, hipStream_t s) {
   if (m.nt <= 0) return;
-  if (fp32)
-    k_elem_eval<float><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, make_spsd, psi, P, H, K);
-  else
-    k_elem_eval<double><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, make_spsd, psi, P, H, K);
+  dispatch_elem(m.model, fp32, [&](auto k) {
+    using Kd = decltype(k);
+    k_elem_eval<typename Kd::type, Kd::model><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, make_spsd, psi, P, H, K);
+  });
 }
 
 // ------------------------------------------------------------------------------------------------------
 // adjoint of one step (element part) and the Dirichlet mask of its right-hand side
 
+template <int Model>
 __global__ __launch_bounds__(kBlock) void k_elem_adjoint(MeshView m, const double* __restrict__ u,
                                                          const double* __restrict__ lv, double dt2,
                                                          const int* __restrict__ gperm, double* __restrict__ gcontrib,
                                                          double* __restrict__ g_lam, double* __restrict__ g_mu) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e < m.nt) elem_adjoint_tet(m, e, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+  if (e < m.nt) elem_adjoint_tet<Model>(m, e, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
 }
 
 void launch_elem_adjoint(const MeshView& m, const double* u, const double* lv, double dt2, const int* gperm,
                          double* gcontrib, double* g_lam, double* g_mu, hipStream_t s) {
   if (m.nt <= 0) return;
-  k_elem_adjoint<<<nblocks(m.nt), kBlock, 0, s>>>(m, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+  if (m.model == kLinearElastic)
+    k_elem_adjoint<kLinearElastic><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
+  else
+    k_elem_adjoint<kStableNeoHookean><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, lv, dt2, gperm, gcontrib, g_lam, g_mu);
 }
 
 __global__ void k_mask_dbc(int n, const unsigned char* __restrict__ dof, const double* __restrict__ in,

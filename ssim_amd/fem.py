@@ -37,14 +37,23 @@ class TetFiniteElementSolver_Hip:
     """Implicit (backward Euler) tetrahedral FEM time step on one MI355X (FP64).
 
     method: "newton" (Newton + Jacobi-PCG + backtracking Armijo), "lbfgs" (L-BFGS, m = 8, H0 = diagonal
-    of the rest Hessian) or "ncg" (nonlinear CG, Dai-Kou β, the L-BFGS line-search constants).
+    of the rest Hessian), "ncg" (nonlinear CG, Dai-Kou β, the L-BFGS line-search constants) or
+    "forward_euler" (explicit: next = û + ½dt²·forces(û), finite_elements/time_step.hpp:717-736).
+    material: "stable_neohookean" (elast/stable_neohookean.hpp, the reference bindings' model) or "linear"
+    (elast/linear.hpp).
     """
+
+    METHODS = {"newton": 0, "lbfgs": 1, "ncg": 2, "forward_euler": 3}
+    MATERIALS = {"stable_neohookean": 0, "linear": 1}
 
     def __init__(self, vertices: np.ndarray, elements: np.ndarray, time_step: float = 1e-2,
                  young_modulus: float = 1e6, poisson_ratio: float = 0.33, density: float = 1e2,
-                 method: Literal["newton", "lbfgs", "ncg"] = "newton"):
-        if method not in ("newton", "lbfgs", "ncg"):
+                 method: Literal["newton", "lbfgs", "ncg", "forward_euler"] = "newton",
+                 material: Literal["stable_neohookean", "linear"] = "stable_neohookean"):
+        if method not in self.METHODS:
             raise ValueError(f"Unknown method: {method}")
+        if material not in self.MATERIALS:
+            raise ValueError(f"Unknown material: {material}")
         X = np.ascontiguousarray(vertices, dtype=np.float64)
         C = np.ascontiguousarray(elements, dtype=np.int32)
         if X.ndim != 2 or X.shape[1] != 3:
@@ -59,8 +68,10 @@ class TetFiniteElementSolver_Hip:
                   float(young_modulus), float(poisson_ratio), float(density))
         self._h = h
         self._has_reset = False
-        _lib.call("ssim_set_stepper", self._h, {"newton": 0, "lbfgs": 1, "ncg": 2}[method])
+        _lib.call("ssim_set_stepper", self._h, self.METHODS[method])
         self.method = method
+        _lib.call("ssim_set_material_model", self._h, self.MATERIALS[material])
+        self.material = material
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -282,13 +293,13 @@ class TetFiniteElementSolver_Hip:
         return {k: getattr(info, k) for k, _ in info._fields_}
 
     @classmethod
-    def from_case(cls, case, method="newton", rtol=None):
+    def from_case(cls, case, method="newton", rtol=None, material="stable_neohookean"):
         """Solver for a seeded ``ssim_amd.mesh.BarCase`` (per-tet material, Dirichlet set, loads, initial state)."""
         lam0, mu0 = float(case.lam[0]), float(case.mu[0])
         # any valid (E, ν) for the constructor; the per-tet arrays below define the material
         nu = lam0 / (2 * (lam0 + mu0))
         E = mu0 * 2 * (1 + nu)
-        s = cls(case.X, case.cells, case.dt, E, nu, case.rho, method)
+        s = cls(case.X, case.cells, case.dt, E, nu, case.rho, method, material)
         s.set_material_per_tet(case.lam, case.mu)  # exact per-tet (λ, μ), uniform or not
         clamped = np.nonzero(case.dof_type.reshape(-1, 3).any(axis=1))[0].astype(np.int32)
         if clamped.size:

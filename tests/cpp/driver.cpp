@@ -2,7 +2,7 @@
 // the reference's drivers use basic_time_step (tests/fem_time_step/main.cpp): construct from a mesh, mark the
 // clamped vertices, add gravity, reset with a stepper, then prepare_step / compute_step / step_next.
 //
-// Input file (little-endian): int32 nv, nt, steps, stepper (0 Newton, 1 L-BFGS, 2 NCG); float64 dt, E, nu, rho, rtol,
+// Input file (little-endian): int32 nv, nt, steps, stepper (0 Newton, 1 L-BFGS, 2 NCG, 3 forward Euler); float64 dt, E, nu, rho, rtol,
 // gravity_y; float64 X[nv*3]; int32 cells[nt*4]; uint8 clamped[nv]; float64 u0[nv*3].
 // Output file: float64 u[nv*3] after the last step, then per step int32 newton_iterations, pcg_iterations.
 #include <cstdio>
@@ -50,10 +50,13 @@ int main(int argc, char** argv) {
     ssim::hip::newton_solver newton;
     ssim::hip::lbfgs_solver lbfgs;
     ssim::hip::ncg_solver ncg;
+    ssim::hip::forward_euler_solver euler;
     if (stepper == 1)
       step.reset(lbfgs);
     else if (stepper == 2)
       step.reset(ncg);
+    else if (stepper == 3)
+      step.reset(euler);
     else
       step.reset(newton);
     std::vector<int32_t> counts;
@@ -63,6 +66,8 @@ int main(int argc, char** argv) {
         step.compute_step(lbfgs);
       else if (stepper == 2)
         step.compute_step(ncg);
+      else if (stepper == 3)
+        step.compute_step(euler);
       else
         step.compute_step(newton);
       step.step_next();

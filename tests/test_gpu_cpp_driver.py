@@ -1,6 +1,6 @@
 """The header-only C++ front end (include/ssim_hip.hpp) driven by a C++ program, against the Python front end
 on the same bar: identical positions (bitwise) and iteration counts after 5 steps of the 8×8×32 bar, for the
-Newton, L-BFGS and nonlinear-CG steppers."""
+Newton, L-BFGS, nonlinear-CG and forward-Euler steppers."""
 import os
 import subprocess
 
@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 DRIVER = os.path.join(os.path.dirname(__file__), "cpp", "build", "driver")
 
 
-@pytest.mark.parametrize("stepper", [0, 1, 2])
+@pytest.mark.parametrize("stepper", [0, 1, 2, 3])
 def test_cpp_driver_matches_python_front_end(tmp_path, stepper):
     if not os.path.exists(DRIVER):
         pytest.fail("tests/cpp/build/driver is not built (python tests/cpp/build.py)")
@@ -36,7 +36,8 @@ def test_cpp_driver_matches_python_front_end(tmp_path, stepper):
     u_cpp = raw[:8 * 3 * case.nv].view(np.float64)
     counts = raw[8 * 3 * case.nv:].view(np.int32).reshape(steps, 2)
 
-    s = fem.TetFiniteElementSolver_Hip(case.X, case.cells, dt, E, nu, rho, ["newton", "lbfgs", "ncg"][stepper])
+    method = ["newton", "lbfgs", "ncg", "forward_euler"][stepper]
+    s = fem.TetFiniteElementSolver_Hip(case.X, case.cells, dt, E, nu, rho, method)
     idx = np.nonzero(clamped)[0].astype(np.int32)
     s.mark_dirichlet_batched(idx, np.zeros((3, idx.size)))
     s.add_gravity(np.array([0.0, g, 0.0]))
