@@ -78,6 +78,9 @@ struct ssim_ctx {
 
   // host topology (block adjacency, rows and columns ascending)
   std::vector<int> brow, bcol, kdiag;
+  // blocked-ELL slot of every block of the adjacency: upper blocks (k ≥ kdiag) their upper slot, lower blocks
+  // (k < kdiag) their lower-reference slot
+  std::vector<int> bslot;
   std::vector<int> hwu, hoffu;  // symmetric blocked ELL: upper widths and slot offsets per slice
   long long ncolslots = 0;      // upper slots (with padding)
   long long nlowslots = 0;      // lower reference slots (with padding)

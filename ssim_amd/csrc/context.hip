@@ -69,6 +69,97 @@ void ssim_hip::sync(ssim_ctx* c) {
 
 namespace {
 
+// Blocked-ELL slots of every block.  By default the upper blocks of a row take slots 0, 1, ... in column order
+// (diagonal first) and the lower references likewise.  In the fixed 8/7 layout the slots are aligned by
+// direction instead: slot k ≥ 1 holds, where it can, the block at the column offset w − v that sits at sorted
+// position k in most full rows (on a structured mesh every row then keeps each neighbour direction in the same
+// slot), and the lower reference to the block stored in slot k of the neighbour's row goes to lower slot 7 − k.
+// The fixed-width SpMV pairs upper slot k with lower slot 7 − k, so that for most rows of a slice the lower
+// reads and the neighbouring slice's own reads of the same lines are issued at the same point of the program
+// and meet in L2.  Slots only move blocks; every block is still read and summed once per row.
+void assign_slots(ssim_ctx* c, const std::vector<int>& len) {
+  const int nv = c->nv;
+  c->bslot.assign(std::max(c->brow[nv], 1), 0);
+  for (int v = 0; v < nv; ++v) {
+    const int kd = c->kdiag[v];
+    for (int k = 0; k < len[v]; ++k) c->bslot[c->brow[v] + k] = k < kd ? k : k - kd;
+  }
+  if (!c->fixed_layout) return;
+  static_assert(kFixedWL == kFixedWU - 1, "the slot pairing needs one lower slot per off-diagonal upper slot");
+  // modal column offset of each sorted upper position over the rows with all kFixedWU upper blocks
+  std::vector<std::vector<int>> offs(kFixedWU);
+  for (int v = 0; v < nv; ++v)
+    if (len[v] - c->kdiag[v] == kFixedWU)
+      for (int q = 1; q < kFixedWU; ++q) offs[q].push_back(c->bcol[c->brow[v] + c->kdiag[v] + q] - v);
+  std::vector<long long> modal(kFixedWU, -1);  // −1: no canonical offset (offsets are positive)
+  for (int q = 1; q < kFixedWU; ++q) {
+    auto& o = offs[q];
+    if (o.empty()) continue;
+    std::sort(o.begin(), o.end());
+    int best = o[0], run = 0, best_run = 0;
+    for (size_t i = 0; i < o.size(); ++i) {
+      run = (i > 0 && o[i] == o[i - 1]) ? run + 1 : 1;
+      if (run > best_run) {
+        best_run = run;
+        best = o[i];
+      }
+    }
+    modal[q] = best;
+    o.clear();
+    o.shrink_to_fit();
+  }
+  // upper slots: the diagonal in slot 0, canonical directions in their slot, the rest in the free slots
+  for (int v = 0; v < nv; ++v) {
+    const int kd = c->kdiag[v];
+    bool used[kFixedWU] = {true};
+    int rest[kFixedWU], nrest = 0;
+    c->bslot[c->brow[v] + kd] = 0;
+    for (int k = kd + 1; k < len[v]; ++k) {
+      const long long off = c->bcol[c->brow[v] + k] - v;
+      int q = 1;
+      while (q < kFixedWU && !(modal[q] == off && !used[q])) ++q;
+      if (q < kFixedWU) {
+        c->bslot[c->brow[v] + k] = q;
+        used[q] = true;
+      } else {
+        rest[nrest++] = k;
+      }
+    }
+    for (int i = 0, q = 1; i < nrest; ++i) {
+      while (used[q]) ++q;
+      c->bslot[c->brow[v] + rest[i]] = q;
+      used[q] = true;
+    }
+  }
+  // lower references: the block stored in slot k of row w goes to lower slot kFixedWU − 1 − k where free
+  auto upper_slot_of = [&](int w, int v) -> int {
+    const int* b = c->bcol.data() + c->brow[w];
+    const int* e = c->bcol.data() + c->brow[w + 1];
+    const int* it = std::lower_bound(b, e, v);
+    if (it == e || *it != v) fail(SSIM_ERR_STATE, "internal: missing block in sparsity pattern");
+    return c->bslot[c->brow[w] + (int)(it - b)];
+  };
+  for (int v = 0; v < nv; ++v) {
+    const int kd = c->kdiag[v];
+    bool used[kFixedWL] = {};
+    int rest[kFixedWL], nrest = 0;
+    for (int k = 0; k < kd; ++k) {
+      const int j = kFixedWU - 1 - upper_slot_of(c->bcol[c->brow[v] + k], v);
+      if (j >= 0 && j < kFixedWL && !used[j]) {
+        c->bslot[c->brow[v] + k] = j;
+        used[j] = true;
+      } else {
+        rest[nrest++] = k;
+      }
+    }
+    for (int i = 0, j = 0; i < nrest; ++i) {
+      while (used[j]) ++j;
+      c->bslot[c->brow[v] + rest[i]] = j;
+      used[j] = true;
+    }
+  }
+}
+
 // Sparsity, blocked-ELL layout and the two assembly permutations.
 void build_topology(ssim_ctx* c) {
   const int nv = c->nv, nt = c->nt;
@@ -139,6 +230,7 @@ void build_topology(ssim_ctx* c) {
     }
   if (c->spmv_lanes == 0 && !c->fixed_layout) c->spmv_lanes = 1;
   if (c->spmv_lanes_default) c->spmv_lanes = c->fixed_layout ? 0 : 1;
+  assign_slots(c, len);
   long long offu = 0, offl = 0;
   for (int s = 0; s < c->nslices; ++s) {
     c->hoffu[s] = (int)offu;
@@ -168,10 +260,12 @@ void build_topology(ssim_ctx* c) {
   for (int v = 0; v < nv; ++v) {
     const int s = v / kSlice, lane = v % kSlice;
     const int kd = c->kdiag[v];
-    for (int k = kd; k < len[v]; ++k) hucol[c->hoffu[s] + (k - kd) * kSlice + lane] = c->bcol[c->brow[v] + k];
+    for (int k = kd; k < len[v]; ++k)
+      hucol[c->hoffu[s] + c->bslot[c->brow[v] + k] * kSlice + lane] = c->bcol[c->brow[v] + k];
     for (int k = 0; k < kd; ++k) {
       const int w = c->bcol[c->brow[v] + k];
-      hlref[hoffl[s] + k * kSlice + lane] = make_int2(w, valpos(w, slot(w, v) - c->kdiag[w]));
+      hlref[hoffl[s] + c->bslot[c->brow[v] + k] * kSlice + lane] =
+          make_int2(w, valpos(w, c->bslot[c->brow[w] + slot(w, v)]));
     }
   }
 
@@ -185,7 +279,7 @@ void build_topology(ssim_ctx* c) {
     for (int k = c->kdiag[v]; k < len[v]; ++k) {
       const int ub = ubptr[v] + (k - c->kdiag[v]);
       hub_rc[ub] = make_int2(v, c->bcol[c->brow[v] + k]);
-      hub_pos[ub] = valpos(v, k - c->kdiag[v]);
+      hub_pos[ub] = valpos(v, c->bslot[c->brow[v] + k]);
     }
 
   // Hessian contributions: 10 local pairs (v ≤ w) per tet, sorted by (upper block, tet, pair)
@@ -1160,12 +1254,12 @@ static void export_csr(ssim_ctx* c, const double* dval, int* rowptr, int* colidx
         for (int k = 0; k < len; ++k) {
           const int w = c->bcol[c->brow[v] + k];
           if (k >= kd) {
-            const size_t bs = base(v, k - kd);
+            const size_t bs = base(v, c->bslot[c->brow[v] + k]);
             for (int d = 0; d < 3; ++d) values[p++] = hv[bs + (size_t)(3 * b + d) * kSlice];
           } else {  // lower block (v, w) = transpose of the stored (w, v)
             const int* bw = c->bcol.data() + c->brow[w];
             const int kw = (int)(std::lower_bound(bw, bw + (c->brow[w + 1] - c->brow[w]), v) - bw);
-            const size_t bs = base(w, kw - c->kdiag[w]);
+            const size_t bs = base(w, c->bslot[c->brow[w] + kw]);
             for (int d = 0; d < 3; ++d) values[p++] = hv[bs + (size_t)(3 * d + b) * kSlice];
           }
         }

@@ -491,12 +491,14 @@ __device__ __forceinline__ void bell_row_part(const BellView& A, int v, int j, c
   y[2] = y2;
 }
 
-// Fixed-width variant for layouts padded to exactly WU upper and WL lower slots in every slice (used when the
-// mesh allows it; the Freudenthal bar has 8 and 7): straight-line code, so all column / reference loads of the
-// row go out together, then all gathers and value loads -- a few dependent memory round trips per row instead
-// of two per slot.
+// Fixed-width variant for layouts padded to exactly WU upper and WL = WU − 1 lower slots in every slice (used when
+// the mesh allows it; the Freudenthal bar has 8 and 7): straight-line code, all column / reference loads of the
+// row first.  The slots are aligned by direction (lower slot WU−1−k refers to blocks stored in upper slot k of
+// the neighbour rows), and each pair (lower WU−1−k, upper k) is issued together, the far directions first: the
+// lower reads of a slice then reach L2 at about the time the slice that stores those blocks reads them itself.
 template <int WU, int WL>
 __device__ __forceinline__ void bell_row_fixed(const BellView& A, int v, const double* __restrict__ x, double y[3]) {
+  static_assert(WL == WU - 1, "lower slot WU-1-k pairs with upper slot k");
   const int s = v >> 6, lane = v & 63;
   const int offu = s * WU * kSlice;  // uniform widths: the offsets are implied
   const int* col = A.ucol + offu + lane;
@@ -509,24 +511,28 @@ __device__ __forceinline__ void bell_row_fixed(const BellView& A, int v, const d
 #pragma unroll
   for (int k = 0; k < WL; ++k) cl[k] = ref[k * kSlice];
   double y0 = 0, y1 = 0, y2 = 0;
-#pragma unroll
-  for (int k = 0; k < WU; ++k) {
+  auto upper = [&](int k) {
     const size_t w3 = 3 * (size_t)cu[k];
     const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
     const double* vk = val + (size_t)k * 9 * kSlice;
     y0 += vk[0 * kSlice] * x0 + vk[1 * kSlice] * x1 + vk[2 * kSlice] * x2;
     y1 += vk[3 * kSlice] * x0 + vk[4 * kSlice] * x1 + vk[5 * kSlice] * x2;
     y2 += vk[6 * kSlice] * x0 + vk[7 * kSlice] * x1 + vk[8 * kSlice] * x2;
-  }
-#pragma unroll
-  for (int k = 0; k < WL; ++k) {
+  };
+  auto lower = [&](int k) {
     const size_t w3 = 3 * (size_t)cl[k].x;
     const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
     const double* b = A.val + (size_t)cl[k].y;
     y0 += b[0 * kSlice] * x0 + b[3 * kSlice] * x1 + b[6 * kSlice] * x2;
     y1 += b[1 * kSlice] * x0 + b[4 * kSlice] * x1 + b[7 * kSlice] * x2;
     y2 += b[2 * kSlice] * x0 + b[5 * kSlice] * x1 + b[8 * kSlice] * x2;
+  };
+#pragma unroll
+  for (int k = WU - 1; k >= 1; --k) {
+    lower(WU - 1 - k);
+    upper(k);
   }
+  upper(0);
   y[0] = y0;
   y[1] = y1;
   y[2] = y2;

@@ -50,6 +50,7 @@ struct BellView {
 // Scalars of one PCG solve, resident in device memory.
 struct PcgScalars {
   double rz, pq, alpha, beta, rr, bb, tol2, pad;
+  double rz_hist[2];  // ρ_k in slot k % 2, written by kernel A of iteration k (ρ_0 by the initialisation)
   int done, iters, maxit, breakdown;  // read back together by the host
 };
 

@@ -418,6 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fin(const double* __restric
     it->ka = 0;
     it->kb = 0;
     sc->rz = t[0];
+    sc->rz_hist[0] = t[0];
     sc->bb = t[1];
     sc->rr = t[1];
     sc->tol2 = rtol * rtol * t[1];
@@ -441,7 +442,8 @@ int launch_pcg_init(int n, const double* b, double sign, const double* dinv, dou
 // p_k = z_k + β p_{k−1} and q_k = A p_k = s + β q_{k−1} for the lane's own row (in place), and the partials of
 // p_k·q_k.  For k = 0: p_0 = z_0, q_0 = s.  k is read from the device counter (kernel A reads ka and publishes
 // kb = ka, kernel B reads kb and publishes ka = kb + 1), so every iteration's launch is identical and a chunk of
-// iterations can be replayed from a captured graph; the (r·z, r·r) partials of residual k live in pb[k % 2].
+// iterations can be replayed from a captured graph; the (r·z, r·r) partials of residual k live in pb[k % 2], and
+// ρ_{k−1} is read from the slot the previous launch wrote (the same sum, so only one partial array is summed).
 // At 786k tets the SpMV has about 2.2 waves per SIMD: a register budget for 3 waves keeps the whole grid in
 // one dispatch round while leaving the compiler room to keep many of a row's loads in flight.
 template <int LPR>
@@ -453,7 +455,6 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
   if (blockIdx.x == 0 && threadIdx.x == 0) it->kb = k;
   if (sc->done) return;
   const double* __restrict__ pb_cur = (k & 1) ? pb1 : pb0;
-  const double* __restrict__ pb_prev = (k & 1) ? pb0 : pb1;
   const int lb = logical_block(RowMap<LPR>::blocks(A.nrows), A.xcd_remap);
   int v = 0, j = 0;
   if (lb >= 0) RowMap<LPR>::map(lb, v, j);
@@ -463,20 +464,21 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
   row_combine<LPR>(sv);
   double beta = 0.0;
   if (k > 0) {
-    double t_cur[2], t_prev[2];
+    double t_cur[2];
     sum_partials<2, kSpmvBlock>(pb_cur, nb, t_cur);
-    sum_partials<2, kSpmvBlock>(pb_prev, nb, t_prev);
+    const double rz_prev = sc->rz_hist[(k - 1) & 1];  // ρ_{k−1}, published by this kernel's previous launch
     const bool nan = !(t_cur[1] == t_cur[1]);
     const bool done = t_cur[1] <= sc->tol2 || k >= sc->maxit || nan;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->iters = k;
       sc->rr = t_cur[1];
       sc->rz = t_cur[0];
+      sc->rz_hist[k & 1] = t_cur[0];
       if (nan) sc->breakdown = 2;
       if (done) sc->done = 1;  // only ever 0 → 1: a workgroup that reads it early skips work that is not needed
     }
     if (done) return;
-    beta = t_cur[0] / t_prev[0];
+    beta = t_cur[0] / rz_prev;
   }
   double acc[1] = {0.0};
   if (active) {
