@@ -34,7 +34,9 @@ typedef enum {
   SSIM_STEPPER_NEWTON = 0,
   SSIM_STEPPER_LBFGS = 1,
   SSIM_STEPPER_NCG = 2,
-  SSIM_STEPPER_FORWARD_EULER = 3
+  SSIM_STEPPER_FORWARD_EULER = 3,
+  SSIM_STEPPER_NCG_EXT_PREC = 4 /* NCG preconditioned by z = L·Lᵀ·g (time_step_solver_ncg_with_ext_prec,
+                                   stepper/nonlinear_cg.hpp:41-97; L set by ssim_set_precond_matrix) */
 } ssim_stepper_t;
 
 /* material model, the ElastModel template argument of basic_time_step (finite_elements/time_step.hpp:168-172):
@@ -166,6 +168,10 @@ ssim_status_t ssim_grad_threshold_abs(const ssim_ctx* ctx, double* tol);
  * hessian(): the system matrix of the last update_hessian / Newton iteration (sysmatrix_, fem_host.cpp:169-171);
  * mass(): mass_matrix_ (fem_host.cpp:167). */
 ssim_status_t ssim_csr_nnz(const ssim_ctx* ctx, long long* nnz);
+/* tet_ncg_cuda_ext_ai::set_matrix (bindings/src/fem_cuda.cu:204): the preconditioner factor L of the
+ * SSIM_STEPPER_NCG_EXT_PREC stepper, nnz values in the CSR order of ssim_export_hessian_csr (the system matrix's
+ * pattern, nnz = ssim_csr_nnz); host array.  ssim_reset sets L to the identity (fem_cuda.cu:175-201). */
+ssim_status_t ssim_set_precond_matrix(ssim_ctx* ctx, const double* values, long long nnz);
 ssim_status_t ssim_export_hessian_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
 ssim_status_t ssim_export_mass_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
 

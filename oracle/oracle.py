@@ -71,6 +71,7 @@ def lib():
             "orc_ts_compute_step_ncg": (ctypes.c_double, [_V, _D]),
             "orc_ts_compute_step_fe": (ctypes.c_double, [_V, _D]),
             "orc_ts_set_model": (None, [_V, ctypes.c_int]),
+            "orc_ts_set_precond": (None, [_V, _D]),
             "orc_lin_energy": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double]),
             "orc_lin_stress": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, _D]),
             "orc_lin_hessian": (None, [ctypes.c_double, ctypes.c_double, _D]),
@@ -350,6 +351,17 @@ class TimeStep:
         """"stable_neohookean" or "linear"."""
         lib().orc_ts_set_model(self._h, {"stable_neohookean": 0, "linear": 1}[model])
         self.model = model
+
+    def set_precond(self, values):
+        """NCG preconditioner factor L (z = L·Lᵀ·g) as values in the Hessian's CSR order; None removes it."""
+        if values is None:
+            lib().orc_ts_set_precond(self._h, None)
+            return
+        v = _f64(values).reshape(-1)
+        if v.size != self.nnz:
+            raise ValueError(f"expected {self.nnz} values, got {v.size}")
+        self._prec = v
+        lib().orc_ts_set_precond(self._h, _dp(v))
 
     def set_polish(self, steps):
         lib().orc_ts_set_polish(self._h, int(steps))

@@ -335,6 +335,7 @@ struct orc_ts {
   int max_newton, pcg_maxit, ls_max;
   int polish; /* full Newton steps after the Newton loop (0: the reference's algorithm) */
   int model;  /* 0: stable Neo-Hookean, 1: linear elasticity */
+  double* prec; /* NCG preconditioner factor L in the CSR order of the Hessian (NULL: none) */
   double pcg_rtol, ls_c, ls_beta;
   int lbfgs_m, lbfgs_maxit, lbfgs_ls_max;
   double lbfgs_c, lbfgs_beta;
@@ -496,6 +497,7 @@ orc_ts* orc_ts_create(const double* X, int nv, const int* cells, int nt, double 
   ts->ls_max = 20;
   ts->polish = 0;
   ts->model = 0;
+  ts->prec = NULL;
   ts->lbfgs_m = 8;
   ts->lbfgs_maxit = 1000;
   ts->lbfgs_c = 1e-4;
@@ -509,7 +511,7 @@ void orc_ts_destroy(orc_ts* ts) {
   void* ptrs[] = {ts->X, ts->lam, ts->mu, ts->B, ts->V, ts->cells, ts->rowptr, ts->colidx, ts->brow, ts->bcol,
                   ts->tet_kw, ts->inc_ptr, ts->inc, ts->mass, ts->dof, ts->dbc, ts->u, ts->u_prev, ts->u_hat,
                   ts->u_next, ts->vel, ts->accel, ts->g, ts->tmp, ts->Kloc, ts->floc, ts->psi, ts->Hvals,
-                  ts->hrest_diag};
+                  ts->hrest_diag, ts->prec};
   for (unsigned k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
   free(ts);
 }
@@ -570,6 +572,15 @@ void orc_ts_set_solver(orc_ts* ts, int max_newton, double pcg_rtol, int pcg_maxi
   ts->ls_max = ls_max;
 }
 void orc_ts_set_polish(orc_ts* ts, int polish) { ts->polish = polish; }
+/* tet_ncg_cuda_ext_ai::set_matrix (bindings/src/fem_cuda.cu:204): L in the Hessian's CSR order; NULL removes it */
+void orc_ts_set_precond(orc_ts* ts, const double* values) {
+  free(ts->prec);
+  ts->prec = NULL;
+  if (!values) return;
+  ts->prec = (double*)malloc(sizeof(double) * (ts->nnz > 0 ? ts->nnz : 1));
+  memcpy(ts->prec, values, sizeof(double) * ts->nnz);
+}
+
 void orc_ts_set_model(orc_ts* ts, int model) {
   ts->model = model;
   if (ts->hrest_diag) {
@@ -932,6 +943,16 @@ double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats) {
   return gn;
 }
 
+/* z = L·(Lᵀ·g) with L in CSR (the preconditioner of time_step_solver_ncg_with_ext_prec,
+ * stepper/nonlinear_cg.hpp:54-59: gemv with the transpose, then gemv): a = Lᵀg, out = L a */
+static void prec_apply(const orc_ts* ts, const double* g, double* a, double* out) {
+  const int n = ts->n;
+  for (int i = 0; i < n; ++i) a[i] = 0.0;
+  for (int r = 0; r < n; ++r)
+    for (int k = ts->rowptr[r]; k < ts->rowptr[r + 1]; ++k) a[ts->colidx[k]] += ts->prec[k] * g[r];
+  csr_spmv(n, ts->rowptr, ts->colidx, ts->prec, a, out);
+}
+
 /* Nonlinear CG stepper (stepper/nonlinear_cg.hpp:9-37: mathprim ncg_optimizer with the Dai-Kou strategy,
  * learning rate 1e-2, backtracking line search, at most 1000 iterations, tol = threshold·mass).  mathprim is not
  * available, so this follows the published method: Dai & Kou, "A nonlinear conjugate gradient algorithm with an
@@ -940,17 +961,26 @@ double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats) {
  * dₖ₊₁ = −gₖ₊₁ + β dₖ, restarting with d = −g when dᵀy ≤ 0 or d is not a descent direction.  Step lengths:
  * backtracking Armijo (the L-BFGS constants: c = 1e-4, ×0.5, at most lbfgs_ls_max trials) from a first trial of
  * the learning rate 1e-2 in the first iteration and αₖ₋₁·gₖ₋₁ᵀdₖ₋₁/gₖᵀdₖ after it (Nocedal & Wright, eq. 3.60),
- * capped at 1. */
+ * capped at 1.
+ * With a preconditioner P = L·Lᵀ (set by orc_ts_set_precond; time_step_solver_ncg_with_ext_prec,
+ * stepper/nonlinear_cg.hpp:41-97) P·g replaces g in the direction and the restart, and the first term of β is
+ * g₊ᵀPy/dᵀy − yᵀPy·g₊ᵀd/(dᵀy)², evaluated as aᵀ(a − b) and ‖a − b‖² with a = Lᵀg₊, b = Lᵀg (the Euclidean floor is
+ * kept); with L = I every quantity equals the unpreconditioned one. */
 double orc_ts_compute_step_ncg(orc_ts* ts, double* stats) {
   const int n = ts->n;
+  const int pc = ts->prec != NULL;
   double* u = ts->u_next;
   double* d = (double*)malloc(sizeof(double) * n);
   double* ut = (double*)malloc(sizeof(double) * n);
   double* gt = (double*)malloc(sizeof(double) * n);
+  double* pa = (double*)malloc(sizeof(double) * n); /* Lᵀg */
+  double* pb = (double*)malloc(sizeof(double) * n); /* Lᵀg₊ */
+  double* pg = (double*)malloc(sizeof(double) * n); /* P·g */
   int it = 0, evals = 0, converged = 0, ls_fail = 0;
   double E = orc_ts_energy_grad(ts, u, ts->g);
   evals++;
-  for (int i = 0; i < n; ++i) d[i] = -ts->g[i];
+  if (pc) prec_apply(ts, ts->g, pa, pg);
+  for (int i = 0; i < n; ++i) d[i] = pc ? -pg[i] : -ts->g[i];
   double alpha_prev = 1e-2, gd_prev = 0.0;
   for (; it < ts->lbfgs_maxit;) {
     const double gg = dot_n(n, ts->g, ts->g);
@@ -960,8 +990,8 @@ double orc_ts_compute_step_ncg(orc_ts* ts, double* stats) {
     }
     double gd = dot_n(n, ts->g, d);
     if (!(gd < 0)) { /* not a descent direction: restart */
-      for (int i = 0; i < n; ++i) d[i] = -ts->g[i];
-      gd = -gg;
+      for (int i = 0; i < n; ++i) d[i] = pc ? -pg[i] : -ts->g[i];
+      gd = pc ? -dot_n(n, pa, pa) : -gg;
     }
     double alpha = it == 0 ? 1e-2 : alpha_prev * gd_prev / gd;
     if (!(alpha > 0) || alpha > 1.0) alpha = 1.0;
@@ -978,13 +1008,16 @@ double orc_ts_compute_step_ncg(orc_ts* ts, double* stats) {
       alpha *= ts->lbfgs_beta;
     }
     if (!ok) ls_fail++;
-    /* β from y = g_new − g: dᵀy, g_newᵀy, yᵀy, g_newᵀd, dᵀd */
+    if (pc) prec_apply(ts, gt, pb, pg);
+    /* dᵀy, g₊ᵀPy, yᵀPy, g₊ᵀd, dᵀd with y = g₊ − g (P = I: a = g₊, b = g) */
     double dy = 0, gy = 0, yy = 0, gnd = 0, dd = 0;
     for (int i = 0; i < n; ++i) {
       const double y = gt[i] - ts->g[i];
+      const double ai = pc ? pb[i] : gt[i];
+      const double ya = ai - (pc ? pa[i] : ts->g[i]);
       dy += d[i] * y;
-      gy += gt[i] * y;
-      yy += y * y;
+      gy += ai * ya;
+      yy += ya * ya;
       gnd += gt[i] * d[i];
       dd += d[i] * d[i];
     }
@@ -994,9 +1027,14 @@ double orc_ts_compute_step_ncg(orc_ts* ts, double* stats) {
       const double floor_ = 0.5 * gnd / dd;
       beta = bdk > floor_ ? bdk : floor_;
     }
-    for (int i = 0; i < n; ++i) d[i] = -gt[i] + beta * d[i];
+    for (int i = 0; i < n; ++i) d[i] = -(pc ? pg[i] : gt[i]) + beta * d[i];
     memcpy(u, ut, sizeof(double) * n);
     memcpy(ts->g, gt, sizeof(double) * n);
+    if (pc) {
+      double* t = pa;
+      pa = pb;
+      pb = t;
+    }
     E = Et;
     alpha_prev = alpha;
     gd_prev = gd;
@@ -1015,6 +1053,9 @@ double orc_ts_compute_step_ncg(orc_ts* ts, double* stats) {
   free(d);
   free(ut);
   free(gt);
+  free(pa);
+  free(pb);
+  free(pg);
   return gn;
 }
 

@@ -58,6 +58,8 @@ void orc_ts_set_solver(orc_ts* ts, int max_newton, double pcg_rtol, int pcg_maxi
 void orc_ts_set_polish(orc_ts* ts, int polish);
 /* material model: 0 stable Neo-Hookean (default), 1 linear elasticity */
 void orc_ts_set_model(orc_ts* ts, int model);
+/* NCG preconditioner factor L (z = L·Lᵀ·g) in the Hessian's CSR order; NULL removes it */
+void orc_ts_set_precond(orc_ts* ts, const double* values);
 double orc_lin_energy(const double F[9], double lam, double mu);
 double orc_lin_stress(const double F[9], double lam, double mu, double P[9]);
 void orc_lin_hessian(double lam, double mu, double H[81]);

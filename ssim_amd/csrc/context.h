@@ -104,6 +104,9 @@ struct ssim_ctx {
   ssim_hip::DBuf<int2> ub_rc, lref;
   ssim_hip::DBuf<double> gcontrib, hcontrib, val, massval;
   ssim_hip::DBuf<double> adj_lam, adj_mu;  // per-tet material gradients when the caller passes no buffer
+  // preconditioner factor L of the externally preconditioned NCG: block CSR on the block adjacency
+  ssim_hip::DBuf<int> pre_brow, pre_bcol, pre_tpos;
+  ssim_hip::DBuf<double> pre_val, pre_a, pre_b, pre_pg;
   // device: scalars
   ssim_hip::DBuf<ssim_hip::PcgScalars> pcg;
   ssim_hip::DBuf<ssim_hip::PcgIter> pcg_it;

@@ -355,6 +355,10 @@ void upload_material(ssim_ctx* c) {
 }  // namespace
 
 // reset(): finite_elements/time_step.hpp:368-494
+namespace {
+void set_precond(ssim_ctx* c, const double* values);
+}  // namespace
+
 void ssim_hip::do_reset(ssim_ctx* c) {
   check_material(c->youngs, c->poisson, c->density);
   if (0.5 - c->poisson < 1e-4)
@@ -391,6 +395,9 @@ void ssim_hip::do_reset(ssim_ctx* c) {
   // multiple of 8 (XCD remapping)
   c->pmax = std::max<size_t>((size_t)max_partials(std::max({c->nt, c->n, c->nub, 1})), 4 * (size_t)c->nslices) + 8;
   c->partials.alloc(8 * c->pmax);
+  // the preconditioner factor is rebuilt on the new pattern and set to the identity (fem_cuda.cu:175-201)
+  for (ssim_hip::DBuf<int>* b : {&c->pre_brow, &c->pre_bcol, &c->pre_tpos}) b->release();
+  if (c->pre_val.p || c->stepper == SSIM_STEPPER_NCG_EXT_PREC) set_precond(c, nullptr);
   c->drop_graphs();  // buffers may have moved
   c->pcg.alloc(1);
   c->pcg_it.alloc(1);
@@ -676,11 +683,61 @@ void lbfgs_solve(ssim_ctx* c) {
   c->stats.newton_iterations = it;
 }
 
+// The preconditioner factor L of the externally preconditioned NCG lives on the block adjacency (the system
+// matrix's pattern, fem_cuda.cu:175-201) as block CSR, with tpos[k] the index of the mirrored block (w, v) of
+// block k = (v, w) for the transposed product.  `values` null: the identity.
+void set_precond(ssim_ctx* c, const double* values) {
+  const int nv = c->nv;
+  const long long nb = c->brow[nv];
+  if (!c->pre_brow.p) {
+    std::vector<int> tpos(std::max<long long>(nb, 1));
+    for (int v = 0; v < nv; ++v)
+      for (int k = c->brow[v]; k < c->brow[v + 1]; ++k) {
+        const int w = c->bcol[k];
+        const int* b = c->bcol.data() + c->brow[w];
+        const int* e = c->bcol.data() + c->brow[w + 1];
+        tpos[k] = c->brow[w] + (int)(std::lower_bound(b, e, v) - b);
+      }
+    c->pre_brow.upload(c->brow.data(), nv + 1, c->stream);
+    c->pre_bcol.upload(c->bcol.data(), std::max<long long>(nb, 1), c->stream);
+    c->pre_tpos.upload(tpos, c->stream);
+    for (ssim_hip::DBuf<double>* b : {&c->pre_a, &c->pre_b, &c->pre_pg}) b->alloc(std::max(c->n, 1));
+    sync(c);
+  }
+  // user CSR order: scalar row 3v+b holds, for each neighbour k of v in column order, the entries (b, 0..2);
+  // it starts at 9·brow[v] + 3·b·len(v)
+  std::vector<double> hv(9 * (size_t)std::max<long long>(nb, 1), 0.0);
+  for (int v = 0; v < nv; ++v) {
+    const int len = c->brow[v + 1] - c->brow[v];
+    for (int k = 0; k < len; ++k) {
+      double* B = hv.data() + 9 * (size_t)(c->brow[v] + k);
+      if (!values) {
+        if (c->bcol[c->brow[v] + k] == v) B[0] = B[4] = B[8] = 1.0;
+        continue;
+      }
+      for (int b = 0; b < 3; ++b)
+        for (int d = 0; d < 3; ++d) B[3 * b + d] = values[9 * (size_t)c->brow[v] + 3 * (size_t)b * len + 3 * k + d];
+    }
+  }
+  c->pre_val.upload(hv, c->stream);
+  sync(c);
+}
+
+// z = L·(Lᵀ·g): a = Lᵀ g, out = L a
+void apply_precond(ssim_ctx* c, const double* g, double* a, double* out) {
+  launch_bsr_spmv(c->nv, c->pre_brow.p, c->pre_bcol.p, c->pre_tpos.p, c->pre_val.p, g, a, true, c->stream);
+  launch_bsr_spmv(c->nv, c->pre_brow.p, c->pre_bcol.p, c->pre_tpos.p, c->pre_val.p, a, out, false, c->stream);
+  HIPCHK(hipGetLastError());
+}
+
 // Nonlinear CG (stepper/nonlinear_cg.hpp:9-37: Dai-Kou strategy, learning rate 1e-2, backtracking line search,
 // tol = threshold·mass), the published DK+ method: β = max(gₖ₊₁ᵀy/dᵀy − ‖y‖²·gₖ₊₁ᵀd/(dᵀy)², ½·gₖ₊₁ᵀd/‖d‖²),
 // d = −gₖ₊₁ + βd, restarts with d = −g when dᵀy ≤ 0 or d is not a descent direction; backtracking Armijo with the
 // L-BFGS constants from a first trial of 1e-2, then αₖ₋₁·gₖ₋₁ᵀdₖ₋₁/gₖᵀdₖ capped at 1.
-void ncg_solve(ssim_ctx* c) {
+// With the external preconditioner P = L·Lᵀ (stepper/nonlinear_cg.hpp:41-97) the preconditioned gradient P·g
+// replaces g in the direction and the first term of β reads gₖ₊₁ᵀPy/dᵀy − yᵀPy·gₖ₊₁ᵀd/(dᵀy)² (the Euclidean
+// floor is kept); with L = I every quantity is the unpreconditioned one.
+void ncg_solve(ssim_ctx* c, bool precond) {
   const int n = c->n;
   hipStream_t s = c->stream;
   double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
@@ -690,8 +747,17 @@ void ncg_solve(ssim_ctx* c) {
   double* dots = c->lb_scal.p;  // five reductions of one iteration
   // their partials (5 per vector block) fit the dot-product slot: pmax ≥ 4·slices = n/48 > 5·(n/1024 + 1)
   if (5 * (size_t)pcg_vector_blocks(n) > c->pmax) fail(SSIM_ERR_STATE, "internal: partial buffer too small");
+  if (precond && !c->pre_val.p) set_precond(c, nullptr);
+  double* pa = precond ? c->pre_a.p : nullptr;  // Lᵀ·g of the current gradient
+  double* pb = precond ? c->pre_b.p : nullptr;  // Lᵀ·g of the trial gradient
+  double* pg = precond ? c->pre_pg.p : nullptr;  // P·g
   EvalOut cur = eval_energy_grad(c, u, g);
-  launch_axpby(n, -1.0, g, 0.0, d, s);
+  if (precond) {
+    apply_precond(c, g, pa, pg);
+    launch_axpby(n, -1.0, pg, 0.0, d, s);
+  } else {
+    launch_axpby(n, -1.0, g, 0.0, d, s);
+  }
   double alpha_prev = 1e-2, gd_prev = 0.0;
   int it = 0;
   for (; it < c->prm.lbfgs_maxit; ++it) {
@@ -701,9 +767,9 @@ void ncg_solve(ssim_ctx* c) {
       break;
     }
     double gd = device_dot(c, g, d);
-    if (!(gd < 0)) {  // not a descent direction: restart along −g
-      launch_axpby(n, -1.0, g, 0.0, d, s);
-      gd = -cur.gg;
+    if (!(gd < 0)) {  // not a descent direction: restart along −P·g
+      launch_axpby(n, -1.0, precond ? pg : g, 0.0, d, s);
+      gd = precond ? -device_dot(c, pa, pa) : -cur.gg;
     }
     double alpha = it == 0 ? 1e-2 : alpha_prev * gd_prev / gd;
     if (!(alpha > 0) || alpha > 1.0) alpha = 1.0;
@@ -719,16 +785,18 @@ void ncg_solve(ssim_ctx* c) {
       alpha *= c->prm.ls_beta;
     }
     if (!ok) c->stats.ls_failures++;
-    launch_ncg_dots(n, c->gtry.p, g, d, c->part_d(), dots, s);
+    if (precond) apply_precond(c, c->gtry.p, pb, pg);
+    launch_ncg_dots(n, c->gtry.p, g, d, precond ? pb : c->gtry.p, precond ? pa : g, c->part_d(), dots, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->pin, dots, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
     sync(c);
     const double dy = c->pin[0], gy = c->pin[1], yy = c->pin[2], gnd = c->pin[3], dd = c->pin[4];
     double beta = 0.0;
     if (dy > 0) beta = std::max(gy / dy - yy * gnd / (dy * dy), 0.5 * gnd / dd);
-    launch_axpby(n, -1.0, c->gtry.p, beta, d, s);
+    launch_axpby(n, -1.0, precond ? pg : c->gtry.p, beta, d, s);
     HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (precond) std::swap(pa, pb);
     cur = trial;
     alpha_prev = alpha;
     gd_prev = gd;
@@ -959,7 +1027,7 @@ ssim_status_t ssim_get_solver_params(const ssim_ctx* ctx, ssim_solver_params* p)
 ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
   if (stepper != SSIM_STEPPER_NEWTON && stepper != SSIM_STEPPER_LBFGS && stepper != SSIM_STEPPER_NCG &&
-      stepper != SSIM_STEPPER_FORWARD_EULER) {
+      stepper != SSIM_STEPPER_FORWARD_EULER && stepper != SSIM_STEPPER_NCG_EXT_PREC) {
     g_err = "unknown stepper";
     return SSIM_ERR_INVALID_ARGUMENT;
   }
@@ -1018,8 +1086,8 @@ ssim_status_t ssim_compute_step(ssim_ctx* ctx, double* residual_out) {
       newton_solve(ctx);
     else if (ctx->stepper == SSIM_STEPPER_LBFGS)
       lbfgs_solve(ctx);
-    else if (ctx->stepper == SSIM_STEPPER_NCG)
-      ncg_solve(ctx);
+    else if (ctx->stepper == SSIM_STEPPER_NCG || ctx->stepper == SSIM_STEPPER_NCG_EXT_PREC)
+      ncg_solve(ctx, ctx->stepper == SSIM_STEPPER_NCG_EXT_PREC);
     else
       forward_euler_solve(ctx);
     // compute_step: final update_energy_and_gradients and ‖forces‖ (time_step.hpp:520-528)
@@ -1219,6 +1287,19 @@ ssim_status_t ssim_csr_nnz(const ssim_ctx* ctx, long long* nnz) {
   if (!ctx || !nnz) return SSIM_ERR_INVALID_ARGUMENT;
   *nnz = 9LL * (ctx->brow.empty() ? 0 : ctx->brow[ctx->nv]);
   return SSIM_OK;
+}
+
+ssim_status_t ssim_set_precond_matrix(ssim_ctx* ctx, const double* values, long long nnz) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    if (!values) fail(SSIM_ERR_INVALID_ARGUMENT, "values is null");
+    const long long want = 9LL * ctx->brow[ctx->nv];
+    if (nnz != want)
+      fail(SSIM_ERR_INVALID_ARGUMENT, "the preconditioner needs " + std::to_string(want) + " values (the CSR pattern of "
+                                       "the system matrix), got " + std::to_string(nnz));
+    set_precond(ctx, values);
+  });
 }
 
 // blocked-ELL values (device) → scalar CSR in row/column order

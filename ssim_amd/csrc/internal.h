@@ -135,8 +135,10 @@ void launch_elem_adjoint(const MeshView& m, const double* u, const double* lv, d
 void launch_mask_dbc(int n, const unsigned char* dof, const double* in, double* out, hipStream_t s);
 
 // nonlinear CG: out[0..5) = (dᵀy, gtᵀy, yᵀy, gtᵀd, dᵀd), y = gt − g (part holds vector-block partials of width 5)
-void launch_ncg_dots(int n, const double* gt, const double* g, const double* d, double* part, double* out,
-                     hipStream_t s);
+void launch_ncg_dots(int n, const double* gt, const double* g, const double* d, const double* a, const double* b,
+                     double* part, double* out, hipStream_t s);
+void launch_bsr_spmv(int nv, const int* brow, const int* bcol, const int* tpos, const double* val, const double* x,
+                     double* y, bool trans, hipStream_t s);
 // y = a·x + b·y (y is not read when b = 0)
 void launch_axpby(int n, double a, const double* x, double b, double* y, hipStream_t s);
 

@@ -590,17 +590,21 @@ int pcg_spmv_grid(const BellView& A) {
 // nonlinear-CG helpers
 
 // partials of (dᵀy, gtᵀy, yᵀy, gtᵀd, dᵀd) with y = gt − g
+// With a preconditioner P = L·Lᵀ, a = Lᵀ·g_new and b = Lᵀ·g give g_newᵀ·P·y = aᵀ(a − b) and yᵀ·P·y = ‖a − b‖²;
+// without one the caller passes a = g_new, b = g.
 __global__ __launch_bounds__(kBlock) void k_ncg_dots(int n, const double* __restrict__ gt, const double* __restrict__ g,
-                                                     const double* __restrict__ d, double* __restrict__ part) {
+                                                     const double* __restrict__ d, const double* __restrict__ a,
+                                                     const double* __restrict__ b, double* __restrict__ part) {
   double v[5] = {0, 0, 0, 0, 0};
 #pragma unroll
   for (int j = 0; j < kVec; ++j) {
     const int i = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
     if (i < n) {
       const double gi = gt[i], di = d[i], y = gi - g[i];
+      const double ai = a[i], ya = ai - b[i];
       v[0] += di * y;
-      v[1] += gi * y;
-      v[2] += y * y;
+      v[1] += ai * ya;
+      v[2] += ya * ya;
       v[3] += gi * di;
       v[4] += di * di;
     }
@@ -614,11 +618,50 @@ __global__ __launch_bounds__(kBlock) void k_finalize5(const double* __restrict__
   if (threadIdx.x < 5) out[threadIdx.x] = t[threadIdx.x];
 }
 
-void launch_ncg_dots(int n, const double* gt, const double* g, const double* d, double* part, double* out,
-                     hipStream_t s) {
+void launch_ncg_dots(int n, const double* gt, const double* g, const double* d, const double* a, const double* b,
+                     double* part, double* out, hipStream_t s) {
   const int nb = vblocks(n);
-  k_ncg_dots<<<nb, kBlock, 0, s>>>(n, gt, g, d, part);
+  k_ncg_dots<<<nb, kBlock, 0, s>>>(n, gt, g, d, a, b, part);
   k_finalize5<<<1, kBlock, 0, s>>>(part, nb, out);
+}
+
+// Block-CSR product with a general matrix on the block adjacency (blocks row-major, 9 doubles each):
+// y_v = Σ_k B_k·x_{w_k} over the neighbours w_k = bcol[k] of v, with B_k = block k (TRANS = false) or the
+// transpose of block tpos[k] = (w_k, v) (TRANS = true, i.e. y = Lᵀx).  One thread per block row; used by the
+// externally preconditioned NCG (z = L·Lᵀ·g).
+template <bool TRANS>
+__global__ __launch_bounds__(kBlock) void k_bsr_spmv(int nv, const int* __restrict__ brow, const int* __restrict__ bcol,
+                                                     const int* __restrict__ tpos, const double* __restrict__ val,
+                                                     const double* __restrict__ x, double* __restrict__ y) {
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nv) return;
+  double y0 = 0, y1 = 0, y2 = 0;
+  for (int k = brow[v]; k < brow[v + 1]; ++k) {
+    const size_t w3 = 3 * (size_t)bcol[k];
+    const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
+    const double* B = val + 9 * (size_t)(TRANS ? tpos[k] : k);
+    if (TRANS) {
+      y0 += B[0] * x0 + B[3] * x1 + B[6] * x2;
+      y1 += B[1] * x0 + B[4] * x1 + B[7] * x2;
+      y2 += B[2] * x0 + B[5] * x1 + B[8] * x2;
+    } else {
+      y0 += B[0] * x0 + B[1] * x1 + B[2] * x2;
+      y1 += B[3] * x0 + B[4] * x1 + B[5] * x2;
+      y2 += B[6] * x0 + B[7] * x1 + B[8] * x2;
+    }
+  }
+  y[3 * (size_t)v] = y0;
+  y[3 * (size_t)v + 1] = y1;
+  y[3 * (size_t)v + 2] = y2;
+}
+
+void launch_bsr_spmv(int nv, const int* brow, const int* bcol, const int* tpos, const double* val, const double* x,
+                     double* y, bool trans, hipStream_t s) {
+  if (nv <= 0) return;
+  if (trans)
+    k_bsr_spmv<true><<<nblocks(nv), kBlock, 0, s>>>(nv, brow, bcol, tpos, val, x, y);
+  else
+    k_bsr_spmv<false><<<nblocks(nv), kBlock, 0, s>>>(nv, brow, bcol, tpos, val, x, y);
 }
 
 // y = a·x + b·y (b = 0 does not read y)

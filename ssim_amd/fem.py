@@ -38,17 +38,19 @@ class TetFiniteElementSolver_Hip:
 
     method: "newton" (Newton + Jacobi-PCG + backtracking Armijo), "lbfgs" (L-BFGS, m = 8, H0 = diagonal
     of the rest Hessian), "ncg" (nonlinear CG, Dai-Kou β, the L-BFGS line-search constants) or
-    "forward_euler" (explicit: next = û + ½dt²·forces(û), finite_elements/time_step.hpp:717-736).
+    "forward_euler" (explicit: next = û + ½dt²·forces(û), finite_elements/time_step.hpp:717-736) or "ncg_ext_ai"
+    (NCG preconditioned by z = L·Lᵀ·g with L set by ``set_matrix``, the reference's tet_ncg_cuda_ext_ai,
+    bindings/src/fem_cuda.cu:172-244; L is the identity after ``reset``).
     material: "stable_neohookean" (elast/stable_neohookean.hpp, the reference bindings' model) or "linear"
     (elast/linear.hpp).
     """
 
-    METHODS = {"newton": 0, "lbfgs": 1, "ncg": 2, "forward_euler": 3}
+    METHODS = {"newton": 0, "lbfgs": 1, "ncg": 2, "forward_euler": 3, "ncg_ext_ai": 4}
     MATERIALS = {"stable_neohookean": 0, "linear": 1}
 
     def __init__(self, vertices: np.ndarray, elements: np.ndarray, time_step: float = 1e-2,
                  young_modulus: float = 1e6, poisson_ratio: float = 0.33, density: float = 1e2,
-                 method: Literal["newton", "lbfgs", "ncg", "forward_euler"] = "newton",
+                 method: Literal["newton", "lbfgs", "ncg", "forward_euler", "ncg_ext_ai"] = "newton",
                  material: Literal["stable_neohookean", "linear"] = "stable_neohookean"):
         if method not in self.METHODS:
             raise ValueError(f"Unknown method: {method}")
@@ -125,6 +127,12 @@ class TetFiniteElementSolver_Hip:
     def mark_general_batched(self, verts: np.ndarray):
         v = np.ascontiguousarray(verts, dtype=np.int32).reshape(-1)
         _lib.call("ssim_mark_general", self._h, _ip(v), v.size)
+
+    def set_matrix(self, values: np.ndarray):
+        """Preconditioner factor L of the "ncg_ext_ai" method: the values of a CSR matrix with the system matrix's
+        pattern (``hessian()`` order, pyssim/fem/__init__.py:151-157; fem_cuda.cu:204)."""
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        _lib.call("ssim_set_precond_matrix", self._h, _dp(v), ctypes.c_longlong(v.size))
 
     def set_rtol(self, rtol: float):
         _lib.call("ssim_set_threshold", self._h, float(rtol))
