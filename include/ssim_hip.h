@@ -35,8 +35,11 @@ typedef enum {
   SSIM_STEPPER_LBFGS = 1,
   SSIM_STEPPER_NCG = 2,
   SSIM_STEPPER_FORWARD_EULER = 3,
-  SSIM_STEPPER_NCG_EXT_PREC = 4 /* NCG preconditioned by z = L·Lᵀ·g (time_step_solver_ncg_with_ext_prec,
-                                   stepper/nonlinear_cg.hpp:41-97; L set by ssim_set_precond_matrix) */
+  SSIM_STEPPER_NCG_EXT_PREC = 4, /* NCG preconditioned by z = L·Lᵀ·g (time_step_solver_ncg_with_ext_prec,
+                                    stepper/nonlinear_cg.hpp:41-97; L set by ssim_set_precond_matrix) */
+  SSIM_STEPPER_LBFGS_PD = 5       /* L-BFGS preconditioned by solves with the projective-dynamics matrix
+                                    M + dt²κK built at reset (time_step_solver_pd, stepper/projective_dynamics.hpp:9-88;
+                                    tet_lbfgs_pd, bindings/src/fem_host.cpp:223) */
 } ssim_stepper_t;
 
 /* material model, the ElastModel template argument of basic_time_step (finite_elements/time_step.hpp:168-172):
@@ -173,6 +176,9 @@ ssim_status_t ssim_csr_nnz(const ssim_ctx* ctx, long long* nnz);
  * pattern, nnz = ssim_csr_nnz); host array.  ssim_reset sets L to the identity (fem_cuda.cu:175-201). */
 ssim_status_t ssim_set_precond_matrix(ssim_ctx* ctx, const double* values, long long nnz);
 ssim_status_t ssim_export_hessian_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
+/* the projective-dynamics matrix of SSIM_STEPPER_LBFGS_PD (time_step_solver_pd::pd_matrix_,
+ * stepper/projective_dynamics.hpp:51-87) in the same CSR layout; built at reset or on first use */
+ssim_status_t ssim_export_pd_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
 ssim_status_t ssim_export_mass_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
 
 /* Element-level outputs at deformation u (host, 3·nv doubles): Ψ [nt], P [nt,3,3], H [nt,9,9] (SPD-projected

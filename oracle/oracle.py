@@ -72,6 +72,9 @@ def lib():
             "orc_ts_compute_step_fe": (ctypes.c_double, [_V, _D]),
             "orc_ts_set_model": (None, [_V, ctypes.c_int]),
             "orc_ts_set_precond": (None, [_V, _D]),
+            "orc_ts_build_pd": (None, [_V, ctypes.c_double]),
+            "orc_ts_pd_values": (None, [_V, _D]),
+            "orc_ts_compute_step_lbfgs_pd": (ctypes.c_double, [_V, _D]),
             "orc_lin_energy": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double]),
             "orc_lin_stress": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, _D]),
             "orc_lin_hessian": (None, [ctypes.c_double, ctypes.c_double, _D]),
@@ -363,6 +366,18 @@ class TimeStep:
         self._prec = v
         lib().orc_ts_set_precond(self._h, _dp(v))
 
+    def build_pd(self, kappa):
+        """Projective-dynamics matrix M + dt²κK with the diffusivity κ (Dirichlet-filtered, current marks)."""
+        lib().orc_ts_build_pd(self._h, float(kappa))
+
+    def pd_csr(self):
+        import scipy.sparse as sp
+
+        rowptr, colidx = self.csr_pattern()
+        vals = np.zeros(self.nnz)
+        lib().orc_ts_pd_values(self._h, _dp(vals))
+        return sp.csr_matrix((vals, colidx, rowptr), shape=(self.n, self.n))
+
     def set_polish(self, steps):
         lib().orc_ts_set_polish(self._h, int(steps))
 
@@ -405,7 +420,8 @@ class TimeStep:
     def compute_step(self, method="newton"):
         st = np.zeros(6)
         fn = {"newton": lib().orc_ts_compute_step, "lbfgs": lib().orc_ts_compute_step_lbfgs,
-              "ncg": lib().orc_ts_compute_step_ncg, "forward_euler": lib().orc_ts_compute_step_fe}[method]
+              "ncg": lib().orc_ts_compute_step_ncg, "forward_euler": lib().orc_ts_compute_step_fe,
+              "lbfgs_pd": lib().orc_ts_compute_step_lbfgs_pd}[method]
         gn = fn(self._h, _dp(st))
         return gn, dict(newton=int(st[0]), pcg=int(st[1]), evals=int(st[2]), converged=bool(st[3]),
                         grad_norm=float(st[4]), ls_fail=int(st[5]))

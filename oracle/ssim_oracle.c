@@ -336,6 +336,8 @@ struct orc_ts {
   int polish; /* full Newton steps after the Newton loop (0: the reference's algorithm) */
   int model;  /* 0: stable Neo-Hookean, 1: linear elasticity */
   double* prec; /* NCG preconditioner factor L in the CSR order of the Hessian (NULL: none) */
+  double* pd_vals; /* projective-dynamics matrix M + dt²κL (Hessian CSR order), built by orc_ts_build_pd */
+  int lbfgs_pd;    /* L-BFGS preconditioned by solves with pd_vals instead of the rest-Hessian diagonal */
   double pcg_rtol, ls_c, ls_beta;
   int lbfgs_m, lbfgs_maxit, lbfgs_ls_max;
   double lbfgs_c, lbfgs_beta;
@@ -498,6 +500,8 @@ orc_ts* orc_ts_create(const double* X, int nv, const int* cells, int nt, double 
   ts->polish = 0;
   ts->model = 0;
   ts->prec = NULL;
+  ts->pd_vals = NULL;
+  ts->lbfgs_pd = 0;
   ts->lbfgs_m = 8;
   ts->lbfgs_maxit = 1000;
   ts->lbfgs_c = 1e-4;
@@ -511,7 +515,7 @@ void orc_ts_destroy(orc_ts* ts) {
   void* ptrs[] = {ts->X, ts->lam, ts->mu, ts->B, ts->V, ts->cells, ts->rowptr, ts->colidx, ts->brow, ts->bcol,
                   ts->tet_kw, ts->inc_ptr, ts->inc, ts->mass, ts->dof, ts->dbc, ts->u, ts->u_prev, ts->u_hat,
                   ts->u_next, ts->vel, ts->accel, ts->g, ts->tmp, ts->Kloc, ts->floc, ts->psi, ts->Hvals,
-                  ts->hrest_diag, ts->prec};
+                  ts->hrest_diag, ts->prec, ts->pd_vals};
   for (unsigned k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
   free(ts);
 }
@@ -854,6 +858,50 @@ void orc_ts_rest_hessian_diag(orc_ts* ts, double* diag) {
  * with H0 = diag(H_rest)⁻¹ plugged in where the reference's l_bfgs_preconditioner::apply_impl(z, q, s, y)
  * hook sits (stepper/projective_dynamics.hpp:15-35), backtracking Armijo line search, stopping when
  * ‖g‖₂ < tol_abs (stepper/backward_euler.hpp:48-60). */
+/* Projective-dynamics matrix of time_step_solver_pd::reset_impl (stepper/projective_dynamics.hpp:51-87):
+ * A = M + dt²·κ·K on every dof, M the consistent mass ρV(1+δ_ij)/20 (mass_integrator, bilinears.hpp:140-148)
+ * and K the P1 Laplacian V·∇N_i·∇N_j (laplace_integrator, bilinears.hpp:158-167, 260-264) with the diffusivity κ
+ * of the uniform material (λ + 2μ, elast/basic_elast.hpp:225), Dirichlet rows and columns set to the identity. */
+void orc_ts_build_pd(orc_ts* ts, double kappa) {
+  const int nv = ts->nv;
+  const double c = ts->dt * ts->dt * kappa;
+  if (!ts->pd_vals) ts->pd_vals = (double*)malloc(sizeof(double) * (ts->nnz > 0 ? ts->nnz : 1));
+  memcpy(ts->pd_vals, ts->mass, sizeof(double) * ts->nnz);
+  for (int v = 0; v < nv; ++v)
+    for (int q = ts->inc_ptr[v]; q < ts->inc_ptr[v + 1]; ++q) {
+      const int e = ts->inc[q] / 4, i = ts->inc[q] % 4;
+      const double* B = ts->B + 9 * e;
+      double D[4][3];
+      for (int k = 0; k < 3; ++k) {
+        D[0][k] = -B[0 * 3 + k] - B[1 * 3 + k] - B[2 * 3 + k];
+        for (int r = 1; r < 4; ++r) D[r][k] = B[(r - 1) * 3 + k];
+      }
+      for (int j = 0; j < 4; ++j) {
+        const double lap = ts->V[e] * (D[i][0] * D[j][0] + D[i][1] * D[j][1] + D[i][2] * D[j][2]);
+        const int kw = ts->tet_kw[16 * e + 4 * i + j];
+        for (int a = 0; a < 3; ++a) ts->pd_vals[ts->rowptr[3 * v + a] + 3 * kw + a] += c * lap;
+      }
+    }
+  for (int r = 0; r < ts->n; ++r)
+    for (int k = ts->rowptr[r]; k < ts->rowptr[r + 1]; ++k) {
+      const int col = ts->colidx[k];
+      if (ts->dof[r] || ts->dof[col]) ts->pd_vals[k] = (r == col) ? 1.0 : 0.0;
+    }
+}
+void orc_ts_pd_values(const orc_ts* ts, double* vals) {
+  if (ts->pd_vals) memcpy(vals, ts->pd_vals, sizeof(double) * ts->nnz);
+}
+
+/* time_step_solver_pd (stepper/projective_dynamics.hpp:36-50): L-BFGS whose two-loop middle step solves with the
+ * projective-dynamics matrix (a direct solver in the reference; here Jacobi-PCG to a relative residual of 1e-12). */
+double orc_ts_compute_step_lbfgs_pd(orc_ts* ts, double* stats) {
+  if (!ts->pd_vals) return -1.0;
+  ts->lbfgs_pd = 1;
+  const double r = orc_ts_compute_step_lbfgs(ts, stats);
+  ts->lbfgs_pd = 0;
+  return r;
+}
+
 double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats) {
   const int n = ts->n, m = ts->lbfgs_m;
   if (!ts->hrest_diag) {
@@ -868,7 +916,7 @@ double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats) {
   double* q = (double*)malloc(sizeof(double) * n);
   double* ut = (double*)malloc(sizeof(double) * n);
   double* gt = (double*)malloc(sizeof(double) * n);
-  int it = 0, evals = 0, converged = 0, ls_fail = 0, hist = 0, head = 0;
+  int it = 0, evals = 0, converged = 0, ls_fail = 0, hist = 0, head = 0, pd_total = 0;
   double E = orc_ts_energy_grad(ts, u, ts->g);
   evals++;
   for (; it < ts->lbfgs_maxit;) {
@@ -884,7 +932,12 @@ double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats) {
       const double* y = Y + (size_t)idx * n;
       for (int i = 0; i < n; ++i) q[i] -= a[idx] * y[i];
     }
-    for (int i = 0; i < n; ++i) q[i] /= ts->hrest_diag[i];
+    if (ts->lbfgs_pd) { /* r = A_pd⁻¹ q */
+      pd_total += orc_pcg(n, ts->rowptr, ts->colidx, ts->pd_vals, q, ut, 1e-12, 100000, NULL);
+      memcpy(q, ut, sizeof(double) * n);
+    } else {
+      for (int i = 0; i < n; ++i) q[i] /= ts->hrest_diag[i];
+    }
     for (int k = hist - 1; k >= 0; --k) {
       const int idx = (head - 1 - k + m) % m;
       const double bb = rho[idx] * dot_n(n, Y + (size_t)idx * n, q);
@@ -927,7 +980,7 @@ double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats) {
   const double gn = sqrt(dot_n(n, ts->g, ts->g));
   if (stats) {
     stats[0] = it;
-    stats[1] = 0;
+    stats[1] = pd_total;
     stats[2] = evals;
     stats[3] = converged;
     stats[4] = gn;

@@ -60,6 +60,10 @@ void orc_ts_set_polish(orc_ts* ts, int polish);
 void orc_ts_set_model(orc_ts* ts, int model);
 /* NCG preconditioner factor L (z = L·Lᵀ·g) in the Hessian's CSR order; NULL removes it */
 void orc_ts_set_precond(orc_ts* ts, const double* values);
+/* projective dynamics: A = M + dt²κK (Dirichlet-filtered) and L-BFGS preconditioned by solves with it */
+void orc_ts_build_pd(orc_ts* ts, double kappa);
+void orc_ts_pd_values(const orc_ts* ts, double* vals);
+double orc_ts_compute_step_lbfgs_pd(orc_ts* ts, double* stats);
 double orc_lin_energy(const double F[9], double lam, double mu);
 double orc_lin_stress(const double F[9], double lam, double mu, double P[9]);
 void orc_lin_hessian(double lam, double mu, double H[81]);

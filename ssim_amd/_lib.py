@@ -97,6 +97,7 @@ SIGNATURES = {
     "ssim_grad_threshold_abs": [_V, _D],
     "ssim_csr_nnz": [_V, ctypes.POINTER(ctypes.c_longlong)],
     "ssim_export_hessian_csr": [_V, _I, _I, _D],
+    "ssim_export_pd_csr": [_V, _I, _I, _D],
     "ssim_export_mass_csr": [_V, _I, _I, _D],
     "ssim_eval_elements": [_V, _D, ctypes.c_int, _D, _D, _D, _D],
     "ssim_pcg_solve": [_V, _D, _D, ctypes.c_double, ctypes.c_int, _I],

@@ -107,6 +107,8 @@ struct ssim_ctx {
   // preconditioner factor L of the externally preconditioned NCG: block CSR on the block adjacency
   ssim_hip::DBuf<int> pre_brow, pre_bcol, pre_tpos;
   ssim_hip::DBuf<double> pre_val, pre_a, pre_b, pre_pg;
+  // projective-dynamics matrix M + dt²κK (blocked-ELL values and Jacobi) and the two-loop vector of its L-BFGS
+  ssim_hip::DBuf<double> pd_val, pd_dinv, pd_q;
   // device: scalars
   ssim_hip::DBuf<ssim_hip::PcgScalars> pcg;
   ssim_hip::DBuf<ssim_hip::PcgIter> pcg_it;
@@ -115,6 +117,7 @@ struct ssim_ctx {
   struct PcgGraph {
     int count;
     const double* x;
+    const double* val;
     hipGraphExec_t exec;
   };
   std::vector<PcgGraph> pcg_graphs;

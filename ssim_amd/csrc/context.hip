@@ -357,6 +357,7 @@ void upload_material(ssim_ctx* c) {
 // reset(): finite_elements/time_step.hpp:368-494
 namespace {
 void set_precond(ssim_ctx* c, const double* values);
+void build_pd(ssim_ctx* c);
 }  // namespace
 
 void ssim_hip::do_reset(ssim_ctx* c) {
@@ -395,6 +396,9 @@ void ssim_hip::do_reset(ssim_ctx* c) {
   // multiple of 8 (XCD remapping)
   c->pmax = std::max<size_t>((size_t)max_partials(std::max({c->nt, c->n, c->nub, 1})), 4 * (size_t)c->nslices) + 8;
   c->partials.alloc(8 * c->pmax);
+  // the projective-dynamics matrix is built at reset (projective_dynamics.hpp:51-87), for that stepper
+  c->pd_val.release();
+  if (c->stepper == SSIM_STEPPER_LBFGS_PD) build_pd(c);
   // the preconditioner factor is rebuilt on the new pattern and set to the identity (fem_cuda.cu:175-201)
   for (ssim_hip::DBuf<int>* b : {&c->pre_brow, &c->pre_bcol, &c->pre_tpos}) b->release();
   if (c->pre_val.p || c->stepper == SSIM_STEPPER_NCG_EXT_PREC) set_precond(c, nullptr);
@@ -470,25 +474,27 @@ void ssim_hip::assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int
 namespace {
 
 // Enqueues `count` PCG iterations (kernels A and B) on stream s.
-void enqueue_pcg_iterations(ssim_ctx* c, const BellView& A, double* x, int nb, int na, int count, hipStream_t s) {
+void enqueue_pcg_iterations(ssim_ctx* c, const BellView& A, const double* dinv, double* x, int nb, int na, int count,
+                            hipStream_t s) {
   for (int j = 0; j < count; ++j) {
     launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
                     c->part_pa(), s);
-    launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(0),
+    launch_pcg_update(c->n, dinv, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(0),
                       c->part_pb(1), c->pcg_it.p, c->pcg.p, s);
   }
 }
 
-// The graph of `count` iterations writing x, captured on first use (buffers are fixed until the next reset,
-// which drops the cache).
-hipGraphExec_t pcg_chunk_graph(ssim_ctx* c, const BellView& A, double* x, int nb, int na, int count) {
+// The graph of `count` iterations on matrix A writing x, captured on first use (buffers are fixed until the
+// next reset, which drops the cache).
+hipGraphExec_t pcg_chunk_graph(ssim_ctx* c, const BellView& A, const double* dinv, double* x, int nb, int na,
+                               int count) {
   for (const auto& g : c->pcg_graphs)
-    if (g.count == count && g.x == x) return g.exec;
+    if (g.count == count && g.x == x && g.val == A.val) return g.exec;
   if (!c->cap_stream) HIPCHK(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
   HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
   hipGraph_t graph = nullptr;
   try {
-    enqueue_pcg_iterations(c, A, x, nb, na, count, c->cap_stream);
+    enqueue_pcg_iterations(c, A, dinv, x, nb, na, count, c->cap_stream);
   } catch (...) {
     (void)hipStreamEndCapture(c->cap_stream, &graph);
     if (graph) (void)hipGraphDestroy(graph);
@@ -499,15 +505,19 @@ hipGraphExec_t pcg_chunk_graph(ssim_ctx* c, const BellView& A, double* x, int nb
   const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) fail(SSIM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-  c->pcg_graphs.push_back({count, x, exec});
+  c->pcg_graphs.push_back({count, x, A.val, exec});
   return exec;
 }
 
-// Jacobi-PCG on the current matrix: x = H⁻¹ (sign·b).  Returns iterations.
-int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit) {
+// Jacobi-PCG: x = A⁻¹ (sign·b) for the blocked-ELL values `val` with inverse diagonal `dinv` (default: the
+// current Hessian).  Returns iterations.
+int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit,
+              const double* val = nullptr, const double* dinv = nullptr) {
   hipStream_t s = c->stream;
-  const BellView A = c->bell(c->val.p);
-  const int nb = launch_pcg_init(c->n, b, sign, c->dinv.p, x, c->r.p, c->z.p, rtol, maxit, c->pcg.p, c->pcg_it.p,
+  if (!val) val = c->val.p;
+  if (!dinv) dinv = c->dinv.p;
+  const BellView A = c->bell(val);
+  const int nb = launch_pcg_init(c->n, b, sign, dinv, x, c->r.p, c->z.p, rtol, maxit, c->pcg.p, c->pcg_it.p,
                                  c->part_pb(0), s);
   HIPCHK(hipGetLastError());
   for (auto& e : c->poll_ev)
@@ -523,14 +533,14 @@ int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol,
                       c->part_pa(), s);
     });
     timed(c, SSIM_KERNEL_PCG_UPDATE, [&] {
-      launch_pcg_update(c->n, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(0),
+      launch_pcg_update(c->n, dinv, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(0),
                         c->part_pb(1), c->pcg_it.p, c->pcg.p, s);
     });
     if (count > 1) {
       if (c->pcg_graph)
-        HIPCHK(hipGraphLaunch(pcg_chunk_graph(c, A, x, nb, na, count - 1), s));
+        HIPCHK(hipGraphLaunch(pcg_chunk_graph(c, A, dinv, x, nb, na, count - 1), s));
       else
-        enqueue_pcg_iterations(c, A, x, nb, na, count - 1, s);
+        enqueue_pcg_iterations(c, A, dinv, x, nb, na, count - 1, s);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(flags + 4 * slot, &c->pcg.p->done, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -604,13 +614,36 @@ void newton_solve(ssim_ctx* c) {
   }
 }
 
+// Projective-dynamics matrix of time_step_solver_pd::reset_impl (stepper/projective_dynamics.hpp:51-87):
+// M + dt²·κ·K on every dof, K the P1 Laplacian, Dirichlet rows and columns set to the identity, with κ the
+// diffusivity λ + 2μ of the reference's uniform_elasticity(), which builds the material from (E, ν) in the
+// (λ, μ) slots (finite_elements/time_step.hpp:356, elast/basic_elast.hpp:225): κ = E + 2ν.
+void build_pd(ssim_ctx* c) {
+  hipStream_t s = c->stream;
+  c->pd_val.alloc(c->val.count);
+  c->pd_dinv.alloc(std::max(c->n, 1));
+  c->pd_q.alloc(std::max(c->n, 1));
+  HIPCHK(hipMemsetAsync(c->pd_val.p, 0, sizeof(double) * c->pd_val.count, s));  // padding slots stay zero
+  const double kappa = c->youngs + 2.0 * c->poisson;
+  launch_elem_pd(c->mesh(), c->dt * c->dt * kappa, c->hperm.p, c->hcontrib.p, s);
+  launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->pd_val.p,
+                     c->pd_dinv.p, nullptr, s);
+  HIPCHK(hipGetLastError());
+  sync(c);
+}
+
 // L-BFGS two-loop recursion (Nocedal & Wright, Alg. 7.4) with H0 = diag(H_rest)⁻¹ and backtracking Armijo;
 // stepper/backward_euler.hpp:48-60 (tol = threshold·mass).  The two-loop coefficients stay on the device;
-// ρ_i = 1/(y_iᵀs_i) is read back once per accepted pair.
-void lbfgs_solve(ssim_ctx* c) {
+// ρ_i = 1/(y_iᵀs_i) is read back once per accepted pair.  With `pd` the middle step solves with the
+// projective-dynamics matrix instead (time_step_solver_pd, stepper/projective_dynamics.hpp:31-50: a direct solve
+// in the reference, Jacobi-PCG to a relative residual of kPdRtol here).
+constexpr double kPdRtol = 1e-12;
+void lbfgs_solve(ssim_ctx* c, bool pd) {
   const int n = c->n, m = std::max(1, c->prm.lbfgs_m);
   hipStream_t s = c->stream;
-  if (!c->hrest.p) {
+  if (pd && !c->pd_val.p) build_pd(c);
+  double* qv = pd ? c->pd_q.p : c->q.p;  // the PD solve's PCG uses c->q itself
+  if (!pd && !c->hrest.p) {
     // H0: diagonal of the SPD-projected, Dirichlet-filtered Hessian at rest (u = 0)
     c->hrest.alloc(n);
     HIPCHK(hipMemsetAsync(c->tmp.p, 0, sizeof(double) * n, s));
@@ -637,14 +670,17 @@ void lbfgs_solve(ssim_ctx* c) {
       c->stats.converged = 1;
       break;
     }
-    HIPCHK(hipMemcpyAsync(c->q.p, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(qv, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     for (int k = 0; k < hist; ++k) {
       const int idx = (head - 1 - k + m) % m;
       // a_i = ρ_i s_iᵀq ; q −= a_i y_i
-      launch_dot(n, c->lb_S.p + (size_t)idx * n, c->q.p, c->part_d(), av + idx, s);
-      launch_axpy_coef(n, -rho_h[idx], av + idx, 0.0, nullptr, c->lb_Y.p + (size_t)idx * n, c->q.p, s);
+      launch_dot(n, c->lb_S.p + (size_t)idx * n, qv, c->part_d(), av + idx, s);
+      launch_axpy_coef(n, -rho_h[idx], av + idx, 0.0, nullptr, c->lb_Y.p + (size_t)idx * n, qv, s);
     }
-    launch_scale_div(n, c->q.p, c->hrest.p, d, s);  // r = H0 q
+    if (pd)  // r = A_pd⁻¹ q
+      c->stats.pcg_iterations += pcg_solve(c, qv, 1.0, d, kPdRtol, c->prm.pcg_maxit, c->pd_val.p, c->pd_dinv.p);
+    else
+      launch_scale_div(n, qv, c->hrest.p, d, s);  // r = H0 q
     for (int k = hist - 1; k >= 0; --k) {
       const int idx = (head - 1 - k + m) % m;
       // b = ρ_i y_iᵀr ; r += s_i (a_i − b)
@@ -1027,7 +1063,8 @@ ssim_status_t ssim_get_solver_params(const ssim_ctx* ctx, ssim_solver_params* p)
 ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
   if (stepper != SSIM_STEPPER_NEWTON && stepper != SSIM_STEPPER_LBFGS && stepper != SSIM_STEPPER_NCG &&
-      stepper != SSIM_STEPPER_FORWARD_EULER && stepper != SSIM_STEPPER_NCG_EXT_PREC) {
+      stepper != SSIM_STEPPER_FORWARD_EULER && stepper != SSIM_STEPPER_NCG_EXT_PREC &&
+      stepper != SSIM_STEPPER_LBFGS_PD) {
     g_err = "unknown stepper";
     return SSIM_ERR_INVALID_ARGUMENT;
   }
@@ -1084,8 +1121,8 @@ ssim_status_t ssim_compute_step(ssim_ctx* ctx, double* residual_out) {
     ctx->stats = ssim_step_stats{};
     if (ctx->stepper == SSIM_STEPPER_NEWTON)
       newton_solve(ctx);
-    else if (ctx->stepper == SSIM_STEPPER_LBFGS)
-      lbfgs_solve(ctx);
+    else if (ctx->stepper == SSIM_STEPPER_LBFGS || ctx->stepper == SSIM_STEPPER_LBFGS_PD)
+      lbfgs_solve(ctx, ctx->stepper == SSIM_STEPPER_LBFGS_PD);
     else if (ctx->stepper == SSIM_STEPPER_NCG || ctx->stepper == SSIM_STEPPER_NCG_EXT_PREC)
       ncg_solve(ctx, ctx->stepper == SSIM_STEPPER_NCG_EXT_PREC);
     else
@@ -1353,6 +1390,15 @@ ssim_status_t ssim_export_hessian_csr(ssim_ctx* ctx, int* rowptr, int* colidx, d
   return run([&] {
     require_reset(ctx);
     export_csr(ctx, ctx->val.p, rowptr, colidx, values);
+  });
+}
+
+ssim_status_t ssim_export_pd_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    if (!ctx->pd_val.p) build_pd(ctx);
+    export_csr(ctx, ctx->pd_val.p, rowptr, colidx, values);
   });
 }
 

@@ -74,6 +74,36 @@ int launch_elem_grad(const MeshView& m, const double* u, const double* uhat, dou
   return nb;
 }
 
+// Projective-dynamics matrix (stepper/projective_dynamics.hpp:51-87): the 10 oriented blocks of tet e,
+// (ρV(1+δ_vw)/20 + c·V·∇N_v·∇N_w)·I₃ with c = dt²κ (consistent mass plus the scaled P1 Laplacian on every dof),
+// to the Hessian's contribution slots hperm[10e+k]; the Hessian reduce then assembles them.
+__global__ __launch_bounds__(kBlock) void k_elem_pd(MeshView m, double c, const int* __restrict__ hperm,
+                                                    double* __restrict__ hcontrib) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= m.nt || m.cells[e].x < 0) return;
+  const TetLocal t = load_tet(m, e);
+  double D[4][3];
+  shape_grads(t.B, D);
+  const double mm = m.rho * t.V / 20.0;
+  const int* pp = hperm + 10 * (size_t)e;
+#pragma unroll
+  for (int w = 0; w < 4; ++w)
+#pragma unroll
+    for (int v = 0; v <= w; ++v) {
+      const double val = (v == w ? 2.0 * mm : mm) + c * t.V * (D[v][0] * D[w][0] + D[v][1] * D[w][1] + D[v][2] * D[w][2]);
+      const size_t pos = 9 * (size_t)pp[pair_index(v, w)];
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) hcontrib[pos + 3 * b + d] = b == d ? val : 0.0;
+    }
+}
+
+void launch_elem_pd(const MeshView& m, double c, const int* hperm, double* hcontrib, hipStream_t s) {
+  if (m.nt <= 0) return;
+  k_elem_pd<<<nblocks(m.nt), kBlock, 0, s>>>(m, c, hperm, hcontrib);
+}
+
 // segmented reduce of the vertex-sorted gradient contributions; Dirichlet dofs zeroed (boundary.hpp:45-51)
 __global__ __launch_bounds__(kBlock) void k_grad_reduce(int nv, const int* __restrict__ gseg,
                                                         const double* __restrict__ gcontrib,

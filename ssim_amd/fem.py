@@ -40,17 +40,19 @@ class TetFiniteElementSolver_Hip:
     of the rest Hessian), "ncg" (nonlinear CG, Dai-Kou β, the L-BFGS line-search constants) or
     "forward_euler" (explicit: next = û + ½dt²·forces(û), finite_elements/time_step.hpp:717-736) or "ncg_ext_ai"
     (NCG preconditioned by z = L·Lᵀ·g with L set by ``set_matrix``, the reference's tet_ncg_cuda_ext_ai,
-    bindings/src/fem_cuda.cu:172-244; L is the identity after ``reset``).
+    bindings/src/fem_cuda.cu:172-244; L is the identity after ``reset``) or "lbfgs_pd" (L-BFGS preconditioned by
+    solves with the projective-dynamics matrix M + dt²κK built at ``reset``, the reference's tet_lbfgs_pd,
+    bindings/src/fem_host.cpp:223, stepper/projective_dynamics.hpp:9-88).
     material: "stable_neohookean" (elast/stable_neohookean.hpp, the reference bindings' model) or "linear"
     (elast/linear.hpp).
     """
 
-    METHODS = {"newton": 0, "lbfgs": 1, "ncg": 2, "forward_euler": 3, "ncg_ext_ai": 4}
+    METHODS = {"newton": 0, "lbfgs": 1, "ncg": 2, "forward_euler": 3, "ncg_ext_ai": 4, "lbfgs_pd": 5}
     MATERIALS = {"stable_neohookean": 0, "linear": 1}
 
     def __init__(self, vertices: np.ndarray, elements: np.ndarray, time_step: float = 1e-2,
                  young_modulus: float = 1e6, poisson_ratio: float = 0.33, density: float = 1e2,
-                 method: Literal["newton", "lbfgs", "ncg", "forward_euler", "ncg_ext_ai"] = "newton",
+                 method: Literal["newton", "lbfgs", "ncg", "forward_euler", "ncg_ext_ai", "lbfgs_pd"] = "newton",
                  material: Literal["stable_neohookean", "linear"] = "stable_neohookean"):
         if method not in self.METHODS:
             raise ValueError(f"Unknown method: {method}")
@@ -194,6 +196,10 @@ class TetFiniteElementSolver_Hip:
 
     def hessian(self):
         return self._csr("ssim_export_hessian_csr")
+
+    def pd_matrix(self):
+        """The projective-dynamics matrix of the "lbfgs_pd" method (scipy CSR, the Hessian's pattern)."""
+        return self._csr("ssim_export_pd_csr")
 
     def hessian_nonzeros(self) -> np.ndarray:
         return self.hessian().data.copy()
