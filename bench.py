@@ -5,11 +5,12 @@ Workload: cantilever bar of 32×32×128 Freudenthal cubes (140,481 vertices, 786
 iterations with Jacobi-PCG rtol 1e-8 and backtracking Armijo (c = 0.3).  The Newton stopping rule is the
 reference's ‖g‖₂ < threshold × total mass (time_step.hpp:482-492) with threshold 1e-6: at 786k tets the
 reference drivers' 1e-3 (tests/fem_cu/main.cu:60) stops most steps at the predictor, 1e-6 makes every step
-converge the nonlinear solve (about 2 Newton iterations and 1,700 PCG iterations per step).  All inputs are resident in HBM when the timed
-region starts; the timed region is K whole steps (prepare + Newton solve + step_next).
+converge the nonlinear solve (about 3 Newton iterations and 2,800 PCG iterations per step).  All inputs are
+resident in HBM when the timed region starts; the timed region is K whole steps (prepare + Newton solve + step_next).
 
 Prints one JSON line.  ``roofline`` is for the dominant kernel (the PCG SpMV), whose average duration is
-measured with hipEvents inside the timed region; ``cpu_baseline`` is the CPU FP64 oracle timed on a bounded
+measured with hipEvents inside the timed region (stamped at the sampled launches' own start and end by
+hipExtLaunchKernelGGL); ``cpu_baseline`` is the CPU FP64 oracle timed on a bounded
 sample of the same workload on this host (see ``cpu_sample`` below).
 
 Two more objects in the same line cover the batched and differentiable variants of the step:

@@ -218,6 +218,20 @@ void timed(ssim_ctx* c, ssim_kernel_t k, F&& launch) {
   }
 }
 
+// As timed(), for a launcher that takes the event pair and has the runtime stamp it at the kernel's own start
+// and end (hipExtLaunchKernelGGL): the interval then excludes the dispatch gap after the preceding kernel.
+template <typename F>
+void timed_ext(ssim_ctx* c, ssim_kernel_t k, F&& launch) {
+  auto& t = c->kt[k];
+  const bool arm = c->timing && !t.pending;
+  if (arm && !t.ev[0]) {
+    HIPCHK(hipEventCreate(&t.ev[0]));
+    HIPCHK(hipEventCreate(&t.ev[1]));
+  }
+  launch(arm ? t.ev[0] : nullptr, arm ? t.ev[1] : nullptr);
+  if (arm) t.pending = true;
+}
+
 void do_reset(ssim_ctx* c);
 void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_dbc, double dt2, double* values);
 template <typename F>

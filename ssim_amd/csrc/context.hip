@@ -528,13 +528,13 @@ int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol,
   // on its own (it carries the sampled kernel timers); the others are replayed from a captured graph.
   int* flags = reinterpret_cast<int*>(c->pin);  // two slots of 4 ints: done, iters, maxit, breakdown
   auto enqueue = [&](int count, int slot) {
-    timed(c, SSIM_KERNEL_PCG_SPMV, [&] {
+    timed_ext(c, SSIM_KERNEL_PCG_SPMV, [&](hipEvent_t e0, hipEvent_t e1) {
       launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
-                      c->part_pa(), s);
+                      c->part_pa(), s, e0, e1);
     });
-    timed(c, SSIM_KERNEL_PCG_UPDATE, [&] {
+    timed_ext(c, SSIM_KERNEL_PCG_UPDATE, [&](hipEvent_t e0, hipEvent_t e1) {
       launch_pcg_update(c->n, dinv, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(0),
-                        c->part_pb(1), c->pcg_it.p, c->pcg.p, s);
+                        c->part_pb(1), c->pcg_it.p, c->pcg.p, s, e0, e1);
     });
     if (count > 1) {
       if (c->pcg_graph)

@@ -5,6 +5,7 @@
 // (wavefront shuffles, a fixed-order workgroup tree and a last-workgroup final pass) and atomic-free
 // assembly (every element writes its contributions to precomputed, target-sorted slots; a segmented reduce
 // sums each target's contiguous run).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -583,28 +584,39 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
   store_partials<2>(v, pb_next);
 }
 
+// With an event pair the launch goes through hipExtLaunchKernelGGL, which stamps the events at the kernel's own
+// start and end (the interval a profiler reports) instead of around the stream commands.
+template <typename K, typename... Args>
+void launch_timed(K kernel, int grid, int block, hipStream_t s, hipEvent_t e0, hipEvent_t e1, Args... args) {
+  if (e0)
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, e0, e1, 0, args...);
+  else
+    kernel<<<grid, block, 0, s>>>(args...);
+}
+
 int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb0, const double* pb1,
-                    int nb, PcgIter* it, PcgScalars* sc, double* pa, hipStream_t s) {
+                    int nb, PcgIter* it, PcgScalars* sc, double* pa, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  const int rows = A.nrows > 0 ? A.nrows : 1;
   if (A.lanes_per_row == 4) {
-    const int grid = grid_for(RowMap<4>::blocks(A.nrows > 0 ? A.nrows : 1), A.xcd_remap);
-    k_pcg_spmv<4><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb0, pb1, nb, it, sc, pa);
+    const int grid = grid_for(RowMap<4>::blocks(rows), A.xcd_remap);
+    launch_timed(k_pcg_spmv<4>, grid, kSpmvBlock, s, e0, e1, A, z, p, q, pb0, pb1, nb, it, sc, pa);
     return grid;
   }
   if (A.lanes_per_row == 0) {
-    const int grid = grid_for(RowMap<0>::blocks(A.nrows > 0 ? A.nrows : 1), A.xcd_remap);
-    k_pcg_spmv<0><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb0, pb1, nb, it, sc, pa);
+    const int grid = grid_for(RowMap<0>::blocks(rows), A.xcd_remap);
+    launch_timed(k_pcg_spmv<0>, grid, kSpmvBlock, s, e0, e1, A, z, p, q, pb0, pb1, nb, it, sc, pa);
     return grid;
   }
-  const int grid = grid_for(RowMap<1>::blocks(A.nrows > 0 ? A.nrows : 1), A.xcd_remap);
-  k_pcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, z, p, q, pb0, pb1, nb, it, sc, pa);
+  const int grid = grid_for(RowMap<1>::blocks(rows), A.xcd_remap);
+  launch_timed(k_pcg_spmv<1>, grid, kSpmvBlock, s, e0, e1, A, z, p, q, pb0, pb1, nb, it, sc, pa);
   return grid;
 }
 
 void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
                        const double* pa, int na, double* pb0, double* pb1, PcgIter* it, PcgScalars* sc,
-                       hipStream_t s) {
+                       hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const int nb = vblocks(n);
-  k_pcg_update<<<nb, kBlock, 0, s>>>(n, dinv, x, r, z, p, q, pa, na, pb0, pb1, nb, it, sc);
+  launch_timed(k_pcg_update, nb, kBlock, s, e0, e1, n, dinv, x, r, z, p, q, pa, na, pb0, pb1, nb, it, sc);
 }
 
 int pcg_vector_blocks(int n) { return vblocks(n); }
