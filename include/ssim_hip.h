@@ -37,9 +37,11 @@ typedef enum {
   SSIM_STEPPER_FORWARD_EULER = 3,
   SSIM_STEPPER_NCG_EXT_PREC = 4, /* NCG preconditioned by z = L·Lᵀ·g (time_step_solver_ncg_with_ext_prec,
                                     stepper/nonlinear_cg.hpp:41-97; L set by ssim_set_precond_matrix) */
-  SSIM_STEPPER_LBFGS_PD = 5       /* L-BFGS preconditioned by solves with the projective-dynamics matrix
+  SSIM_STEPPER_LBFGS_PD = 5,      /* L-BFGS preconditioned by solves with the projective-dynamics matrix
                                     M + dt²κK built at reset (time_step_solver_pd, stepper/projective_dynamics.hpp:9-88;
                                     tet_lbfgs_pd, bindings/src/fem_host.cpp:223) */
+  SSIM_STEPPER_LBFGS_H0 = 6       /* L-BFGS preconditioned by solves with the SPD Hessian at the start of each step
+                                    (time_step_solver_h0, stepper/projective_dynamics.hpp:91-148) */
 } ssim_stepper_t;
 
 /* material model, the ElastModel template argument of basic_time_step (finite_elements/time_step.hpp:168-172):

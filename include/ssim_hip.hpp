@@ -7,8 +7,8 @@
 // restore, the setters (which throw std::invalid_argument where the reference does, time_step.hpp:316-347) and
 // the vertex-array views (device pointers of [nv, 3] row-major doubles, time_step.hpp:275-311).  Steppers are the
 // tag types newton_solver (time_step_solver_backward_euler, stepper/backward_euler.hpp:9-41), lbfgs_solver
-// (time_step_solver_lbfgs, backward_euler.hpp:43-67), lbfgs_pd_solver, ncg_solver, ncg_ext_prec_solver and
-// forward_euler_solver, passed to reset / compute_step as in the reference.
+// (time_step_solver_lbfgs, backward_euler.hpp:43-67), lbfgs_pd_solver, lbfgs_h0_solver, ncg_solver,
+// ncg_ext_prec_solver and forward_euler_solver, passed to reset / compute_step as in the reference.
 //
 // ssim::hip::batch is the batched extension (many meshes per set of launches) and has no reference counterpart.
 #ifndef SSIM_HIP_HPP
@@ -64,6 +64,10 @@ struct forward_euler_solver {  // time_step_solver_forward_euler (finite_element
 // time_step_solver_pd (stepper/projective_dynamics.hpp:9-88), bound as tet_lbfgs_pd
 struct lbfgs_pd_solver {
   static constexpr ssim_stepper_t kind = SSIM_STEPPER_LBFGS_PD;
+};
+// time_step_solver_h0 (stepper/projective_dynamics.hpp:91-148)
+struct lbfgs_h0_solver {
+  static constexpr ssim_stepper_t kind = SSIM_STEPPER_LBFGS_H0;
 };
 // time_step_solver_ncg_with_ext_prec (stepper/nonlinear_cg.hpp:41-97); L from time_step::set_matrix
 struct ncg_ext_prec_solver {

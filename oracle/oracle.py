@@ -75,6 +75,7 @@ def lib():
             "orc_ts_build_pd": (None, [_V, ctypes.c_double]),
             "orc_ts_pd_values": (None, [_V, _D]),
             "orc_ts_compute_step_lbfgs_pd": (ctypes.c_double, [_V, _D]),
+            "orc_ts_compute_step_lbfgs_h0": (ctypes.c_double, [_V, _D]),
             "orc_lin_energy": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double]),
             "orc_lin_stress": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, _D]),
             "orc_lin_hessian": (None, [ctypes.c_double, ctypes.c_double, _D]),
@@ -421,7 +422,7 @@ class TimeStep:
         st = np.zeros(6)
         fn = {"newton": lib().orc_ts_compute_step, "lbfgs": lib().orc_ts_compute_step_lbfgs,
               "ncg": lib().orc_ts_compute_step_ncg, "forward_euler": lib().orc_ts_compute_step_fe,
-              "lbfgs_pd": lib().orc_ts_compute_step_lbfgs_pd}[method]
+              "lbfgs_pd": lib().orc_ts_compute_step_lbfgs_pd, "lbfgs_h0": lib().orc_ts_compute_step_lbfgs_h0}[method]
         gn = fn(self._h, _dp(st))
         return gn, dict(newton=int(st[0]), pcg=int(st[1]), evals=int(st[2]), converged=bool(st[3]),
                         grad_norm=float(st[4]), ls_fail=int(st[5]))

@@ -337,7 +337,7 @@ struct orc_ts {
   int model;  /* 0: stable Neo-Hookean, 1: linear elasticity */
   double* prec; /* NCG preconditioner factor L in the CSR order of the Hessian (NULL: none) */
   double* pd_vals; /* projective-dynamics matrix M + dt²κL (Hessian CSR order), built by orc_ts_build_pd */
-  int lbfgs_pd;    /* L-BFGS preconditioned by solves with pd_vals instead of the rest-Hessian diagonal */
+  int lbfgs_pd;    /* L-BFGS middle step: 0 rest-Hessian diagonal, 1 solve with pd_vals, 2 solve with the Hessian */
   double pcg_rtol, ls_c, ls_beta;
   int lbfgs_m, lbfgs_maxit, lbfgs_ls_max;
   double lbfgs_c, lbfgs_beta;
@@ -894,6 +894,15 @@ void orc_ts_pd_values(const orc_ts* ts, double* vals) {
 
 /* time_step_solver_pd (stepper/projective_dynamics.hpp:36-50): L-BFGS whose two-loop middle step solves with the
  * projective-dynamics matrix (a direct solver in the reference; here Jacobi-PCG to a relative residual of 1e-12). */
+/* time_step_solver_h0 (stepper/projective_dynamics.hpp:91-148): L-BFGS whose two-loop middle step solves with the
+ * SPD-projected, Dirichlet-filtered Hessian at the start of the step (factorised in the reference). */
+double orc_ts_compute_step_lbfgs_h0(orc_ts* ts, double* stats) {
+  ts->lbfgs_pd = 2;
+  const double r = orc_ts_compute_step_lbfgs(ts, stats);
+  ts->lbfgs_pd = 0;
+  return r;
+}
+
 double orc_ts_compute_step_lbfgs_pd(orc_ts* ts, double* stats) {
   if (!ts->pd_vals) return -1.0;
   ts->lbfgs_pd = 1;
@@ -917,6 +926,7 @@ double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats) {
   double* ut = (double*)malloc(sizeof(double) * n);
   double* gt = (double*)malloc(sizeof(double) * n);
   int it = 0, evals = 0, converged = 0, ls_fail = 0, hist = 0, head = 0, pd_total = 0;
+  if (ts->lbfgs_pd == 2) orc_ts_hessian(ts, u, 1, 1, ts->Hvals);
   double E = orc_ts_energy_grad(ts, u, ts->g);
   evals++;
   for (; it < ts->lbfgs_maxit;) {
@@ -932,8 +942,9 @@ double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats) {
       const double* y = Y + (size_t)idx * n;
       for (int i = 0; i < n; ++i) q[i] -= a[idx] * y[i];
     }
-    if (ts->lbfgs_pd) { /* r = A_pd⁻¹ q */
-      pd_total += orc_pcg(n, ts->rowptr, ts->colidx, ts->pd_vals, q, ut, 1e-12, 100000, NULL);
+    if (ts->lbfgs_pd) { /* r = A⁻¹ q, A the PD matrix (1) or the Hessian at the start of the step (2) */
+      pd_total += orc_pcg(n, ts->rowptr, ts->colidx, ts->lbfgs_pd == 2 ? ts->Hvals : ts->pd_vals, q, ut, 1e-12,
+                          100000, NULL);
       memcpy(q, ut, sizeof(double) * n);
     } else {
       for (int i = 0; i < n; ++i) q[i] /= ts->hrest_diag[i];

@@ -64,6 +64,8 @@ void orc_ts_set_precond(orc_ts* ts, const double* values);
 void orc_ts_build_pd(orc_ts* ts, double kappa);
 void orc_ts_pd_values(const orc_ts* ts, double* vals);
 double orc_ts_compute_step_lbfgs_pd(orc_ts* ts, double* stats);
+/* L-BFGS preconditioned by solves with the SPD Hessian at the start of the step */
+double orc_ts_compute_step_lbfgs_h0(orc_ts* ts, double* stats);
 double orc_lin_energy(const double F[9], double lam, double mu);
 double orc_lin_stress(const double F[9], double lam, double mu, double P[9]);
 void orc_lin_hessian(double lam, double mu, double H[81]);

@@ -637,13 +637,21 @@ void build_pd(ssim_ctx* c) {
 // ρ_i = 1/(y_iᵀs_i) is read back once per accepted pair.  With `pd` the middle step solves with the
 // projective-dynamics matrix instead (time_step_solver_pd, stepper/projective_dynamics.hpp:31-50: a direct solve
 // in the reference, Jacobi-PCG to a relative residual of kPdRtol here).
+// With `h0` it solves with the SPD-projected, Dirichlet-filtered Hessian at the step's starting point instead
+// (time_step_solver_h0, stepper/projective_dynamics.hpp:91-148: update_hessian(false, true) and a factorisation per
+// step; the reference binds no Python class for it).
 constexpr double kPdRtol = 1e-12;
-void lbfgs_solve(ssim_ctx* c, bool pd) {
+void lbfgs_solve(ssim_ctx* c, bool pd, bool h0 = false) {
   const int n = c->n, m = std::max(1, c->prm.lbfgs_m);
   hipStream_t s = c->stream;
   if (pd && !c->pd_val.p) build_pd(c);
-  double* qv = pd ? c->pd_q.p : c->q.p;  // the PD solve's PCG uses c->q itself
-  if (!pd && !c->hrest.p) {
+  if (h0) {
+    c->pd_q.alloc(std::max(n, 1));
+    assemble_hessian(c, c->a(SSIM_ARRAY_NEXT_DEFORM), 1, 1, c->dt * c->dt, c->val.p);
+  }
+  const bool solve = pd || h0;
+  double* qv = solve ? c->pd_q.p : c->q.p;  // the solve's PCG uses c->q itself
+  if (!solve && !c->hrest.p) {
     // H0: diagonal of the SPD-projected, Dirichlet-filtered Hessian at rest (u = 0)
     c->hrest.alloc(n);
     HIPCHK(hipMemsetAsync(c->tmp.p, 0, sizeof(double) * n, s));
@@ -679,6 +687,8 @@ void lbfgs_solve(ssim_ctx* c, bool pd) {
     }
     if (pd)  // r = A_pd⁻¹ q
       c->stats.pcg_iterations += pcg_solve(c, qv, 1.0, d, kPdRtol, c->prm.pcg_maxit, c->pd_val.p, c->pd_dinv.p);
+    else if (h0)  // r = H(u_start)⁻¹ q
+      c->stats.pcg_iterations += pcg_solve(c, qv, 1.0, d, kPdRtol, c->prm.pcg_maxit);
     else
       launch_scale_div(n, qv, c->hrest.p, d, s);  // r = H0 q
     for (int k = hist - 1; k >= 0; --k) {
@@ -1064,7 +1074,7 @@ ssim_status_t ssim_set_stepper(ssim_ctx* ctx, ssim_stepper_t stepper) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
   if (stepper != SSIM_STEPPER_NEWTON && stepper != SSIM_STEPPER_LBFGS && stepper != SSIM_STEPPER_NCG &&
       stepper != SSIM_STEPPER_FORWARD_EULER && stepper != SSIM_STEPPER_NCG_EXT_PREC &&
-      stepper != SSIM_STEPPER_LBFGS_PD) {
+      stepper != SSIM_STEPPER_LBFGS_PD && stepper != SSIM_STEPPER_LBFGS_H0) {
     g_err = "unknown stepper";
     return SSIM_ERR_INVALID_ARGUMENT;
   }
@@ -1121,8 +1131,9 @@ ssim_status_t ssim_compute_step(ssim_ctx* ctx, double* residual_out) {
     ctx->stats = ssim_step_stats{};
     if (ctx->stepper == SSIM_STEPPER_NEWTON)
       newton_solve(ctx);
-    else if (ctx->stepper == SSIM_STEPPER_LBFGS || ctx->stepper == SSIM_STEPPER_LBFGS_PD)
-      lbfgs_solve(ctx, ctx->stepper == SSIM_STEPPER_LBFGS_PD);
+    else if (ctx->stepper == SSIM_STEPPER_LBFGS || ctx->stepper == SSIM_STEPPER_LBFGS_PD ||
+             ctx->stepper == SSIM_STEPPER_LBFGS_H0)
+      lbfgs_solve(ctx, ctx->stepper == SSIM_STEPPER_LBFGS_PD, ctx->stepper == SSIM_STEPPER_LBFGS_H0);
     else if (ctx->stepper == SSIM_STEPPER_NCG || ctx->stepper == SSIM_STEPPER_NCG_EXT_PREC)
       ncg_solve(ctx, ctx->stepper == SSIM_STEPPER_NCG_EXT_PREC);
     else

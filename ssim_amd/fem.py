@@ -42,17 +42,19 @@ class TetFiniteElementSolver_Hip:
     (NCG preconditioned by z = L·Lᵀ·g with L set by ``set_matrix``, the reference's tet_ncg_cuda_ext_ai,
     bindings/src/fem_cuda.cu:172-244; L is the identity after ``reset``) or "lbfgs_pd" (L-BFGS preconditioned by
     solves with the projective-dynamics matrix M + dt²κK built at ``reset``, the reference's tet_lbfgs_pd,
-    bindings/src/fem_host.cpp:223, stepper/projective_dynamics.hpp:9-88).
+    bindings/src/fem_host.cpp:223, stepper/projective_dynamics.hpp:9-88) or "lbfgs_h0" (L-BFGS preconditioned by
+    solves with the SPD Hessian at the start of each step, time_step_solver_h0, projective_dynamics.hpp:91-148).
     material: "stable_neohookean" (elast/stable_neohookean.hpp, the reference bindings' model) or "linear"
     (elast/linear.hpp).
     """
 
-    METHODS = {"newton": 0, "lbfgs": 1, "ncg": 2, "forward_euler": 3, "ncg_ext_ai": 4, "lbfgs_pd": 5}
+    METHODS = {"newton": 0, "lbfgs": 1, "ncg": 2, "forward_euler": 3, "ncg_ext_ai": 4, "lbfgs_pd": 5, "lbfgs_h0": 6}
     MATERIALS = {"stable_neohookean": 0, "linear": 1}
 
     def __init__(self, vertices: np.ndarray, elements: np.ndarray, time_step: float = 1e-2,
                  young_modulus: float = 1e6, poisson_ratio: float = 0.33, density: float = 1e2,
-                 method: Literal["newton", "lbfgs", "ncg", "forward_euler", "ncg_ext_ai", "lbfgs_pd"] = "newton",
+                 method: Literal["newton", "lbfgs", "ncg", "forward_euler", "ncg_ext_ai", "lbfgs_pd",
+                                 "lbfgs_h0"] = "newton",
                  material: Literal["stable_neohookean", "linear"] = "stable_neohookean"):
         if method not in self.METHODS:
             raise ValueError(f"Unknown method: {method}")

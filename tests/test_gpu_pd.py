@@ -87,3 +87,27 @@ def test_trajectory_with_the_reset_built_matrix():
         g.step()
         o.step("lbfgs_pd")
         assert rel_err(X + g.get_array("deformation"), X + o.get_state()[0]) <= 1e-6, step
+
+
+def test_lbfgs_h0_matches_oracle():
+    case = _case()
+    for maxit in (1, 5000):
+        g = fem.TetFiniteElementSolver_Hip.from_case(case, method="lbfgs_h0", rtol=1e-8)
+        g.set_solver_params(lbfgs_maxit=maxit)
+        o = oracle.TimeStep.from_case(case)
+        o.set_threshold(1e-8)
+        o.set_lbfgs(max_iter=maxit)
+        g.prepare_step()
+        g.compute_step()
+        o.prepare_step()
+        _, so = o.compute_step("lbfgs_h0")
+        uh = o.get_vectors()[1]
+        sg = g.last_stats()
+        if maxit == 1:
+            assert rel_err(g.get_array("next_deform") - uh, o.get_vectors()[0] - uh) <= 1e-9
+        else:
+            assert sg["converged"] and so["converged"]
+            X = case.X.reshape(-1)
+            assert rel_err(X + g.get_array("next_deform"), X + o.get_vectors()[0]) <= 1e-6
+            assert abs(sg["newton_iterations"] - so["newton"]) <= 3
+            print(f"[gpu] lbfgs_h0: {sg['newton_iterations']} iterations (oracle {so['newton']})", flush=True)

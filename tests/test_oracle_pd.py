@@ -99,3 +99,25 @@ def test_lbfgs_pd_reaches_the_newton_minimiser():
     assert sn["converged"] and sp_["converged"], sp_
     X = case.X.reshape(-1)
     assert rel_err(X + pd.get_vectors()[0], X + newton.get_vectors()[0]) <= 1e-6
+
+
+def test_lbfgs_h0_first_step_is_the_newton_direction():
+    """time_step_solver_h0 (stepper/projective_dynamics.hpp:91-148): the two-loop middle step solves with the
+    SPD Hessian at the start of the step, so the first step is −α·H⁻¹g (sparse direct solve), and the iteration
+    reaches the Newton minimiser."""
+    case = _case()
+    ts = _ts(case, max_iter=1)
+    _, uh, _ = ts.get_vectors()
+    _, g0 = ts.energy_grad(uh)
+    H = ts.hessian_csr(uh, True, True)
+    ts.compute_step("lbfgs_h0")
+    step = ts.get_vectors()[0] - uh
+    d0 = -spla.spsolve(H.tocsc(), g0)
+    alpha = (step @ d0) / (d0 @ d0)
+    assert rel_err(step, alpha * d0) <= 1e-10
+    newton, h0 = _ts(case), _ts(case)
+    newton.compute_step("newton")
+    _, s = h0.compute_step("lbfgs_h0")
+    assert s["converged"]
+    X = case.X.reshape(-1)
+    assert rel_err(X + h0.get_vectors()[0], X + newton.get_vectors()[0]) <= 1e-6
