@@ -616,15 +616,22 @@ void newton_solve(ssim_ctx* c) {
 
 // Projective-dynamics matrix of time_step_solver_pd::reset_impl (stepper/projective_dynamics.hpp:51-87):
 // M + dt²·κ·K on every dof, K the P1 Laplacian, Dirichlet rows and columns set to the identity, with κ the
-// diffusivity λ + 2μ of the reference's uniform_elasticity(), which builds the material from (E, ν) in the
-// (λ, μ) slots (finite_elements/time_step.hpp:356, elast/basic_elast.hpp:225): κ = E + 2ν.
+// approximated diffusivity of the reference's uniform_elasticity(), which builds the material from (E, ν) in the
+// (λ, μ) slots (finite_elements/time_step.hpp:356): for stable Neo-Hookean λ + 5μ/6 + 2·(4μ/3)
+// (elast/stable_neohookean.hpp:109-114), κ = E + 3.5ν; for linear elasticity λ + 2μ (elast/basic_elast.hpp:225),
+// κ = E + 2ν.
+double pd_diffusivity(ssim_material_t model, double lam, double mu) {
+  if (model == SSIM_MATERIAL_LINEAR) return lam + 2.0 * mu;
+  return (lam + 5.0 / 6.0 * mu) + 2.0 * (4.0 / 3.0 * mu);
+}
+
 void build_pd(ssim_ctx* c) {
   hipStream_t s = c->stream;
   c->pd_val.alloc(c->val.count);
   c->pd_dinv.alloc(std::max(c->n, 1));
   c->pd_q.alloc(std::max(c->n, 1));
   HIPCHK(hipMemsetAsync(c->pd_val.p, 0, sizeof(double) * c->pd_val.count, s));  // padding slots stay zero
-  const double kappa = c->youngs + 2.0 * c->poisson;
+  const double kappa = pd_diffusivity(c->model, c->youngs, c->poisson);
   launch_elem_pd(c->mesh(), c->dt * c->dt * kappa, c->hperm.p, c->hcontrib.p, s);
   launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->pd_val.p,
                      c->pd_dinv.p, nullptr, s);

@@ -111,3 +111,15 @@ def test_lbfgs_h0_matches_oracle():
             assert rel_err(X + g.get_array("next_deform"), X + o.get_vectors()[0]) <= 1e-6
             assert abs(sg["newton_iterations"] - so["newton"]) <= 3
             print(f"[gpu] lbfgs_h0: {sg['newton_iterations']} iterations (oracle {so['newton']})", flush=True)
+
+
+def test_pd_matrix_follows_the_material_model():
+    """κ is the model's approximated diffusivity: E + 3.5ν for stable Neo-Hookean, E + 2ν for linear elasticity."""
+    case = _case()
+    E, nu = _e_nu(case)
+    for model in ("stable_neohookean", "linear"):
+        g = fem.TetFiniteElementSolver_Hip.from_case(case, method="lbfgs_pd", material=model)
+        g.reset()
+        o = oracle.TimeStep.from_case(case)
+        o.build_pd(pd_kappa(E, nu, model))
+        assert rel_err(g.pd_matrix().data, o.pd_csr().data) <= 1e-12, model

@@ -5,7 +5,8 @@ Pins: the matrix M + dt²κK against the closed form on the reference tet (K = V
 zero row sums and the patch test of K (K·x = 0 on interior rows for linear x), symmetry and the Dirichlet
 identity rows; the first L-BFGS step against −A⁻¹g from a sparse direct solve; convergence to the Newton
 minimiser.  The diffusivity κ follows the reference's uniform_elasticity(), which builds the material from
-(E, ν) in the (λ, μ) slots (finite_elements/time_step.hpp:356), so κ = λ + 2μ reads E + 2ν.
+(E, ν) in the (λ, μ) slots (finite_elements/time_step.hpp:356), so κ = λ + 5μ/6 + 2·(4μ/3) of the stable
+Neo-Hookean model reads E + 3.5ν.
 """
 import numpy as np
 import scipy.sparse.linalg as spla
@@ -15,9 +16,13 @@ from helpers import rel_err
 from ssim_amd import mesh
 
 
-def pd_kappa(E, nu):
-    """approximated_diffusivity (elast/basic_elast.hpp:225) of ElastModel{E, ν}: λ + 2μ with λ = E, μ = ν."""
-    return E + 2.0 * nu
+def pd_kappa(E, nu, model="stable_neohookean"):
+    """approximated_diffusivity of ElastModel{E, ν}, i.e. λ = E, μ = ν: stable Neo-Hookean overrides it with
+    λ + 5μ/6 + 2·(4μ/3) (elast/stable_neohookean.hpp:109-114), linear elasticity keeps λ + 2μ
+    (elast/basic_elast.hpp:225)."""
+    if model == "linear":
+        return E + 2.0 * nu
+    return (E + 5.0 / 6.0 * nu) + 2.0 * (4.0 / 3.0 * nu)
 
 
 def test_pd_matrix_closed_form_on_the_reference_tet():
