@@ -79,6 +79,12 @@ def lib():
             "orc_lin_energy": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double]),
             "orc_lin_stress": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, _D]),
             "orc_lin_hessian": (None, [ctypes.c_double, ctypes.c_double, _D]),
+            "orc_snh2_energy": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double]),
+            "orc_snh2_stress": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, _D]),
+            "orc_snh2_hessian": (None, [_D, ctypes.c_double, ctypes.c_double, _D]),
+            "orc_lin2_energy": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double]),
+            "orc_lin2_stress": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, _D]),
+            "orc_lin2_hessian": (None, [ctypes.c_double, ctypes.c_double, _D]),
             "orc_ts_step_next": (None, [_V]),
             "orc_ts_rest_hessian_diag": (None, [_V, _D]),
             "orc_ts_spmv": (None, [_V, _D, _D, _D]),
@@ -132,6 +138,24 @@ def lin_hessian(lam, mu):
     H = np.zeros(81)
     lib().orc_lin_hessian(lam, mu, _dp(H))
     return H.reshape(9, 9)
+
+
+def material2(model, F, lam, mu):
+    """2D model ("stable_neohookean" or "linear"): (Ψ, P (2,2), H (4,4)) at the row-major 2x2 F."""
+    F = _f64(F).reshape(4)
+    P, H = np.zeros(4), np.zeros(16)
+    if model == "linear":
+        e = lib().orc_lin2_stress(_dp(F), lam, mu, _dp(P))
+        lib().orc_lin2_hessian(lam, mu, _dp(H))
+    else:
+        e = lib().orc_snh2_stress(_dp(F), lam, mu, _dp(P))
+        lib().orc_snh2_hessian(_dp(F), lam, mu, _dp(H))
+    return e, P.reshape(2, 2), H.reshape(4, 4)
+
+
+def energy2(model, F, lam, mu):
+    F = _f64(F).reshape(4)
+    return (lib().orc_lin2_energy if model == "linear" else lib().orc_snh2_energy)(_dp(F), lam, mu)
 
 
 def snh_hessian(F, lam, mu):

@@ -225,6 +225,85 @@ void orc_lin_hessian(double lam, double mu, double H[81]) {
     }
 }
 
+/* 2D models, F row-major [F00, F01, F10, F11], Hessian 4x4 in that flattening.
+ * Stable Neo-Hookean with Ndim = 2 (elast/stable_neohookean.hpp:23-107): δ the positive root of
+ * (2t+1)δ² + (2t+2)δ − 2t = 0 with t = μ/λ, α = (1 − 1/(2+δ))μ/λ + 1,
+ * Ψ = ½(μ(Ic − 2) + λ(J − α)² − μ ln(Ic + δ)) − ½(λ(1 − α)² − μ ln(2 + δ)),
+ * P = λ(J − α)·cof F + μ(1 − 1/(Ic+δ))·F, cof F = [[F11, −F10], [−F01, F00]] (internal_physic.hpp:32-37). */
+static double snh2_delta(double lam, double mu) {
+  const double t = mu / lam;
+  const double a = 2 * t + 1, b = 2 * t + 2, c = -2 * t;
+  return (-b + sqrt(b * b - 4 * a * c)) / (2 * a);
+}
+
+double orc_snh2_energy(const double F[4], double lam, double mu) {
+  const double del = snh2_delta(lam, mu);
+  const double Ic = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+  const double J = F[0] * F[3] - F[1] * F[2];
+  const double alpha = (1 - 1 / (2 + del)) * mu / lam + 1;
+  const double zero = 0.5 * (lam * (1 - alpha) * (1 - alpha) - mu * log(2 + del));
+  return 0.5 * (mu * (Ic - 2) + lam * (J - alpha) * (J - alpha) - mu * log(Ic + del)) - zero;
+}
+
+double orc_snh2_stress(const double F[4], double lam, double mu, double P[4]) {
+  const double del = snh2_delta(lam, mu);
+  const double Ic = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+  const double J = F[0] * F[3] - F[1] * F[2];
+  const double alpha = (1 - 1 / (2 + del)) * mu / lam + 1;
+  const double cof[4] = {F[3], -F[2], -F[1], F[0]};
+  for (int i = 0; i < 4; ++i) P[i] = lam * (J - alpha) * cof[i] + mu * (1 - 1 / (Ic + del)) * F[i];
+  return orc_snh2_energy(F, lam, mu);
+}
+
+/* H = μ(1 − 1/(Ic+δ))I + 2μ/(Ic+δ)² f fᵀ + λ g gᵀ + λ(J − α) ∂²J/∂F² (stable_neohookean.hpp:64-92, add_HJ
+ * internal_physic.hpp:54-60) */
+void orc_snh2_hessian(const double F[4], double lam, double mu, double H[16]) {
+  const double del = snh2_delta(lam, mu);
+  const double Ic = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+  const double J = F[0] * F[3] - F[1] * F[2];
+  const double alpha = (1 - 1 / (2 + del)) * mu / lam + 1;
+  const double g[4] = {F[3], -F[2], -F[1], F[0]};
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      H[4 * i + j] = (i == j ? mu * (1 - 1 / (Ic + del)) : 0.0) + 2 * mu / ((Ic + del) * (Ic + del)) * F[i] * F[j] +
+                     lam * g[i] * g[j];
+  const double s = lam * (J - alpha);
+  H[3] += s;
+  H[12] += s;
+  H[6] -= s;
+  H[9] -= s;
+}
+
+/* linear elasticity with Ndim = 2 (elast/linear.hpp:24-69) */
+double orc_lin2_energy(const double F[4], double lam, double mu) {
+  double n2 = 0;
+  const double tr = F[0] + F[3] - 2;
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c) {
+      const double e = 0.5 * (F[2 * a + c] + F[2 * c + a]) - (a == c ? 1.0 : 0.0);
+      n2 += e * e;
+    }
+  return mu * n2 + 0.5 * lam * tr * tr;
+}
+
+double orc_lin2_stress(const double F[4], double lam, double mu, double P[4]) {
+  const double tr = F[0] + F[3] - 2;
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c)
+      P[2 * a + c] = 2 * mu * (0.5 * (F[2 * a + c] + F[2 * c + a]) - (a == c ? 1.0 : 0.0)) + (a == c ? lam * tr : 0.0);
+  return orc_lin2_energy(F, lam, mu);
+}
+
+void orc_lin2_hessian(double lam, double mu, double H[16]) {
+  for (int k = 0; k < 16; ++k) H[k] = 0.0;
+  for (int a = 0; a < 4; ++a) H[5 * a] = mu;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) {
+      H[(2 * i + j) * 4 + 2 * j + i] += mu;
+      H[(2 * i + i) * 4 + 2 * j + j] += lam;
+    }
+}
+
 static void inv3(const double A[9], double Ai[9]) {
   const double d = det3(A);
   Ai[0] = (A[4] * A[8] - A[5] * A[7]) / d;

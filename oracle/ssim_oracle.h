@@ -69,6 +69,13 @@ double orc_ts_compute_step_lbfgs_h0(orc_ts* ts, double* stats);
 double orc_lin_energy(const double F[9], double lam, double mu);
 double orc_lin_stress(const double F[9], double lam, double mu, double P[9]);
 void orc_lin_hessian(double lam, double mu, double H[81]);
+/* 2D models (F row-major 2x2, Hessian 4x4) */
+double orc_snh2_energy(const double F[4], double lam, double mu);
+double orc_snh2_stress(const double F[4], double lam, double mu, double P[4]);
+void orc_snh2_hessian(const double F[4], double lam, double mu, double H[16]);
+double orc_lin2_energy(const double F[4], double lam, double mu);
+double orc_lin2_stress(const double F[4], double lam, double mu, double P[4]);
+void orc_lin2_hessian(double lam, double mu, double H[16]);
 /* forward Euler: u_next = û + ½dt²·g(û) (time_step.hpp:717-736) */
 double orc_ts_compute_step_fe(orc_ts* ts, double* stats);
 void orc_ts_set_lbfgs(orc_ts* ts, int m, int max_iter, double ls_c, double ls_beta, int ls_max);

@@ -1037,6 +1037,21 @@ ssim_status_t ssim_set_material_per_tet(ssim_ctx* ctx, const double* lambda, con
     ctx->hrest.release();
   });
 }
+ssim_status_t ssim_material_eval(int dim, ssim_material_t model, long long n, const double* F, const double* lambda,
+                                 const double* mu, int make_spsd, double* psi, double* P, double* H, void* stream) {
+  return run([&] {
+    if (dim != 2 && dim != 3) fail(SSIM_ERR_INVALID_ARGUMENT, "dim must be 2 or 3");
+    if (model != SSIM_MATERIAL_STABLE_NEOHOOKEAN && model != SSIM_MATERIAL_LINEAR)
+      fail(SSIM_ERR_INVALID_ARGUMENT, "unknown material model");
+    if (n < 0) fail(SSIM_ERR_INVALID_ARGUMENT, "n must be non-negative");
+    if (n > 0 && (!F || !lambda || !mu)) fail(SSIM_ERR_INVALID_ARGUMENT, "null input array");
+    if (make_spsd && dim != 3) fail(SSIM_ERR_INVALID_ARGUMENT, "the SPD projection is defined for dim 3 only");
+    if (n == 0) return;
+    launch_material_eval(dim, model, n, F, lambda, mu, make_spsd, psi, P, H, static_cast<hipStream_t>(stream));
+    HIPCHK(hipGetLastError());
+  });
+}
+
 ssim_status_t ssim_set_material_model(ssim_ctx* ctx, ssim_material_t model) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
   if (model != SSIM_MATERIAL_STABLE_NEOHOOKEAN && model != SSIM_MATERIAL_LINEAR) {

@@ -396,4 +396,82 @@ SSIM_HD void material_hessian(const T F[9], T lam, T mu, int make_spsd, T floor_
   }
 }
 
+// ------------------------------------------------------------------------------------------------------
+// 2D models (Ndim = 2 of elast/stable_neohookean.hpp and elast/linear.hpp; the reference has no 2D time step,
+// they are exercised by element tests, tests/fem_intro/main.cpp:155-300).  F row-major [F00, F01, F10, F11];
+// the 4x4 Hessian is full, H[4·i + j] = ∂²Ψ/∂F_i∂F_j in that flattening.
+
+// δ of the 2D stable Neo-Hookean model: the positive root of (2t+1)δ² + (2t+2)δ − 2t = 0, t = μ/λ
+// (elast/stable_neohookean.hpp:96-107; δ = 1 in 3D)
+template <typename T>
+SSIM_HD T snh2_delta(T lam, T mu) {
+  const T t = mu / lam;
+  const T a = T(2) * t + T(1), b = T(2) * t + T(2), c = -T(2) * t;
+  return (-b + sqrt(b * b - T(4) * a * c)) / (T(2) * a);
+}
+
+// Ψ = ½(μ(Ic − 2) + λ(J − α)² − μ·ln(Ic + δ)) − Ψ(I), α = (1 − 1/(2+δ))·μ/λ + 1, P = λ(J − α)·cof F +
+// μ(1 − 1/(Ic+δ))·F with cof F = [[F11, −F10], [−F01, F00]] (elast/stable_neohookean.hpp:23-62, 2D
+// partial_determinant elast/internal_physic.hpp:32-37)
+template <typename T>
+SSIM_HD T snh2_stress(const T F[4], T lam, T mu, T P[4]) {
+  const T del = snh2_delta(lam, mu);
+  const T Ic = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+  const T J = F[0] * F[3] - F[1] * F[2];
+  const T alpha = (T(1) - T(1) / (T(2) + del)) * mu / lam + T(1);
+  const T Jm = J - alpha;
+  const T C[4] = {F[3], -F[2], -F[1], F[0]};
+  const T d2 = mu * (T(1) - T(1) / (Ic + del));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) P[i] = lam * Jm * C[i] + d2 * F[i];
+  const T zero_energy = T(0.5) * (lam * (T(1) - alpha) * (T(1) - alpha) - mu * log(T(2) + del));
+  return T(0.5) * (mu * (Ic - T(2)) + lam * Jm * Jm - mu * log(Ic + del)) - zero_energy;
+}
+
+// H = μ(1 − 1/(Ic+δ))·I + 2μ/(Ic+δ)²·f fᵀ + λ·g gᵀ + λ(J − α)·H_J, f = vec F, g = vec cof F, H_J = ∂²J/∂F²
+// (entries (F00, F11) = 1, (F01, F10) = −1; add_HJ, elast/internal_physic.hpp:54-60)
+template <typename T>
+SSIM_HD void snh2_hessian(const T F[4], T lam, T mu, T H[16]) {
+  const T del = snh2_delta(lam, mu);
+  const T Ic = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+  const T J = F[0] * F[3] - F[1] * F[2];
+  const T alpha = (T(1) - T(1) / (T(2) + del)) * mu / lam + T(1);
+  const T C[4] = {F[3], -F[2], -F[1], F[0]};
+  const T d2 = mu * (T(1) - T(1) / (Ic + del)), c4 = mu * (T(2) / (Ic + del) / (Ic + del)), s3 = lam * (J - alpha);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) H[4 * i + j] = (i == j ? d2 : T(0)) + c4 * F[i] * F[j] + lam * C[i] * C[j];
+  H[0 * 4 + 3] += s3;
+  H[3 * 4 + 0] += s3;
+  H[1 * 4 + 2] -= s3;
+  H[2 * 4 + 1] -= s3;
+}
+
+template <typename T>
+SSIM_HD T linear2_stress(const T F[4], T lam, T mu, T P[4]) {
+  const T tr = F[0] + F[3] - T(2);
+  T n2 = 0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const T e = T(0.5) * (F[2 * a + c] + F[2 * c + a]) - (a == c ? T(1) : T(0));
+      n2 += e * e;
+      P[2 * a + c] = T(2) * mu * e + (a == c ? lam * tr : T(0));
+    }
+  return mu * n2 + T(0.5) * lam * tr * tr;
+}
+
+template <typename T>
+SSIM_HD void linear2_hessian(T lam, T mu, T H[16]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int a = i / 2, b = i % 2, c = j / 2, d = j % 2;
+      H[4 * i + j] = (i == j ? mu : T(0)) + (a == d && b == c ? mu : T(0)) + (a == b && c == d ? lam : T(0));
+    }
+}
+
 }  // namespace ssim_hip

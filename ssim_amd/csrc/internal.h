@@ -139,6 +139,8 @@ void launch_mask_dbc(int n, const unsigned char* dof, const double* in, double* 
 // nonlinear CG: out[0..5) = (dᵀy, gtᵀy, yᵀy, gtᵀd, dᵀd), y = gt − g (part holds vector-block partials of width 5)
 void launch_ncg_dots(int n, const double* gt, const double* g, const double* d, const double* a, const double* b,
                      double* part, double* out, hipStream_t s);
+void launch_material_eval(int dim, int model, long long n, const double* F, const double* lam, const double* mu,
+                          int make_spsd, double* psi, double* P, double* H, hipStream_t s);
 void launch_elem_pd(const MeshView& m, double c, const int* hperm, double* hcontrib, hipStream_t s);
 void launch_bsr_spmv(int nv, const int* brow, const int* bcol, const int* tpos, const double* val, const double* x,
                      double* y, bool trans, hipStream_t s);

@@ -247,6 +247,67 @@ void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double*
 }
 
 // ------------------------------------------------------------------------------------------------------
+// batched material evaluation (vmap over the models' stress / Hessian operators), one item per thread
+
+template <int DIM, int Model>
+__global__ __launch_bounds__(kBlock) void k_material_eval(long long n, const double* __restrict__ F,
+                                                          const double* __restrict__ lam, const double* __restrict__ mu,
+                                                          int make_spsd, double* __restrict__ psi,
+                                                          double* __restrict__ P, double* __restrict__ H) {
+  const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  constexpr int D2 = DIM * DIM;
+  double f[D2], p[D2];
+#pragma unroll
+  for (int k = 0; k < D2; ++k) f[k] = F[i * D2 + k];
+  const double l = lam[i], m = mu[i];
+  double e;
+  if constexpr (DIM == 2) {
+    e = Model == kLinearElastic ? linear2_stress(f, l, m, p) : snh2_stress(f, l, m, p);
+    if (H) {
+      double h[16];
+      if (Model == kLinearElastic)
+        linear2_hessian(l, m, h);
+      else
+        snh2_hessian(f, l, m, h);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) H[i * 16 + k] = h[k];
+    }
+  } else {
+    e = material_stress<Model>(f, l, m, p);
+    if (H) {
+      double h[45];
+      material_hessian<Model>(f, l, m, make_spsd, kSpdFloor, h);
+#pragma unroll
+      for (int a = 0; a < 9; ++a)
+#pragma unroll
+        for (int b = 0; b < 9; ++b) H[i * 81 + 9 * a + b] = h[sym9(a, b)];
+    }
+  }
+  if (psi) psi[i] = e;
+  if (P)
+#pragma unroll
+    for (int k = 0; k < D2; ++k) P[i * D2 + k] = p[k];
+}
+
+void launch_material_eval(int dim, int model, long long n, const double* F, const double* lam, const double* mu,
+                          int make_spsd, double* psi, double* P, double* H, hipStream_t s) {
+  if (n <= 0) return;
+  const int nb = (int)((n + kBlock - 1) / kBlock);
+  if (dim == 2) {
+    if (model == kLinearElastic)
+      k_material_eval<2, kLinearElastic><<<nb, kBlock, 0, s>>>(n, F, lam, mu, make_spsd, psi, P, H);
+    else
+      k_material_eval<2, kStableNeoHookean><<<nb, kBlock, 0, s>>>(n, F, lam, mu, make_spsd, psi, P, H);
+  } else {
+    if (model == kLinearElastic)
+      k_material_eval<3, kLinearElastic><<<nb, kBlock, 0, s>>>(n, F, lam, mu, make_spsd, psi, P, H);
+    else
+      k_material_eval<3, kStableNeoHookean><<<nb, kBlock, 0, s>>>(n, F, lam, mu, make_spsd, psi, P, H);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------------
 // adjoint of one step (element part) and the Dirichlet mask of its right-hand side
 
 template <int Model>
