@@ -48,11 +48,12 @@ def spmv_algorithmic_bytes(info):
     """Bytes one launch of the PCG SpMV kernel must move, for the format it reads (symmetric BSR3: only blocks
     with row <= column are stored).  76 B per stored block (72 B of values + a 4 B column index), 8 B per
     lower-block reference (column + value offset), the gathered input z once (n·8 B), and the lane's own rows
-    of p and q read and written (4·n·8 B).  Padding slots are not counted."""
+    of p, q and x read and written (6·n·8 B; the kernel applies the previous iteration's x += α·p).  Padding
+    slots are not counted."""
     n = 3 * info["nv"]
     upper = info["upper_blocks"]
     lower = info["nonzero_blocks"] - upper
-    return 76 * upper + 8 * lower + 5 * n * 8
+    return 76 * upper + 8 * lower + 7 * n * 8
 
 
 def load_traffic():
@@ -296,7 +297,7 @@ def main():
         "unit": "GB/s",
         "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
         "traffic": traffic,
-        "kernel": "k_pcg_spmv (symmetric blocked-ELL 3x3 SpMV, p/q update, p·q partials)",
+        "kernel": "k_pcg_spmv (symmetric blocked-ELL 3x3 SpMV, x/p/q update, p·q partials)",
         "bytes_per_launch": bytes_per_launch,
         "avg_launch_ms": spmv_ms,
         "sampled_launches": spmv_n,

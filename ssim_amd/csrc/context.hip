@@ -478,8 +478,8 @@ void enqueue_pcg_iterations(ssim_ctx* c, const BellView& A, const double* dinv, 
                             hipStream_t s) {
   for (int j = 0; j < count; ++j) {
     launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
-                    c->part_pa(), s);
-    launch_pcg_update(c->n, dinv, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(0),
+                    c->part_pa(), x, s);
+    launch_pcg_update(c->n, dinv, c->r.p, c->z.p, c->q.p, c->part_pa(), na, c->part_pb(0),
                       c->part_pb(1), c->pcg_it.p, c->pcg.p, s);
   }
 }
@@ -530,10 +530,10 @@ int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol,
   auto enqueue = [&](int count, int slot) {
     timed_ext(c, SSIM_KERNEL_PCG_SPMV, [&](hipEvent_t e0, hipEvent_t e1) {
       launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
-                      c->part_pa(), s, e0, e1);
+                      c->part_pa(), x, s, e0, e1);
     });
     timed_ext(c, SSIM_KERNEL_PCG_UPDATE, [&](hipEvent_t e0, hipEvent_t e1) {
-      launch_pcg_update(c->n, dinv, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), na, c->part_pb(0),
+      launch_pcg_update(c->n, dinv, c->r.p, c->z.p, c->q.p, c->part_pa(), na, c->part_pb(0),
                         c->part_pb(1), c->pcg_it.p, c->pcg.p, s, e0, e1);
     });
     if (count > 1) {

@@ -120,9 +120,9 @@ void launch_axpy_to(int n, const double* x, double alpha, const double* d, doubl
 int launch_pcg_init(int n, const double* b, double sign, const double* dinv, double* x, double* r, double* z,
                     double rtol, int maxit, PcgScalars* sc, PcgIter* it, double* pb0, hipStream_t s);
 int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb0, const double* pb1,
-                    int nb, PcgIter* it, PcgScalars* sc, double* pa, hipStream_t s,
+                    int nb, PcgIter* it, PcgScalars* sc, double* pa, double* x, hipStream_t s,
                     hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
+void launch_pcg_update(int n, const double* dinv, double* r, double* z, const double* q,
                        const double* pa, int na, double* pb0, double* pb1, PcgIter* it, PcgScalars* sc,
                        hipStream_t s,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);

@@ -530,9 +530,10 @@ int launch_pcg_init(int n, const double* b, double sign, const double* dinv, dou
 }
 
 // Iteration k, kernel A: s = A z_k, then (after the loop, so the partial-sum latency overlaps other
-// workgroups' SpMV work) the convergence test on ‖r_k‖² from the previous update's partials, β = ρ_k/ρ_{k−1},
-// p_k = z_k + β p_{k−1} and q_k = A p_k = s + β q_{k−1} for the lane's own row (in place), and the partials of
-// p_k·q_k.  For k = 0: p_0 = z_0, q_0 = s.  k is read from the device counter (kernel A reads ka and publishes
+// workgroups' SpMV work) the convergence test on ‖r_k‖² from the previous update's partials, x_k = x_{k−1} +
+// α_{k−1}·p_{k−1} (α published by the previous kernel B; applied here, where p_{k−1} is read anyway, also on the
+// iteration that stops), β = ρ_k/ρ_{k−1}, p_k = z_k + β p_{k−1} and q_k = A p_k = s + β q_{k−1} for the lane's
+// own row (in place), and the partials of p_k·q_k.  For k = 0: p_0 = z_0, q_0 = s.  k is read from the device counter (kernel A reads ka and publishes
 // kb = ka, kernel B reads kb and publishes ka = kb + 1), so every iteration's launch is identical and a chunk of
 // iterations can be replayed from a captured graph; the (r·z, r·r) partials of residual k live in pb[k % 2], and
 // ρ_{k−1} is read from the slot the previous launch wrote (the same sum, so only one partial array is summed).
@@ -542,7 +543,7 @@ template <int LPR>
 __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_pcg_spmv(
     BellView A, const double* __restrict__ z, double* __restrict__ p, double* __restrict__ q,
     const double* __restrict__ pb0, const double* __restrict__ pb1, int nb, PcgIter* it, PcgScalars* sc,
-    double* __restrict__ pa) {
+    double* __restrict__ pa, double* __restrict__ x) {
   const int k = it->ka;
   if (blockIdx.x == 0 && threadIdx.x == 0) it->kb = k;
   if (sc->done) return;
@@ -569,6 +570,16 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
       if (nan) sc->breakdown = 2;
       if (done) sc->done = 1;  // only ever 0 → 1: a workgroup that reads it early skips work that is not needed
     }
+    // x_k = x_{k−1} + α_{k−1}·p_{k−1} for the own row (kernel B published α_{k−1})
+    if (active) {
+      const double al = sc->alpha;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        if (LPR <= 1 || a == j) {
+          const size_t i = 3 * (size_t)v + a;
+          x[i] += al * p[i];
+        }
+    }
     if (done) return;
     beta = t_cur[0] / rz_prev;
   }
@@ -592,11 +603,12 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
   store_partials<1, kSpmvBlock>(acc, pa);
 }
 
-// Iteration k, kernel B: α = ρ_k/(p_k·q); x += α p_k, r −= α q, z = D⁻¹ r; partials of (r·z, r·r).
+// Iteration k, kernel B: α = ρ_k/(p_k·q) with ρ_k from the slot kernel A wrote; r −= α q, z = D⁻¹ r; partials
+// of (r·z, r·r); α is published for the next kernel A, which applies x += α p_k.
 // The streaming loads are issued before the partial sums so that their latency overlaps.
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __restrict__ dinv, double* __restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __restrict__ dinv,
                                                        double* __restrict__ r, double* __restrict__ z,
-                                                       const double* __restrict__ p, const double* __restrict__ q,
+                                                       const double* __restrict__ q,
                                                        const double* __restrict__ pa, int na, double* __restrict__ pb0,
                                                        double* __restrict__ pb1, int nb, PcgIter* it,
                                                        PcgScalars* sc) {
@@ -605,21 +617,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
   if (sc->done) return;
   const double* __restrict__ pb_cur = (k & 1) ? pb1 : pb0;
   double* __restrict__ pb_next = (k & 1) ? pb0 : pb1;
-  double xv[kVec], rv[kVec], pv[kVec], qv[kVec], dv[kVec];
+  double rv[kVec], qv[kVec], dv[kVec];
 #pragma unroll
   for (int j = 0; j < kVec; ++j) {
     const int i = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
     if (i < n) {
-      xv[j] = x[i];
       rv[j] = r[i];
-      pv[j] = p[i];
       qv[j] = q[i];
       dv[j] = dinv[i];
     }
   }
-  double pq[1], rz[2];
+  double pq[1], rz[1];
   sum_partials<1>(pa, na, pq);
-  sum_partials<2>(pb_cur, nb, rz);
+  rz[0] = sc->rz_hist[k & 1];
   if (!(pq[0] > 0.0)) {  // not positive definite (or NaN): every workgroup sees the same pq
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->breakdown = 1;
@@ -628,12 +638,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
     return;
   }
   const double alpha = rz[0] / pq[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha = alpha;
   double v[2] = {0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < kVec; ++j) {
     const int i = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
     if (i < n) {
-      x[i] = xv[j] + alpha * pv[j];
       const double ri = rv[j] - alpha * qv[j];
       const double zi = dv[j] * ri;
       r[i] = ri;
@@ -656,28 +666,29 @@ void launch_timed(K kernel, int grid, int block, hipStream_t s, hipEvent_t e0, h
 }
 
 int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb0, const double* pb1,
-                    int nb, PcgIter* it, PcgScalars* sc, double* pa, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                    int nb, PcgIter* it, PcgScalars* sc, double* pa, double* x, hipStream_t s, hipEvent_t e0,
+                    hipEvent_t e1) {
   const int rows = A.nrows > 0 ? A.nrows : 1;
   if (A.lanes_per_row == 4) {
     const int grid = grid_for(RowMap<4>::blocks(rows), A.xcd_remap);
-    launch_timed(k_pcg_spmv<4>, grid, kSpmvBlock, s, e0, e1, A, z, p, q, pb0, pb1, nb, it, sc, pa);
+    launch_timed(k_pcg_spmv<4>, grid, kSpmvBlock, s, e0, e1, A, z, p, q, pb0, pb1, nb, it, sc, pa, x);
     return grid;
   }
   if (A.lanes_per_row == 0) {
     const int grid = grid_for(RowMap<0>::blocks(rows), A.xcd_remap);
-    launch_timed(k_pcg_spmv<0>, grid, kSpmvBlock, s, e0, e1, A, z, p, q, pb0, pb1, nb, it, sc, pa);
+    launch_timed(k_pcg_spmv<0>, grid, kSpmvBlock, s, e0, e1, A, z, p, q, pb0, pb1, nb, it, sc, pa, x);
     return grid;
   }
   const int grid = grid_for(RowMap<1>::blocks(rows), A.xcd_remap);
-  launch_timed(k_pcg_spmv<1>, grid, kSpmvBlock, s, e0, e1, A, z, p, q, pb0, pb1, nb, it, sc, pa);
+  launch_timed(k_pcg_spmv<1>, grid, kSpmvBlock, s, e0, e1, A, z, p, q, pb0, pb1, nb, it, sc, pa, x);
   return grid;
 }
 
-void launch_pcg_update(int n, const double* dinv, double* x, double* r, double* z, const double* p, const double* q,
+void launch_pcg_update(int n, const double* dinv, double* r, double* z, const double* q,
                        const double* pa, int na, double* pb0, double* pb1, PcgIter* it, PcgScalars* sc,
                        hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const int nb = vblocks(n);
-  launch_timed(k_pcg_update, nb, kBlock, s, e0, e1, n, dinv, x, r, z, p, q, pa, na, pb0, pb1, nb, it, sc);
+  launch_timed(k_pcg_update, nb, kBlock, s, e0, e1, n, dinv, r, z, q, pa, na, pb0, pb1, nb, it, sc);
 }
 
 int pcg_vector_blocks(int n) { return vblocks(n); }
