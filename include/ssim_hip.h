@@ -255,8 +255,11 @@ typedef enum {
   SSIM_TUNE_PCG_GRAPH = 3,     /* 1: replay PCG chunks from captured hipGraphs; 0 (default): launch each kernel
                                   (measured the same speed for one 786k-tet mesh and for 256 batched meshes: the gap
                                   between dependent kernels is not the host's launch cost) */
-  SSIM_TUNE_COMPACT_PCT = 4    /* batches: compact the PCG grids to the meshes still iterating once those hold at
+  SSIM_TUNE_COMPACT_PCT = 4,   /* batches: compact the PCG grids to the meshes still iterating once those hold at
                                   most this percentage of the current grid (default 70; 0 disables) */
+  SSIM_TUNE_LBFGS_GRAM = 5     /* L-BFGS with the diagonal H0 and m ≤ 8: 1 (default) runs the two-loop recursion on
+                                  Gram matrices of the stored pairs (one fused dot-product pass and one fused
+                                  direction pass per iteration), 0 runs it vector by vector */
 } ssim_tune_t;
 ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value);
 ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value);

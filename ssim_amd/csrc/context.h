@@ -123,8 +123,9 @@ struct ssim_ctx {
   std::vector<PcgGraph> pcg_graphs;
   hipStream_t cap_stream = nullptr;  // private stream the chunks are captured on
   int pcg_graph = 0;                 // replay PCG chunks from captured graphs
+  int lbfgs_gram = 1;                // two-loop recursion on Gram matrices (diagonal H0, m ≤ kLbfgsSlots)
   ssim_hip::DBuf<ssim_hip::EvalScalars> ev;
-  ssim_hip::DBuf<double> partials, lb_S, lb_Y, lb_scal;
+  ssim_hip::DBuf<double> partials, lb_S, lb_Y, lb_scal, lb_part;
   size_t pmax = 0;  // partials per slot
   // partial-sum slots: pcg p·q, pcg (r·z, r·r) ping-pong, energy, ‖g‖², dot
   double* part_pa() { return partials.p; }

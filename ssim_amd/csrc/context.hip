@@ -639,6 +639,167 @@ void build_pd(ssim_ctx* c) {
   sync(c);
 }
 
+// H0 of the L-BFGS stepper: diagonal of the SPD-projected, Dirichlet-filtered Hessian at rest (u = 0)
+void ensure_hrest(ssim_ctx* c) {
+  if (c->hrest.p) return;
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  c->hrest.alloc(n);
+  HIPCHK(hipMemsetAsync(c->tmp.p, 0, sizeof(double) * n, s));
+  const double dt2 = c->dt * c->dt;
+  launch_elem_hess(c->mesh(), c->tmp.p, dt2, 1, c->hperm.p, c->hcontrib.p, c->prm.fp32_elements != 0, s);
+  launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->val.p, c->dinv.p,
+                     c->hrest.p, s);
+  HIPCHK(hipGetLastError());
+}
+
+// The same L-BFGS iteration with the two-loop recursion carried out on Gram matrices of the stored pairs (the
+// vector-free formulation of Chen, Wang, Zhou & Luo, NIPS 2014, with a diagonal H0 = D): with SY_ij = s_iᵀy_j,
+// YDY_ij = y_iᵀDy_j, SG_i = s_iᵀg, YDG_i = y_iᵀDg,
+//   a_i = ρ_i (SG_i − Σ_{j newer} a_j SY_ij)                                  (newest to oldest)
+//   b_i = ρ_i (YDG_i − Σ_j a_j YDY_ij + Σ_{j older} c_j SY_ji),  c_j = a_j − b_j   (oldest to newest)
+//   r = D(g − Σ_j a_j y_j) + Σ_j c_j s_j,    gᵀr = gᵀDg − Σ_j a_j YDG_j + Σ_j c_j SG_j,
+// which is the recursion of lbfgs_solve term by term.  Per iteration one pass over the stored vectors computes
+// the new Gram entries (s_jᵀg, y_jᵀDg, gᵀDg and the newest pair's row and column), the host runs the scalar
+// recursion, and one pass forms r.  The 2·kLbfgsSlots history vectors are zeroed once, so unused slots hold finite
+// values.  If a line search fails, the rest of the solve computes the directions vector by vector on the same
+// history (the recursion of lbfgs_solve).
+void lbfgs_solve_gram(ssim_ctx* c) {
+  const int n = c->n, m = std::max(1, c->prm.lbfgs_m);
+  constexpr int M = kLbfgsSlots;
+  hipStream_t s = c->stream;
+  if (c->lb_S.count < (size_t)M * n || c->lb_Y.count < (size_t)M * n) {
+    c->lb_S.alloc((size_t)M * n);
+    c->lb_Y.alloc((size_t)M * n);
+    HIPCHK(hipMemsetAsync(c->lb_S.p, 0, sizeof(double) * M * (size_t)n, s));
+    HIPCHK(hipMemsetAsync(c->lb_Y.p, 0, sizeof(double) * M * (size_t)n, s));
+  }
+  c->lb_part.alloc((size_t)kLbfgsDots * lbfgs_gram_blocks(n));
+  c->lb_scal.alloc(std::max<size_t>(c->lb_scal.count, kLbfgsDots + 2 * M + 8));
+  double* av = c->lb_scal.p + kLbfgsDots;  // device two-loop coefficients of the vector-by-vector recursion
+  double* bv = av + M;
+  bool vector_mode = false;
+  std::vector<double> SY((size_t)M * M, 0.0), YDY((size_t)M * M, 0.0), rho(M, 0.0), gram(kLbfgsDots);
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* g = c->a(SSIM_ARRAY_FORCES);
+  double* d = c->dir.p;  // r ≈ H⁻¹g; the search direction is −r
+  EvalOut cur = eval_energy_grad(c, u, g);
+  int it = 0, hist = 0, head = 0, fresh = -1;  // fresh: slot of the pair written by the last line search
+  auto accept = [&](double ys) {
+    if (!(ys > 1e-300)) return false;  // as lbfgs_solve: a pair with yᵀs ≤ 0 is dropped
+    rho[fresh] = 1.0 / ys;
+    head = (head + 1) % m;
+    if (hist < m) hist++;
+    return true;
+  };
+  for (; it < c->prm.lbfgs_maxit; ++it) {
+    if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+    if (std::sqrt(cur.gg) < c->tol_abs) {
+      c->stats.converged = 1;
+      break;
+    }
+    double gd = 0.0;
+    if (vector_mode) {
+      if (fresh >= 0)
+        accept(device_dot(c, c->lb_Y.p + (size_t)fresh * n, c->lb_S.p + (size_t)fresh * n));
+      fresh = -1;
+      HIPCHK(hipMemcpyAsync(c->q.p, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      for (int k = 0; k < hist; ++k) {
+        const int idx = (head - 1 - k + m) % m;
+        launch_dot(n, c->lb_S.p + (size_t)idx * n, c->q.p, c->part_d(), av + idx, s);
+        launch_axpy_coef(n, -rho[idx], av + idx, 0.0, nullptr, c->lb_Y.p + (size_t)idx * n, c->q.p, s);
+      }
+      launch_scale_div(n, c->q.p, c->hrest.p, d, s);
+      for (int k = hist - 1; k >= 0; --k) {
+        const int idx = (head - 1 - k + m) % m;
+        launch_dot(n, c->lb_Y.p + (size_t)idx * n, d, c->part_d(), bv + idx, s);
+        launch_axpy_coef(n, rho[idx], av + idx, rho[idx], bv + idx, c->lb_S.p + (size_t)idx * n, d, s);
+      }
+      HIPCHK(hipGetLastError());
+      gd = -device_dot(c, g, d);
+    } else {
+      launch_lbfgs_gram(n, c->lb_S.p, c->lb_Y.p, g, c->hrest.p, fresh, c->lb_part.p, c->lb_scal.p, s);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(gram.data(), c->lb_scal.p, sizeof(double) * kLbfgsDots, hipMemcpyDeviceToHost, s));
+      sync(c);
+      const double* SG = gram.data();
+      const double* YDG = SG + M;
+      const double GDG = SG[2 * M];
+      if (fresh >= 0) {
+        const double* SYk = SG + 2 * M + 1;  // s_jᵀy_k
+        const double* SkY = SG + 3 * M + 1;  // s_kᵀy_j
+        const double* YDYk = SG + 4 * M + 1;
+        for (int j = 0; j < M; ++j) {  // row and column of the slot; used only if the pair is kept
+          SY[(size_t)j * M + fresh] = SYk[j];
+          SY[(size_t)fresh * M + j] = SkY[j];
+          YDY[(size_t)fresh * M + j] = YDY[(size_t)j * M + fresh] = YDYk[j];
+        }
+        accept(SkY[fresh]);
+        fresh = -1;
+      }
+      // scalar two-loop recursion over the history, slot of the k-th newest pair: (head − 1 − k) mod m
+      LbfgsCoef cf{};
+      double a[M] = {}, b[M] = {};
+      for (int k = 0; k < hist; ++k) {
+        const int i = (head - 1 - k + m) % m;
+        double t = SG[i];
+        for (int q = 0; q < k; ++q) {
+          const int j = (head - 1 - q + m) % m;
+          t -= a[j] * SY[(size_t)i * M + j];
+        }
+        a[i] = rho[i] * t;
+      }
+      for (int k = hist - 1; k >= 0; --k) {
+        const int i = (head - 1 - k + m) % m;
+        double t = YDG[i];
+        for (int q = 0; q < hist; ++q) {
+          const int j = (head - 1 - q + m) % m;
+          t -= a[j] * YDY[(size_t)i * M + j];
+        }
+        for (int q = k + 1; q < hist; ++q) {  // older pairs, already updated
+          const int j = (head - 1 - q + m) % m;
+          t += (a[j] - b[j]) * SY[(size_t)j * M + i];
+        }
+        b[i] = rho[i] * t;
+      }
+      double gr = GDG;
+      for (int k = 0; k < hist; ++k) {
+        const int i = (head - 1 - k + m) % m;
+        cf.a[i] = a[i];
+        cf.c[i] = a[i] - b[i];
+        cf.mask |= 1u << i;
+        gr += -a[i] * YDG[i] + (a[i] - b[i]) * SG[i];
+      }
+      launch_lbfgs_dir(n, c->lb_S.p, c->lb_Y.p, g, c->hrest.p, cf, d, s);
+      HIPCHK(hipGetLastError());
+      gd = -gr;
+    }
+    double alpha = 1.0;
+    EvalOut trial{0, 0};
+    bool ok = false;
+    for (int ls = 0; ls < c->prm.lbfgs_ls_max; ++ls) {
+      launch_axpy_to(n, u, -alpha, d, c->utry.p, s);
+      trial = eval_energy_grad(c, c->utry.p, c->gtry.p);
+      if (trial.E <= cur.E + c->prm.lbfgs_c * alpha * gd) {
+        ok = true;
+        break;
+      }
+      alpha *= c->prm.ls_beta;
+    }
+    if (!ok) {
+      c->stats.ls_failures++;
+      vector_mode = true;
+    }
+    launch_sub(n, c->utry.p, u, c->lb_S.p + (size_t)head * n, s);
+    launch_sub(n, c->gtry.p, g, c->lb_Y.p + (size_t)head * n, s);
+    fresh = head;
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    cur = trial;
+  }
+  c->stats.newton_iterations = it;
+}
+
 // L-BFGS two-loop recursion (Nocedal & Wright, Alg. 7.4) with H0 = diag(H_rest)⁻¹ and backtracking Armijo;
 // stepper/backward_euler.hpp:48-60 (tol = threshold·mass).  The two-loop coefficients stay on the device;
 // ρ_i = 1/(y_iᵀs_i) is read back once per accepted pair.  With `pd` the middle step solves with the
@@ -658,15 +819,10 @@ void lbfgs_solve(ssim_ctx* c, bool pd, bool h0 = false) {
   }
   const bool solve = pd || h0;
   double* qv = solve ? c->pd_q.p : c->q.p;  // the solve's PCG uses c->q itself
-  if (!solve && !c->hrest.p) {
-    // H0: diagonal of the SPD-projected, Dirichlet-filtered Hessian at rest (u = 0)
-    c->hrest.alloc(n);
-    HIPCHK(hipMemsetAsync(c->tmp.p, 0, sizeof(double) * n, s));
-    const double dt2 = c->dt * c->dt;
-    launch_elem_hess(c->mesh(), c->tmp.p, dt2, 1, c->hperm.p, c->hcontrib.p, c->prm.fp32_elements != 0, s);
-    launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->val.p, c->dinv.p,
-                       c->hrest.p, s);
-    HIPCHK(hipGetLastError());
+  if (!solve) ensure_hrest(c);
+  if (!solve && c->lbfgs_gram && m <= kLbfgsSlots) {
+    lbfgs_solve_gram(c);
+    return;
   }
   c->lb_S.alloc((size_t)m * n);
   c->lb_Y.alloc((size_t)m * n);
@@ -1566,6 +1722,9 @@ ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value) {
       case SSIM_TUNE_PCG_GRAPH:
         ctx->pcg_graph = value != 0;
         break;
+      case SSIM_TUNE_LBFGS_GRAM:
+        ctx->lbfgs_gram = value != 0;
+        break;
       default:
         fail(SSIM_ERR_INVALID_ARGUMENT, "unknown tuning key");
     }
@@ -1589,6 +1748,9 @@ ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value) 
       return SSIM_OK;
     case SSIM_TUNE_PCG_GRAPH:
       *value = ctx->pcg_graph;
+      return SSIM_OK;
+    case SSIM_TUNE_LBFGS_GRAM:
+      *value = ctx->lbfgs_gram;
       return SSIM_OK;
     default:
       g_err = "unknown tuning key";

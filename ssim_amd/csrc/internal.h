@@ -141,6 +141,18 @@ void launch_ncg_dots(int n, const double* gt, const double* g, const double* d, 
                      double* part, double* out, hipStream_t s);
 void launch_material_eval(int dim, int model, long long n, const double* F, const double* lam, const double* mu,
                           int make_spsd, double* psi, double* P, double* H, hipStream_t s);
+// L-BFGS in Gram form: at most kLbfgsSlots pairs; kLbfgsDots dot products per iteration (see kernels.hip)
+constexpr int kLbfgsSlots = 8;
+constexpr int kLbfgsDots = 6 * kLbfgsSlots + 1;
+struct LbfgsCoef {
+  double a[kLbfgsSlots], c[kLbfgsSlots];
+  unsigned mask;
+};
+void launch_lbfgs_gram(int n, const double* S, const double* Y, const double* g, const double* h, int k, double* part,
+                       double* out, hipStream_t s);
+int lbfgs_gram_blocks(int n);
+void launch_lbfgs_dir(int n, const double* S, const double* Y, const double* g, const double* h, const LbfgsCoef& cf,
+                      double* r, hipStream_t s);
 void launch_elem_pd(const MeshView& m, double c, const int* hperm, double* hcontrib, hipStream_t s);
 void launch_bsr_spmv(int nv, const int* brow, const int* bcol, const int* tpos, const double* val, const double* x,
                      double* y, bool trans, hipStream_t s);

@@ -790,6 +790,81 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
 }
 
 // ------------------------------------------------------------------------------------------------------
+// L-BFGS in Gram form (H0 = D = diag(h)⁻¹, history of at most kLbfgsSlots pairs in slots 0..m−1)
+
+// Partials of the 6m + 1 dot products one iteration needs (m = kLbfgsSlots, every slot, used or not):
+//   [0, m)   s_jᵀg        [m, 2m)  y_jᵀDg       [2m]  gᵀDg
+//   [2m+1+j] s_jᵀy_k      [3m+1+j] s_kᵀy_j      [4m+1+j] y_kᵀDy_j      (k = the newest slot; zero when k < 0)
+__global__ __launch_bounds__(kBlock) void k_lbfgs_gram(int n, const double* __restrict__ S, const double* __restrict__ Y,
+                                                       const double* __restrict__ g, const double* __restrict__ h,
+                                                       int k, double* __restrict__ part) {
+  constexpr int M = kLbfgsSlots;
+  double v[kLbfgsDots];
+#pragma unroll
+  for (int j = 0; j < kLbfgsDots; ++j) v[j] = 0.0;
+#pragma unroll 1
+  for (int jj = 0; jj < kVec; ++jj) {
+    const int i = (blockIdx.x * kVec + jj) * kBlock + threadIdx.x;
+    if (i >= n) break;
+    const double gi = g[i], di = 1.0 / h[i];
+    const double dg = gi * di;
+    const double sk = k >= 0 ? S[(size_t)k * n + i] : 0.0, yk = k >= 0 ? Y[(size_t)k * n + i] : 0.0;
+    const double dyk = yk * di;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const double sj = S[(size_t)j * n + i], yj = Y[(size_t)j * n + i];
+      v[j] += sj * gi;
+      v[M + j] += yj * dg;
+      v[2 * M + 1 + j] += sj * yk;
+      v[3 * M + 1 + j] += sk * yj;
+      v[4 * M + 1 + j] += dyk * yj;
+    }
+    v[2 * M] += gi * dg;
+  }
+  store_partials<kLbfgsDots>(v, part);
+}
+
+__global__ __launch_bounds__(kBlock) void k_lbfgs_gram_fin(const double* __restrict__ part, int np,
+                                                           double* __restrict__ out) {
+  double t[kLbfgsDots];
+  sum_partials<kLbfgsDots, kBlock, 1>(part, np, t);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < kLbfgsDots; ++j) out[j] = t[j];
+}
+
+void launch_lbfgs_gram(int n, const double* S, const double* Y, const double* g, const double* h, int k, double* part,
+                       double* out, hipStream_t s) {
+  const int nb = vblocks(n);
+  k_lbfgs_gram<<<nb, kBlock, 0, s>>>(n, S, Y, g, h, k, part);
+  k_lbfgs_gram_fin<<<1, kBlock, 0, s>>>(part, nb, out);
+}
+
+int lbfgs_gram_blocks(int n) { return vblocks(n); }
+
+// r = D·(g − Σ_j a_j y_j) + Σ_j c_j s_j over the slots in `mask`
+__global__ __launch_bounds__(kBlock) void k_lbfgs_dir(int n, const double* __restrict__ S, const double* __restrict__ Y,
+                                                      const double* __restrict__ g, const double* __restrict__ h,
+                                                      LbfgsCoef cf, double* __restrict__ r) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double q = g[i], t = 0.0;
+#pragma unroll
+  for (int j = 0; j < kLbfgsSlots; ++j)
+    if (cf.mask & (1u << j)) {
+      q -= cf.a[j] * Y[(size_t)j * n + i];
+      t += cf.c[j] * S[(size_t)j * n + i];
+    }
+  r[i] = q / h[i] + t;
+}
+
+void launch_lbfgs_dir(int n, const double* S, const double* Y, const double* g, const double* h, const LbfgsCoef& cf,
+                      double* r, hipStream_t s) {
+  if (n <= 0) return;
+  k_lbfgs_dir<<<nblocks(n), kBlock, 0, s>>>(n, S, Y, g, h, cf, r);
+}
+
+// ------------------------------------------------------------------------------------------------------
 // L-BFGS helpers
 
 __global__ void k_scale_div(int n, const double* __restrict__ q, const double* __restrict__ diag,

@@ -279,3 +279,22 @@ def test_irregular_mesh_spmv_variants_match_oracle(lanes):
         _, st = o2.step()
         assert g2.last_stats()["newton_iterations"] == st["newton"], step
         assert rel_err(_positions(case, g2.deformation().T.reshape(-1)), _positions(case, o2.get_state()[0])) <= 1e-8
+
+
+def test_lbfgs_gram_and_vector_recursions_agree():
+    """The two-loop recursion on Gram matrices (the default) and vector by vector are the same arithmetic up to
+    the order of the sums: at threshold 1e-8 they take the same iterations and agree within 1e-9 (positions).
+    (Near the limit of double precision, threshold 1e-10 on this cube, both iterate erratically.)"""
+    case = mesh.twisted_cube(n=4, seed=3)
+    runs = []
+    for gram in (1, 0):
+        g = fem.TetFiniteElementSolver_Hip.from_case(case, method="lbfgs", rtol=1e-8)
+        g.set_tuning(lbfgs_gram=gram)
+        its = []
+        for _ in range(3):
+            g.step()
+            its.append(g.last_stats()["newton_iterations"])
+        runs.append((_positions(case, g.deformation().T.reshape(-1)), its))
+    assert rel_err(runs[0][0], runs[1][0]) <= 1e-9
+    assert all(abs(a - b) <= 2 for a, b in zip(runs[0][1], runs[1][1])), (runs[0][1], runs[1][1])
+    print(f"[gpu] L-BFGS iterations per step, Gram {runs[0][1]} vs vector {runs[1][1]}", flush=True)
