@@ -101,7 +101,90 @@ def cpu_sample(case, counts, threads, rtol):
     }
 
 
-def bench_batched(rtol, steps=3, seed=0):
+def _oracle(threads):
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    return oracle
+
+
+def cpu_batched(cases, rtol, threads, sample=16):
+    """The oracle's Newton step on the first `sample` meshes of the batch, scaled to the batch by tet count."""
+    oracle = _oracle(threads)
+    t, tets = 0.0, 0
+    for c in cases[:sample]:
+        ts = oracle.TimeStep.from_case(c)
+        ts.set_threshold(rtol)
+        t0 = time.perf_counter()
+        ts.step()
+        t += time.perf_counter() - t0
+        tets += c.nt
+    total = sum(c.nt for c in cases)
+    per_step = t * total / tets
+    return {"value": 1.0 / per_step, "unit": "batch steps/s", "cores": oracle.lib().orc_num_threads(), "kind": "port",
+            "sample": f"oracle Newton step of the first {sample} meshes ({tets} tets, {t:.2f} s), scaled by tet count "
+                      f"to the {total} tets of the batch"}
+
+
+def cpu_lbfgs(case, rtol, threads, its_per_step, sample_its=20):
+    """The oracle's L-BFGS on the same cube for `sample_its` iterations; composed with the GPU's iterations per
+    step."""
+    oracle = _oracle(threads)
+    ts = oracle.TimeStep.from_case(case)
+    ts.set_threshold(rtol)
+    ts.set_lbfgs(max_iter=1)  # builds the rest-Hessian H0 once, outside the timed sample
+    ts.prepare_step()
+    ts.compute_step("lbfgs")
+    ts.set_lbfgs(max_iter=sample_its)
+    ts.prepare_step()
+    t0 = time.perf_counter()
+    _, st = ts.compute_step("lbfgs")
+    t = time.perf_counter() - t0
+    per_it = t / max(st["newton"], 1)
+    return {"value": 1.0 / (per_it * its_per_step), "unit": "steps/s", "cores": oracle.lib().orc_num_threads(),
+            "kind": "port",
+            "sample": f"oracle L-BFGS, {st['newton']} iterations on the same cube ({per_it * 1e3:.1f} ms each), "
+                      f"composed with the GPU's {its_per_step:.0f} iterations per step"}
+
+
+def cpu_adjoint(rc, rtol, threads, adj_pcg_per_step, steps=2):
+    """CPU time of the rollout and its adjoint, composed from the oracle on one bar: `steps` measured Newton steps
+    (forward), and per step one unprojected Hessian assembly plus the GPU adjoint's PCG iterations per step at the
+    oracle's time per PCG iteration (backward); scaled to the rollout's bars and steps."""
+    oracle = _oracle(threads)
+    from ssim_amd import adjoint
+
+    case = rc.cases[0]
+    lam, mu = adjoint.lame_from_log_e(rc.log_e[:case.nt], rc.nu[:case.nt])
+    ts = oracle.TimeStep(case.X, case.cells, case.dt, lam, mu, case.rho)
+    ts.set_dirichlet(case.dof_type, case.dbc)
+    ts.set_ext_accel(case.accel)
+    ts.set_state(case.u0, None, case.v0)
+    ts.set_threshold(rtol)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts.step()
+    t_fwd = (time.perf_counter() - t0) / steps
+    u = ts.get_state()[0]
+    t0 = time.perf_counter()
+    vals = ts.hessian(u, False, True)
+    t_hess = time.perf_counter() - t0
+    rowptr, colidx = ts.csr_pattern()
+    _, g = ts.energy_grad(u)
+    t0 = time.perf_counter()
+    _, it, _ = oracle.pcg(rowptr, colidx, vals, g, rtol=0.0, maxit=20)
+    t_pcg = (time.perf_counter() - t0) / max(it, 1)
+    t_bwd = t_hess + adj_pcg_per_step * t_pcg
+    total = (t_fwd + t_bwd) * len(rc.cases) * rc.steps
+    return {"value": total, "unit": "s per rollout (forward + adjoint)", "cores": oracle.lib().orc_num_threads(),
+            "kind": "port",
+            "sample": f"oracle on one bar: Newton step {t_fwd:.2f} s (measured over {steps}), adjoint step = Hessian "
+                      f"{t_hess:.2f} s + {adj_pcg_per_step:.0f} PCG iterations (the GPU adjoint's per step) x "
+                      f"{t_pcg * 1e3:.1f} ms; scaled to {len(rc.cases)} bars x {rc.steps} steps"}
+
+
+def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
     """Batched vs sequential per-mesh GPU stepping of 256 uneven bars; per-mesh agreement after one step."""
     import torch
 
@@ -154,10 +237,11 @@ def bench_batched(rtol, steps=3, seed=0):
         "setup_s": setup_b,
         "per_mesh_max_position_rel_err": worst,
         "newton_count_mismatches": mism,
+        "cpu_baseline": cpu_batched(cases, rtol, cpu_threads) if cpu_threads else None,
     }
 
 
-def bench_lbfgs(rtol, n=64, steps=3, seed=0):
+def bench_lbfgs(rtol, n=64, steps=3, seed=0, cpu_threads=None):
     """L-BFGS steps/s on the twisted cube of n³ cubes (per-tet E log-uniform, ν uniform, z = 0 face clamped)."""
     import torch
 
@@ -186,10 +270,11 @@ def bench_lbfgs(rtol, n=64, steps=3, seed=0):
         "timed_steps": steps,
         "lbfgs_iterations": its,
         "energy_evals": evals,
+        "cpu_baseline": cpu_lbfgs(case, rtol, cpu_threads, its / steps) if cpu_threads else None,
     }
 
 
-def bench_adjoint(rtol, nbar=32, seed=0):
+def bench_adjoint(rtol, nbar=32, seed=0, cpu_threads=None):
     """Forward and adjoint (backward) time of a 20-step differentiable rollout of 32 bars of 16x16x64 cubes."""
     import torch
 
@@ -234,6 +319,8 @@ def bench_adjoint(rtol, nbar=32, seed=0):
         "adjoint_pcg_iterations_lockstep": runner.adjoint_pcg_iterations,
         "gradient_norm_log_e": float(le.grad.norm().item()),
         "gradient_norm_v0": float(v0.grad.norm().item()),
+        "cpu_baseline": (cpu_adjoint(rc, rtol, cpu_threads, runner.adjoint_pcg_iterations / rc.steps)
+                         if cpu_threads else None),
     }
 
 
@@ -307,9 +394,17 @@ def main():
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
         cpu = cpu_sample(case, counts, args.cpu_threads, args.rtol)
     del solver
-    batched = None if args.no_batched else bench_batched(args.rtol)
-    adj = None if args.no_adjoint else bench_adjoint(args.rtol)
-    lbfgs = None if args.no_lbfgs else bench_lbfgs(args.rtol)
+    cpu_threads = None if args.no_cpu else args.cpu_threads
+    batched = None if args.no_batched else bench_batched(args.rtol, cpu_threads=cpu_threads)
+    adj = None if args.no_adjoint else bench_adjoint(args.rtol, cpu_threads=cpu_threads)
+    lbfgs = None if args.no_lbfgs else bench_lbfgs(args.rtol, cpu_threads=cpu_threads)
+    # GPU / CPU ratios of the extension legs (their cpu_baseline objects are bounded samples, as for the main line)
+    if batched and batched.get("cpu_baseline"):
+        batched["gpu_over_cpu"] = batched["batched_steps_per_s"] / batched["cpu_baseline"]["value"]
+    if lbfgs and lbfgs.get("cpu_baseline"):
+        lbfgs["gpu_over_cpu"] = lbfgs["steps_per_s"] / lbfgs["cpu_baseline"]["value"]
+    if adj and adj.get("cpu_baseline"):
+        adj["gpu_over_cpu"] = adj["cpu_baseline"]["value"] / (adj["forward_s"] + adj["backward_s"])
     value = args.steps / elapsed
     line = {
         "metric": METRIC,
