@@ -45,9 +45,9 @@ __global__ __launch_bounds__(kBlock) void k_bgrad_reduce(BatchView bt, const int
                                                          const double* __restrict__ gcontrib,
                                                          const unsigned char* __restrict__ dof, double* __restrict__ g,
                                                          double* __restrict__ part_g) {
-  if (!bt.mask[bt.vwg_mesh[blockIdx.x]]) return;
-  const int v = blockIdx.x * kBlock + threadIdx.x;
-  double gg[1] = {grad_reduce_vertex(v, gseg, gcontrib, dof, g)};
+  if (!bt.mask[bt.vwg_mesh[blockIdx.x]]) return;  // uniform over the workgroup
+  const int v0 = blockIdx.x * kBlock;
+  double gg[1] = {grad_reduce_vertices(v0, v0 + kBlock, gseg, gcontrib, dof, g)};
   store_partials<1>(gg, part_g);
 }
 
@@ -108,11 +108,14 @@ __global__ __launch_bounds__(kBlock) void k_bhess_reduce(BatchView bt, int nub, 
                                                          const int* __restrict__ ub_pos,
                                                          const unsigned char* __restrict__ dof, int filter_dbc,
                                                          double* __restrict__ val, double* __restrict__ dinv) {
-  const int ub = blockIdx.x * kBlock + threadIdx.x;
-  if (ub >= nub) return;
-  const int2 rc = ub_rc[ub];
-  if (!bt.mask[bt.vwg_mesh[rc.x / kBatchRows]]) return;
-  hess_reduce_block(ub, rc, hseg, hcontrib, ub_pos, dof, filter_dbc, val, dinv, nullptr);
+  const int j0 = blockIdx.x * kBlock, ub = j0 + threadIdx.x;
+  const int2 rc = ub < nub ? ub_rc[ub] : make_int2(0, 0);
+  const bool active = ub < nub && bt.mask[bt.vwg_mesh[rc.x / kBatchRows]];
+  if (!__syncthreads_or(active)) return;  // every mesh of this workgroup has converged
+  double S[9];
+  staged_run_sum<9>(hseg, j0, min(j0 + kBlock, nub), hcontrib, S);
+  if (!active) return;
+  hess_store_block(ub, rc, S, ub_pos, dof, filter_dbc, val, dinv, nullptr);
 }
 
 // ------------------------------------------------------------------------------------------------------

@@ -790,11 +790,9 @@ void lbfgs_solve_gram(ssim_ctx* c) {
       c->stats.ls_failures++;
       vector_mode = true;
     }
-    launch_sub(n, c->utry.p, u, c->lb_S.p + (size_t)head * n, s);
-    launch_sub(n, c->gtry.p, g, c->lb_Y.p + (size_t)head * n, s);
+    launch_lbfgs_shift(n, c->utry.p, c->gtry.p, u, g, c->lb_S.p + (size_t)head * n, c->lb_Y.p + (size_t)head * n, s);
+    HIPCHK(hipGetLastError());
     fresh = head;
-    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     cur = trial;
   }
   c->stats.newton_iterations = it;

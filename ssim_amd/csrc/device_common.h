@@ -359,43 +359,81 @@ __device__ __forceinline__ void elem_adjoint_tet(const MeshView& m, int e, const
   }
 }
 
-// segmented reduce of vertex v's gradient contributions, zeroed on Dirichlet dofs (boundary.hpp:45-51);
-// returns the vertex's share of ‖g‖²
-__device__ __forceinline__ double grad_reduce_vertex(int v, const int* __restrict__ gseg,
-                                                     const double* __restrict__ gcontrib,
-                                                     const unsigned char* __restrict__ dof, double* __restrict__ g) {
-  double gg = 0.0;
-  double acc[3] = {0, 0, 0};
-  const int lo = gseg[v], hi = gseg[v + 1];
-  for (int q = lo; q < hi; ++q) {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) acc[a] += gcontrib[3 * (size_t)q + a];
+// LDS-staged segmented sums.  Thread t of the workgroup owns segment j = j0 + t (j < j1), the record run
+// [seg[j], seg[j+1]) of W doubles per record.  The runs of consecutive segments are contiguous, so the workgroup
+// copies its whole span [seg[j0], seg[j1]) through LDS with coalesced loads, kStageDoubles at a time, and each
+// thread adds its own records in index order: the sums are bitwise those of a sequential walk over the run, but
+// the loads are no longer one scattered walk per lane.  Every thread of the workgroup must call it (it
+// synchronises the workgroup); threads with j ≥ j1 get zeros.
+constexpr int kStageDoubles = 4096;  // 32 KB of LDS per workgroup
+
+template <int W>
+__device__ __forceinline__ void staged_run_sum(const int* __restrict__ seg, int j0, int j1,
+                                               const double* __restrict__ rec, double (&acc)[W]) {
+  constexpr int kStage = kStageDoubles / W;  // records per stage
+  constexpr int U = 8;                       // loads in flight per thread
+  __shared__ double lds[kStage * W];
+  const int j = j0 + (int)threadIdx.x;
+  const int span_lo = seg[j0], span_hi = seg[j1];
+  int lo = 0, hi = 0;
+  if (j < j1) {
+    lo = seg[j];
+    hi = seg[j + 1];
   }
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double val = dof[3 * (size_t)v + a] ? 0.0 : acc[a];
-    g[3 * (size_t)v + a] = val;
-    gg += val * val;
+  for (int k = 0; k < W; ++k) acc[k] = 0.0;
+  for (int s0 = span_lo; s0 < span_hi; s0 += kStage) {
+    const int cnt = min(kStage, span_hi - s0) * W;
+    const double* __restrict__ src = rec + (size_t)s0 * W;
+    for (int t0 = 0; t0 < cnt; t0 += U * kBlock) {
+      double buf[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * kBlock + (int)threadIdx.x;
+        buf[u] = t < cnt ? src[t] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * kBlock + (int)threadIdx.x;
+        if (t < cnt) lds[t] = buf[u];
+      }
+    }
+    __syncthreads();
+    const int a = max(lo, s0) - s0, b = min(hi, s0 + kStage) - s0;
+    for (int q = a; q < b; ++q)
+#pragma unroll
+      for (int k = 0; k < W; ++k) acc[k] += lds[q * W + k];
+    __syncthreads();
+  }
+}
+
+// segmented reduce of the gradient contributions of vertices [v0, v1), one per thread, zeroed on Dirichlet dofs
+// (boundary.hpp:45-51); returns the thread's vertex's share of ‖g‖²
+__device__ __forceinline__ double grad_reduce_vertices(int v0, int v1, const int* __restrict__ gseg,
+                                                       const double* __restrict__ gcontrib,
+                                                       const unsigned char* __restrict__ dof,
+                                                       double* __restrict__ g) {
+  double acc[3];
+  staged_run_sum<3>(gseg, v0, v1, gcontrib, acc);
+  const int v = v0 + (int)threadIdx.x;
+  double gg = 0.0;
+  if (v < v1) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double val = dof[3 * (size_t)v + a] ? 0.0 : acc[a];
+      g[3 * (size_t)v + a] = val;
+      gg += val * val;
+    }
   }
   return gg;
 }
 
-// upper block ub = (rc.x, rc.y): sum of its contribution run, Dirichlet identity filter, store into the
+// upper block ub = (rc.x, rc.y) with its summed contributions S: Dirichlet identity filter, store into the
 // blocked-ELL values; Jacobi inverse diagonal (and optionally the diagonal) for a diagonal block
-__device__ __forceinline__ void hess_reduce_block(int ub, int2 rc, const int* __restrict__ hseg,
-                                                  const double* __restrict__ hcontrib, const int* __restrict__ ub_pos,
-                                                  const unsigned char* __restrict__ dof, int filter_dbc,
-                                                  double* __restrict__ val, double* __restrict__ dinv,
-                                                  double* __restrict__ diag) {
-  double S[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) S[k] = 0;
-  const int lo = hseg[ub], hi = hseg[ub + 1];
-  for (int q = lo; q < hi; ++q) {
-    const double* src = hcontrib + 9 * (size_t)q;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) S[k] += src[k];
-  }
+__device__ __forceinline__ void hess_store_block(int ub, int2 rc, double (&S)[9], const int* __restrict__ ub_pos,
+                                                 const unsigned char* __restrict__ dof, int filter_dbc,
+                                                 double* __restrict__ val, double* __restrict__ dinv,
+                                                 double* __restrict__ diag) {
   if (filter_dbc) {
 #pragma unroll
     for (int b = 0; b < 3; ++b)

@@ -153,6 +153,8 @@ void launch_lbfgs_gram(int n, const double* S, const double* Y, const double* g,
 int lbfgs_gram_blocks(int n);
 void launch_lbfgs_dir(int n, const double* S, const double* Y, const double* g, const double* h, const LbfgsCoef& cf,
                       double* r, hipStream_t s);
+void launch_lbfgs_shift(int n, const double* ut, const double* gt, double* u, double* g, double* sv, double* yv,
+                        hipStream_t s);
 void launch_elem_pd(const MeshView& m, double c, const int* hperm, double* hcontrib, hipStream_t s);
 void launch_bsr_spmv(int nv, const int* brow, const int* bcol, const int* tpos, const double* val, const double* x,
                      double* y, bool trans, hipStream_t s);

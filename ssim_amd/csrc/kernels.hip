@@ -105,14 +105,14 @@ void launch_elem_pd(const MeshView& m, double c, const int* hperm, double* hcont
   k_elem_pd<<<nblocks(m.nt), kBlock, 0, s>>>(m, c, hperm, hcontrib);
 }
 
-// segmented reduce of the vertex-sorted gradient contributions; Dirichlet dofs zeroed (boundary.hpp:45-51)
+// segmented reduce of the vertex-sorted gradient contributions, staged through LDS; Dirichlet dofs zeroed
+// (boundary.hpp:45-51)
 __global__ __launch_bounds__(kBlock) void k_grad_reduce(int nv, const int* __restrict__ gseg,
                                                         const double* __restrict__ gcontrib,
                                                         const unsigned char* __restrict__ dof, double* __restrict__ g,
                                                         double* __restrict__ part_g) {
-  const int v = blockIdx.x * kBlock + threadIdx.x;
-  double gg[1] = {0.0};
-  if (v < nv) gg[0] = grad_reduce_vertex(v, gseg, gcontrib, dof, g);
+  const int v0 = blockIdx.x * kBlock;
+  double gg[1] = {grad_reduce_vertices(v0, min(v0 + kBlock, nv), gseg, gcontrib, dof, g)};
   store_partials<1>(gg, part_g);
 }
 
@@ -178,9 +178,11 @@ __global__ __launch_bounds__(kBlock) void k_hess_reduce(int nub, const int* __re
                                                         const unsigned char* __restrict__ dof, int filter_dbc,
                                                         double* __restrict__ val, double* __restrict__ dinv,
                                                         double* __restrict__ diag) {
-  const int ub = blockIdx.x * kBlock + threadIdx.x;
+  const int j0 = blockIdx.x * kBlock, ub = j0 + threadIdx.x;
+  double S[9];
+  staged_run_sum<9>(hseg, j0, min(j0 + kBlock, nub), hcontrib, S);
   if (ub >= nub) return;
-  hess_reduce_block(ub, ub_rc[ub], hseg, hcontrib, ub_pos, dof, filter_dbc, val, dinv, diag);
+  hess_store_block(ub, ub_rc[ub], S, ub_pos, dof, filter_dbc, val, dinv, diag);
 }
 
 void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const int2* ub_rc, const int* ub_pos,
@@ -795,52 +797,141 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
 // Partials of the 6m + 1 dot products one iteration needs (m = kLbfgsSlots, every slot, used or not):
 //   [0, m)   s_jᵀg        [m, 2m)  y_jᵀDg       [2m]  gᵀDg
 //   [2m+1+j] s_jᵀy_k      [3m+1+j] s_kᵀy_j      [4m+1+j] y_kᵀDy_j      (k = the newest slot; zero when k < 0)
+// Each of the workgroup's four waves owns one group of them over the same kGramChunk elements (s_jᵀg; y_jᵀDg and
+// gᵀDg; s_jᵀy_k; s_kᵀy_j and y_kᵀDy_j), so a lane keeps at most 2m accumulators and every wave loads about
+// m + 2 vectors; the waves of a workgroup read the same elements, which the caches serve once.  A lane loads all
+// kGramU of its elements before it multiplies, so one wave has all its loads in flight at once.
+constexpr int kGramU = 4;
+constexpr int kGramChunk = 64 * kGramU;
+
 __global__ __launch_bounds__(kBlock) void k_lbfgs_gram(int n, const double* __restrict__ S, const double* __restrict__ Y,
                                                        const double* __restrict__ g, const double* __restrict__ h,
                                                        int k, double* __restrict__ part) {
-  constexpr int M = kLbfgsSlots;
-  double v[kLbfgsDots];
+  constexpr int M = kLbfgsSlots, U = kGramU;
+  static_assert(kBlock == 256, "four waves per workgroup, one per group of dot products");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int idx[U];
+  bool ok[U];
 #pragma unroll
-  for (int j = 0; j < kLbfgsDots; ++j) v[j] = 0.0;
-#pragma unroll 1
-  for (int jj = 0; jj < kVec; ++jj) {
-    const int i = (blockIdx.x * kVec + jj) * kBlock + threadIdx.x;
-    if (i >= n) break;
-    const double gi = g[i], di = 1.0 / h[i];
-    const double dg = gi * di;
-    const double sk = k >= 0 ? S[(size_t)k * n + i] : 0.0, yk = k >= 0 ? Y[(size_t)k * n + i] : 0.0;
-    const double dyk = yk * di;
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-      const double sj = S[(size_t)j * n + i], yj = Y[(size_t)j * n + i];
-      v[j] += sj * gi;
-      v[M + j] += yj * dg;
-      v[2 * M + 1 + j] += sj * yk;
-      v[3 * M + 1 + j] += sk * yj;
-      v[4 * M + 1 + j] += dyk * yj;
-    }
-    v[2 * M] += gi * dg;
+  for (int u = 0; u < U; ++u) {
+    const int i = blockIdx.x * kGramChunk + u * 64 + lane;
+    ok[u] = i < n;
+    idx[u] = ok[u] ? i : 0;
   }
-  store_partials<kLbfgsDots>(v, part);
+  // vector j at this lane's elements; zero past the end
+  auto load = [&](const double* __restrict__ v, double (&out)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) out[u] = ok[u] ? v[idx[u]] : 0.0;
+  };
+  double acc[2 * M];
+#pragma unroll
+  for (int j = 0; j < 2 * M; ++j) acc[j] = 0.0;
+  double gdg = 0.0;
+  const bool pair = k >= 0;
+  if (wave == 0) {
+    double gv[U], sv[M][U];
+    load(g, gv);
+#pragma unroll
+    for (int j = 0; j < M; ++j) load(S + (size_t)j * n, sv[j]);
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc[j] += sv[j][u] * gv[u];
+  } else if (wave == 1) {
+    double gv[U], hv[U], yv[M][U];
+    load(g, gv);
+    load(h, hv);
+#pragma unroll
+    for (int j = 0; j < M; ++j) load(Y + (size_t)j * n, yv[j]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const double dg = ok[u] ? gv[u] * (1.0 / hv[u]) : 0.0;
+#pragma unroll
+      for (int j = 0; j < M; ++j) acc[j] += yv[j][u] * dg;
+      gdg += gv[u] * dg;
+    }
+  } else if (wave == 2) {
+    if (pair) {
+      double ykv[U], sv[M][U];
+      load(Y + (size_t)k * n, ykv);
+#pragma unroll
+      for (int j = 0; j < M; ++j) load(S + (size_t)j * n, sv[j]);
+#pragma unroll
+      for (int j = 0; j < M; ++j)
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[j] += sv[j][u] * ykv[u];
+    }
+  } else {
+    if (pair) {
+      double skv[U], ykv[U], hv[U], yv[M][U];
+      load(S + (size_t)k * n, skv);
+      load(Y + (size_t)k * n, ykv);
+      load(h, hv);
+#pragma unroll
+      for (int j = 0; j < M; ++j) load(Y + (size_t)j * n, yv[j]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double dyk = ok[u] ? ykv[u] * (1.0 / hv[u]) : 0.0;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+          acc[j] += skv[u] * yv[j][u];
+          acc[M + j] += dyk * yv[j][u];
+        }
+      }
+    }
+  }
+  const int cnt = wave == 3 ? 2 * M : M;
+  const int base = wave == 0 ? 0 : wave == 1 ? M : wave == 2 ? 2 * M + 1 : 3 * M + 1;
+  double* out = part + (size_t)blockIdx.x * kLbfgsDots;
+#pragma unroll
+  for (int j = 0; j < 2 * M; ++j) {
+    const double t = wave_sum(acc[j]);
+    if (lane == 0 && j < cnt) out[base + j] = t;
+  }
+  if (wave == 1) {
+    const double t = wave_sum(gdg);
+    if (lane == 0) out[2 * M] = t;
+  }
 }
 
+// one workgroup per dot product: the sum of its partials in a fixed order
 __global__ __launch_bounds__(kBlock) void k_lbfgs_gram_fin(const double* __restrict__ part, int np,
                                                            double* __restrict__ out) {
-  double t[kLbfgsDots];
-  sum_partials<kLbfgsDots, kBlock, 1>(part, np, t);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int j = 0; j < kLbfgsDots; ++j) out[j] = t[j];
+  const int d = blockIdx.x;
+  double t[1] = {0.0};
+  for (int b = threadIdx.x; b < np; b += kBlock) t[0] += part[(size_t)b * kLbfgsDots + d];
+  block_sum<1>(t);
+  if (threadIdx.x == 0) out[d] = t[0];
 }
 
 void launch_lbfgs_gram(int n, const double* S, const double* Y, const double* g, const double* h, int k, double* part,
                        double* out, hipStream_t s) {
-  const int nb = vblocks(n);
+  const int nb = lbfgs_gram_blocks(n);
   k_lbfgs_gram<<<nb, kBlock, 0, s>>>(n, S, Y, g, h, k, part);
-  k_lbfgs_gram_fin<<<1, kBlock, 0, s>>>(part, nb, out);
+  k_lbfgs_gram_fin<<<kLbfgsDots, kBlock, 0, s>>>(part, nb, out);
 }
 
-int lbfgs_gram_blocks(int n) { return vblocks(n); }
+int lbfgs_gram_blocks(int n) { return (n + kGramChunk - 1) / kGramChunk; }
+
+// new pair and iterate in one pass: s = u_t − u, y = g_t − g, then u = u_t, g = g_t
+__global__ __launch_bounds__(kBlock) void k_lbfgs_shift(int n, const double* __restrict__ ut,
+                                                        const double* __restrict__ gt, double* __restrict__ u,
+                                                        double* __restrict__ g, double* __restrict__ sv,
+                                                        double* __restrict__ yv) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double a = ut[i], b = gt[i];
+  sv[i] = a - u[i];
+  yv[i] = b - g[i];
+  u[i] = a;
+  g[i] = b;
+}
+
+void launch_lbfgs_shift(int n, const double* ut, const double* gt, double* u, double* g, double* sv, double* yv,
+                        hipStream_t s) {
+  if (n <= 0) return;
+  k_lbfgs_shift<<<nblocks(n), kBlock, 0, s>>>(n, ut, gt, u, g, sv, yv);
+}
 
 // r = D·(g − Σ_j a_j y_j) + Σ_j c_j s_j over the slots in `mask`
 __global__ __launch_bounds__(kBlock) void k_lbfgs_dir(int n, const double* __restrict__ S, const double* __restrict__ Y,
