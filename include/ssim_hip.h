@@ -247,7 +247,7 @@ ssim_status_t ssim_get_layout_info(const ssim_ctx* ctx, ssim_layout_info* info);
 
 /* ---- launch tuning (no reference counterpart: the reference's launch shapes live inside mathprim) ---- */
 typedef enum {
-  SSIM_TUNE_XCD_REMAP = 0,     /* 1 (default): each of the 8 XCDs takes a contiguous range of SpMV row slices */
+  SSIM_TUNE_XCD_REMAP = 0,     /* 1 (default): each of the 8 XCDs takes a contiguous range of SpMV row slices and of tets */
   SSIM_TUNE_PCG_CHUNK_MAX = 1, /* PCG iterations enqueued per done-flag poll (default 64) */
   SSIM_TUNE_SPMV_LANES = 2,    /* SpMV variant: 0 = one lane per row, fully unrolled over a layout padded to 8
                                   upper / 7 lower blocks (default when every row fits), 1 = one lane per row with

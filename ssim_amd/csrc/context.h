@@ -172,6 +172,7 @@ struct ssim_ctx {
     m.nt = nt;
     m.rho = rho_reset;
     m.model = static_cast<int>(model);
+    m.xcd_remap = xcd_remap;
     return m;
   }
   ssim_hip::BellView bell(const double* values) const {

@@ -23,7 +23,8 @@ struct MeshView {
   const double* mu;     // [nt]
   int nv, nt;
   double rho;
-  int model = 0;  // MaterialModel (elem_math.h)
+  int model = 0;      // MaterialModel (elem_math.h)
+  int xcd_remap = 0;  // 1: the element kernels give each of the 8 XCDs a contiguous range of tets
 };
 
 // Symmetric blocked ELLPACK of 3x3 blocks.  Rows (vertices) are grouped in slices of 64 lanes.

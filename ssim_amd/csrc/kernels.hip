@@ -14,7 +14,7 @@ namespace ssim_hip {
 
 int max_partials(int n) { return (n + kBlock - 1) / kBlock + 1; }
 
-static inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+__host__ __device__ static inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 // ------------------------------------------------------------------------------------------------------
 // rest shape
@@ -59,10 +59,12 @@ __global__ __launch_bounds__(kBlock) void k_elem_grad(MeshView m, const double* 
                                                       const double* __restrict__ uhat, double dt2,
                                                       const int* __restrict__ gperm, double* __restrict__ gcontrib,
                                                       double* __restrict__ part_e) {
-  const int e = blockIdx.x * kBlock + threadIdx.x;
+  const int lb = logical_block(nblocks(m.nt > 0 ? m.nt : 1), m.xcd_remap);
+  if (lb < 0) return;
+  const int e = lb * kBlock + threadIdx.x;
   double energy[1] = {0.0};
   if (e < m.nt) energy[0] = elem_grad_tet<T, Model>(m, e, u, uhat, dt2, gperm, gcontrib);
-  store_partials<1>(energy, part_e);
+  store_partials<1>(energy, part_e, lb);
 }
 
 int launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
@@ -70,7 +72,8 @@ int launch_elem_grad(const MeshView& m, const double* u, const double* uhat, dou
   const int nb = nblocks(m.nt > 0 ? m.nt : 1);
   dispatch_elem(m.model, fp32, [&](auto k) {
     using K = decltype(k);
-    k_elem_grad<typename K::type, K::model><<<nb, kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, part_e);
+    k_elem_grad<typename K::type, K::model>
+        <<<grid_for(nb, m.xcd_remap), kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, part_e);
   });
   return nb;
 }
@@ -155,8 +158,9 @@ template <typename T, int Model>
 __global__ __launch_bounds__(kBlock) void k_elem_hess(MeshView m, const double* __restrict__ u, double dt2,
                                                       int make_spsd, const int* __restrict__ hperm,
                                                       double* __restrict__ hcontrib) {
-  const int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e >= m.nt) return;
+  const int lb = logical_block(nblocks(m.nt), m.xcd_remap);
+  const int e = lb * kBlock + threadIdx.x;
+  if (lb < 0 || e >= m.nt) return;
   elem_hess_tet<T, Model>(m, e, u, dt2, make_spsd, hperm, hcontrib);
 }
 
@@ -165,7 +169,8 @@ void launch_elem_hess(const MeshView& m, const double* u, double dt2, int make_s
   if (m.nt <= 0) return;
   dispatch_elem(m.model, fp32, [&](auto k) {
     using K = decltype(k);
-    k_elem_hess<typename K::type, K::model><<<nblocks(m.nt), kBlock, 0, s>>>(m, u, dt2, make_spsd, hperm, hcontrib);
+    k_elem_hess<typename K::type, K::model>
+        <<<grid_for(nblocks(m.nt), m.xcd_remap), kBlock, 0, s>>>(m, u, dt2, make_spsd, hperm, hcontrib);
   });
 }
 
