@@ -635,7 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
     }
   }
   double pq[1], rz[1];
-  sum_partials<1>(pa, na, pq);
+  sum_partials<1, kBlock, 16>(pa, na, pq);  // one pass over up to 4,096 partials
   rz[0] = sc->rz_hist[k & 1];
   if (!(pq[0] > 0.0)) {  // not positive definite (or NaN): every workgroup sees the same pq
     if (blockIdx.x == 0 && threadIdx.x == 0) {
