@@ -64,9 +64,40 @@ def load_traffic():
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def host_cpu():
+    """Model, sockets and physical cores of the host (from /proc/cpuinfo) and the CPUs this process may use."""
+    model, phys = None, set()
+    pid = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    pid = v
+                elif k == "core id":
+                    core = v
+                elif not k and pid is not None:
+                    phys.add((pid, core))
+                    pid = core = None
+        if pid is not None:
+            phys.add((pid, core))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "sockets": len({p for p, _ in phys}) or None, "physical_cores": len(phys) or None,
+            "usable_cpus": usable}
+
+
 def cpu_sample(case, counts, threads, rtol):
     """Times the oracle's phases on the same mesh (one Hessian assembly, two energy+gradient evaluations, 20 PCG
-    iterations) and composes the CPU time of the GPU-timed steps from their Newton / PCG / evaluation counts."""
+    iterations) and composes the CPU time of the GPU-timed steps from their Newton / PCG / evaluation counts; the
+    same phases once more on one thread (one evaluation, one assembly, 5 PCG iterations)."""
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -75,29 +106,45 @@ def cpu_sample(case, counts, threads, rtol):
     ts.set_threshold(rtol)
     ts.prepare_step()
     u = case.u0.copy()
-    t0 = time.perf_counter()
-    E, g = ts.energy_grad(u)
-    E, g = ts.energy_grad(u)
-    t_eval = (time.perf_counter() - t0) / 2
-    t0 = time.perf_counter()
-    vals = ts.hessian(u, True, True)
-    t_hess = time.perf_counter() - t0
     rowptr, colidx = ts.csr_pattern()
-    t0 = time.perf_counter()
-    _, it, _ = oracle.pcg(rowptr, colidx, vals, -g, rtol=0.0, maxit=20)
-    t_pcg = (time.perf_counter() - t0) / max(it, 1)
+
+    def phases(n_eval, pcg_its):
+        t0 = time.perf_counter()
+        for _ in range(n_eval):
+            E, g = ts.energy_grad(u)
+        t_eval = (time.perf_counter() - t0) / n_eval
+        t0 = time.perf_counter()
+        vals = ts.hessian(u, True, True)
+        t_hess = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, it, _ = oracle.pcg(rowptr, colidx, vals, -g, rtol=0.0, maxit=pcg_its)
+        t_pcg = (time.perf_counter() - t0) / max(it, 1)
+        return t_eval, t_hess, t_pcg
+
     steps = counts["steps"]
-    total = (counts["energy_evals"] * t_eval + counts["hessian_evals"] * t_hess + counts["pcg_iterations"] * t_pcg)
-    per_step = total / steps
+
+    def per_step(t_eval, t_hess, t_pcg):
+        return (counts["energy_evals"] * t_eval + counts["hessian_evals"] * t_hess
+                + counts["pcg_iterations"] * t_pcg) / steps
+
+    cores = oracle.lib().orc_num_threads()
+    t_eval, t_hess, t_pcg = phases(2, 20)
+    oracle.lib().orc_set_num_threads(1)
+    t1 = phases(1, 5)
+    oracle.lib().orc_set_num_threads(cores)
     return {
-        "value": 1.0 / per_step,
+        "value": 1.0 / per_step(t_eval, t_hess, t_pcg),
         "unit": "steps/s",
-        "cores": oracle.lib().orc_num_threads(),
+        "cores": cores,
         "kind": "port",
         "sample": (f"oracle phases on the same 786k-tet mesh: energy+grad {t_eval * 1e3:.1f} ms, SPD Hessian assembly "
                    f"{t_hess * 1e3:.1f} ms, PCG iteration {t_pcg * 1e3:.2f} ms; composed with the timed GPU steps' "
                    f"counts ({counts['energy_evals']} evals, {counts['hessian_evals']} Hessians, "
                    f"{counts['pcg_iterations']} PCG iterations over {steps} steps)"),
+        "one_thread": {"value": 1.0 / per_step(*t1), "unit": "steps/s",
+                       "sample": (f"the same phases on 1 thread: energy+grad {t1[0] * 1e3:.1f} ms, SPD Hessian "
+                                  f"assembly {t1[1] * 1e3:.1f} ms, PCG iteration {t1[2] * 1e3:.2f} ms")},
+        "host": host_cpu(),
     }
 
 

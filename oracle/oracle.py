@@ -89,6 +89,7 @@ def lib():
             "orc_ts_rest_hessian_diag": (None, [_V, _D]),
             "orc_ts_spmv": (None, [_V, _D, _D, _D]),
             "orc_num_threads": (ctypes.c_int, []),
+            "orc_set_num_threads": (None, [ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

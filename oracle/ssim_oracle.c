@@ -44,6 +44,14 @@ int orc_num_threads(void) {
 #endif
 }
 
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------------------------------------ */
 /* Lamé parameters: elast/basic_elast.hpp:11-20                                                      */
 double orc_mu(double E, double nu) { return E / (2 * (1 + nu)); }

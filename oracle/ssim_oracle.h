@@ -102,6 +102,7 @@ void orc_ts_rest_hessian_diag(orc_ts* ts, double* diag);
 /* timing helpers for the CPU baseline: one PCG iteration-equivalent (SpMV + vector work) etc. */
 void orc_ts_spmv(const orc_ts* ts, const double* vals, const double* x, double* y);
 int orc_num_threads(void);
+void orc_set_num_threads(int n); /* thread count of the following parallel loops (OpenMP builds) */
 
 #ifdef __cplusplus
 }
