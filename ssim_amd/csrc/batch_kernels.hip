@@ -353,8 +353,8 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const int*
     x[i] = xv[j] + alpha * pv[j];
     const double ri = rv[j] - alpha * qv[j];
     const double zi = dv[j] * ri;
-    r[i] = ri;
-    z[i] = zi;
+    __builtin_nontemporal_store(ri, &r[i]);  // as k_pcg_update
+    __builtin_nontemporal_store(zi, &z[i]);
     v[0] += ri * zi;
     v[1] += ri * ri;
   }
