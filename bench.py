@@ -321,6 +321,40 @@ def bench_lbfgs(rtol, n=64, steps=3, seed=0, cpu_threads=None):
     }
 
 
+def bench_fp32(case, solver, done_steps, rtol, total=60):
+    """The FP32 element-kernel variant on the same bar: ``solver`` (FP64, ``done_steps`` steps taken) is carried on
+    to ``total`` steps, a fresh solver with FP32 element kernels takes ``total`` steps from the same initial
+    state (timed), and the positions are compared."""
+    import numpy as np
+    import torch
+
+    from ssim_amd import fem
+
+    for _ in range(max(0, total - done_steps)):
+        solver.step()
+    x64 = case.X.reshape(-1) + solver.get_array("deformation")
+    s = fem.TetFiniteElementSolver_Hip.from_case(case, rtol=rtol)
+    s.set_solver_params(fp32_elements=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(total):
+        s.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    x32 = case.X.reshape(-1) + s.get_array("deformation")
+    err = float(np.abs(x32 - x64).max())
+    print(f"[bench] fp32 elements: {total / dt:.2f} steps/s, max position error {err:.3e} m", file=sys.stderr,
+          flush=True)
+    return {
+        "workload": (f"the same bar, {total} steps from the same initial state with the element kernels (F, Psi, P, H, "
+                     "H+, K) in FP32; reductions, assembly sums, PCG and positions in FP64"),
+        "steps": total,
+        "steps_per_s": total / dt,
+        "max_position_error_m": err,
+        "max_position_error_rel": err / float(np.abs(x64).max()),
+    }
+
+
 def bench_adjoint(rtol, nbar=32, seed=0, cpu_threads=None):
     """Forward and adjoint (backward) time of a 20-step differentiable rollout of 32 bars of 16x16x64 cubes."""
     import torch
@@ -384,6 +418,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--no-lbfgs", action="store_true", help="skip the L-BFGS twisted-cube leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 element-variant leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build's time step runs on a single device "
@@ -440,6 +475,7 @@ def main():
     if not args.no_cpu:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
         cpu = cpu_sample(case, counts, args.cpu_threads, args.rtol)
+    fp32 = None if args.no_fp32 else bench_fp32(case, solver, args.warmup + args.steps, args.rtol)
     del solver
     cpu_threads = None if args.no_cpu else args.cpu_threads
     batched = None if args.no_batched else bench_batched(args.rtol, cpu_threads=cpu_threads)
@@ -482,6 +518,7 @@ def main():
         "batched": batched,
         "adjoint": adj,
         "lbfgs": lbfgs,
+        "fp32_elements": fp32,
     }
     print(json.dumps(line), flush=True)
 
