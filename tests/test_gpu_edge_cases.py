@@ -1,6 +1,7 @@
 """Edge cases of the HIP path against the CPU FP64 oracle: a vertex shared by 1,500 tets (gradient and Hessian
 runs longer than one LDS stage of the segmented reductions, a blocked-ELL row of 4,501 blocks), a single tet,
-and a mesh whose every dof is Dirichlet.
+and a mesh whose every dof is Dirichlet; the first two and a small bar also in one batch, against the single-mesh
+path.
 
 Bars as in the parity tests: energy, gradient and Hessian ≤ 1e-10 relative (norm-wise); positions after every
 Newton step ≤ 1e-8 relative with identical Newton counts; a fully clamped mesh lands exactly on its targets.
@@ -10,7 +11,7 @@ import pytest
 
 import oracle
 from helpers import rel_err
-from ssim_amd import fem, mesh
+from ssim_amd import batch, fem, mesh
 
 pytestmark = pytest.mark.gpu
 
@@ -105,7 +106,8 @@ def test_high_valence_vertex_newton_steps_match_oracle():
         assert rel_err(_x(case, g.get_array("deformation")), _x(case, o.get_state()[0])) <= 1e-8, step
 
 
-def test_single_tet_trajectory_matches_oracle():
+def tet_case():
+    """One 1 cm tet, vertex 0 clamped, gravity along −y, seeded jitter of ±0.5 mm on the free vertices."""
     X, cells = mesh.tetra()
     X = X * 0.01
     lam, mu = mesh.lame(np.full(1, 1e5), np.full(1, 0.45))
@@ -115,7 +117,11 @@ def test_single_tet_trajectory_matches_oracle():
     u0[3:] = np.random.default_rng(1).uniform(-5e-4, 5e-4, 9)
     accel = np.zeros(12)
     accel[1::3] = -9.8
-    case = mesh.BarCase(X, cells, lam, mu, 1000.0, 1.0 / 60, dof, np.zeros(12), accel, u0, np.zeros(12))
+    return mesh.BarCase(X, cells, lam, mu, 1000.0, 1.0 / 60, dof, np.zeros(12), accel, u0, np.zeros(12), name="tet")
+
+
+def test_single_tet_trajectory_matches_oracle():
+    case = tet_case()
     g, o = _pair(case)
     for step in range(20):
         g.step()
@@ -137,3 +143,34 @@ def test_fully_clamped_mesh_lands_on_its_targets():
         assert np.array_equal(g.get_array("deformation"), case.dbc), step
         assert rel_err(g.get_array("velocity"), o.get_state()[2]) <= 1e-12, step
     assert g.last_stats()["converged"]
+
+
+def _batch_vs_single(cases, rtol, steps=3, **prm):
+    B = batch.TetFiniteElementBatch_Hip.from_cases(cases, rtol=rtol)
+    if prm:
+        B.set_solver_params(**prm)
+    singles = []
+    for c in cases:
+        s = fem.TetFiniteElementSolver_Hip.from_case(c, rtol=rtol)
+        if prm:
+            s.set_solver_params(**prm)
+        singles.append(s)
+    for step in range(steps):
+        B.step()
+        for m, (c, s) in enumerate(zip(cases, singles)):
+            s.step()
+            assert B.last_stats(m)["newton_iterations"] == s.last_stats()["newton_iterations"], (step, c.name)
+            err = rel_err(_x(c, B.get_array("deformation", m)), _x(c, s.get_array("deformation")))
+            assert err <= 1e-8, (step, c.name, err)
+
+
+def test_star_and_bar_in_one_batch_match_the_single_mesh_path():
+    """The 600-tet star (a blocked-ELL row of 1,801 blocks; Hessian runs across LDS stages in the batched reduce)
+    beside a small bar, solved to 1e-12 and stopped at 1e-9 x mass as in the single-mesh star test."""
+    _batch_vs_single([star_case(600), mesh.cantilever_bar(3, 2, 2, seed=5)], 1e-9, pcg_rtol=1e-12, max_newton=20)
+
+
+def test_single_tet_and_bar_in_one_batch_match_the_single_mesh_path():
+    """A one-tet mesh (one slice with 4 live rows, one vertex workgroup with 4 live vertices) beside a small bar, at
+    the benchmark threshold 1e-6 x mass."""
+    _batch_vs_single([tet_case(), mesh.cantilever_bar(3, 2, 2, seed=5)], 1e-6, steps=10)
