@@ -190,6 +190,16 @@ ssim_status_t ssim_export_hessian_csr(ssim_ctx* ctx, int* rowptr, int* colidx, d
  * stepper/projective_dynamics.hpp:51-87) in the same CSR layout; built at reset or on first use */
 ssim_status_t ssim_export_pd_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
 ssim_status_t ssim_export_mass_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
+/* mass_matrix_filtered() (time_step.hpp:310-311, built at :460-466): the mass matrix with the Dirichlet rows and
+ * columns replaced by the identity (boundary.hpp:53-67), for the current dof flags; same CSR layout */
+ssim_status_t ssim_export_mass_filtered_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values);
+/* dminv() and rest_vol() (time_step.hpp:296-301): Dm⁻¹ per tet, [nt,3,3] row-major with Dm = [X1−X0, X2−X0,
+ * X3−X0] (def_grad.hpp:56-91), and V = |det Dm|/6 per tet (rest_vol.hpp:34-54); host arrays, either may be NULL;
+ * computed by ssim_reset */
+ssim_status_t ssim_get_rest_shape(ssim_ctx* ctx, double* dminv, double* rest_vol);
+/* mesh() (time_step.hpp:275-276): device pointers to the rest positions [nv,3] (double) and the cells [nt,4] (int),
+ * owned by ctx and valid until ssim_destroy; either output may be NULL */
+ssim_status_t ssim_device_mesh(const ssim_ctx* ctx, const double** vertices, const int** cells);
 
 /* Element-level outputs at deformation u (host, 3·nv doubles): Ψ [nt], P [nt,3,3], H [nt,9,9] (SPD-projected
  * with floor 1e-3 when make_spsd; wrapped_pfpx, time_step.hpp:41-59) and the local stiffness blocks

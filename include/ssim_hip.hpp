@@ -49,6 +49,21 @@ struct vertex_view {
   long long numel() const noexcept { return 3LL * rows; }
 };
 
+// Device view of the mesh (rest positions [nv,3] and cells [nt,4]), owned by the time_step.
+struct mesh_view {
+  const double* vertices = nullptr;
+  const int* cells = nullptr;
+  int nv = 0, nt = 0;
+};
+
+// Host copy of a scalar CSR matrix (rows and columns ascending; the layout of the reference's sys_matrix views).
+struct csr_matrix {
+  int rows = 0;
+  std::vector<int> outer_ptrs, inner_indices;
+  std::vector<double> values;
+  long long nnz() const noexcept { return static_cast<long long>(values.size()); }
+};
+
 struct newton_solver {
   static constexpr ssim_stepper_t kind = SSIM_STEPPER_NEWTON;
 };
@@ -201,7 +216,53 @@ class time_step {
     return s;
   }
 
+  // the remaining views of time_step.hpp:275-311, as device views (mesh) or host copies (the rest)
+  mesh_view mesh() const {
+    mesh_view m;
+    check(ssim_device_mesh(ctx_, &m.vertices, &m.cells));
+    m.nv = num_vertices();
+    m.nt = num_cells();
+    return m;
+  }
+  std::vector<unsigned char> dof_type() const {
+    std::vector<unsigned char> out(3 * static_cast<size_t>(num_vertices()));
+    check(ssim_get_dof_type(ctx_, out.data()));
+    return out;
+  }
+  std::vector<double> dminv() const {  // [nt,3,3] row-major
+    std::vector<double> out(9 * static_cast<size_t>(num_cells()));
+    check(ssim_get_rest_shape(ctx_, out.data(), nullptr));
+    return out;
+  }
+  std::vector<double> rest_vol() const {  // [nt]
+    std::vector<double> out(static_cast<size_t>(num_cells()));
+    check(ssim_get_rest_shape(ctx_, nullptr, out.data()));
+    return out;
+  }
+  // local_stiff() [nt,12,12]: pfpxᵀ·H·pfpx at next_deform (time_step.hpp:641-644), computed on request -- the
+  // Hessian kernels assemble it without storing it
+  std::vector<double> local_stiff(bool make_spsd = true) const {
+    const std::vector<double> u = get(SSIM_ARRAY_NEXT_DEFORM);
+    std::vector<double> out(144 * static_cast<size_t>(num_cells()));
+    check(ssim_eval_elements(ctx_, u.data(), make_spsd ? 1 : 0, nullptr, nullptr, nullptr, out.data()));
+    return out;
+  }
+  csr_matrix sysmatrix() const { return csr(ssim_export_hessian_csr); }
+  csr_matrix mass_matrix() const { return csr(ssim_export_mass_csr); }
+  csr_matrix mass_matrix_filtered() const { return csr(ssim_export_mass_filtered_csr); }
+
  private:
+  csr_matrix csr(ssim_status_t (*exporter)(ssim_ctx*, int*, int*, double*)) const {
+    csr_matrix m;
+    long long nnz = 0;
+    check(ssim_csr_nnz(ctx_, &nnz));
+    m.rows = 3 * num_vertices();
+    m.outer_ptrs.resize(static_cast<size_t>(m.rows) + 1);
+    m.inner_indices.resize(static_cast<size_t>(nnz));
+    m.values.resize(static_cast<size_t>(nnz));
+    check(exporter(ctx_, m.outer_ptrs.data(), m.inner_indices.data(), m.values.data()));
+    return m;
+  }
   time_step& chk(ssim_status_t st) {
     check(st);
     return *this;

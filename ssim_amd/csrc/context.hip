@@ -1589,22 +1589,62 @@ ssim_status_t ssim_export_pd_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double
   });
 }
 
-ssim_status_t ssim_export_mass_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values) {
+namespace {
+// the mass matrix is the Hessian assembly with dt = 0; `filtered` applies the Dirichlet identity rows and columns
+// (mass_matrix_filtered_, time_step.hpp:460-466) with the current dof flags
+ssim_status_t export_mass(ssim_ctx* ctx, int filtered, int* rowptr, int* colidx, double* values) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
   return run([&] {
     require_reset(ctx);
     if (values) {
-      // the mass matrix is the Hessian assembly with dt = 0 and no Dirichlet filter
       ctx->massval.alloc(ctx->val.count);
       HIPCHK(hipMemsetAsync(ctx->massval.p, 0, sizeof(double) * ctx->val.count, ctx->stream));
       launch_elem_hess(ctx->mesh(), ctx->a(SSIM_ARRAY_NEXT_DEFORM), 0.0, 0, ctx->hperm.p, ctx->hcontrib.p, false,
                        ctx->stream);
-      launch_hess_reduce(ctx->nub, ctx->hseg.p, ctx->hcontrib.p, ctx->ub_rc.p, ctx->ub_pos.p, ctx->dof.p, 0,
+      launch_hess_reduce(ctx->nub, ctx->hseg.p, ctx->hcontrib.p, ctx->ub_rc.p, ctx->ub_pos.p, ctx->dof.p, filtered,
                          ctx->massval.p, ctx->tmp.p, nullptr, ctx->stream);
       HIPCHK(hipGetLastError());
     }
     export_csr(ctx, ctx->massval.p, rowptr, colidx, values);
   });
+}
+}  // namespace
+
+ssim_status_t ssim_export_mass_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values) {
+  return export_mass(ctx, 0, rowptr, colidx, values);
+}
+
+ssim_status_t ssim_export_mass_filtered_csr(ssim_ctx* ctx, int* rowptr, int* colidx, double* values) {
+  return export_mass(ctx, 1, rowptr, colidx, values);
+}
+
+ssim_status_t ssim_get_rest_shape(ssim_ctx* ctx, double* dminv, double* rest_vol) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  return run([&] {
+    require_reset(ctx);
+    const size_t nt = ctx->nt;
+    if (dminv && nt) {
+      std::vector<double> soa(9 * nt);
+      HIPCHK(hipMemcpyAsync(soa.data(), ctx->B.p, sizeof(double) * 9 * nt, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      for (size_t e = 0; e < nt; ++e)
+        for (int k = 0; k < 9; ++k) dminv[9 * e + k] = soa[(size_t)k * nt + e];
+    }
+    if (rest_vol && nt) {
+      HIPCHK(hipMemcpyAsync(rest_vol, ctx->V.p, sizeof(double) * nt, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+  });
+}
+
+ssim_status_t ssim_device_mesh(const ssim_ctx* ctx, const double** vertices, const int** cells) {
+  if (!ctx || (!vertices && !cells)) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  if (vertices) *vertices = ctx->X.p;
+  if (cells) *cells = reinterpret_cast<const int*>(ctx->cells.p);
+  return SSIM_OK;
 }
 
 ssim_status_t ssim_eval_elements(ssim_ctx* ctx, const double* u, int make_spsd, double* psi, double* P, double* H,

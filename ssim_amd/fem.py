@@ -199,6 +199,17 @@ class TetFiniteElementSolver_Hip:
     def hessian(self):
         return self._csr("ssim_export_hessian_csr")
 
+    def mass_matrix_filtered(self):
+        """mass_matrix_filtered() (time_step.hpp:310-311): M with identity Dirichlet rows and columns."""
+        return self._csr("ssim_export_mass_filtered_csr")
+
+    def rest_shape(self):
+        """(dminv [nt,3,3], rest_vol [nt]): Dm⁻¹ and |det Dm|/6 per tet (time_step.hpp:296-301)."""
+        dminv = np.empty((self._nt, 3, 3))
+        vol = np.empty(self._nt)
+        _lib.call("ssim_get_rest_shape", self._h, _dp(dminv), _dp(vol))
+        return dminv, vol
+
     def pd_matrix(self):
         """The projective-dynamics matrix of the "lbfgs_pd" method (scipy CSR, the Hessian's pattern)."""
         return self._csr("ssim_export_pd_csr")

@@ -4,7 +4,10 @@
 //
 // Input file (little-endian): int32 nv, nt, steps, stepper (0 Newton, 1 L-BFGS, 2 NCG, 3 forward Euler); float64 dt, E, nu, rho, rtol,
 // gravity_y; float64 X[nv*3]; int32 cells[nt*4]; uint8 clamped[nv]; float64 u0[nv*3].
-// Output file: float64 u[nv*3] after the last step, then per step int32 newton_iterations, pcg_iterations.
+// Output file: float64 u[nv*3] after the last step, then per step int32 newton_iterations, pcg_iterations, then
+// float64 sums of absolute values over the remaining views: rest_vol, dminv, sysmatrix nnz and values, mass_matrix,
+// mass_matrix_filtered, dof_type, local_stiff.
+#include <cmath>
 #include <cstdio>
 #include <cstdint>
 #include <vector>
@@ -84,10 +87,23 @@ int main(int argc, char** argv) {
       threw = true;
     }
     if (!threw) throw std::runtime_error("set_poisson(0.5) did not throw");
+    // the remaining views of basic_time_step (time_step.hpp:275-311)
+    const ssim::hip::mesh_view mv = step.mesh();
+    if (mv.nv != nv || mv.nt != nt || !mv.vertices || !mv.cells) throw std::runtime_error("mesh() view mismatch");
+    auto sum = [](const auto& v) {
+      double t = 0;
+      for (auto x : v) t += std::fabs(static_cast<double>(x));
+      return t;
+    };
+    const ssim::hip::csr_matrix H = step.sysmatrix();
+    const double views[8] = {sum(step.rest_vol()), sum(step.dminv()), static_cast<double>(H.nnz()), sum(H.values),
+                             sum(step.mass_matrix().values), sum(step.mass_matrix_filtered().values),
+                             sum(step.dof_type()), sum(step.local_stiff())};
     FILE* out = std::fopen(argv[2], "wb");
     if (!out) throw std::runtime_error("cannot open output");
     std::fwrite(u.data(), sizeof(double), u.size(), out);
     std::fwrite(counts.data(), sizeof(int32_t), counts.size(), out);
+    std::fwrite(views, sizeof(double), 8, out);
     std::fclose(out);
     std::printf("driver: %d vertices, %d tets, %d steps done\n", nv, nt, steps);
   } catch (const std::exception& e) {

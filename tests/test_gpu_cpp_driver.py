@@ -1,6 +1,7 @@
 """The header-only C++ front end (include/ssim_hip.hpp) driven by a C++ program, against the Python front end
 on the same bar: identical positions (bitwise) and iteration counts after 5 steps of the 8×8×32 bar, for the
-Newton, L-BFGS, nonlinear-CG and forward-Euler steppers."""
+Newton, L-BFGS, nonlinear-CG and forward-Euler steppers; the remaining views of the header (mesh, dof_type, dminv,
+rest_vol, local_stiff, sysmatrix, mass_matrix, mass_matrix_filtered) agree with the Python front end's."""
 import os
 import subprocess
 
@@ -34,7 +35,9 @@ def test_cpp_driver_matches_python_front_end(tmp_path, stepper):
     assert res.returncode == 0, res.stderr
     raw = np.fromfile(out, dtype=np.uint8)
     u_cpp = raw[:8 * 3 * case.nv].view(np.float64)
-    counts = raw[8 * 3 * case.nv:].view(np.int32).reshape(steps, 2)
+    off = 8 * 3 * case.nv
+    counts = raw[off:off + 8 * steps].view(np.int32).reshape(steps, 2)
+    views = raw[off + 8 * steps:].view(np.float64)
 
     method = ["newton", "lbfgs", "ncg", "forward_euler"][stepper]
     s = fem.TetFiniteElementSolver_Hip(case.X, case.cells, dt, E, nu, rho, method)
@@ -51,3 +54,13 @@ def test_cpp_driver_matches_python_front_end(tmp_path, stepper):
         py_counts.append((st["newton_iterations"], st["pcg_iterations"]))
     assert np.array_equal(counts, np.array(py_counts, dtype=np.int32))
     assert np.array_equal(u_cpp, s.get_array("deformation"))
+    dminv, vol = s.rest_shape()
+    H = s.hessian()
+    K = s.eval_elements(s.get_array("next_deform"), make_spsd=True, want_H=False, want_K=True)["K"]
+    dof = np.zeros(3 * case.nv, dtype=np.uint8)
+    dof[np.repeat(clamped, 3)] = 1
+    a = lambda v: np.abs(v).sum()
+    expect = [a(vol), a(dminv), H.nnz, a(H.data), a(s.mass_matrix().data), a(s.mass_matrix_filtered().data), a(dof),
+              a(K)]
+    assert views.shape == (8,)
+    np.testing.assert_allclose(views, expect, rtol=1e-10, atol=0)  # sequential vs pairwise sums of ~3e5 terms

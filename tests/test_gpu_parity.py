@@ -9,6 +9,7 @@ import os
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import oracle
 from helpers import per_item_rel_err, rel_err
@@ -106,6 +107,27 @@ def test_energy_gradient_hessian_mass_match_oracle(bar):
     assert np.array_equal(M.indptr, Ho.indptr) and np.array_equal(M.indices, Ho.indices)
     assert rel_err(M.data, o.mass_values()) <= 1e-12
     assert abs(g.total_mass() - o.total_mass()) <= 1e-12 * o.total_mass()
+
+
+def test_rest_shape_and_filtered_mass_match(bar):
+    """dminv / rest_vol against numpy (Dm⁻¹ and |det Dm|/6, ΣV = the bar's volume) and mass_matrix_filtered against
+    the oracle's mass matrix with identity Dirichlet rows and columns (time_step.hpp:460-466)."""
+    g, o = _pair(bar)
+    dminv, vol = g.rest_shape()
+    X, C = bar.X, bar.cells
+    Dm = np.stack([X[C[:, 1]] - X[C[:, 0]], X[C[:, 2]] - X[C[:, 0]], X[C[:, 3]] - X[C[:, 0]]], axis=2)
+    assert rel_err(dminv.reshape(-1), np.linalg.inv(Dm).reshape(-1)) <= 1e-13
+    assert rel_err(vol, np.abs(np.linalg.det(Dm)) / 6) <= 1e-13
+    ext = X.max(axis=0) - X.min(axis=0)
+    assert abs(vol.sum() - ext.prod()) <= 1e-12 * ext.prod()
+    M = o.mass_csr().tocsr()
+    dof = bar.dof_type.astype(bool)
+    keep = sp.diags((~dof).astype(float))
+    Mf = (keep @ M @ keep + sp.diags(dof.astype(float))).tocsr()
+    Gf = g.mass_matrix_filtered()
+    assert np.array_equal(Gf.indptr, M.indptr) and np.array_equal(Gf.indices, M.indices)
+    assert rel_err(Gf.toarray().reshape(-1), Mf.toarray().reshape(-1)) <= 1e-12
+    assert np.array_equal(Gf.diagonal()[dof], np.ones(dof.sum()))
 
 
 def test_spmv_and_pcg_match_oracle(bar):
