@@ -338,3 +338,9 @@ class TetFiniteElementSolver_Hip:
             s.set_rtol(rtol)
         s.reset()
         return s
+
+
+# The reference's class names (pyssim/fem/__init__.py:11 and :97) map onto the one GPU class, so code written
+# against either runs unchanged; the "Host" name keeps its interface, and its arithmetic runs on the GPU.
+TetFiniteElementSolver_Cuda = TetFiniteElementSolver_Hip
+TetFiniteElementSolver_Host = TetFiniteElementSolver_Hip

@@ -92,3 +92,12 @@ def test_one_hip_runtime_per_process():
             "print(len({l.split()[-1] for l in maps if 'libamdhip64' in l}))\n") % ROOT
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True).stdout
     assert out.strip() == "1", out
+
+
+def test_reference_class_names_map_to_the_gpu_class():
+    """pyssim.fem's TetFiniteElementSolver_Host / _Cuda (pyssim/fem/__init__.py:11, 97) are the GPU class."""
+    from ssim_amd import fem
+
+    assert fem.TetFiniteElementSolver_Cuda is fem.TetFiniteElementSolver_Hip
+    assert fem.TetFiniteElementSolver_Host is fem.TetFiniteElementSolver_Hip
+    assert callable(fem.unit_box)
