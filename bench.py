@@ -56,6 +56,23 @@ def spmv_algorithmic_bytes(info):
     return 76 * upper + 8 * lower + 7 * n * 8
 
 
+def step_accounting(info, counts, elapsed):
+    """Bytes one time step must move by the per-step accounting sum over Newton iterations of
+    [K2 + N_pcg·(K3 + K4) + N_ls·K1] + K1, with K1 (energy + gradient) = 112·nt + 4·n·8 (per-tet geometry, cells
+    and (λ, μ); X, u, û read, g written), K2 (SPD Hessian assembly) = 112·nt + 2·n·8 + nnz·8 + n, K3 + K4 (one
+    PCG iteration) = the SpMV kernel's bytes + 5·n·8 for the update kernel, and the counts of the timed steps."""
+    n, nt = 3 * info["nv"], info["nt"]
+    nnz = 9 * info["nonzero_blocks"]
+    k1 = 112 * nt + 4 * n * 8
+    k2 = 112 * nt + 2 * n * 8 + nnz * 8 + n
+    k34 = spmv_algorithmic_bytes(info) + 5 * n * 8
+    total = counts["pcg_iterations"] * k34 + counts["hessian_evals"] * k2 + counts["energy_evals"] * k1
+    per_step = total / counts["steps"]
+    achieved = total / elapsed / 1e9
+    return {"bytes_per_step": per_step, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "k1": k1, "k2": k2, "k3_k4": k34}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "k_pcg_spmv_786k_traffic.json")
     if not os.path.exists(path):
@@ -519,6 +536,7 @@ def main():
             "kernel_avg_ms": {k: v[0] for k, v in ktimes.items()},
         },
         "roofline": roofline,
+        "step_accounting": step_accounting(info, counts, elapsed),
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adj,
