@@ -1,7 +1,7 @@
 """ctypes front end of the CPU FP64 oracle (oracle/ssim_oracle.c).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
-the checker.  The product (ssim_amd) never imports it.
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, which
+compare the HIP path against it or time it.  The product (ssim_amd) never imports it.
 """
 from __future__ import annotations
 

@@ -1,7 +1,7 @@
 /*
  * ssim_oracle.c -- CPU FP64 restatement of the reference FEM soft-body implicit time step.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never part of the product.  Only tests/,
+ * TEST INFRASTRUCTURE ONLY: what the HIP path is tested against, never part of the product.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it (through oracle/oracle.py).
  *
  * Citations are relative to the reference tree (Adversarr/ssim).  Conventions follow the reference:

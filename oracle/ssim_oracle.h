@@ -2,7 +2,7 @@
  * ssim_oracle.h -- CPU FP64 restatement of the reference FEM soft-body time step.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (ssim_amd/, include/) may link, load or call this
- * code.  It is the checker that the HIP path is compared against: tests/ and bench.py's cpu_baseline
+ * code.  The tests compare the HIP path against it; tests/ and bench.py's cpu_baseline
  * leg load it through ctypes (oracle/oracle.py).
  *
  * Every routine restates one piece of the reference (citations are relative to the reference tree,

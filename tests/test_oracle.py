@@ -1,4 +1,4 @@
-"""Pins the CPU FP64 oracle (oracle/ssim_oracle.c) before it is trusted as the checker of the HIP path.
+"""Pins the CPU FP64 oracle (oracle/ssim_oracle.c) before the HIP path is tested against it.
 
 Each check is independent of the oracle's own formulas: reference known-answer tests, torch autograd fixtures
 (tests/golden/snh_autograd.json), numpy eigh, central finite differences, closed forms, dense brute force and
