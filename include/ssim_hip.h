@@ -115,6 +115,10 @@ ssim_status_t ssim_set_damping(ssim_ctx* ctx, double damping);
 /* set_threshold (time_step.hpp:343-347) / set_rtol (bindings/src/fem_host.cpp:146); like the reference, the
  * absolute tolerance rtol·mass is recomputed at the next reset */
 ssim_status_t ssim_set_threshold(ssim_ctx* ctx, double rtol);
+/* the getters time_step(), youngs(), poisson(), density(), damping() and grad_convergence_threshold()
+ * (time_step.hpp:312, 349-353): the values last set; any output may be NULL */
+ssim_status_t ssim_get_physical_params(const ssim_ctx* ctx, double* time_step, double* youngs, double* poisson,
+                                       double* density, double* damping, double* threshold);
 /* per-tet Lamé parameters (extension: the reference's per-call (λ, μ) overloads,
  * elast/basic_elast.hpp:187-194); NULL restores the uniform (E, ν) material */
 ssim_status_t ssim_set_material_per_tet(ssim_ctx* ctx, const double* lambda, const double* mu);

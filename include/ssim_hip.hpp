@@ -163,6 +163,18 @@ class time_step {
   time_step& set_matrix(const double* values, long long nnz) { return chk(ssim_set_precond_matrix(ctx_, values, nnz)); }
   // the ElastModel template argument of basic_time_step, chosen at run time
   time_step& set_material(ssim_material_t model) { return chk(ssim_set_material_model(ctx_, model)); }
+  // getters (time_step.hpp:312-313, 349-353, 361); the reference's time_step() is time_step_size() here, since
+  // a member cannot take the class's name
+  double time_step_size() const { return param(0); }
+  double youngs() const { return param(1); }
+  double poisson() const { return param(2); }
+  double density() const { return param(3); }
+  double damping() const { return param(4); }
+  double grad_convergence_threshold() const { return param(5); }
+  long long total_dofs() const { return 3LL * num_vertices(); }
+  // update_deformation_gradient() (time_step.hpp:571-575): F is formed inside every element kernel from the
+  // current next_deform, so there is nothing to precompute; kept so that reference call sites compile
+  void update_deformation_gradient() const noexcept {}
   ssim_solver_params solver_params() const {
     ssim_solver_params p{};
     check(ssim_get_solver_params(ctx_, &p));
@@ -262,6 +274,11 @@ class time_step {
     m.values.resize(static_cast<size_t>(nnz));
     check(exporter(ctx_, m.outer_ptrs.data(), m.inner_indices.data(), m.values.data()));
     return m;
+  }
+  double param(int which) const {
+    double v[6] = {};
+    check(ssim_get_physical_params(ctx_, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]));
+    return v[which];
   }
   time_step& chk(ssim_status_t st) {
     check(st);

@@ -102,6 +102,7 @@ SIGNATURES = {
     "ssim_export_mass_csr": [_V, _I, _I, _D],
     "ssim_export_mass_filtered_csr": [_V, _I, _I, _D],
     "ssim_get_rest_shape": [_V, _D, _D],
+    "ssim_get_physical_params": [_V, _D, _D, _D, _D, _D, _D],
     "ssim_device_mesh": [_V, ctypes.POINTER(_D), ctypes.POINTER(_I)],
     "ssim_eval_elements": [_V, _D, ctypes.c_int, _D, _D, _D, _D],
     "ssim_pcg_solve": [_V, _D, _D, ctypes.c_double, ctypes.c_int, _I],

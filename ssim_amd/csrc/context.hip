@@ -1172,6 +1172,20 @@ ssim_status_t ssim_set_threshold(ssim_ctx* ctx, double rtol) {
     ctx->rtol = rtol;
   });
 }
+ssim_status_t ssim_get_physical_params(const ssim_ctx* ctx, double* time_step, double* youngs, double* poisson,
+                                       double* density, double* damping, double* threshold) {
+  if (!ctx) {
+    g_err = "null context";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  if (time_step) *time_step = ctx->dt;
+  if (youngs) *youngs = ctx->youngs;
+  if (poisson) *poisson = ctx->poisson;
+  if (density) *density = ctx->density;
+  if (damping) *damping = ctx->damping;
+  if (threshold) *threshold = ctx->rtol;
+  return SSIM_OK;
+}
 ssim_status_t ssim_set_material_per_tet(ssim_ctx* ctx, const double* lambda, const double* mu) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
   return run([&] {

@@ -87,6 +87,12 @@ int main(int argc, char** argv) {
       threw = true;
     }
     if (!threw) throw std::runtime_error("set_poisson(0.5) did not throw");
+    // the getters return what was set (time_step.hpp:349-353)
+    if (step.time_step_size() != prm[0] || step.youngs() != prm[1] || step.poisson() != prm[2] ||
+        step.density() != prm[3] || step.grad_convergence_threshold() != prm[4] || step.damping() != 0.0 ||
+        step.total_dofs() != 3LL * nv)
+      throw std::runtime_error("getter mismatch");
+    step.update_deformation_gradient();
     // the remaining views of basic_time_step (time_step.hpp:275-311)
     const ssim::hip::mesh_view mv = step.mesh();
     if (mv.nv != nv || mv.nt != nt || !mv.vertices || !mv.cells) throw std::runtime_error("mesh() view mismatch");
