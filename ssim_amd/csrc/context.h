@@ -236,6 +236,23 @@ void timed_ext(ssim_ctx* c, ssim_kernel_t k, F&& launch) {
 
 void do_reset(ssim_ctx* c);
 void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_dbc, double dt2, double* values);
+void check_material(double youngs, double poisson, double density);  // time_step.hpp:369-374
+void require_reset(ssim_ctx* c);
+void upload_material(ssim_ctx* c);
+
+// stepper drivers (solvers.hip)
+struct EvalOut {
+  double E, gg;
+};
+EvalOut eval_energy_grad(ssim_ctx* c, const double* u, double* g);
+int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit,
+              const double* val = nullptr, const double* dinv = nullptr);
+void newton_solve(ssim_ctx* c);
+void lbfgs_solve(ssim_ctx* c, bool pd, bool h0 = false);
+void ncg_solve(ssim_ctx* c, bool precond);
+void forward_euler_solve(ssim_ctx* c);
+void set_precond(ssim_ctx* c, const double* values);
+void build_pd(ssim_ctx* c);
 template <typename F>
 ssim_status_t run(F&& f) {
   try {

@@ -1,0 +1,618 @@
+// solvers.hip -- the host drivers of the steppers: energy and gradient evaluation, Hessian assembly, Jacobi-PCG
+// in chunks of device-resident iterations, Newton with backtracking (stepper/backward_euler.hpp:9-41), L-BFGS in
+// the Gram and the vector-by-vector form with the diagonal, projective-dynamics and Hessian-at-start
+// preconditioners (backward_euler.hpp:43-67, stepper/projective_dynamics.hpp:9-150), nonlinear CG with an
+// optional external preconditioner (stepper/nonlinear_cg.hpp:9-97) and forward Euler (time_step.hpp:717-736).
+// The host decides control flow only; the arithmetic runs in the kernels of kernels.hip.
+#include "context.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace ssim_hip;
+
+namespace ssim_hip {
+
+// update_energy_and_gradients at deformation u → gradient into g; returns E and ‖g‖²
+EvalOut eval_energy_grad(ssim_ctx* c, const double* u, double* g) {
+  hipStream_t s = c->stream;
+  const double dt2 = c->dt * c->dt;
+  int ne = 0, ng = 0;
+  timed(c, SSIM_KERNEL_ELEM_GRAD, [&] {
+    ne = launch_elem_grad(c->mesh(), u, c->a(SSIM_ARRAY_INERTIA), dt2, c->gperm.p, c->gcontrib.p, c->part_e(),
+                          c->prm.fp32_elements != 0, s);
+  });
+  timed(c, SSIM_KERNEL_GRAD_REDUCE, [&] {
+    ng = launch_grad_reduce(c->nv, c->gseg.p, c->gcontrib.p, c->dof.p, g, c->part_g(), s);
+  });
+  launch_finalize(c->part_e(), ne, 1, c->part_g(), ng, &c->ev.p->energy, s);  // energy, gg
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->pin, c->ev.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  sync(c);
+  c->stats.energy_evals++;
+  return EvalOut{c->pin[0], c->pin[1]};
+}
+
+static double device_dot(ssim_ctx* c, const double* a, const double* b) {
+  launch_dot(c->n, a, b, c->part_d(), &c->ev.p->dot, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->pin, &c->ev.p->dot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  return c->pin[0];
+}
+
+
+// update_hessian at deformation u into the blocked-ELL values and the Jacobi diagonal
+void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_dbc, double dt2,
+                                double* values) {
+  hipStream_t s = c->stream;
+  timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
+    launch_elem_hess(c->mesh(), u, dt2, make_spsd, c->hperm.p, c->hcontrib.p, c->prm.fp32_elements != 0, s);
+  });
+  timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
+    launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, filter_dbc, values,
+                       c->dinv.p, nullptr, s);
+  });
+  HIPCHK(hipGetLastError());
+  c->stats.hessian_evals++;
+}
+
+
+// Enqueues `count` PCG iterations (kernels A and B) on stream s.
+static void enqueue_pcg_iterations(ssim_ctx* c, const BellView& A, const double* dinv, double* x, int nb, int na, int count,
+                            hipStream_t s) {
+  for (int j = 0; j < count; ++j) {
+    launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
+                    c->part_pa(), x, s);
+    launch_pcg_update(c->n, dinv, c->r.p, c->z.p, c->q.p, c->part_pa(), na, c->part_pb(0),
+                      c->part_pb(1), c->pcg_it.p, c->pcg.p, s);
+  }
+}
+
+// The graph of `count` iterations on matrix A writing x, captured on first use (buffers are fixed until the
+// next reset, which drops the cache).
+static hipGraphExec_t pcg_chunk_graph(ssim_ctx* c, const BellView& A, const double* dinv, double* x, int nb, int na,
+                               int count) {
+  for (const auto& g : c->pcg_graphs)
+    if (g.count == count && g.x == x && g.val == A.val) return g.exec;
+  if (!c->cap_stream) HIPCHK(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+  hipGraph_t graph = nullptr;
+  try {
+    enqueue_pcg_iterations(c, A, dinv, x, nb, na, count, c->cap_stream);
+  } catch (...) {
+    (void)hipStreamEndCapture(c->cap_stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(c->cap_stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) fail(SSIM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  c->pcg_graphs.push_back({count, x, A.val, exec});
+  return exec;
+}
+
+// Jacobi-PCG: x = A⁻¹ (sign·b) for the blocked-ELL values `val` with inverse diagonal `dinv` (default: the
+// current Hessian).  Returns iterations.
+int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit, const double* val,
+              const double* dinv) {
+  hipStream_t s = c->stream;
+  if (!val) val = c->val.p;
+  if (!dinv) dinv = c->dinv.p;
+  const BellView A = c->bell(val);
+  const int nb = launch_pcg_init(c->n, b, sign, dinv, x, c->r.p, c->z.p, rtol, maxit, c->pcg.p, c->pcg_it.p,
+                                 c->part_pb(0), s);
+  HIPCHK(hipGetLastError());
+  for (auto& e : c->poll_ev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const int na = pcg_spmv_grid(A);
+  // Chunks of iterations are enqueued one ahead of the done-flag poll, so the GPU never drains while the host
+  // reads a flag; kernels enqueued after convergence return at entry.  The first iteration of a chunk is launched
+  // on its own (it carries the sampled kernel timers); the others are replayed from a captured graph.
+  int* flags = reinterpret_cast<int*>(c->pin);  // two slots of 4 ints: done, iters, maxit, breakdown
+  auto enqueue = [&](int count, int slot) {
+    timed_ext(c, SSIM_KERNEL_PCG_SPMV, [&](hipEvent_t e0, hipEvent_t e1) {
+      launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
+                      c->part_pa(), x, s, e0, e1);
+    });
+    timed_ext(c, SSIM_KERNEL_PCG_UPDATE, [&](hipEvent_t e0, hipEvent_t e1) {
+      launch_pcg_update(c->n, dinv, c->r.p, c->z.p, c->q.p, c->part_pa(), na, c->part_pb(0),
+                        c->part_pb(1), c->pcg_it.p, c->pcg.p, s, e0, e1);
+    });
+    if (count > 1) {
+      if (c->pcg_graph)
+        HIPCHK(hipGraphLaunch(pcg_chunk_graph(c, A, dinv, x, nb, na, count - 1), s));
+      else
+        enqueue_pcg_iterations(c, A, dinv, x, nb, na, count - 1, s);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(flags + 4 * slot, &c->pcg.p->done, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(c->poll_ev[slot], s));
+  };
+  int chunk = 8, launched = 0, slot = 0;
+  enqueue(chunk, slot);
+  launched += chunk;
+  for (;;) {
+    const int next = std::min(chunk * 2, std::max(8, c->pcg_chunk_max));
+    enqueue(next, slot ^ 1);
+    launched += next;
+    HIPCHK(hipEventSynchronize(c->poll_ev[slot]));
+    if (c->timing) harvest_timers(c);
+    const int* f = flags + 4 * slot;
+    if (f[0]) {
+      if (f[3] == 1) fail(SSIM_ERR_NUMERIC, "PCG breakdown: pᵀAp ≤ 0 (matrix not positive definite)");
+      if (f[3] == 2) fail(SSIM_ERR_NUMERIC, "PCG breakdown: NaN residual");
+      return f[1];
+    }
+    if (launched > maxit + 2 * std::max(8, c->pcg_chunk_max) + 16) fail(SSIM_ERR_STATE, "internal: PCG did not stop");
+    chunk = next;
+    slot ^= 1;
+  }
+}
+
+// Newton with backtracking Armijo line search: stepper/backward_euler.hpp:15-33 (tol = threshold·mass,
+// Armijo 0.3) with the iteration cap of the benchmark configuration.
+void newton_solve(ssim_ctx* c) {
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* g = c->a(SSIM_ARRAY_FORCES);
+  EvalOut cur = eval_energy_grad(c, u, g);
+  int it = 0;
+  for (; it < c->prm.max_newton; ++it) {
+    if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+    if (std::sqrt(cur.gg) < c->tol_abs) {
+      c->stats.converged = 1;
+      break;
+    }
+    assemble_hessian(c, u, 1, 1, c->dt * c->dt, c->val.p);
+    c->stats.pcg_iterations += pcg_solve(c, g, -1.0, c->dir.p, c->prm.pcg_rtol, c->prm.pcg_maxit);
+    const double gd = device_dot(c, g, c->dir.p);
+    double alpha = 1.0;
+    EvalOut trial{0, 0};
+    bool ok = false;
+    for (int ls = 0; ls < c->prm.ls_max; ++ls) {
+      launch_axpy_to(n, u, alpha, c->dir.p, c->utry.p, s);
+      trial = eval_energy_grad(c, c->utry.p, c->gtry.p);
+      if (trial.E <= cur.E + c->prm.ls_c * alpha * gd) {
+        ok = true;
+        break;
+      }
+      alpha *= c->prm.ls_beta;
+    }
+    if (!ok) c->stats.ls_failures++;
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    cur = trial;
+  }
+  c->stats.newton_iterations = it;
+  // polish: full Newton steps with the unprojected Hessian (positive definite at the converged minimum, so the
+  // steps converge quadratically), no line search
+  for (int k = 0; k < c->prm.newton_polish; ++k) {
+    assemble_hessian(c, u, 0, 1, c->dt * c->dt, c->val.p);
+    c->stats.pcg_iterations += pcg_solve(c, g, -1.0, c->dir.p, c->prm.pcg_rtol, c->prm.pcg_maxit);
+    launch_axpy_to(n, u, 1.0, c->dir.p, c->utry.p, s);
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    eval_energy_grad(c, u, g);
+  }
+}
+
+// Projective-dynamics matrix of time_step_solver_pd::reset_impl (stepper/projective_dynamics.hpp:51-87):
+// M + dt²·κ·K on every dof, K the P1 Laplacian, Dirichlet rows and columns set to the identity, with κ the
+// approximated diffusivity of the reference's uniform_elasticity(), which builds the material from (E, ν) in the
+// (λ, μ) slots (finite_elements/time_step.hpp:356): for stable Neo-Hookean λ + 5μ/6 + 2·(4μ/3)
+// (elast/stable_neohookean.hpp:109-114), κ = E + 3.5ν; for linear elasticity λ + 2μ (elast/basic_elast.hpp:225),
+// κ = E + 2ν.
+static double pd_diffusivity(ssim_material_t model, double lam, double mu) {
+  if (model == SSIM_MATERIAL_LINEAR) return lam + 2.0 * mu;
+  return (lam + 5.0 / 6.0 * mu) + 2.0 * (4.0 / 3.0 * mu);
+}
+
+void build_pd(ssim_ctx* c) {
+  hipStream_t s = c->stream;
+  c->pd_val.alloc(c->val.count);
+  c->pd_dinv.alloc(std::max(c->n, 1));
+  c->pd_q.alloc(std::max(c->n, 1));
+  HIPCHK(hipMemsetAsync(c->pd_val.p, 0, sizeof(double) * c->pd_val.count, s));  // padding slots stay zero
+  const double kappa = pd_diffusivity(c->model, c->youngs, c->poisson);
+  launch_elem_pd(c->mesh(), c->dt * c->dt * kappa, c->hperm.p, c->hcontrib.p, s);
+  launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->pd_val.p,
+                     c->pd_dinv.p, nullptr, s);
+  HIPCHK(hipGetLastError());
+  sync(c);
+}
+
+// H0 of the L-BFGS stepper: diagonal of the SPD-projected, Dirichlet-filtered Hessian at rest (u = 0)
+static void ensure_hrest(ssim_ctx* c) {
+  if (c->hrest.p) return;
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  c->hrest.alloc(n);
+  HIPCHK(hipMemsetAsync(c->tmp.p, 0, sizeof(double) * n, s));
+  const double dt2 = c->dt * c->dt;
+  launch_elem_hess(c->mesh(), c->tmp.p, dt2, 1, c->hperm.p, c->hcontrib.p, c->prm.fp32_elements != 0, s);
+  launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->val.p, c->dinv.p,
+                     c->hrest.p, s);
+  HIPCHK(hipGetLastError());
+}
+
+// The same L-BFGS iteration with the two-loop recursion carried out on Gram matrices of the stored pairs (the
+// vector-free formulation of Chen, Wang, Zhou & Luo, NIPS 2014, with a diagonal H0 = D): with SY_ij = s_iᵀy_j,
+// YDY_ij = y_iᵀDy_j, SG_i = s_iᵀg, YDG_i = y_iᵀDg,
+//   a_i = ρ_i (SG_i − Σ_{j newer} a_j SY_ij)                                  (newest to oldest)
+//   b_i = ρ_i (YDG_i − Σ_j a_j YDY_ij + Σ_{j older} c_j SY_ji),  c_j = a_j − b_j   (oldest to newest)
+//   r = D(g − Σ_j a_j y_j) + Σ_j c_j s_j,    gᵀr = gᵀDg − Σ_j a_j YDG_j + Σ_j c_j SG_j,
+// which is the recursion of lbfgs_solve term by term.  Per iteration one pass over the stored vectors computes
+// the new Gram entries (s_jᵀg, y_jᵀDg, gᵀDg and the newest pair's row and column), the host runs the scalar
+// recursion, and one pass forms r.  The 2·kLbfgsSlots history vectors are zeroed once, so unused slots hold finite
+// values.  If a line search fails, the rest of the solve computes the directions vector by vector on the same
+// history (the recursion of lbfgs_solve).
+static void lbfgs_solve_gram(ssim_ctx* c) {
+  const int n = c->n, m = std::max(1, c->prm.lbfgs_m);
+  constexpr int M = kLbfgsSlots;
+  hipStream_t s = c->stream;
+  if (c->lb_S.count < (size_t)M * n || c->lb_Y.count < (size_t)M * n) {
+    c->lb_S.alloc((size_t)M * n);
+    c->lb_Y.alloc((size_t)M * n);
+    HIPCHK(hipMemsetAsync(c->lb_S.p, 0, sizeof(double) * M * (size_t)n, s));
+    HIPCHK(hipMemsetAsync(c->lb_Y.p, 0, sizeof(double) * M * (size_t)n, s));
+  }
+  c->lb_part.alloc((size_t)kLbfgsDots * lbfgs_gram_blocks(n));
+  c->lb_scal.alloc(std::max<size_t>(c->lb_scal.count, kLbfgsDots + 2 * M + 8));
+  double* av = c->lb_scal.p + kLbfgsDots;  // device two-loop coefficients of the vector-by-vector recursion
+  double* bv = av + M;
+  bool vector_mode = false;
+  std::vector<double> SY((size_t)M * M, 0.0), YDY((size_t)M * M, 0.0), rho(M, 0.0), gram(kLbfgsDots);
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* g = c->a(SSIM_ARRAY_FORCES);
+  double* d = c->dir.p;  // r ≈ H⁻¹g; the search direction is −r
+  EvalOut cur = eval_energy_grad(c, u, g);
+  int it = 0, hist = 0, head = 0, fresh = -1;  // fresh: slot of the pair written by the last line search
+  auto accept = [&](double ys) {
+    if (!(ys > 1e-300)) return false;  // as lbfgs_solve: a pair with yᵀs ≤ 0 is dropped
+    rho[fresh] = 1.0 / ys;
+    head = (head + 1) % m;
+    if (hist < m) hist++;
+    return true;
+  };
+  for (; it < c->prm.lbfgs_maxit; ++it) {
+    if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+    if (std::sqrt(cur.gg) < c->tol_abs) {
+      c->stats.converged = 1;
+      break;
+    }
+    double gd = 0.0;
+    if (vector_mode) {
+      if (fresh >= 0)
+        accept(device_dot(c, c->lb_Y.p + (size_t)fresh * n, c->lb_S.p + (size_t)fresh * n));
+      fresh = -1;
+      HIPCHK(hipMemcpyAsync(c->q.p, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      for (int k = 0; k < hist; ++k) {
+        const int idx = (head - 1 - k + m) % m;
+        launch_dot(n, c->lb_S.p + (size_t)idx * n, c->q.p, c->part_d(), av + idx, s);
+        launch_axpy_coef(n, -rho[idx], av + idx, 0.0, nullptr, c->lb_Y.p + (size_t)idx * n, c->q.p, s);
+      }
+      launch_scale_div(n, c->q.p, c->hrest.p, d, s);
+      for (int k = hist - 1; k >= 0; --k) {
+        const int idx = (head - 1 - k + m) % m;
+        launch_dot(n, c->lb_Y.p + (size_t)idx * n, d, c->part_d(), bv + idx, s);
+        launch_axpy_coef(n, rho[idx], av + idx, rho[idx], bv + idx, c->lb_S.p + (size_t)idx * n, d, s);
+      }
+      HIPCHK(hipGetLastError());
+      gd = -device_dot(c, g, d);
+    } else {
+      launch_lbfgs_gram(n, c->lb_S.p, c->lb_Y.p, g, c->hrest.p, fresh, c->lb_part.p, c->lb_scal.p, s);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(gram.data(), c->lb_scal.p, sizeof(double) * kLbfgsDots, hipMemcpyDeviceToHost, s));
+      sync(c);
+      const double* SG = gram.data();
+      const double* YDG = SG + M;
+      const double GDG = SG[2 * M];
+      if (fresh >= 0) {
+        const double* SYk = SG + 2 * M + 1;  // s_jᵀy_k
+        const double* SkY = SG + 3 * M + 1;  // s_kᵀy_j
+        const double* YDYk = SG + 4 * M + 1;
+        for (int j = 0; j < M; ++j) {  // row and column of the slot; used only if the pair is kept
+          SY[(size_t)j * M + fresh] = SYk[j];
+          SY[(size_t)fresh * M + j] = SkY[j];
+          YDY[(size_t)fresh * M + j] = YDY[(size_t)j * M + fresh] = YDYk[j];
+        }
+        accept(SkY[fresh]);
+        fresh = -1;
+      }
+      // scalar two-loop recursion over the history, slot of the k-th newest pair: (head − 1 − k) mod m
+      LbfgsCoef cf{};
+      double a[M] = {}, b[M] = {};
+      for (int k = 0; k < hist; ++k) {
+        const int i = (head - 1 - k + m) % m;
+        double t = SG[i];
+        for (int q = 0; q < k; ++q) {
+          const int j = (head - 1 - q + m) % m;
+          t -= a[j] * SY[(size_t)i * M + j];
+        }
+        a[i] = rho[i] * t;
+      }
+      for (int k = hist - 1; k >= 0; --k) {
+        const int i = (head - 1 - k + m) % m;
+        double t = YDG[i];
+        for (int q = 0; q < hist; ++q) {
+          const int j = (head - 1 - q + m) % m;
+          t -= a[j] * YDY[(size_t)i * M + j];
+        }
+        for (int q = k + 1; q < hist; ++q) {  // older pairs, already updated
+          const int j = (head - 1 - q + m) % m;
+          t += (a[j] - b[j]) * SY[(size_t)j * M + i];
+        }
+        b[i] = rho[i] * t;
+      }
+      double gr = GDG;
+      for (int k = 0; k < hist; ++k) {
+        const int i = (head - 1 - k + m) % m;
+        cf.a[i] = a[i];
+        cf.c[i] = a[i] - b[i];
+        cf.mask |= 1u << i;
+        gr += -a[i] * YDG[i] + (a[i] - b[i]) * SG[i];
+      }
+      launch_lbfgs_dir(n, c->lb_S.p, c->lb_Y.p, g, c->hrest.p, cf, d, s);
+      HIPCHK(hipGetLastError());
+      gd = -gr;
+    }
+    double alpha = 1.0;
+    EvalOut trial{0, 0};
+    bool ok = false;
+    for (int ls = 0; ls < c->prm.lbfgs_ls_max; ++ls) {
+      launch_axpy_to(n, u, -alpha, d, c->utry.p, s);
+      trial = eval_energy_grad(c, c->utry.p, c->gtry.p);
+      if (trial.E <= cur.E + c->prm.lbfgs_c * alpha * gd) {
+        ok = true;
+        break;
+      }
+      alpha *= c->prm.ls_beta;
+    }
+    if (!ok) {
+      c->stats.ls_failures++;
+      vector_mode = true;
+    }
+    launch_lbfgs_shift(n, c->utry.p, c->gtry.p, u, g, c->lb_S.p + (size_t)head * n, c->lb_Y.p + (size_t)head * n, s);
+    HIPCHK(hipGetLastError());
+    fresh = head;
+    cur = trial;
+  }
+  c->stats.newton_iterations = it;
+}
+
+// L-BFGS two-loop recursion (Nocedal & Wright, Alg. 7.4) with H0 = diag(H_rest)⁻¹ and backtracking Armijo;
+// stepper/backward_euler.hpp:48-60 (tol = threshold·mass).  The two-loop coefficients stay on the device;
+// ρ_i = 1/(y_iᵀs_i) is read back once per accepted pair.  With `pd` the middle step solves with the
+// projective-dynamics matrix instead (time_step_solver_pd, stepper/projective_dynamics.hpp:31-50: a direct solve
+// in the reference, Jacobi-PCG to a relative residual of kPdRtol here).
+// With `h0` it solves with the SPD-projected, Dirichlet-filtered Hessian at the step's starting point instead
+// (time_step_solver_h0, stepper/projective_dynamics.hpp:91-148: update_hessian(false, true) and a factorisation per
+// step; the reference binds no Python class for it).
+constexpr double kPdRtol = 1e-12;
+void lbfgs_solve(ssim_ctx* c, bool pd, bool h0) {
+  const int n = c->n, m = std::max(1, c->prm.lbfgs_m);
+  hipStream_t s = c->stream;
+  if (pd && !c->pd_val.p) build_pd(c);
+  if (h0) {
+    c->pd_q.alloc(std::max(n, 1));
+    assemble_hessian(c, c->a(SSIM_ARRAY_NEXT_DEFORM), 1, 1, c->dt * c->dt, c->val.p);
+  }
+  const bool solve = pd || h0;
+  double* qv = solve ? c->pd_q.p : c->q.p;  // the solve's PCG uses c->q itself
+  if (!solve) ensure_hrest(c);
+  if (!solve && c->lbfgs_gram && m <= kLbfgsSlots) {
+    lbfgs_solve_gram(c);
+    return;
+  }
+  c->lb_S.alloc((size_t)m * n);
+  c->lb_Y.alloc((size_t)m * n);
+  c->lb_scal.alloc(2 * (size_t)m + 8);  // a[m], b[m]
+  double* av = c->lb_scal.p;
+  double* bv = av + m;
+  std::vector<double> rho_h(m, 0.0);
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* g = c->a(SSIM_ARRAY_FORCES);
+  double* d = c->dir.p;  // holds r ≈ H⁻¹g; the search direction is −r
+  EvalOut cur = eval_energy_grad(c, u, g);
+  int it = 0, hist = 0, head = 0;
+  for (; it < c->prm.lbfgs_maxit; ++it) {
+    if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+    if (std::sqrt(cur.gg) < c->tol_abs) {
+      c->stats.converged = 1;
+      break;
+    }
+    HIPCHK(hipMemcpyAsync(qv, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    for (int k = 0; k < hist; ++k) {
+      const int idx = (head - 1 - k + m) % m;
+      // a_i = ρ_i s_iᵀq ; q −= a_i y_i
+      launch_dot(n, c->lb_S.p + (size_t)idx * n, qv, c->part_d(), av + idx, s);
+      launch_axpy_coef(n, -rho_h[idx], av + idx, 0.0, nullptr, c->lb_Y.p + (size_t)idx * n, qv, s);
+    }
+    if (pd)  // r = A_pd⁻¹ q
+      c->stats.pcg_iterations += pcg_solve(c, qv, 1.0, d, kPdRtol, c->prm.pcg_maxit, c->pd_val.p, c->pd_dinv.p);
+    else if (h0)  // r = H(u_start)⁻¹ q
+      c->stats.pcg_iterations += pcg_solve(c, qv, 1.0, d, kPdRtol, c->prm.pcg_maxit);
+    else
+      launch_scale_div(n, qv, c->hrest.p, d, s);  // r = H0 q
+    for (int k = hist - 1; k >= 0; --k) {
+      const int idx = (head - 1 - k + m) % m;
+      // b = ρ_i y_iᵀr ; r += s_i (a_i − b)
+      launch_dot(n, c->lb_Y.p + (size_t)idx * n, d, c->part_d(), bv + idx, s);
+      launch_axpy_coef(n, rho_h[idx], av + idx, rho_h[idx], bv + idx, c->lb_S.p + (size_t)idx * n, d, s);
+    }
+    HIPCHK(hipGetLastError());
+    const double gd = -device_dot(c, g, d);
+    double alpha = 1.0;
+    EvalOut trial{0, 0};
+    bool ok = false;
+    for (int ls = 0; ls < c->prm.lbfgs_ls_max; ++ls) {
+      launch_axpy_to(n, u, -alpha, d, c->utry.p, s);
+      trial = eval_energy_grad(c, c->utry.p, c->gtry.p);
+      if (trial.E <= cur.E + c->prm.lbfgs_c * alpha * gd) {
+        ok = true;
+        break;
+      }
+      alpha *= c->prm.ls_beta;
+    }
+    if (!ok) c->stats.ls_failures++;
+    double* sv = c->lb_S.p + (size_t)head * n;
+    double* yv = c->lb_Y.p + (size_t)head * n;
+    launch_sub(n, c->utry.p, u, sv, s);
+    launch_sub(n, c->gtry.p, g, yv, s);
+    const double ys = device_dot(c, yv, sv);
+    if (ys > 1e-300) {
+      rho_h[head] = 1.0 / ys;
+      head = (head + 1) % m;
+      if (hist < m) hist++;
+    }
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    cur = trial;
+  }
+  c->stats.newton_iterations = it;
+}
+
+// The preconditioner factor L of the externally preconditioned NCG lives on the block adjacency (the system
+// matrix's pattern, fem_cuda.cu:175-201) as block CSR, with tpos[k] the index of the mirrored block (w, v) of
+// block k = (v, w) for the transposed product.  `values` null: the identity.
+void set_precond(ssim_ctx* c, const double* values) {
+  const int nv = c->nv;
+  const long long nb = c->brow[nv];
+  if (!c->pre_brow.p) {
+    std::vector<int> tpos(std::max<long long>(nb, 1));
+    for (int v = 0; v < nv; ++v)
+      for (int k = c->brow[v]; k < c->brow[v + 1]; ++k) {
+        const int w = c->bcol[k];
+        const int* b = c->bcol.data() + c->brow[w];
+        const int* e = c->bcol.data() + c->brow[w + 1];
+        tpos[k] = c->brow[w] + (int)(std::lower_bound(b, e, v) - b);
+      }
+    c->pre_brow.upload(c->brow.data(), nv + 1, c->stream);
+    c->pre_bcol.upload(c->bcol.data(), std::max<long long>(nb, 1), c->stream);
+    c->pre_tpos.upload(tpos, c->stream);
+    for (ssim_hip::DBuf<double>* b : {&c->pre_a, &c->pre_b, &c->pre_pg}) b->alloc(std::max(c->n, 1));
+    sync(c);
+  }
+  // user CSR order: scalar row 3v+b holds, for each neighbour k of v in column order, the entries (b, 0..2);
+  // it starts at 9·brow[v] + 3·b·len(v)
+  std::vector<double> hv(9 * (size_t)std::max<long long>(nb, 1), 0.0);
+  for (int v = 0; v < nv; ++v) {
+    const int len = c->brow[v + 1] - c->brow[v];
+    for (int k = 0; k < len; ++k) {
+      double* B = hv.data() + 9 * (size_t)(c->brow[v] + k);
+      if (!values) {
+        if (c->bcol[c->brow[v] + k] == v) B[0] = B[4] = B[8] = 1.0;
+        continue;
+      }
+      for (int b = 0; b < 3; ++b)
+        for (int d = 0; d < 3; ++d) B[3 * b + d] = values[9 * (size_t)c->brow[v] + 3 * (size_t)b * len + 3 * k + d];
+    }
+  }
+  c->pre_val.upload(hv, c->stream);
+  sync(c);
+}
+
+// z = L·(Lᵀ·g): a = Lᵀ g, out = L a
+static void apply_precond(ssim_ctx* c, const double* g, double* a, double* out) {
+  launch_bsr_spmv(c->nv, c->pre_brow.p, c->pre_bcol.p, c->pre_tpos.p, c->pre_val.p, g, a, true, c->stream);
+  launch_bsr_spmv(c->nv, c->pre_brow.p, c->pre_bcol.p, c->pre_tpos.p, c->pre_val.p, a, out, false, c->stream);
+  HIPCHK(hipGetLastError());
+}
+
+// Nonlinear CG (stepper/nonlinear_cg.hpp:9-37: Dai-Kou strategy, learning rate 1e-2, backtracking line search,
+// tol = threshold·mass), the published DK+ method: β = max(gₖ₊₁ᵀy/dᵀy − ‖y‖²·gₖ₊₁ᵀd/(dᵀy)², ½·gₖ₊₁ᵀd/‖d‖²),
+// d = −gₖ₊₁ + βd, restarts with d = −g when dᵀy ≤ 0 or d is not a descent direction; backtracking Armijo with the
+// L-BFGS constants from a first trial of 1e-2, then αₖ₋₁·gₖ₋₁ᵀdₖ₋₁/gₖᵀdₖ capped at 1.
+// With the external preconditioner P = L·Lᵀ (stepper/nonlinear_cg.hpp:41-97) the preconditioned gradient P·g
+// replaces g in the direction and the first term of β reads gₖ₊₁ᵀPy/dᵀy − yᵀPy·gₖ₊₁ᵀd/(dᵀy)² (the Euclidean
+// floor is kept); with L = I every quantity is the unpreconditioned one.
+void ncg_solve(ssim_ctx* c, bool precond) {
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* g = c->a(SSIM_ARRAY_FORCES);
+  double* d = c->dir.p;
+  c->lb_scal.alloc(std::max<size_t>(c->lb_scal.count, 16));
+  double* dots = c->lb_scal.p;  // five reductions of one iteration
+  // their partials (5 per vector block) fit the dot-product slot: pmax ≥ 4·slices = n/48 > 5·(n/1024 + 1)
+  if (5 * (size_t)pcg_vector_blocks(n) > c->pmax) fail(SSIM_ERR_STATE, "internal: partial buffer too small");
+  if (precond && !c->pre_val.p) set_precond(c, nullptr);
+  double* pa = precond ? c->pre_a.p : nullptr;  // Lᵀ·g of the current gradient
+  double* pb = precond ? c->pre_b.p : nullptr;  // Lᵀ·g of the trial gradient
+  double* pg = precond ? c->pre_pg.p : nullptr;  // P·g
+  EvalOut cur = eval_energy_grad(c, u, g);
+  if (precond) {
+    apply_precond(c, g, pa, pg);
+    launch_axpby(n, -1.0, pg, 0.0, d, s);
+  } else {
+    launch_axpby(n, -1.0, g, 0.0, d, s);
+  }
+  double alpha_prev = 1e-2, gd_prev = 0.0;
+  int it = 0;
+  for (; it < c->prm.lbfgs_maxit; ++it) {
+    if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+    if (std::sqrt(cur.gg) < c->tol_abs) {
+      c->stats.converged = 1;
+      break;
+    }
+    double gd = device_dot(c, g, d);
+    if (!(gd < 0)) {  // not a descent direction: restart along −P·g
+      launch_axpby(n, -1.0, precond ? pg : g, 0.0, d, s);
+      gd = precond ? -device_dot(c, pa, pa) : -cur.gg;
+    }
+    double alpha = it == 0 ? 1e-2 : alpha_prev * gd_prev / gd;
+    if (!(alpha > 0) || alpha > 1.0) alpha = 1.0;
+    EvalOut trial{0, 0};
+    bool ok = false;
+    for (int ls = 0; ls < c->prm.lbfgs_ls_max; ++ls) {
+      launch_axpy_to(n, u, alpha, d, c->utry.p, s);
+      trial = eval_energy_grad(c, c->utry.p, c->gtry.p);
+      if (trial.E <= cur.E + c->prm.lbfgs_c * alpha * gd) {
+        ok = true;
+        break;
+      }
+      alpha *= c->prm.ls_beta;
+    }
+    if (!ok) c->stats.ls_failures++;
+    if (precond) apply_precond(c, c->gtry.p, pb, pg);
+    launch_ncg_dots(n, c->gtry.p, g, d, precond ? pb : c->gtry.p, precond ? pa : g, c->part_d(), dots, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->pin, dots, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    const double dy = c->pin[0], gy = c->pin[1], yy = c->pin[2], gnd = c->pin[3], dd = c->pin[4];
+    double beta = 0.0;
+    if (dy > 0) beta = std::max(gy / dy - yy * gnd / (dy * dy), 0.5 * gnd / dd);
+    launch_axpby(n, -1.0, precond ? pg : c->gtry.p, beta, d, s);
+    HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    if (precond) std::swap(pa, pb);
+    cur = trial;
+    alpha_prev = alpha;
+    gd_prev = gd;
+  }
+  c->stats.newton_iterations = it;
+}
+
+// Forward Euler (time_step_solver_forward_euler, time_step.hpp:717-736): one evaluation of the forces at the
+// predictor held in next_deform, then next_deform += ½·dt²·forces.  The forces array is the DBC-filtered
+// gradient of the incremental potential (time_step.hpp:577-624) and the update adds it with a positive sign,
+// exactly as the reference does; Dirichlet dofs keep their targets because their gradient entries are zero.
+void forward_euler_solve(ssim_ctx* c) {
+  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* f = c->a(SSIM_ARRAY_FORCES);
+  const EvalOut cur = eval_energy_grad(c, u, f);
+  if (!std::isfinite(cur.E) || !std::isfinite(cur.gg)) fail(SSIM_ERR_NUMERIC, "non-finite energy or gradient");
+  launch_axpby(c->n, 0.5 * c->dt * c->dt, f, 1.0, u, c->stream);
+  HIPCHK(hipGetLastError());
+  c->stats.newton_iterations = 1;
+  c->stats.converged = 1;
+}
+
+}  // namespace ssim_hip

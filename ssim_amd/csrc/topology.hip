@@ -1,0 +1,364 @@
+// topology.hip -- what reset builds on the host and uploads once: the per-vertex and per-block contribution
+// slots of the segmented reductions (global_composer.hpp:36-128 restated without maps), the symmetric
+// blocked-ELL layout of the system matrix with its slot assignment, the per-tet material, the rest shapes and
+// the Jacobi diagonal of the rest Hessian (finite_elements/time_step.hpp:368-494).
+#include "context.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace ssim_hip;
+
+namespace {
+
+// Blocked-ELL slots of every block.  By default the upper blocks of a row take slots 0, 1, ... in column order
+// (diagonal first) and the lower references likewise.  In the fixed 8/7 layout the slots are aligned by
+// direction instead: slot k ≥ 1 holds, where it can, the block at the column offset w − v that sits at sorted
+// position k in most full rows (on a structured mesh every row then keeps each neighbour direction in the same
+// slot), and the lower reference to the block stored in slot k of the neighbour's row goes to lower slot 7 − k.
+// The fixed-width SpMV pairs upper slot k with lower slot 7 − k, so that for most rows of a slice the lower
+// reads and the neighbouring slice's own reads of the same lines are issued at the same point of the program
+// and meet in L2.  Slots only move blocks; every block is still read and summed once per row.
+void assign_slots(ssim_ctx* c, const std::vector<int>& len) {
+  const int nv = c->nv;
+  c->bslot.assign(std::max(c->brow[nv], 1), 0);
+  for (int v = 0; v < nv; ++v) {
+    const int kd = c->kdiag[v];
+    for (int k = 0; k < len[v]; ++k) c->bslot[c->brow[v] + k] = k < kd ? k : k - kd;
+  }
+  if (!c->fixed_layout) return;
+  static_assert(kFixedWL == kFixedWU - 1, "the slot pairing needs one lower slot per off-diagonal upper slot");
+  // modal column offset of each sorted upper position over the rows with all kFixedWU upper blocks
+  std::vector<std::vector<int>> offs(kFixedWU);
+  for (int v = 0; v < nv; ++v)
+    if (len[v] - c->kdiag[v] == kFixedWU)
+      for (int q = 1; q < kFixedWU; ++q) offs[q].push_back(c->bcol[c->brow[v] + c->kdiag[v] + q] - v);
+  std::vector<long long> modal(kFixedWU, -1);  // −1: no canonical offset (offsets are positive)
+  for (int q = 1; q < kFixedWU; ++q) {
+    auto& o = offs[q];
+    if (o.empty()) continue;
+    std::sort(o.begin(), o.end());
+    int best = o[0], run = 0, best_run = 0;
+    for (size_t i = 0; i < o.size(); ++i) {
+      run = (i > 0 && o[i] == o[i - 1]) ? run + 1 : 1;
+      if (run > best_run) {
+        best_run = run;
+        best = o[i];
+      }
+    }
+    modal[q] = best;
+    o.clear();
+    o.shrink_to_fit();
+  }
+  // upper slots: the diagonal in slot 0, canonical directions in their slot, the rest in the free slots
+  for (int v = 0; v < nv; ++v) {
+    const int kd = c->kdiag[v];
+    bool used[kFixedWU] = {true};
+    int rest[kFixedWU], nrest = 0;
+    c->bslot[c->brow[v] + kd] = 0;
+    for (int k = kd + 1; k < len[v]; ++k) {
+      const long long off = c->bcol[c->brow[v] + k] - v;
+      int q = 1;
+      while (q < kFixedWU && !(modal[q] == off && !used[q])) ++q;
+      if (q < kFixedWU) {
+        c->bslot[c->brow[v] + k] = q;
+        used[q] = true;
+      } else {
+        rest[nrest++] = k;
+      }
+    }
+    for (int i = 0, q = 1; i < nrest; ++i) {
+      while (used[q]) ++q;
+      c->bslot[c->brow[v] + rest[i]] = q;
+      used[q] = true;
+    }
+  }
+  // lower references: the block stored in slot k of row w goes to lower slot kFixedWU − 1 − k where free
+  auto upper_slot_of = [&](int w, int v) -> int {
+    const int* b = c->bcol.data() + c->brow[w];
+    const int* e = c->bcol.data() + c->brow[w + 1];
+    const int* it = std::lower_bound(b, e, v);
+    if (it == e || *it != v) fail(SSIM_ERR_STATE, "internal: missing block in sparsity pattern");
+    return c->bslot[c->brow[w] + (int)(it - b)];
+  };
+  for (int v = 0; v < nv; ++v) {
+    const int kd = c->kdiag[v];
+    bool used[kFixedWL] = {};
+    int rest[kFixedWL], nrest = 0;
+    for (int k = 0; k < kd; ++k) {
+      const int j = kFixedWU - 1 - upper_slot_of(c->bcol[c->brow[v] + k], v);
+      if (j >= 0 && j < kFixedWL && !used[j]) {
+        c->bslot[c->brow[v] + k] = j;
+        used[j] = true;
+      } else {
+        rest[nrest++] = k;
+      }
+    }
+    for (int i = 0, j = 0; i < nrest; ++i) {
+      while (used[j]) ++j;
+      c->bslot[c->brow[v] + rest[i]] = j;
+      used[j] = true;
+    }
+  }
+}
+
+// Sparsity, blocked-ELL layout and the two assembly permutations.
+void build_topology(ssim_ctx* c) {
+  const int nv = c->nv, nt = c->nt;
+  const int* cells = c->hcells.data();
+  // block adjacency: neighbours (self included, also for a vertex in no tet) of every vertex, ascending.
+  // Tets with a negative first index are absent (padding of a batched layout) and take no part.
+  std::vector<int> cnt(nv + 1, 1);
+  cnt[0] = 0;
+  for (int e = 0; e < nt; ++e)
+    if (cells[4 * e] >= 0)
+      for (int i = 0; i < 4; ++i) cnt[cells[4 * e + i] + 1] += 4;
+  for (int v = 0; v < nv; ++v) cnt[v + 1] += cnt[v];
+  std::vector<int> cand(std::max(cnt[nv], 1));
+  std::vector<int> fillp(cnt.begin(), cnt.end() - 1);
+  for (int v = 0; v < nv; ++v) cand[fillp[v]++] = v;
+  for (int e = 0; e < nt; ++e)
+    if (cells[4 * e] >= 0)
+      for (int i = 0; i < 4; ++i) {
+        const int v = cells[4 * e + i];
+        for (int j = 0; j < 4; ++j) cand[fillp[v]++] = cells[4 * e + j];
+      }
+  c->brow.assign(nv + 1, 0);
+  std::vector<int> len(nv, 0);
+  for (int v = 0; v < nv; ++v) {
+    int* b = cand.data() + cnt[v];
+    int* e = cand.data() + cnt[v + 1];
+    std::sort(b, e);
+    len[v] = (int)(std::unique(b, e) - b);
+    c->brow[v + 1] = c->brow[v] + len[v];
+  }
+  c->bcol.resize(std::max(c->brow[nv], 1));
+  for (int v = 0; v < nv; ++v) std::copy(cand.begin() + cnt[v], cand.begin() + cnt[v] + len[v], c->bcol.begin() + c->brow[v]);
+  cand.clear();
+  cand.shrink_to_fit();
+  auto slot = [&](int v, int w) -> int {  // index of w in the sorted neighbour list of v
+    const int* b = c->bcol.data() + c->brow[v];
+    const int* e = c->bcol.data() + c->brow[v + 1];
+    const int* it = std::lower_bound(b, e, w);
+    if (it == e || *it != w) fail(SSIM_ERR_STATE, "internal: missing block in sparsity pattern");
+    return (int)(it - b);
+  };
+  c->kdiag.resize(nv);
+  for (int v = 0; v < nv; ++v) c->kdiag[v] = slot(v, v);
+
+  // symmetric blocked ELL: slices of 64 rows; the upper part (w ≥ v, diagonal first) is stored, the lower
+  // part refers to the stored transposes
+  c->nslices = (nv + kSlice - 1) / kSlice;
+  const int ns = std::max(c->nslices, 1);
+  c->hwu.assign(ns, 0);
+  c->hoffu.assign(ns + 1, 0);
+  std::vector<int> hwl(ns, 0), hoffl(ns + 1, 0);
+  int max_wu = 0, max_wl = 0;
+  for (int s = 0; s < c->nslices; ++s) {
+    for (int l = 0; l < kSlice && s * kSlice + l < nv; ++l) {
+      const int v = s * kSlice + l;
+      c->hwu[s] = std::max(c->hwu[s], len[v] - c->kdiag[v]);
+      hwl[s] = std::max(hwl[s], c->kdiag[v]);
+    }
+    max_wu = std::max(max_wu, c->hwu[s]);
+    max_wl = std::max(max_wl, hwl[s]);
+  }
+  // rows narrow enough for the fixed-width kernel: pad every slice to the same widths
+  c->fixed_layout = max_wu <= kFixedWU && max_wl <= kFixedWL;
+  if (c->fixed_layout)
+    for (int s = 0; s < c->nslices; ++s) {
+      c->hwu[s] = kFixedWU;
+      hwl[s] = kFixedWL;
+    }
+  if (c->spmv_lanes == 0 && !c->fixed_layout) c->spmv_lanes = 1;
+  if (c->spmv_lanes_default) c->spmv_lanes = c->fixed_layout ? 0 : 1;
+  assign_slots(c, len);
+  long long offu = 0, offl = 0;
+  for (int s = 0; s < c->nslices; ++s) {
+    c->hoffu[s] = (int)offu;
+    hoffl[s] = (int)offl;
+    offu += (long long)c->hwu[s] * kSlice;
+    offl += (long long)hwl[s] * kSlice;
+  }
+  c->ncolslots = offu;
+  c->nlowslots = offl;
+
+  if (9 * offu + 9 * kSlice >= (1LL << 31) || offl >= (1LL << 31))
+    fail(SSIM_ERR_INVALID_ARGUMENT, "mesh too large for 32-bit blocked-ELL offsets");
+  auto valpos = [&](int v, int ku) -> int {  // value base of upper slot ku of row v
+    const int s = v / kSlice, lane = v % kSlice;
+    return 9 * c->hoffu[s] + ku * 9 * kSlice + lane;
+  };
+  // padding: upper slots → (v, zero values); lower references → (v, a zero block after the last slice)
+  std::vector<int> hucol(std::max<long long>(offu, 1), 0);
+  std::vector<int2> hlref(std::max<long long>(offl, 1), make_int2(0, 0));
+  const long long zero_block = 9 * offu;
+  for (int s = 0; s < c->nslices; ++s)
+    for (int l = 0; l < kSlice; ++l) {
+      const int v = std::min(s * kSlice + l, nv - 1);
+      for (int k = 0; k < c->hwu[s]; ++k) hucol[c->hoffu[s] + k * kSlice + l] = v;
+      for (int k = 0; k < hwl[s]; ++k) hlref[hoffl[s] + k * kSlice + l] = make_int2(v, (int)(zero_block + l));
+    }
+  for (int v = 0; v < nv; ++v) {
+    const int s = v / kSlice, lane = v % kSlice;
+    const int kd = c->kdiag[v];
+    for (int k = kd; k < len[v]; ++k)
+      hucol[c->hoffu[s] + c->bslot[c->brow[v] + k] * kSlice + lane] = c->bcol[c->brow[v] + k];
+    for (int k = 0; k < kd; ++k) {
+      const int w = c->bcol[c->brow[v] + k];
+      hlref[hoffl[s] + c->bslot[c->brow[v] + k] * kSlice + lane] =
+          make_int2(w, valpos(w, c->bslot[c->brow[w] + slot(w, v)]));
+    }
+  }
+
+  // upper blocks (r ≤ c) in row-major order, with the value base of (r,c)
+  std::vector<int> ubptr(nv + 1, 0);
+  for (int v = 0; v < nv; ++v) ubptr[v + 1] = ubptr[v] + (len[v] - c->kdiag[v]);
+  c->nub = ubptr[nv];
+  std::vector<int2> hub_rc(std::max(c->nub, 1));
+  std::vector<int> hub_pos(std::max(c->nub, 1));
+  for (int v = 0; v < nv; ++v)
+    for (int k = c->kdiag[v]; k < len[v]; ++k) {
+      const int ub = ubptr[v] + (k - c->kdiag[v]);
+      hub_rc[ub] = make_int2(v, c->bcol[c->brow[v] + k]);
+      hub_pos[ub] = valpos(v, c->bslot[c->brow[v] + k]);
+    }
+
+  // Hessian contributions: 10 local pairs (v ≤ w) per tet, sorted by (upper block, tet, pair)
+  std::vector<int> hcount(c->nub + 1, 0);
+  std::vector<int> hub_of(10 * (size_t)nt, -1);
+  for (int e = 0; e < nt; ++e)
+    if (cells[4 * e] >= 0)
+      for (int w = 0; w < 4; ++w)
+        for (int v = 0; v <= w; ++v) {
+          const int a = cells[4 * e + v], b = cells[4 * e + w];
+          const int r = std::min(a, b), cc = std::max(a, b);
+          const int ub = ubptr[r] + (slot(r, cc) - c->kdiag[r]);
+          hub_of[10 * (size_t)e + w * (w + 1) / 2 + v] = ub;
+          hcount[ub + 1]++;
+        }
+  for (int u = 0; u < c->nub; ++u) hcount[u + 1] += hcount[u];
+  std::vector<int> hperm(10 * (size_t)nt, -1);
+  {
+    std::vector<int> pos(hcount.begin(), hcount.end() - 1);
+    for (size_t k = 0; k < 10 * (size_t)nt; ++k)
+      if (hub_of[k] >= 0) hperm[k] = pos[hub_of[k]]++;
+  }
+  // gradient contributions: 4 per tet, sorted by (vertex, tet, corner)
+  std::vector<int> gseg(nv + 1, 0);
+  for (int e = 0; e < nt; ++e)
+    if (cells[4 * e] >= 0)
+      for (int i = 0; i < 4; ++i) gseg[cells[4 * e + i] + 1]++;
+  for (int v = 0; v < nv; ++v) gseg[v + 1] += gseg[v];
+  std::vector<int> gperm(4 * (size_t)nt, -1);
+  {
+    std::vector<int> pos(gseg.begin(), gseg.end() - 1);
+    for (int e = 0; e < nt; ++e)
+      if (cells[4 * e] >= 0)
+        for (int i = 0; i < 4; ++i) gperm[4 * e + i] = pos[cells[4 * e + i]]++;
+  }
+  hipStream_t s = c->stream;
+  c->wu.upload(c->hwu.data(), ns, s);
+  c->offu.upload(c->hoffu.data(), ns, s);
+  c->ucol.upload(hucol, s);
+  c->wl.upload(hwl.data(), ns, s);
+  c->offl.upload(hoffl.data(), ns, s);
+  c->lref.upload(hlref, s);
+  c->ub_rc.upload(hub_rc, s);
+  c->ub_pos.upload(hub_pos, s);
+  c->hseg.upload(hcount, s);
+  c->hperm.upload(hperm, s);
+  c->gseg.upload(gseg, s);
+  c->gperm.upload(gperm, s);
+  c->val.alloc(9 * (size_t)offu + 9 * kSlice);  // + the zero block of the padded lower references
+  HIPCHK(hipMemsetAsync(c->val.p, 0, c->val.count * sizeof(double), s));
+  c->gcontrib.alloc(12 * (size_t)std::max(nt, 1));
+  c->hcontrib.alloc(90 * (size_t)std::max(nt, 1));
+  sync(c);  // host staging vectors go out of scope
+}
+
+}  // namespace
+
+void ssim_hip::upload_material(ssim_ctx* c) {
+  std::vector<double> lam(c->nt), mu(c->nt);
+  if (c->per_tet) {
+    lam = c->hlam;
+    mu = c->hmu;
+  } else {
+    // compute_lambda / compute_mu: elast/basic_elast.hpp:11-20
+    const double l = c->youngs * c->poisson / ((1 + c->poisson) * (1 - 2 * c->poisson));
+    const double m = c->youngs / (2 * (1 + c->poisson));
+    std::fill(lam.begin(), lam.end(), l);
+    std::fill(mu.begin(), mu.end(), m);
+  }
+  c->lam.upload(lam, c->stream);
+  c->mu.upload(mu, c->stream);
+  sync(c);
+}
+
+// reset(): finite_elements/time_step.hpp:368-494
+
+void ssim_hip::do_reset(ssim_ctx* c) {
+  check_material(c->youngs, c->poisson, c->density);
+  if (0.5 - c->poisson < 1e-4)
+    fprintf(stderr, "Warning: got extremely high Poisson's ratio, may cause numerical instability. %lf\n", c->poisson);
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  c->rho_reset = c->density;
+  // vertex buffers: inertia, next, forces zeroed; the others kept if already allocated (:384-413)
+  for (int w = 0; w < SSIM_ARRAY_COUNT; ++w) {
+    const bool keep = (w == SSIM_ARRAY_EXT_ACCEL || w == SSIM_ARRAY_DBC_VALUES || w == SSIM_ARRAY_PREV_DEFORM ||
+                       w == SSIM_ARRAY_DEFORMATION || w == SSIM_ARRAY_VELOCITY) &&
+                      c->arr[w].p && c->arr[w].count == (size_t)n;
+    if (!keep) {
+      c->arr[w].alloc(n);
+      HIPCHK(hipMemsetAsync(c->arr[w].p, 0, sizeof(double) * std::max(n, 1), s));
+    }
+  }
+  if (!(c->dof.p && c->dof.count == (size_t)n)) {
+    c->dof.alloc(n);
+    HIPCHK(hipMemsetAsync(c->dof.p, 0, std::max(n, 1), s));
+  }
+  for (DBuf<double>* b : {&c->utry, &c->gtry, &c->dir, &c->r, &c->z, &c->p, &c->q, &c->tmp, &c->dinv}) {
+    b->alloc(n);
+    HIPCHK(hipMemsetAsync(b->p, 0, sizeof(double) * std::max(n, 1), s));
+  }
+  c->hrest.release();
+  c->B.alloc(9 * (size_t)std::max(c->nt, 1));
+  c->V.alloc(std::max(c->nt, 1));
+  upload_material(c);
+  launch_rest_shape(c->X.p, c->cells.p, c->nt, c->B.p, c->V.p, s);
+  HIPCHK(hipGetLastError());
+  build_topology(c);
+  // the SpMV grid has one workgroup per 64-row slice (or per 16 rows with 4 lanes per row), rounded up to a
+  // multiple of 8 (XCD remapping)
+  c->pmax = std::max<size_t>((size_t)max_partials(std::max({c->nt, c->n, c->nub, 1})), 4 * (size_t)c->nslices) + 8;
+  c->partials.alloc(8 * c->pmax);
+  // the projective-dynamics matrix is built at reset (projective_dynamics.hpp:51-87), for that stepper
+  c->pd_val.release();
+  if (c->stepper == SSIM_STEPPER_LBFGS_PD) build_pd(c);
+  // the preconditioner factor is rebuilt on the new pattern and set to the identity (fem_cuda.cu:175-201)
+  for (ssim_hip::DBuf<int>* b : {&c->pre_brow, &c->pre_bcol, &c->pre_tpos}) b->release();
+  if (c->pre_val.p || c->stepper == SSIM_STEPPER_NCG_EXT_PREC) set_precond(c, nullptr);
+  c->drop_graphs();  // buffers may have moved
+  c->pcg.alloc(1);
+  c->pcg_it.alloc(1);
+  c->ev.alloc(1);
+  HIPCHK(hipMemsetAsync(c->pcg.p, 0, sizeof(PcgScalars), s));
+  HIPCHK(hipMemsetAsync(c->pcg_it.p, 0, sizeof(PcgIter), s));
+  HIPCHK(hipMemsetAsync(c->ev.p, 0, sizeof(EvalScalars), s));
+  // total mass Σ ρV (= asum(M.values)/3, time_step.hpp:490)
+  std::vector<double> hV(c->nt);
+  if (c->nt) HIPCHK(hipMemcpyAsync(hV.data(), c->V.p, sizeof(double) * c->nt, hipMemcpyDeviceToHost, s));
+  sync(c);
+  double mass = 0;
+  for (int e = 0; e < c->nt; ++e) mass += c->density * hV[e];
+  c->total_mass = mass;
+  c->tol_abs = c->rtol * mass;
+  c->has_reset = true;
+}
+
