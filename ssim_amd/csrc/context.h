@@ -1,5 +1,6 @@
 // context.h -- state of a libssim_hip.so context (one mesh, or a padded concatenation of meshes for batching),
-// shared by context.hip (single-mesh ABI) and batch.hip (batched ABI).
+// shared by context.hip (single-mesh ABI), topology.hip (reset-time setup), solvers.hip (stepper drivers) and
+// batch.hip (batched ABI).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -195,7 +196,7 @@ struct ssim_ctx {
 
 
 namespace ssim_hip {
-// helpers defined in context.hip, used by batch.hip
+// helpers defined in context.hip, topology.hip and solvers.hip, used across them and by batch.hip
 void sync(ssim_ctx* c);
 void default_params(ssim_solver_params& p);
 // marks (mark) or releases vertices as Dirichlet; targets (3 per vertex) may be null for zero targets
