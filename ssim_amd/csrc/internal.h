@@ -151,7 +151,7 @@ struct LbfgsCoef {
 };
 void launch_lbfgs_gram(int n, const double* S, const double* Y, const double* g, const double* h, int k, double* part,
                        double* out, hipStream_t s);
-int lbfgs_gram_blocks(int n);
+int lbfgs_gram_blocks(int n);  // partial rows (of kLbfgsDots doubles) that launch_lbfgs_gram writes
 void launch_lbfgs_dir(int n, const double* S, const double* Y, const double* g, const double* h, const LbfgsCoef& cf,
                       double* r, hipStream_t s);
 void launch_lbfgs_shift(int n, const double* ut, const double* gt, double* u, double* g, double* sv, double* yv,

@@ -804,98 +804,72 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
 // Partials of the 6m + 1 dot products one iteration needs (m = kLbfgsSlots, every slot, used or not):
 //   [0, m)   s_jᵀg        [m, 2m)  y_jᵀDg       [2m]  gᵀDg
 //   [2m+1+j] s_jᵀy_k      [3m+1+j] s_kᵀy_j      [4m+1+j] y_kᵀDy_j      (k = the newest slot; zero when k < 0)
-// Each of the workgroup's four waves owns one group of them over the same kGramChunk elements (s_jᵀg; y_jᵀDg and
-// gᵀDg; s_jᵀy_k; s_kᵀy_j and y_kᵀDy_j), so a lane keeps at most 2m accumulators and every wave loads about
-// m + 2 vectors; the waves of a workgroup read the same elements, which the caches serve once.  A lane loads all
-// kGramU of its elements before it multiplies, so one wave has all its loads in flight at once.
-constexpr int kGramU = 4;
-constexpr int kGramChunk = 64 * kGramU;
+// Two wave roles split the dot products by the stored vectors they read, so that each S_j and Y_j is loaded by
+// one wave only: role 0 reads S_j, g and y_k (s_jᵀg, s_jᵀy_k: 2m accumulators), role 1 reads Y_j, g, H0, s_k and
+// y_k (y_jᵀDg, gᵀDg, s_kᵀy_j, y_kᵀDy_j: 3m + 1 accumulators).  Waves 0 and 1 of a workgroup take one set of
+// elements, waves 2 and 3 the next, in a grid-stride loop over a grid of at most kGramMaxBlocks workgroups (all
+// resident at once); each trip takes two elements per lane, whose loads are all in flight together.  Partial
+// row 2·blockIdx + set holds the workgroup's sums over its element set.
+constexpr int kGramMaxBlocks = 768;  // 3 workgroups per CU at 130 VGPRs: all resident
+static int gram_grid(int n) { return std::max(1, std::min(kGramMaxBlocks, (n + 255) / 256)); }
 
 __global__ __launch_bounds__(kBlock) void k_lbfgs_gram(int n, const double* __restrict__ S, const double* __restrict__ Y,
                                                        const double* __restrict__ g, const double* __restrict__ h,
                                                        int k, double* __restrict__ part) {
-  constexpr int M = kLbfgsSlots, U = kGramU;
-  static_assert(kBlock == 256, "four waves per workgroup, one per group of dot products");
+  constexpr int M = kLbfgsSlots;
+  static_assert(kBlock == 256, "two element sets of two waves per workgroup");
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int idx[U];
-  bool ok[U];
+  const int role = wave & 1, set = wave >> 1;
+  const int stride = gridDim.x * 128;
+  double acc[3 * M];
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int i = blockIdx.x * kGramChunk + u * 64 + lane;
-    ok[u] = i < n;
-    idx[u] = ok[u] ? i : 0;
-  }
-  // vector j at this lane's elements; zero past the end
-  auto load = [&](const double* __restrict__ v, double (&out)[U]) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) out[u] = ok[u] ? v[idx[u]] : 0.0;
-  };
-  double acc[2 * M];
-#pragma unroll
-  for (int j = 0; j < 2 * M; ++j) acc[j] = 0.0;
+  for (int j = 0; j < 3 * M; ++j) acc[j] = 0.0;
   double gdg = 0.0;
   const bool pair = k >= 0;
-  if (wave == 0) {
-    double gv[U], sv[M][U];
-    load(g, gv);
+  auto body = [&](int i) {
+    const double gi = g[i];
+    if (role == 0) {
+      const double yk = pair ? Y[(size_t)k * n + i] : 0.0;
 #pragma unroll
-    for (int j = 0; j < M; ++j) load(S + (size_t)j * n, sv[j]);
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-#pragma unroll
-      for (int u = 0; u < U; ++u) acc[j] += sv[j][u] * gv[u];
-  } else if (wave == 1) {
-    double gv[U], hv[U], yv[M][U];
-    load(g, gv);
-    load(h, hv);
-#pragma unroll
-    for (int j = 0; j < M; ++j) load(Y + (size_t)j * n, yv[j]);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const double dg = ok[u] ? gv[u] * (1.0 / hv[u]) : 0.0;
-#pragma unroll
-      for (int j = 0; j < M; ++j) acc[j] += yv[j][u] * dg;
-      gdg += gv[u] * dg;
-    }
-  } else if (wave == 2) {
-    if (pair) {
-      double ykv[U], sv[M][U];
-      load(Y + (size_t)k * n, ykv);
-#pragma unroll
-      for (int j = 0; j < M; ++j) load(S + (size_t)j * n, sv[j]);
-#pragma unroll
-      for (int j = 0; j < M; ++j)
-#pragma unroll
-        for (int u = 0; u < U; ++u) acc[j] += sv[j][u] * ykv[u];
-    }
-  } else {
-    if (pair) {
-      double skv[U], ykv[U], hv[U], yv[M][U];
-      load(S + (size_t)k * n, skv);
-      load(Y + (size_t)k * n, ykv);
-      load(h, hv);
-#pragma unroll
-      for (int j = 0; j < M; ++j) load(Y + (size_t)j * n, yv[j]);
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const double dyk = ok[u] ? ykv[u] * (1.0 / hv[u]) : 0.0;
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-          acc[j] += skv[u] * yv[j][u];
-          acc[M + j] += dyk * yv[j][u];
-        }
+      for (int j = 0; j < M; ++j) {
+        const double sj = S[(size_t)j * n + i];
+        acc[j] += sj * gi;
+        acc[M + j] += sj * yk;
       }
+    } else {
+      const double di = 1.0 / h[i], dg = gi * di;
+      const double sk = pair ? S[(size_t)k * n + i] : 0.0, dyk = pair ? Y[(size_t)k * n + i] * di : 0.0;
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const double yj = Y[(size_t)j * n + i];
+        acc[j] += yj * dg;
+        acc[M + j] += sk * yj;
+        acc[2 * M + j] += dyk * yj;
+      }
+      gdg += gi * dg;
+    }
+  };
+  int i = (blockIdx.x * 2 + set) * 64 + lane;
+  for (; i + stride < n; i += 2 * stride) {
+    body(i);
+    body(i + stride);
+  }
+  if (i < n) body(i);
+  double* out = part + (size_t)(blockIdx.x * 2 + set) * kLbfgsDots;
+  // role 0: s_jᵀg → [j], s_jᵀy_k → [2m+1+j]; role 1: y_jᵀDg → [m+j], s_kᵀy_j → [3m+1+j], y_kᵀDy_j → [4m+1+j]
+#pragma unroll
+  for (int j = 0; j < 3 * M; ++j) {
+    const double t = wave_sum(acc[j]);
+    if (lane != 0) continue;
+    const int q = j / M, r = j % M;
+    if (role == 0) {
+      if (q == 0) out[r] = t;
+      else if (q == 1) out[2 * M + 1 + r] = t;
+    } else {
+      out[(q == 0 ? M : q == 1 ? 3 * M + 1 : 4 * M + 1) + r] = t;
     }
   }
-  const int cnt = wave == 3 ? 2 * M : M;
-  const int base = wave == 0 ? 0 : wave == 1 ? M : wave == 2 ? 2 * M + 1 : 3 * M + 1;
-  double* out = part + (size_t)blockIdx.x * kLbfgsDots;
-#pragma unroll
-  for (int j = 0; j < 2 * M; ++j) {
-    const double t = wave_sum(acc[j]);
-    if (lane == 0 && j < cnt) out[base + j] = t;
-  }
-  if (wave == 1) {
+  if (role == 1) {
     const double t = wave_sum(gdg);
     if (lane == 0) out[2 * M] = t;
   }
@@ -913,12 +887,12 @@ __global__ __launch_bounds__(kBlock) void k_lbfgs_gram_fin(const double* __restr
 
 void launch_lbfgs_gram(int n, const double* S, const double* Y, const double* g, const double* h, int k, double* part,
                        double* out, hipStream_t s) {
-  const int nb = lbfgs_gram_blocks(n);
-  k_lbfgs_gram<<<nb, kBlock, 0, s>>>(n, S, Y, g, h, k, part);
-  k_lbfgs_gram_fin<<<kLbfgsDots, kBlock, 0, s>>>(part, nb, out);
+  k_lbfgs_gram<<<gram_grid(n), kBlock, 0, s>>>(n, S, Y, g, h, k, part);
+  k_lbfgs_gram_fin<<<kLbfgsDots, kBlock, 0, s>>>(part, lbfgs_gram_blocks(n), out);
 }
 
-int lbfgs_gram_blocks(int n) { return (n + kGramChunk - 1) / kGramChunk; }
+// partial rows written by k_lbfgs_gram: two per workgroup
+int lbfgs_gram_blocks(int n) { return 2 * gram_grid(n); }
 
 // new pair and iterate in one pass: s = u_t − u, y = g_t − g, then u = u_t, g = g_t
 __global__ __launch_bounds__(kBlock) void k_lbfgs_shift(int n, const double* __restrict__ ut,
