@@ -361,17 +361,17 @@ __device__ __forceinline__ void elem_adjoint_tet(const MeshView& m, int e, const
 
 // LDS-staged segmented sums.  Thread t of the workgroup owns segment j = j0 + t (j < j1), the record run
 // [seg[j], seg[j+1]) of W doubles per record.  The runs of consecutive segments are contiguous, so the workgroup
-// copies its whole span [seg[j0], seg[j1]) through LDS with coalesced loads, kStageDoubles at a time, and each
-// thread adds its own records in index order: the sums are bitwise those of a sequential walk over the run, but
-// the loads are no longer one scattered walk per lane.  Every thread of the workgroup must call it (it
-// synchronises the workgroup); threads with j ≥ j1 get zeros.
+// copies its whole span [seg[j0], seg[j1]) through LDS with coalesced loads, kStageDoubles at a time (all of a
+// stage's loads in flight together), and each thread adds its own records in index order: the sums are bitwise
+// those of a sequential walk over the run, but the loads are no longer one scattered walk per lane.  Every thread
+// of the workgroup must call it (it synchronises the workgroup); threads with j ≥ j1 get zeros.
 constexpr int kStageDoubles = 4096;  // 32 KB of LDS per workgroup
 
 template <int W>
 __device__ __forceinline__ void staged_run_sum(const int* __restrict__ seg, int j0, int j1,
                                                const double* __restrict__ rec, double (&acc)[W]) {
   constexpr int kStage = kStageDoubles / W;  // records per stage
-  constexpr int U = 8;                       // loads in flight per thread
+  constexpr int U = kStageDoubles / kBlock;  // loads in flight per thread: a whole stage at once
   __shared__ double lds[kStage * W];
   const int j = j0 + (int)threadIdx.x;
   const int span_lo = seg[j0], span_hi = seg[j1];
