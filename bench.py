@@ -443,7 +443,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 element-variant leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: this build's time step runs on a single device "
+        print(f"bench.py measures one GPU only: the time step runs on a single device "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
