@@ -540,28 +540,43 @@ ssim_status_t ssim_get_dof_type(const ssim_ctx* ctx, unsigned char* host_out) {
 }
 
 ssim_status_t ssim_num_vertices(const ssim_ctx* ctx, int* nv) {
-  if (!ctx || !nv) return SSIM_ERR_INVALID_ARGUMENT;
+  if (!ctx || !nv) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
   *nv = ctx->nv;
   return SSIM_OK;
 }
 ssim_status_t ssim_num_cells(const ssim_ctx* ctx, int* nt) {
-  if (!ctx || !nt) return SSIM_ERR_INVALID_ARGUMENT;
+  if (!ctx || !nt) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
   *nt = ctx->nt;
   return SSIM_OK;
 }
 ssim_status_t ssim_total_mass(const ssim_ctx* ctx, double* mass) {
-  if (!ctx || !mass) return SSIM_ERR_INVALID_ARGUMENT;
+  if (!ctx || !mass) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
   *mass = ctx->total_mass;
   return SSIM_OK;
 }
 ssim_status_t ssim_grad_threshold_abs(const ssim_ctx* ctx, double* tol) {
-  if (!ctx || !tol) return SSIM_ERR_INVALID_ARGUMENT;
+  if (!ctx || !tol) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
   *tol = ctx->tol_abs;
   return SSIM_OK;
 }
 
 ssim_status_t ssim_csr_nnz(const ssim_ctx* ctx, long long* nnz) {
-  if (!ctx || !nnz) return SSIM_ERR_INVALID_ARGUMENT;
+  if (!ctx || !nnz) {
+    g_err = "null argument";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
   *nnz = 9LL * (ctx->brow.empty() ? 0 : ctx->brow[ctx->nv]);
   return SSIM_OK;
 }
