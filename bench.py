@@ -19,7 +19,10 @@ Two more objects in the same line cover the batched and differentiable variants 
 after another through the single-mesh path on the GPU, with the per-mesh agreement of the two after one step;
 ``adjoint``: a 20-step rollout of 32 bars of 16×16×64 cubes (per-tet log E uniform in [ln 1e4, ln 1e6], ν = 0.45)
 with the loss Σ‖u − t‖² over the free-end vertices, timed forward and backward (adjoint PCG per step);
-``lbfgs``: L-BFGS steps of the twisted 64³-cube (1,572,864 tets, per-tet material).
+``lbfgs``: L-BFGS steps of the twisted 64³-cube (1,572,864 tets, per-tet material);
+``fp32_elements``: 60 steps of the same bar with FP32 element kernels against FP64, both timed, with the largest
+position difference; ``step_accounting``: the bytes a step must move by the per-step accounting (K1, K2, K3+K4
+times the timed counts) over the step time.
 """
 from __future__ import annotations
 

@@ -1,0 +1,23 @@
+"""The bench's byte accounting at the 786k-tet bar (32×32×128 cubes), computed from the mesh on the host: the
+SpMV kernel's algorithmic bytes (76 B per stored upper block, 8 B per lower reference, 7·n·8 B of vectors) and the
+per-step terms K1 (energy + gradient, per-tet material: 112·nt + 4·n·8) and K2 (SPD Hessian assembly:
+112·nt + 2·n·8 + nnz·8 + n), which reproduce the per-unit figures 101.6 MB and 241.5 MB of the accounting."""
+import numpy as np
+
+import bench
+from ssim_amd import mesh
+
+
+def test_config2_byte_accounting():
+    X, C = mesh.bar_freudenthal(128, 32, 32)
+    nv, nt = X.shape[0], C.shape[0]
+    e = np.concatenate([C[:, [i, j]] for i in range(4) for j in range(i + 1, 4)])
+    edges = np.unique(np.sort(e, axis=1), axis=0).shape[0]
+    info = dict(nv=nv, nt=nt, upper_blocks=nv + edges, nonzero_blocks=nv + 2 * edges)
+    assert (nv, nt, edges) == (140481, 786432, 945344)
+    assert bench.spmv_algorithmic_bytes(info) == 113_686_260
+    counts = dict(steps=1, pcg_iterations=1, hessian_evals=0, energy_evals=0)
+    acc = bench.step_accounting(info, counts, 1.0)
+    assert round(acc["k1"] / 1e6, 1) == 101.6 and round(acc["k2"] / 1e6, 1) == 241.5
+    assert acc["k3_k4"] == 113_686_260 + 5 * 3 * nv * 8
+    assert acc["bytes_per_step"] == acc["k3_k4"]
