@@ -15,8 +15,9 @@ sample of the same workload on this host (see ``cpu_sample`` below).
 
 Two more objects in the same line cover the batched and differentiable variants of the step:
 ``batched``: 256 bars of uneven size (dims uniform in 4..24, per-bar E log-uniform in [1e4, 1e6] Pa, ν uniform in
-[0.3, 0.48], x = 0 end clamped, gravity) stepped by one set of launches, against stepping the same 256 meshes one
-after another through the single-mesh path on the GPU, with the per-mesh agreement of the two after one step;
+[0.3, 0.48], x = 0 end clamped, gravity) stepped by one set of launches whose knobs the autotuner picks, against
+stepping the same 256 meshes one after another through the single-mesh path on the GPU, with the per-mesh
+agreement of the two after one step;
 ``adjoint``: a 20-step rollout of 32 bars of 16×16×64 cubes (per-tet log E uniform in [ln 1e4, ln 1e6], ν = 0.45)
 with the loss Σ‖u − t‖² over the free-end vertices, timed forward and backward (adjoint PCG per step);
 ``lbfgs``: L-BFGS steps of the twisted 64³-cube (1,572,864 tets, per-tet material);
@@ -273,6 +274,14 @@ def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
         mism += int(B.last_stats(m)["newton_iterations"] != s.last_stats()["newton_iterations"])
     print(f"[bench] batched: per-mesh max position rel err {worst:.3e}, Newton count mismatches {mism}",
           file=sys.stderr, flush=True)
+    # launch knobs of the batch (SpMV variant, XCD mapping, PCG chunking, graph replay, grid compaction) chosen by
+    # the autotuner: one step per candidate from the same state, which it restores afterwards
+    from ssim_amd import autotune
+
+    t0 = time.perf_counter()
+    tuned = autotune.tune(B, repeats=1, cache=None)
+    tune_s = time.perf_counter() - t0
+    print(f"[bench] batched: autotuned {tuned} in {tune_s:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tot = dict(newton_iterations=0, pcg_iterations=0)
@@ -291,6 +300,8 @@ def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
     print(f"[bench] batched {tb:.3f} s/step, sequential {ts:.3f} s/step", file=sys.stderr, flush=True)
     return {
         "workload": "256 bars, dims uniform in 4..24, per-bar E log-uniform [1e4,1e6], nu uniform [0.3,0.48]",
+        "tuning": tuned,
+        "tune_s": tune_s,
         "meshes": len(cases),
         "tets": int(sum(c.nt for c in cases)),
         "vertices": int(sum(c.nv for c in cases)),
