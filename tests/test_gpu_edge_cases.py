@@ -1,7 +1,7 @@
 """Edge cases of the HIP path against the CPU FP64 oracle: a vertex shared by 1,500 tets (gradient and Hessian
 runs longer than one LDS stage of the segmented reductions, a blocked-ELL row of 4,501 blocks), a single tet,
-and a mesh whose every dof is Dirichlet; the first two and a small bar also in one batch, against the single-mesh
-path.
+a mesh whose every dof is Dirichlet and an unstructured (Delaunay) mesh; the first two and a small bar also in one
+batch, against the single-mesh path.
 
 Bars as in the parity tests: energy, gradient and Hessian ≤ 1e-10 relative (norm-wise); positions after every
 Newton step ≤ 1e-8 relative with identical Newton counts; a fully clamped mesh lands exactly on its targets.
@@ -49,6 +49,34 @@ def star_case(k=1500, seed=3):
     accel[1::3] = -9.8
     return mesh.BarCase(X, cells, lam, mu, 1000.0, 1.0 / 60, dof, np.zeros(3 * nv), accel, u0, np.zeros(3 * nv),
                         name=f"star{k}")
+
+
+def delaunay_case(seed=7):
+    """Delaunay tetrahedralisation of a 9×5×5 lattice (spacing 5 mm) whose interior points are jittered by ±0.3 of
+    the spacing: an unstructured mesh with vertex valences from 4 to 21 (zero-volume tets of the flat faces
+    dropped).  The x = 0 face is clamped; gravity along −y; E = 1e5, ν = 0.4."""
+    import scipy.spatial
+
+    rng = np.random.default_rng(seed)
+    h = 0.005
+    g = np.stack(np.meshgrid(np.arange(9), np.arange(5), np.arange(5), indexing="ij"), -1).reshape(-1, 3)
+    X = g.astype(np.float64) * h
+    inner = (g[:, 0] > 0) & (g[:, 0] < 8) & (g[:, 1] > 0) & (g[:, 1] < 4) & (g[:, 2] > 0) & (g[:, 2] < 4)
+    X[inner] += rng.uniform(-0.3 * h, 0.3 * h, (int(inner.sum()), 3))
+    C = scipy.spatial.Delaunay(X).simplices.astype(np.int32)
+    D = np.stack([X[C[:, 1]] - X[C[:, 0]], X[C[:, 2]] - X[C[:, 0]], X[C[:, 3]] - X[C[:, 0]]], axis=2)
+    det = np.linalg.det(D)
+    C[det < 0] = C[det < 0][:, [0, 2, 1, 3]]
+    C = np.ascontiguousarray(C[np.abs(det) > 1e-3 * np.abs(det).mean()])
+    nv, nt = X.shape[0], C.shape[0]
+    lam, mu = mesh.lame(np.full(nt, 1e5), np.full(nt, 0.4))
+    dof = np.repeat((g[:, 0] == 0).astype(np.uint8), 3)
+    u0 = rng.uniform(-2e-4, 2e-4, 3 * nv)
+    u0[dof == 1] = 0.0
+    accel = np.zeros(3 * nv)
+    accel[1::3] = -9.8
+    return mesh.BarCase(X, C, lam, mu, 1000.0, 1.0 / 60, dof, np.zeros(3 * nv), accel, u0, np.zeros(3 * nv),
+                        name="delaunay")
 
 
 def _pair(case, rtol=1e-6, **solver):
@@ -174,3 +202,25 @@ def test_single_tet_and_bar_in_one_batch_match_the_single_mesh_path():
     """A one-tet mesh (one slice with 4 live rows, one vertex workgroup with 4 live vertices) beside a small bar, at
     the benchmark threshold 1e-6 x mass."""
     _batch_vs_single([tet_case(), mesh.cantilever_bar(3, 2, 2, seed=5)], 1e-6, steps=10)
+
+
+def test_unstructured_delaunay_mesh_matches_oracle():
+    """The unstructured mesh (general blocked-ELL layout, rows of 5 to 22 blocks): assembly within 1e-10 and three
+    Newton steps within 1e-8 of the oracle, solved to 1e-12 and stopped at 1e-9 x mass."""
+    case = delaunay_case()
+    print(f"[gpu] delaunay mesh nv={case.nv} nt={case.nt}", flush=True)
+    g, o = _pair(case, rtol=1e-9, pcg_rtol=1e-12, max_newton=20)
+    rng = np.random.default_rng(11)
+    u = case.u0 + 5e-4 * rng.standard_normal(3 * case.nv)
+    u[case.dof_type == 1] = 0
+    g.set_array("next_deform", u)
+    g.update_hessian(True)
+    Ho = o.hessian_csr(u, True, True)
+    Hg = g.hessian()
+    assert np.array_equal(Hg.indptr, Ho.indptr) and np.array_equal(Hg.indices, Ho.indices)
+    assert rel_err(Hg.data, Ho.data) <= 1e-10
+    for step in range(3):  # each step starts from its own predictor (prepare_step)
+        g.step()
+        _, st = o.step()
+        assert g.last_stats()["newton_iterations"] == st["newton"], step
+        assert rel_err(_x(case, g.get_array("deformation")), _x(case, o.get_state()[0])) <= 1e-8, step
