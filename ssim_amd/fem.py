@@ -170,6 +170,13 @@ class TetFiniteElementSolver_Hip:
     def forces(self) -> np.ndarray:
         return np.ascontiguousarray(self._get("forces").reshape(-1, 3).T)
 
+    def save_frame(self, path) -> np.ndarray:
+        """Writes the positions X + u as an (N, 3) float64 .npy file and returns them: the deform_<step>.npy
+        frames of tests/fem_cu/main.cu:86-92, which tests/fem_time_step/visualize.py:9-12 loads."""
+        x = self._X + self._get("deformation").reshape(-1, 3)
+        np.save(path, x)
+        return x
+
     def update_energy_and_gradients(self) -> float:
         e = ctypes.c_double()
         _lib.call("ssim_update_energy_and_gradients", self._h, ctypes.byref(e))

@@ -7,12 +7,16 @@
 // Output file: float64 u[nv*3] after the last step, then per step int32 newton_iterations, pcg_iterations, then
 // float64 sums of absolute values over the remaining views: rest_vol, dminv, sysmatrix nnz and values, mass_matrix,
 // mass_matrix_filtered, dof_type, local_stiff.
+// With a third argument, a folder, the driver also writes the positions X + u after every step to
+// <folder>/deform_<step>.npy as an [nv, 3] float64 array, the frames tests/fem_cu/main.cu:86-92 writes and
+// tests/fem_time_step/visualize.py:9-12 reads.
 #include <cmath>
 #include <cstdio>
 #include <cstdint>
 #include <vector>
 
 #include "ssim_hip.hpp"
+#include "ssim_npy.hpp"
 
 template <typename T>
 static void rd(FILE* f, T* p, size_t n) {
@@ -20,8 +24,8 @@ static void rd(FILE* f, T* p, size_t n) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 3) {
-    std::fprintf(stderr, "usage: %s input output\n", argv[0]);
+  if (argc != 3 && argc != 4) {
+    std::fprintf(stderr, "usage: %s input output [frames_folder]\n", argv[0]);
     return 2;
   }
   try {
@@ -77,6 +81,12 @@ int main(int argc, char** argv) {
       const ssim_step_stats st = step.last_stats();
       counts.push_back(st.newton_iterations);
       counts.push_back(st.pcg_iterations);
+      if (argc == 4) {
+        std::vector<double> x = step.get(SSIM_ARRAY_DEFORMATION);
+        for (size_t i = 0; i < x.size(); ++i) x[i] += X[i];
+        ssim::hip::io::write_npy(std::string(argv[3]) + "/deform_" + std::to_string(s) + ".npy", x.data(),
+                                 (size_t)nv, 3);
+      }
     }
     const std::vector<double> u = step.get(SSIM_ARRAY_DEFORMATION);
     // the setters reject what the reference rejects

@@ -1,7 +1,8 @@
 """The header-only C++ front end (include/ssim_hip.hpp) driven by a C++ program, against the Python front end
 on the same bar: identical positions (bitwise) and iteration counts after 5 steps of the 8×8×32 bar, for the
 Newton, L-BFGS, nonlinear-CG and forward-Euler steppers; the remaining views of the header (mesh, dof_type, dminv,
-rest_vol, local_stiff, sysmatrix, mass_matrix, mass_matrix_filtered) agree with the Python front end's."""
+rest_vol, local_stiff, sysmatrix, mass_matrix, mass_matrix_filtered) agree with the Python front end's; the
+per-step deform_<step>.npy frames the driver writes hold X + u and equal the Python front end's save_frame."""
 import os
 import subprocess
 
@@ -31,7 +32,10 @@ def test_cpp_driver_matches_python_front_end(tmp_path, stepper):
         clamped.astype(np.uint8).tofile(f)
         case.u0.astype(np.float64).tofile(f)
     print(f"[gpu] C++ driver, stepper {stepper}", flush=True)
-    res = subprocess.run([DRIVER, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    frames = tmp_path / "frames"
+    frames.mkdir()
+    args = [DRIVER, str(inp), str(out)] + ([str(frames)] if stepper == 0 else [])
+    res = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stderr
     raw = np.fromfile(out, dtype=np.uint8)
     u_cpp = raw[:8 * 3 * case.nv].view(np.float64)
@@ -54,6 +58,15 @@ def test_cpp_driver_matches_python_front_end(tmp_path, stepper):
         py_counts.append((st["newton_iterations"], st["pcg_iterations"]))
     assert np.array_equal(counts, np.array(py_counts, dtype=np.int32))
     assert np.array_equal(u_cpp, s.get_array("deformation"))
+    if stepper == 0:
+        # the deform_<step>.npy frames: (nv, 3) positions X + u, the last one equal to the Python front end's
+        names = sorted(os.listdir(frames))
+        assert names == sorted(f"deform_{i}.npy" for i in range(steps))
+        last = np.load(frames / f"deform_{steps - 1}.npy")
+        assert last.shape == (case.nv, 3) and last.dtype == np.float64
+        assert np.array_equal(last, s.save_frame(tmp_path / "py_frame.npy"))
+        assert np.array_equal(last, np.load(tmp_path / "py_frame.npy"))
+        assert np.array_equal(last.reshape(-1), case.X.reshape(-1) + u_cpp)
     dminv, vol = s.rest_shape()
     H = s.hessian()
     K = s.eval_elements(s.get_array("next_deform"), make_spsd=True, want_H=False, want_K=True)["K"]
