@@ -139,6 +139,22 @@ ssim_status_t ssim_get_material_model(const ssim_ctx* ctx, ssim_material_t* mode
 ssim_status_t ssim_material_eval(int dim, ssim_material_t model, long long n, const double* F, const double* lambda,
                                  const double* mu, int make_spsd, double* psi, double* P, double* H, void* stream);
 
+/* P1 triangles in 2D: the per-element quantities of basic_time_step<..., PhysicalDim 2, TopologyDim 3>, which
+ * the reference templates but never instantiates: the rest area |det Dm|/2 (finite_elements/rest_vol.hpp:33-50),
+ * Dm⁻¹ and F = Ds·Dm⁻¹ (finite_elements/def_grad.hpp:53-127), the energy A·Ψ(F), the vertex gradient A·P·Dm⁻ᵀ
+ * (stress_element_to_vertex, time_step.hpp:62-83), the 6×6 stiffness A·(∂F/∂x)ᵀ·H·(∂F/∂x) with H optionally
+ * SPD-projected (eigenvalues clamped at 1e-3, time_step.hpp:41-59; ∂F/∂x: compute_pfpx_2,
+ * elast/internal_physic.hpp:78-100), and the P1 consistent mass (times `density`) and Laplacian
+ * (finite_elements/bilinears.hpp:65-91, 191-264).  X [nv][2] rest positions, u [nv][2] displacements,
+ * cells [nt][3], lambda / mu [nt]; outputs, each may be NULL: area [nt], dminv [nt][2][2] and F [nt][2][2]
+ * (row-major), energy [nt], grad [nt][3][2], stiff [nt][6][6] (dof 2·vertex + component), mass [nt][3][3],
+ * laplacian [nt][3][3].  A triangle with a vertex index outside [0, nv) gets NaN outputs.  All arrays in device
+ * memory; launched on `stream` (NULL: the null stream), not synchronised. */
+ssim_status_t ssim_tri_eval(ssim_material_t model, int nv, int nt, const double* X, const int* cells, const double* u,
+                            const double* lambda, const double* mu, double density, int make_spsd, double* area,
+                            double* dminv, double* F, double* energy, double* grad, double* stiff, double* mass,
+                            double* laplacian, void* stream);
+
 /* mark_dirichlet_batched / mark_general_batched: bindings/src/fem_host.cpp:123-144 (targets [count,3]) */
 ssim_status_t ssim_mark_dirichlet(ssim_ctx* ctx, const int* verts, int count, const double* targets);
 ssim_status_t ssim_mark_general(ssim_ctx* ctx, const int* verts, int count);

@@ -88,6 +88,8 @@ def lib():
             "orc_ts_step_next": (None, [_V]),
             "orc_ts_rest_hessian_diag": (None, [_V, _D]),
             "orc_ts_spmv": (None, [_V, _D, _D, _D]),
+            "orc_tri_eval": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I, _D, _D, _D, ctypes.c_double,
+                                    ctypes.c_int] + [_D] * 8),
             "orc_num_threads": (ctypes.c_int, []),
             "orc_set_num_threads": (None, [ctypes.c_int]),
         }
@@ -152,6 +154,25 @@ def material2(model, F, lam, mu):
         e = lib().orc_snh2_stress(_dp(F), lam, mu, _dp(P))
         lib().orc_snh2_hessian(_dp(F), lam, mu, _dp(H))
     return e, P.reshape(2, 2), H.reshape(4, 4)
+
+
+TRI_FIELDS = ("area", "dminv", "F", "energy", "grad", "stiff", "mass", "laplacian")
+TRI_SHAPES = {"area": (), "dminv": (2, 2), "F": (2, 2), "energy": (), "grad": (3, 2), "stiff": (6, 6),
+              "mass": (3, 3), "laplacian": (3, 3)}
+
+
+def tri_eval(X, cells, u, lam, mu, model="stable_neohookean", make_spsd=False, density=1.0):
+    """P1 triangles in 2D, per element: X (nv, 2), cells (nt, 3), u (nv, 2); lam, mu scalars or (nt,).  Returns
+    a dict of TRI_FIELDS with the shapes TRI_SHAPES (leading axis nt)."""
+    X, u = _f64(X).reshape(-1, 2), _f64(u).reshape(-1, 2)
+    C = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 3)
+    nv, nt = X.shape[0], C.shape[0]
+    lam = np.array(np.broadcast_to(_f64(lam), (nt,)))
+    mu = np.array(np.broadcast_to(_f64(mu), (nt,)))
+    out = {k: np.zeros((nt,) + TRI_SHAPES[k]) for k in TRI_FIELDS}
+    lib().orc_tri_eval(1 if model == "linear" else 0, nv, nt, _dp(X), _ip(C), _dp(u), _dp(lam), _dp(mu),
+                       float(density), 1 if make_spsd else 0, *[_dp(out[k]) for k in TRI_FIELDS])
+    return out
 
 
 def energy2(model, F, lam, mu):

@@ -24,6 +24,9 @@
  *   compute_step         the Newton iterate is a stationary point: ‖g‖ below the stopping tolerance, and equals
  *                        scipy.optimize.minimize on a tiny mesh; energy decreases monotonically per iteration.
  *   compute_step_lbfgs   the same stationarity check and agreement with the Newton minimiser.
+ *   orc_tri_eval         (tests/test_oracle_tri.py) area / Dm⁻¹ against numpy, affine F, closed-form mass, the
+ *                        cotangent Laplacian, central FD of the energy and gradient, numpy.linalg.eigh for the
+ *                        clamp, rotation invariance.
  * The solver trajectories of mathprim (the reference's external solver library, not vendored) are
  * parity unpinned: iteration order, stopping norm and line-search constants follow the benchmark configuration.
  */
@@ -1240,5 +1243,145 @@ void orc_ts_step_next(orc_ts* ts) {
     v += -1.0 * ts->u_prev[i];
     v *= 1.0 / ts->dt;
     ts->vel[i] = v;
+  }
+}
+
+/* ---- P1 triangles in 2D: basic_time_step<..., PhysicalDim 2, TopologyDim 3>, element by element ----
+ * Route of the reference: Dm⁻¹ from Dm = [x1−x0, x2−x0] (def_grad.hpp:68-84), the area |det Dm|/2
+ * (rest_vol.hpp:33-50), F = Ds·Dm⁻¹ (def_grad.hpp:109-127), ∂F/∂x as the explicit 4×6 matrix of compute_pfpx_2
+ * (elast/internal_physic.hpp:78-100: row 2i+j for F_ij, column 2a+c for vertex a, component c), the vertex
+ * gradient from stress_element_to_vertex (time_step.hpp:62-83: P·Dm⁻ᵀ for vertices 1, 2 and minus their sum
+ * for vertex 0, times A), the stiffness A·pfpxᵀ·H⁺·pfpx (time_step.hpp:41-59, the clamp through orc_sym_eig),
+ * and the mass and Laplacian from the reference-triangle tables (bilinears.hpp:27-41) through the Jacobian
+ * integrator (bilinears.hpp:65-91, 191-264; det of the Jacobian = 2A). */
+static const double kP12ff[3][3] = {{1.0 / 12, 1.0 / 24, 1.0 / 24}, {1.0 / 24, 1.0 / 12, 1.0 / 24},
+                                    {1.0 / 24, 1.0 / 24, 1.0 / 12}};
+/* ∂φ_i/∂ξ on the reference triangle, φ0 = 1 − ξ − η, φ1 = ξ, φ2 = η */
+static const double kP12grad[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+
+void orc_tri_eval(int model, int nv, int nt, const double* X, const int* cells, const double* u, const double* lam,
+                  const double* mu, double rho, int make_spsd, double* area, double* dminv, double* F,
+                  double* energy, double* grad, double* stiff, double* mass, double* lap) {
+  for (int e = 0; e < nt; ++e) {
+    const int* c = cells + 3 * e;
+    int ok = 1;
+    for (int a = 0; a < 3; ++a) ok &= c[a] >= 0 && c[a] < nv;
+    if (!ok) {
+      const double nan = NAN;
+      if (area) area[e] = nan;
+      if (energy) energy[e] = nan;
+      for (int k = 0; k < 4; ++k) {
+        if (dminv) dminv[4 * e + k] = nan;
+        if (F) F[4 * e + k] = nan;
+      }
+      for (int k = 0; k < 6; ++k)
+        if (grad) grad[6 * e + k] = nan;
+      for (int k = 0; k < 9; ++k) {
+        if (mass) mass[9 * e + k] = nan;
+        if (lap) lap[9 * e + k] = nan;
+      }
+      for (int k = 0; k < 36; ++k)
+        if (stiff) stiff[36 * e + k] = nan;
+      continue;
+    }
+    double Dm[2][2], Ds[2][2];
+    for (int i = 0; i < 2; ++i)
+      for (int k = 0; k < 2; ++k) {
+        Dm[i][k] = X[2 * c[k + 1] + i] - X[2 * c[0] + i];
+        Ds[i][k] = (X[2 * c[k + 1] + i] + u[2 * c[k + 1] + i]) - (X[2 * c[0] + i] + u[2 * c[0] + i]);
+      }
+    const double det = Dm[0][0] * Dm[1][1] - Dm[0][1] * Dm[1][0];
+    double B[2][2]; /* Dm⁻¹ */
+    B[0][0] = Dm[1][1] / det;
+    B[0][1] = -Dm[0][1] / det;
+    B[1][0] = -Dm[1][0] / det;
+    B[1][1] = Dm[0][0] / det;
+    const double A = fabs(det) / 2;
+    double f[4];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) f[2 * i + j] = Ds[i][0] * B[0][j] + Ds[i][1] * B[1][j];
+    double P[4], H[16];
+    double psi;
+    if (model == 1) {
+      psi = orc_lin2_stress(f, lam[e], mu[e], P);
+      orc_lin2_hessian(lam[e], mu[e], H);
+    } else {
+      psi = orc_snh2_stress(f, lam[e], mu[e], P);
+      orc_snh2_hessian(f, lam[e], mu[e], H);
+    }
+    if (area) area[e] = A;
+    if (energy) energy[e] = A * psi;
+    for (int k = 0; k < 4; ++k) {
+      if (dminv) dminv[4 * e + k] = B[k / 2][k % 2];
+      if (F) F[4 * e + k] = f[k];
+    }
+    if (grad) {
+      for (int j = 0; j < 2; ++j) {
+        double tot = 0;
+        for (int v = 1; v < 3; ++v) {
+          /* (P·Dm⁻ᵀ)(j, v−1) = Σ_k P(j,k)·B(v−1,k) */
+          const double val = P[2 * j] * B[v - 1][0] + P[2 * j + 1] * B[v - 1][1];
+          grad[6 * e + 2 * v + j] = A * val;
+          tot += val;
+        }
+        grad[6 * e + j] = -A * tot;
+      }
+    }
+    if (stiff) {
+      const double m_ = B[0][0], n_ = B[0][1], p_ = B[1][0], q_ = B[1][1];
+      double pf[4][6];
+      memset(pf, 0, sizeof pf);
+#define PF(i, j, vert, comp, val) pf[(i) * 2 + (j)][(vert) * 2 + (comp)] = (val)
+      PF(0, 0, 0, 0, -m_ - p_);
+      PF(0, 0, 1, 0, m_);
+      PF(0, 0, 2, 0, p_);
+      PF(1, 0, 0, 1, -m_ - p_);
+      PF(1, 0, 1, 1, m_);
+      PF(1, 0, 2, 1, p_);
+      PF(0, 1, 0, 0, -n_ - q_);
+      PF(0, 1, 1, 0, n_);
+      PF(0, 1, 2, 0, q_);
+      PF(1, 1, 0, 1, -n_ - q_);
+      PF(1, 1, 1, 1, n_);
+      PF(1, 1, 2, 1, q_);
+#undef PF
+      double Hp[16];
+      if (make_spsd) {
+        double w[4], Q[16];
+        orc_sym_eig(4, H, w, Q);
+        for (int i = 0; i < 4; ++i) w[i] = w[i] > 1e-3 ? w[i] : 1e-3;
+        for (int i = 0; i < 4; ++i)
+          for (int j = 0; j < 4; ++j) {
+            double t = 0;
+            for (int k = 0; k < 4; ++k) t += Q[i * 4 + k] * w[k] * Q[j * 4 + k];
+            Hp[i * 4 + j] = t;
+          }
+      } else {
+        memcpy(Hp, H, sizeof Hp);
+      }
+      for (int r = 0; r < 6; ++r)
+        for (int s_ = 0; s_ < 6; ++s_) {
+          double t = 0;
+          for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) t += pf[i][r] * Hp[i * 4 + j] * pf[j][s_];
+          stiff[36 * e + 6 * r + s_] = A * t;
+        }
+    }
+    if (mass || lap) {
+      const double detJ = fabs(det); /* p1_element_2d: det of the Jacobian [x1−x0, x2−x0] */
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          if (mass) mass[9 * e + 3 * i + j] = rho * detJ * kP12ff[i][j];
+          if (lap) {
+            /* Σ_k integrate_pf_pf(i, j, k, k): det·(J⁻¹ column k)ᵀ·T_ij·(J⁻¹ column k), T_ij[P][Q] = ½·∂φ_i/∂ξ_P·∂φ_j/∂ξ_Q */
+            double t = 0;
+            for (int k = 0; k < 2; ++k)
+              for (int P_ = 0; P_ < 2; ++P_)
+                for (int Q_ = 0; Q_ < 2; ++Q_)
+                  t += B[P_][k] * 0.5 * kP12grad[i][P_] * kP12grad[j][Q_] * B[Q_][k];
+            lap[9 * e + 3 * i + j] = detJ * t;
+          }
+        }
+    }
   }
 }

@@ -76,6 +76,12 @@ void orc_snh2_hessian(const double F[4], double lam, double mu, double H[16]);
 double orc_lin2_energy(const double F[4], double lam, double mu);
 double orc_lin2_stress(const double F[4], double lam, double mu, double P[4]);
 void orc_lin2_hessian(double lam, double mu, double H[16]);
+/* P1 triangles in 2D, per element (model 0 stable Neo-Hookean, 1 linear): area [nt], Dm⁻¹ and F [nt][4]
+ * (row-major), A·Ψ [nt], vertex gradient [nt][6], stiffness [nt][36] (dof 2a+c), ρ-scaled mass and Laplacian
+ * [nt][9]; outputs may be NULL; a triangle with an index outside [0, nv) gets NaN */
+void orc_tri_eval(int model, int nv, int nt, const double* X, const int* cells, const double* u, const double* lam,
+                  const double* mu, double rho, int make_spsd, double* area, double* dminv, double* F,
+                  double* energy, double* grad, double* stiff, double* mass, double* lap);
 /* forward Euler: u_next = û + ½dt²·g(û) (time_step.hpp:717-736) */
 double orc_ts_compute_step_fe(orc_ts* ts, double* stats);
 void orc_ts_set_lbfgs(orc_ts* ts, int m, int max_iter, double ls_c, double ls_beta, int ls_max);

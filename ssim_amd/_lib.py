@@ -70,6 +70,8 @@ SIGNATURES = {
     "ssim_set_stepper": [_V, ctypes.c_int],
     "ssim_set_material_model": [_V, ctypes.c_int],
     "ssim_material_eval": [ctypes.c_int, ctypes.c_int, ctypes.c_longlong, _V, _V, _V, ctypes.c_int, _V, _V, _V, _V],
+    "ssim_tri_eval": [ctypes.c_int, ctypes.c_int, ctypes.c_int, _V, _V, _V, _V, _V, ctypes.c_double, ctypes.c_int]
+    + [_V] * 9,
     "ssim_set_precond_matrix": [_V, _D, ctypes.c_longlong],
     "ssim_get_material_model": [_V, ctypes.POINTER(ctypes.c_int)],
     "ssim_mark_dirichlet": [_V, _I, ctypes.c_int, _D],

@@ -259,6 +259,22 @@ ssim_status_t ssim_material_eval(int dim, ssim_material_t model, long long n, co
   });
 }
 
+ssim_status_t ssim_tri_eval(ssim_material_t model, int nv, int nt, const double* X, const int* cells, const double* u,
+                            const double* lambda, const double* mu, double density, int make_spsd, double* area,
+                            double* dminv, double* F, double* energy, double* grad, double* stiff, double* mass,
+                            double* laplacian, void* stream) {
+  return run([&] {
+    if (model != SSIM_MATERIAL_STABLE_NEOHOOKEAN && model != SSIM_MATERIAL_LINEAR)
+      fail(SSIM_ERR_INVALID_ARGUMENT, "unknown material model");
+    if (nv < 0 || nt < 0) fail(SSIM_ERR_INVALID_ARGUMENT, "nv and nt must be non-negative");
+    if (nt > 0 && (!X || !cells || !u || !lambda || !mu)) fail(SSIM_ERR_INVALID_ARGUMENT, "null input array");
+    if (nt == 0) return;
+    const TriOut o{area, dminv, F, energy, grad, stiff, mass, laplacian};
+    launch_tri_eval(model, nv, nt, X, cells, u, lambda, mu, density, make_spsd, o, static_cast<hipStream_t>(stream));
+    HIPCHK(hipGetLastError());
+  });
+}
+
 ssim_status_t ssim_set_material_model(ssim_ctx* ctx, ssim_material_t model) {
   if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
   if (model != SSIM_MATERIAL_STABLE_NEOHOOKEAN && model != SSIM_MATERIAL_LINEAR) {

@@ -474,4 +474,71 @@ SSIM_HD void linear2_hessian(T lam, T mu, T H[16]) {
     }
 }
 
+// Cyclic Jacobi eigen-decomposition of a symmetric 4x4 matrix (Golub & Van Loan, Matrix Computations, 4th ed.,
+// §8.5.2-8.5.3: symmetric 2x2 Schur rotations swept row by row until the off-diagonal mass is below 1e-32 of the
+// total, at most 20 sweeps), then H⁺ = Q·max(Λ, floor)·Qᵀ: the SPD projection of the 2D element Hessian
+// (time_step.hpp:50-56, SelfAdjointEigenSolver with eigenvalues clamped from below at 1e-3).
+template <typename T>
+SSIM_HD void sym4_project_spd(T H[16], T floor_) {
+  T A[16], Q[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    A[k] = H[k];
+    Q[k] = (k % 5 == 0) ? T(1) : T(0);
+  }
+  for (int sweep = 0; sweep < 20; ++sweep) {
+    T off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        tot += A[4 * i + j] * A[4 * i + j];
+        if (i != j) off += A[4 * i + j] * A[4 * i + j];
+      }
+    if (!(off > T(1e-32) * tot)) break;
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = p + 1; q < 4; ++q) {
+        const T apq = A[4 * p + q];
+        if (apq == T(0)) continue;
+        const T tau = (A[4 * q + q] - A[4 * p + p]) / (T(2) * apq);
+        const T t = (tau >= T(0) ? T(1) : T(-1)) / (fabs(tau) + sqrt(T(1) + tau * tau));
+        const T c = T(1) / sqrt(T(1) + t * t), s = t * c;
+        // A <- Jᵀ A J with J the rotation in the (p, q) plane; columns first, then rows
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const T akp = A[4 * k + p], akq = A[4 * k + q];
+          A[4 * k + p] = c * akp - s * akq;
+          A[4 * k + q] = s * akp + c * akq;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const T apk = A[4 * p + k], aqk = A[4 * q + k];
+          A[4 * p + k] = c * apk - s * aqk;
+          A[4 * q + k] = s * apk + c * aqk;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const T qkp = Q[4 * k + p], qkq = Q[4 * k + q];
+          Q[4 * k + p] = c * qkp - s * qkq;
+          Q[4 * k + q] = s * qkp + c * qkq;
+        }
+      }
+  }
+  T w[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[k] = A[5 * k] > floor_ ? A[5 * k] : floor_;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = i; j < 4; ++j) {
+      T v = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v += Q[4 * i + k] * w[k] * Q[4 * j + k];
+      H[4 * i + j] = v;
+      H[4 * j + i] = v;
+    }
+}
+
 }  // namespace ssim_hip

@@ -142,6 +142,13 @@ void launch_ncg_dots(int n, const double* gt, const double* g, const double* d, 
                      double* part, double* out, hipStream_t s);
 void launch_material_eval(int dim, int model, long long n, const double* F, const double* lam, const double* mu,
                           int make_spsd, double* psi, double* P, double* H, hipStream_t s);
+// outputs of the P1 triangle evaluation, per triangle (each may be null): area, Dm⁻¹ (row-major 2x2), F
+// (row-major), A·Ψ, vertex gradient [3][2], stiffness [6][6] (dof 2a+c), consistent mass [3][3], Laplacian [3][3]
+struct TriOut {
+  double *area, *dminv, *F, *energy, *grad, *stiff, *mass, *laplacian;
+};
+void launch_tri_eval(int model, int nv, int nt, const double* X, const int* cells, const double* u,
+                     const double* lam, const double* mu, double rho, int make_spsd, const TriOut& o, hipStream_t s);
 // L-BFGS in Gram form: at most kLbfgsSlots pairs; kLbfgsDots dot products per iteration (see kernels.hip)
 constexpr int kLbfgsSlots = 8;
 constexpr int kLbfgsDots = 6 * kLbfgsSlots + 1;

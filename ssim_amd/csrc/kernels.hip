@@ -315,6 +315,134 @@ void launch_material_eval(int dim, int model, long long n, const double* F, cons
 }
 
 // ------------------------------------------------------------------------------------------------------
+// P1 triangles in 2D: the per-element quantities of basic_time_step<..., PhysicalDim 2, TopologyDim 3>.  One
+// thread per triangle.  With D_a the physical gradient of vertex a's hat function (D_0 = −(row 0 + row 1) of
+// Dm⁻¹, D_a = row a−1 for a = 1, 2), ∂F_ij/∂x_ac = δ_ic·D_a(j) (compute_pfpx_2, elast/internal_physic.hpp:78-100),
+// so the vertex gradient is A·P·D_a, the stiffness block (a c, b d) is A·Σ_jl D_a(j)·H[(c j), (d l)]·D_b(l), the
+// consistent mass is ρA/12·(1 + δ_ab) and the Laplacian A·D_a·D_b (bilinears.hpp:65-91 with the tables of
+// :27-41).  A triangle with a vertex index outside [0, nv) is not read: its outputs are NaN.
+
+template <int Model>
+__global__ __launch_bounds__(kBlock) void k_tri_eval(int nv, int nt, const double* __restrict__ X,
+                                                     const int* __restrict__ cells, const double* __restrict__ u,
+                                                     const double* __restrict__ lam, const double* __restrict__ mu,
+                                                     double rho, int make_spsd, TriOut o) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nt) return;
+  int c[3];
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    c[a] = cells[3 * e + a];
+    ok = ok && c[a] >= 0 && c[a] < nv;
+  }
+  if (!ok) {
+    const double nan = __builtin_nan("");
+    if (o.area) o.area[e] = nan;
+    if (o.energy) o.energy[e] = nan;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (o.dminv) o.dminv[4 * e + k] = nan;
+      if (o.F) o.F[4 * e + k] = nan;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      if (o.grad) o.grad[6 * (long long)e + k] = nan;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if (o.mass) o.mass[9 * (long long)e + k] = nan;
+      if (o.laplacian) o.laplacian[9 * (long long)e + k] = nan;
+    }
+    if (o.stiff)
+      for (int k = 0; k < 36; ++k) o.stiff[36 * (long long)e + k] = nan;
+    return;
+  }
+  double x[3][2], y[3][2];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      x[a][i] = X[2 * c[a] + i];
+      y[a][i] = x[a][i] + u[2 * c[a] + i];
+    }
+  // Dm = [x1 − x0, x2 − x0] by columns, its inverse by the adjugate
+  const double m00 = x[1][0] - x[0][0], m01 = x[2][0] - x[0][0], m10 = x[1][1] - x[0][1], m11 = x[2][1] - x[0][1];
+  const double det = m00 * m11 - m01 * m10;
+  const double Di[4] = {m11 / det, -m01 / det, -m10 / det, m00 / det};
+  const double A = 0.5 * fabs(det);
+  double f[4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double s0 = y[1][i] - y[0][i], s1 = y[2][i] - y[0][i];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) f[2 * i + j] = s0 * Di[j] + s1 * Di[2 + j];
+  }
+  const double l = lam[e], m = mu[e];
+  double P[4];
+  const double psi = Model == kLinearElastic ? linear2_stress(f, l, m, P) : snh2_stress(f, l, m, P);
+  double D[3][2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    D[1][j] = Di[j];
+    D[2][j] = Di[2 + j];
+    D[0][j] = -(Di[j] + Di[2 + j]);
+  }
+  if (o.area) o.area[e] = A;
+  if (o.energy) o.energy[e] = A * psi;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (o.dminv) o.dminv[4 * e + k] = Di[k];
+    if (o.F) o.F[4 * e + k] = f[k];
+  }
+  if (o.grad)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) o.grad[6 * (long long)e + 2 * a + i] = A * (P[2 * i] * D[a][0] + P[2 * i + 1] * D[a][1]);
+  if (o.mass || o.laplacian)
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if (o.mass) o.mass[9 * (long long)e + 3 * a + b] = rho * A * (a == b ? 1.0 / 6.0 : 1.0 / 12.0);
+        if (o.laplacian) o.laplacian[9 * (long long)e + 3 * a + b] = A * (D[a][0] * D[b][0] + D[a][1] * D[b][1]);
+      }
+  if (o.stiff) {
+    double H[16];
+    if (Model == kLinearElastic)
+      linear2_hessian(l, m, H);
+    else
+      snh2_hessian(f, l, m, H);
+    if (make_spsd) sym4_project_spd(H, kSpdFloor);
+    // HD[(c j), (b d)] = Σ_l H[(c j), (d l)]·D_b(l), then K = A·Σ_j D_a(j)·HD
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int ci = 0; ci < 2; ++ci)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+          for (int di = 0; di < 2; ++di) {
+            double v = 0;
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+              for (int k = 0; k < 2; ++k) v += D[a][j] * H[4 * (2 * ci + j) + 2 * di + k] * D[b][k];
+            o.stiff[36 * (long long)e + 6 * (2 * a + ci) + 2 * b + di] = A * v;
+          }
+  }
+}
+
+void launch_tri_eval(int model, int nv, int nt, const double* X, const int* cells, const double* u,
+                     const double* lam, const double* mu, double rho, int make_spsd, const TriOut& o, hipStream_t s) {
+  if (nt <= 0) return;
+  if (model == kLinearElastic)
+    k_tri_eval<kLinearElastic><<<nblocks(nt), kBlock, 0, s>>>(nv, nt, X, cells, u, lam, mu, rho, make_spsd, o);
+  else
+    k_tri_eval<kStableNeoHookean><<<nblocks(nt), kBlock, 0, s>>>(nv, nt, X, cells, u, lam, mu, rho, make_spsd, o);
+}
+
+// ------------------------------------------------------------------------------------------------------
 // adjoint of one step (element part) and the Dirichlet mask of its right-hand side
 
 template <int Model>

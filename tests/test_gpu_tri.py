@@ -1,0 +1,54 @@
+"""P1 triangles in 2D on MI355X (ssim_tri_eval) against the oracle's orc_tri_eval: every per-element output for
+both material models, with and without the SPD projection, on a jittered 40×30 grid (2,400 triangles) with
+random displacements large enough to invert some triangles.  Bars: area, Dm⁻¹, F, mass and Laplacian within
+1e-13; energy, gradient and stiffness within 1e-10 (relative, over the whole array).  Also: an out-of-range vertex
+index gives NaN outputs for that triangle only, NULL outputs are skipped, and zero triangles are accepted."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import rel_err
+from ssim_amd import tri
+from test_oracle_tri import _mesh
+
+pytestmark = pytest.mark.gpu
+
+TIGHT = ("area", "dminv", "F", "mass", "laplacian")
+
+
+@pytest.mark.parametrize("model", ["stable_neohookean", "linear"])
+@pytest.mark.parametrize("make_spsd", [False, True])
+def test_triangles_match_oracle(model, make_spsd):
+    X, C, u = _mesh(nx=40, ny=30, seed=11, amp=0.6)
+    rng = np.random.default_rng(2)
+    lam = rng.uniform(1e3, 1e5, len(C))
+    mu = rng.uniform(1e2, 1e4, len(C))
+    print(f"[gpu] triangles {model} spsd={make_spsd}: {len(C)} tris", flush=True)
+    g = tri.evaluate(X, C, u, lam, mu, model=model, make_spsd=make_spsd, density=1200.0)
+    o = oracle.tri_eval(X, C, u, lam, mu, model=model, make_spsd=make_spsd, density=1200.0)
+    assert (o["area"] > 0).all()
+    F = o["F"]
+    assert (F[:, 0, 0] * F[:, 1, 1] - F[:, 0, 1] * F[:, 1, 0] < 0).any()  # some triangles are inverted
+    for k in tri.FIELDS:
+        assert g[k].shape == o[k].shape, k
+        bar = 1e-13 if k in TIGHT else 1e-10
+        assert rel_err(g[k], o[k]) <= bar, (k, rel_err(g[k], o[k]))
+
+
+def test_bad_index_null_outputs_and_empty():
+    X, C, u = _mesh(nx=6, ny=5, seed=4)
+    C = C.copy()
+    C[7, 2] = len(X)
+    C[9, 0] = -3
+    g = tri.evaluate(X, C, u, 1e4, 1e3, make_spsd=True)
+    o = oracle.tri_eval(X, C, u, 1e4, 1e3, make_spsd=True)
+    ok = np.ones(len(C), dtype=bool)
+    ok[[7, 9]] = False
+    for k in tri.FIELDS:
+        assert np.isnan(g[k][~ok]).all(), k
+        assert rel_err(g[k][ok], o[k][ok]) <= 1e-10, k
+    part = tri.evaluate(X, C, u, 1e4, 1e3, want=("energy", "stiff"))
+    assert set(part) == {"energy", "stiff"}
+    assert rel_err(part["stiff"][ok], oracle.tri_eval(X, C, u, 1e4, 1e3)["stiff"][ok]) <= 1e-10
+    empty = tri.evaluate(X, np.zeros((0, 3), dtype=np.int32), u, 1e4, 1e3)
+    assert all(v.shape[0] == 0 for v in empty.values())
