@@ -148,8 +148,10 @@ ssim_status_t ssim_material_eval(int dim, ssim_material_t model, long long n, co
  * (finite_elements/bilinears.hpp:65-91, 191-264).  X [nv][2] rest positions, u [nv][2] displacements,
  * cells [nt][3], lambda / mu [nt]; outputs, each may be NULL: area [nt], dminv [nt][2][2] and F [nt][2][2]
  * (row-major), energy [nt], grad [nt][3][2], stiff [nt][6][6] (dof 2·vertex + component), mass [nt][3][3],
- * laplacian [nt][3][3].  A triangle with a vertex index outside [0, nv) gets NaN outputs.  All arrays in device
- * memory; launched on `stream` (NULL: the null stream), not synchronised. */
+ * laplacian [nt][3][3].  A triangle with a vertex index outside [0, nv) gets NaN outputs; a degenerate one
+ * (det Dm = 0) gets area and mass 0 and non-finite Dm⁻¹, F, energy, gradient, stiffness and Laplacian (the
+ * division by det Dm).  All arrays in device memory; launched on `stream` (NULL: the null stream), not
+ * synchronised. */
 ssim_status_t ssim_tri_eval(ssim_material_t model, int nv, int nt, const double* X, const int* cells, const double* u,
                             const double* lambda, const double* mu, double density, int make_spsd, double* area,
                             double* dminv, double* F, double* energy, double* grad, double* stiff, double* mass,

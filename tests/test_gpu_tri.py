@@ -2,7 +2,8 @@
 both material models, with and without the SPD projection, on a jittered 40×30 grid (2,400 triangles) with
 random displacements large enough to invert some triangles.  Bars: area, Dm⁻¹, F, mass and Laplacian within
 1e-13; energy, gradient and stiffness within 1e-10 (relative, over the whole array).  Also: an out-of-range vertex
-index gives NaN outputs for that triangle only, NULL outputs are skipped, and zero triangles are accepted."""
+index gives NaN outputs for that triangle only, a degenerate triangle the oracle's pattern of non-finite outputs,
+NULL outputs are skipped, and zero triangles are accepted."""
 import numpy as np
 import pytest
 
@@ -40,13 +41,16 @@ def test_bad_index_null_outputs_and_empty():
     C = C.copy()
     C[7, 2] = len(X)
     C[9, 0] = -3
+    C[11] = [0, 6, 12]  # three collinear boundary vertices on y = 0: det Dm = 0
     g = tri.evaluate(X, C, u, 1e4, 1e3, make_spsd=True)
     o = oracle.tri_eval(X, C, u, 1e4, 1e3, make_spsd=True)
     ok = np.ones(len(C), dtype=bool)
-    ok[[7, 9]] = False
+    ok[[7, 9, 11]] = False
     for k in tri.FIELDS:
-        assert np.isnan(g[k][~ok]).all(), k
+        assert np.isnan(g[k][[7, 9]]).all(), k
+        assert np.array_equal(np.isfinite(g[k][11]), np.isfinite(o[k][11])), k
         assert rel_err(g[k][ok], o[k][ok]) <= 1e-10, k
+    assert g["area"][11] == 0.0 and not g["mass"][11].any() and not np.isfinite(g["stiff"][11]).any()
     part = tri.evaluate(X, C, u, 1e4, 1e3, want=("energy", "stiff"))
     assert set(part) == {"energy", "stiff"}
     assert rel_err(part["stiff"][ok], oracle.tri_eval(X, C, u, 1e4, 1e3)["stiff"][ok]) <= 1e-10
