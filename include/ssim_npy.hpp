@@ -31,7 +31,7 @@ inline void write_npy(const std::string& path, const double* data, std::size_t r
   out.write(len_le, 2);
   out.write(dict.data(), static_cast<std::streamsize>(dict.size()));
   static_assert(sizeof(double) == 8, "float64");
-  // the target is little-endian (x86-64 hosts of MI355X), like '<f8'
+  static_assert(__BYTE_ORDER__ == __ORDER_LITTLE_ENDIAN__, "the doubles are written as stored: '<f8' needs a little-endian host");
   out.write(reinterpret_cast<const char*>(data), static_cast<std::streamsize>(sizeof(double) * rows * cols));
   if (!out) throw std::runtime_error("write failed: " + path);
 }
