@@ -84,6 +84,32 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, in
   block_sum<K, NT>(tot);
 }
 
+// Item-major outputs with lane-contiguous stores.  Thread t of the workgroup holds the W values of item first + t
+// (value k is get(k), k a compile-time index after unrolling); the `count` items that exist go to
+// dst[(first + t)·W + k].  One lane per item would store with a stride of W doubles, each store instruction
+// touching 64 cache lines; here the items pass through LDS one wave (64 items) at a time, at an odd stride so that
+// the lane-strided LDS writes spread over the banks, and every thread then stores consecutive doubles of the
+// wave's contiguous output span.  Every thread of the workgroup calls it (it synchronises the workgroup); lds
+// holds 64·(W | 1) doubles.
+template <int W, typename Get>
+__device__ __forceinline__ void store_items(double* __restrict__ dst, long long first, int count, double* lds,
+                                            Get&& get) {
+  constexpr int S = W | 1;
+  const int t = threadIdx.x;
+#pragma unroll 1
+  for (int g0 = 0; g0 < kBlock && g0 < count; g0 += 64) {  // uniform over the workgroup
+    __syncthreads();  // the previous readers of lds are done
+    if (t >= g0 && t < g0 + 64) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) lds[(t - g0) * S + k] = get(k);
+    }
+    __syncthreads();
+    const int n = (count - g0 < 64 ? count - g0 : 64) * W;
+    double* __restrict__ out = dst + (first + g0) * W;
+    for (int idx = t; idx < n; idx += kBlock) out[idx] = lds[(idx / W) * S + idx % W];
+  }
+}
+
 // Host-side dispatch of the element kernels over (arithmetic type, material model): fn(ElemKind<T, Model>{}).
 template <typename T, int Model>
 struct ElemKind {

@@ -261,13 +261,17 @@ __global__ __launch_bounds__(kBlock) void k_material_eval(long long n, const dou
                                                           const double* __restrict__ lam, const double* __restrict__ mu,
                                                           int make_spsd, double* __restrict__ psi,
                                                           double* __restrict__ P, double* __restrict__ H) {
-  const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
   constexpr int D2 = DIM * DIM;
+  __shared__ double lds[64 * (D2 * D2 | 1)];
+  const long long first = (long long)blockIdx.x * kBlock;
+  const long long i = first + threadIdx.x;
+  const int count = n - first < kBlock ? (int)(n - first) : kBlock;
+  const bool live = i < n;
+  // every lane runs the arithmetic (the staged stores synchronise the workgroup); lanes past the end use F = I
   double f[D2], p[D2];
 #pragma unroll
-  for (int k = 0; k < D2; ++k) f[k] = F[i * D2 + k];
-  const double l = lam[i], m = mu[i];
+  for (int k = 0; k < D2; ++k) f[k] = live ? F[i * D2 + k] : (k % (DIM + 1) == 0 ? 1.0 : 0.0);
+  const double l = live ? lam[i] : 1.0, m = live ? mu[i] : 1.0;
   double e;
   if constexpr (DIM == 2) {
     e = Model == kLinearElastic ? linear2_stress(f, l, m, p) : snh2_stress(f, l, m, p);
@@ -277,24 +281,26 @@ __global__ __launch_bounds__(kBlock) void k_material_eval(long long n, const dou
         linear2_hessian(l, m, h);
       else
         snh2_hessian(f, l, m, h);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) H[i * 16 + k] = h[k];
+      store_items<16>(H, first, count, lds, [&](int k) { return h[k]; });
     }
   } else {
     e = material_stress<Model>(f, l, m, p);
     if (H) {
       double h[45];
       material_hessian<Model>(f, l, m, make_spsd, kSpdFloor, h);
-#pragma unroll
-      for (int a = 0; a < 9; ++a)
-#pragma unroll
-        for (int b = 0; b < 9; ++b) H[i * 81 + 9 * a + b] = h[sym9(a, b)];
+      store_items<81>(H, first, count, lds, [&](int k) { return h[sym9(k / 9, k % 9)]; });
     }
   }
-  if (psi) psi[i] = e;
-  if (P)
+  if (live && psi) psi[i] = e;
+  if (P) {
+    if constexpr (DIM == 2) {
+      if (live)
 #pragma unroll
-    for (int k = 0; k < D2; ++k) P[i * D2 + k] = p[k];
+        for (int k = 0; k < D2; ++k) P[i * D2 + k] = p[k];
+    } else {
+      store_items<9>(P, first, count, lds, [&](int k) { return p[k]; });
+    }
+  }
 }
 
 void launch_material_eval(int dim, int model, long long n, const double* F, const double* lam, const double* mu,
@@ -327,44 +333,36 @@ __global__ __launch_bounds__(kBlock) void k_tri_eval(int nv, int nt, const doubl
                                                      const int* __restrict__ cells, const double* __restrict__ u,
                                                      const double* __restrict__ lam, const double* __restrict__ mu,
                                                      double rho, int make_spsd, TriOut o) {
+  __shared__ double lds[64 * 37];
   const int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e >= nt) return;
-  int c[3];
-  bool ok = true;
+  const long long first = (long long)blockIdx.x * kBlock;
+  const int count = nt - (int)first < kBlock ? nt - (int)first : kBlock;
+  const bool live = e < nt;
+  int c[3] = {0, 0, 0};
+  bool ok = live;
+  if (live) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    c[a] = cells[3 * e + a];
-    ok = ok && c[a] >= 0 && c[a] < nv;
+    for (int a = 0; a < 3; ++a) {
+      c[a] = cells[3 * e + a];
+      ok = ok && c[a] >= 0 && c[a] < nv;
+    }
   }
-  if (!ok) {
-    const double nan = __builtin_nan("");
-    if (o.area) o.area[e] = nan;
-    if (o.energy) o.energy[e] = nan;
+  // every lane runs the arithmetic (the stores below synchronise the workgroup); a triangle that is not read
+  // computes from zeros and reports NaN
+  double x[3][2] = {}, y[3][2] = {};
+  double l = 1.0, m = 1.0;
+  if (ok) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (o.dminv) o.dminv[4 * e + k] = nan;
-      if (o.F) o.F[4 * e + k] = nan;
-    }
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int k = 0; k < 6; ++k)
-      if (o.grad) o.grad[6 * (long long)e + k] = nan;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      if (o.mass) o.mass[9 * (long long)e + k] = nan;
-      if (o.laplacian) o.laplacian[9 * (long long)e + k] = nan;
-    }
-    if (o.stiff)
-      for (int k = 0; k < 36; ++k) o.stiff[36 * (long long)e + k] = nan;
-    return;
+      for (int i = 0; i < 2; ++i) {
+        x[a][i] = X[2 * c[a] + i];
+        y[a][i] = x[a][i] + u[2 * c[a] + i];
+      }
+    l = lam[e];
+    m = mu[e];
   }
-  double x[3][2], y[3][2];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      x[a][i] = X[2 * c[a] + i];
-      y[a][i] = x[a][i] + u[2 * c[a] + i];
-    }
+  const double bad = ok ? 0.0 : __builtin_nan("");  // added to every output: NaN for a triangle that was not read
   // Dm = [x1 − x0, x2 − x0] by columns, its inverse by the adjugate
   const double m00 = x[1][0] - x[0][0], m01 = x[2][0] - x[0][0], m10 = x[1][1] - x[0][1], m11 = x[2][1] - x[0][1];
   const double det = m00 * m11 - m01 * m10;
@@ -377,7 +375,6 @@ __global__ __launch_bounds__(kBlock) void k_tri_eval(int nv, int nt, const doubl
 #pragma unroll
     for (int j = 0; j < 2; ++j) f[2 * i + j] = s0 * Di[j] + s1 * Di[2 + j];
   }
-  const double l = lam[e], m = mu[e];
   double P[4];
   const double psi = Model == kLinearElastic ? linear2_stress(f, l, m, P) : snh2_stress(f, l, m, P);
   double D[3][2];
@@ -387,26 +384,28 @@ __global__ __launch_bounds__(kBlock) void k_tri_eval(int nv, int nt, const doubl
     D[2][j] = Di[2 + j];
     D[0][j] = -(Di[j] + Di[2 + j]);
   }
-  if (o.area) o.area[e] = A;
-  if (o.energy) o.energy[e] = A * psi;
+  if (live) {
+    if (o.area) o.area[e] = A + bad;
+    if (o.energy) o.energy[e] = A * psi + bad;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (o.dminv) o.dminv[4 * e + k] = Di[k];
-    if (o.F) o.F[4 * e + k] = f[k];
+    for (int k = 0; k < 4; ++k) {
+      if (o.dminv) o.dminv[4 * (long long)e + k] = Di[k] + bad;
+      if (o.F) o.F[4 * (long long)e + k] = f[k] + bad;
+    }
   }
   if (o.grad)
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int i = 0; i < 2; ++i) o.grad[6 * (long long)e + 2 * a + i] = A * (P[2 * i] * D[a][0] + P[2 * i + 1] * D[a][1]);
-  if (o.mass || o.laplacian)
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        if (o.mass) o.mass[9 * (long long)e + 3 * a + b] = rho * A * (a == b ? 1.0 / 6.0 : 1.0 / 12.0);
-        if (o.laplacian) o.laplacian[9 * (long long)e + 3 * a + b] = A * (D[a][0] * D[b][0] + D[a][1] * D[b][1]);
-      }
+    store_items<6>(o.grad, first, count, lds, [&](int k) {
+      const int a = k / 2, i = k % 2;
+      return A * (P[2 * i] * D[a][0] + P[2 * i + 1] * D[a][1]) + bad;
+    });
+  if (o.mass)
+    store_items<9>(o.mass, first, count, lds,
+                   [&](int k) { return rho * A * (k / 3 == k % 3 ? 1.0 / 6.0 : 1.0 / 12.0) + bad; });
+  if (o.laplacian)
+    store_items<9>(o.laplacian, first, count, lds, [&](int k) {
+      const int a = k / 3, b = k % 3;
+      return A * (D[a][0] * D[b][0] + D[a][1] * D[b][1]) + bad;
+    });
   if (o.stiff) {
     double H[16];
     if (Model == kLinearElastic)
@@ -414,22 +413,16 @@ __global__ __launch_bounds__(kBlock) void k_tri_eval(int nv, int nt, const doubl
     else
       snh2_hessian(f, l, m, H);
     if (make_spsd) sym4_project_spd(H, kSpdFloor);
-    // HD[(c j), (b d)] = Σ_l H[(c j), (d l)]·D_b(l), then K = A·Σ_j D_a(j)·HD
+    // K[(a c), (b d)] = A·Σ_jl D_a(j)·H[(c j), (d l)]·D_b(l)
+    store_items<36>(o.stiff, first, count, lds, [&](int k) {
+      const int a = k / 12, ci = (k / 6) % 2, b = (k % 6) / 2, di = k % 2;
+      double v = 0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int ci = 0; ci < 2; ++ci)
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-#pragma unroll
-          for (int di = 0; di < 2; ++di) {
-            double v = 0;
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-#pragma unroll
-              for (int k = 0; k < 2; ++k) v += D[a][j] * H[4 * (2 * ci + j) + 2 * di + k] * D[b][k];
-            o.stiff[36 * (long long)e + 6 * (2 * a + ci) + 2 * b + di] = A * v;
-          }
+        for (int q = 0; q < 2; ++q) v += D[a][j] * H[4 * (2 * ci + j) + 2 * di + q] * D[b][q];
+      return A * v + bad;
+    });
   }
 }
 

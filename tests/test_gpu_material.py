@@ -49,6 +49,9 @@ def test_3d_models_match_oracle(model):
         assert per_item_rel_err(got["P"], np.array([r[0][1] for r in ref])) <= 1e-12
         H = np.array([oracle.spd_project9(r[1]) if spsd else r[1] for r in ref])
         assert per_item_rel_err(got["H"], H) <= (1e-10 if spsd else 1e-12), spsd
+        # without the Hessian output the stress is the same, bit for bit
+        lean = material.evaluate(F, lam, mu, model, make_spsd=spsd, want_H=False)
+        assert "H" not in lean and np.array_equal(lean["P"], got["P"]) and np.array_equal(lean["psi"], got["psi"])
 
 
 def test_invalid_requests_are_rejected():
