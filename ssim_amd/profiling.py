@@ -5,6 +5,9 @@
   python -m ssim_amd.profiling copy [MiB]                 device-to-device copy of a buffer larger than the
                                                           caches (default 512 MiB), three times, for calibrating
                                                           FETCH_SIZE / WRITE_SIZE against known bytes
+  python -m ssim_amd.profiling evaluators                 the element evaluators (ssim_tri_eval on ~1.0M
+                                                          triangles with every output, ssim_material_eval on 2^20
+                                                          items with Hessians), 50 timed launches per case
 """
 from __future__ import annotations
 
@@ -71,6 +74,72 @@ def copy(mib=512):
     print(f"copied {mib} MiB ({n * 8} bytes) x6")
 
 
+def evaluators(reps=50):
+    """Average launch time (hipEvents on the launching stream) and algorithmic bytes of the element evaluators:
+    per triangle 12 B of cells, 16 B of λ, μ and 70 doubles of outputs, plus X and u once; per material item
+    F, λ, μ, Ψ, P and H once."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from . import _lib
+
+    _lib.load()
+    dev = torch.device("cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ptr = lambda x: ctypes.c_void_p(x.data_ptr())
+    dev_t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+
+    def timed(call):
+        for _ in range(5):
+            call()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            call()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    print(f"{'case':44s} {'items':>9s} {'us/launch':>10s} {'MB':>8s} {'GB/s':>8s} {'% of 8 TB/s':>11s}", flush=True)
+    N = 708  # (N-1)^2 squares, two triangles each
+    xs, ys = np.meshgrid(np.linspace(0, 1, N), np.linspace(0, 1, N), indexing="ij")
+    X = np.stack([xs.ravel(), ys.ravel()], 1)
+    i, j = np.meshgrid(np.arange(N - 1), np.arange(N - 1), indexing="ij")
+    a = (i * N + j).ravel()
+    C = np.concatenate([np.stack([a, a + N, a + N + 1], 1), np.stack([a, a + N + 1, a + 1], 1)]).astype(np.int32)
+    rng = np.random.default_rng(0)
+    nv, nt = len(X), len(C)
+    Xd, Cd, ud = dev_t(X), dev_t(C), dev_t(rng.standard_normal(X.shape) * 1e-4)
+    lam, mu = dev_t(np.full(nt, 1e4)), dev_t(np.full(nt, 1e3))
+    widths = (1, 4, 4, 1, 6, 36, 9, 9)  # area, dminv, F, energy, grad, stiff, mass, laplacian
+    outs = [torch.empty(nt * w, dtype=torch.float64, device=dev) for w in widths]
+    tri_bytes = nt * (12 + 16 + 8 * sum(widths)) + nv * 32
+    for model, name in ((0, "stable_neohookean"), (1, "linear")):
+        for spsd in (0, 1):
+            us = timed(lambda: _lib.call("ssim_tri_eval", model, nv, nt, ptr(Xd), ptr(Cd), ptr(ud), ptr(lam),
+                                         ptr(mu), 1000.0, spsd, *[ptr(o) for o in outs], stream))
+            print(f"{'tri ' + name + (' projected' if spsd else ''):44s} {nt:9d} {us:10.1f} {tri_bytes / 1e6:8.1f} "
+                  f"{tri_bytes / us / 1e3:8.0f} {tri_bytes / us / 8e4:10.1f}%", flush=True)
+    n = 1 << 20
+    for dim in (3, 2):
+        F = dev_t(np.tile(np.eye(dim).ravel(), n) + 1e-2 * rng.standard_normal(n * dim * dim))
+        lm, mm = dev_t(np.full(n, 1e4)), dev_t(np.full(n, 1e3))
+        psi = torch.empty(n, dtype=torch.float64, device=dev)
+        P = torch.empty(n * dim * dim, dtype=torch.float64, device=dev)
+        H = torch.empty(n * dim ** 4, dtype=torch.float64, device=dev)
+        mat_bytes = n * 8 * (2 * dim * dim + 3 + dim ** 4)
+        for model, name in ((0, "stable_neohookean"), (1, "linear")):
+            for spsd in ((0, 1) if dim == 3 else (0,)):
+                us = timed(lambda: _lib.call("ssim_material_eval", dim, model, ctypes.c_longlong(n), ptr(F), ptr(lm),
+                                             ptr(mm), spsd, ptr(psi), ptr(P), ptr(H), stream))
+                label = f"material {dim}D {name}" + (" projected" if spsd else "")
+                print(f"{label:44s} {n:9d} {us:10.1f} {mat_bytes / 1e6:8.1f} {mat_bytes / us / 1e3:8.0f} "
+                      f"{mat_bytes / us / 8e4:10.1f}%", flush=True)
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1] if len(sys.argv) > 1 else ""
     if cmd == "trace":
@@ -79,6 +148,8 @@ if __name__ == "__main__":
         pmc(sys.argv[2:])
     elif cmd == "copy":
         copy(int(sys.argv[2]) if len(sys.argv) > 2 else 512)
+    elif cmd == "evaluators":
+        evaluators()
     else:
         print(__doc__)
         sys.exit(2)
