@@ -474,6 +474,54 @@ SSIM_HD void linear2_hessian(T lam, T mu, T H[16]) {
     }
 }
 
+// The SPD projection of the 2D stable Neo-Hookean Hessian in closed form.  H = d2·I + c4·f fᵀ + λ·g gᵀ + s3·H_J
+// with g = vec cof F = H_J·f, and H_J has the fixed eigenbasis b1 = (e0+e3)/√2, b2 = (e1−e2)/√2 (eigenvalue +1),
+// b3 = (e0−e3)/√2, b4 = (e1+e2)/√2 (−1).  In that basis f = (f₊, f₋) and g = (f₊, −f₋), so (p, 0) with p ⊥ f₊ is an
+// eigenvector for d2 + s3, (0, q) with q ⊥ f₋ one for d2 − s3, and the two left, spanned by (f̂₊, 0) and (0, f̂₋),
+// diagonalise the 2x2 block [[d2 + s3 + (c4+λ)|f₊|², (c4−λ)|f₊||f₋|], [·, d2 − s3 + (c4+λ)|f₋|²]] with one Jacobi
+// rotation.  H⁺ = Σ max(λ_i, floor)·w_i w_iᵀ, the matrix time_step.hpp:50-56 forms with an eigensolver.
+template <typename T>
+SSIM_HD void snh2_project_spd(const T F[4], T lam, T mu, T floor_, T H[16]) {
+  const T del = snh2_delta(lam, mu);
+  const T Ic = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + F[3] * F[3];
+  const T J = F[0] * F[3] - F[1] * F[2];
+  const T alpha = (T(1) - T(1) / (T(2) + del)) * mu / lam + T(1);
+  const T d2 = mu * (T(1) - T(1) / (Ic + del)), c4 = mu * (T(2) / (Ic + del) / (Ic + del)), s3 = lam * (J - alpha);
+  const T r = T(0.70710678118654752440);  // 1/√2
+  const T fp[2] = {r * (F[0] + F[3]), r * (F[1] - F[2])}, fm[2] = {r * (F[0] - F[3]), r * (F[1] + F[2])};
+  const T np_ = sqrt(fp[0] * fp[0] + fp[1] * fp[1]), nm = sqrt(fm[0] * fm[0] + fm[1] * fm[1]);
+  // unit directions (any unit vector where the component vanishes: the block is then already diagonal)
+  const T up[2] = {np_ > T(0) ? fp[0] / np_ : T(1), np_ > T(0) ? fp[1] / np_ : T(0)};
+  const T um[2] = {nm > T(0) ? fm[0] / nm : T(1), nm > T(0) ? fm[1] / nm : T(0)};
+  const T a = d2 + s3 + (c4 + lam) * np_ * np_, c = d2 - s3 + (c4 + lam) * nm * nm, b = (c4 - lam) * np_ * nm;
+  T t = T(0);
+  if (b != T(0)) {
+    const T theta = (c - a) / (T(2) * b);
+    t = fabs(theta) > T(1e150) ? T(0.5) / theta
+                               : (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
+  }
+  const T cs = T(1) / sqrt(t * t + T(1)), sn = t * cs;
+  // eigenvectors in the coordinates (b1, b2, b3, b4) and their eigenvalues
+  const T V[4][4] = {{-up[1], up[0], T(0), T(0)},
+                     {T(0), T(0), -um[1], um[0]},
+                     {cs * up[0], cs * up[1], -sn * um[0], -sn * um[1]},
+                     {sn * up[0], sn * up[1], cs * um[0], cs * um[1]}};
+  const T w[4] = {d2 + s3, d2 - s3, a - t * b, c + t * b};
+#pragma unroll
+  for (int k = 0; k < 16; ++k) H[k] = T(0);
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const T lw = w[n] > floor_ ? w[n] : floor_;
+    // back to F-space coordinates: Σ ṽ_i b_i
+    const T x[4] = {r * (V[n][0] + V[n][2]), r * (V[n][1] + V[n][3]), r * (V[n][3] - V[n][1]),
+                    r * (V[n][0] - V[n][2])};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) H[4 * i + j] += lw * x[i] * x[j];
+  }
+}
+
 // Cyclic Jacobi eigen-decomposition of a symmetric 4x4 matrix (Golub & Van Loan, Matrix Computations, 4th ed.,
 // §8.5.2-8.5.3: symmetric 2x2 Schur rotations swept row by row until the off-diagonal mass is below 1e-32 of the
 // total, at most 20 sweeps), then H⁺ = Q·max(Λ, floor)·Qᵀ: the SPD projection of the 2D element Hessian

@@ -408,11 +408,14 @@ __global__ __launch_bounds__(kBlock) void k_tri_eval(int nv, int nt, const doubl
     });
   if (o.stiff) {
     double H[16];
-    if (Model == kLinearElastic)
+    if (Model == kLinearElastic) {
       linear2_hessian(l, m, H);
-    else
+      if (make_spsd) sym4_project_spd(H, kSpdFloor);
+    } else if (make_spsd) {
+      snh2_project_spd(f, l, m, kSpdFloor, H);
+    } else {
       snh2_hessian(f, l, m, H);
-    if (make_spsd) sym4_project_spd(H, kSpdFloor);
+    }
     // K[(a c), (b d)] = A·Σ_jl D_a(j)·H[(c j), (d l)]·D_b(l)
     store_items<36>(o.stiff, first, count, lds, [&](int k) {
       const int a = k / 12, ci = (k / 6) % 2, b = (k % 6) / 2, di = k % 2;

@@ -56,3 +56,20 @@ def test_bad_index_null_outputs_and_empty():
     assert rel_err(part["stiff"][ok], oracle.tri_eval(X, C, u, 1e4, 1e3)["stiff"][ok]) <= 1e-10
     empty = tri.evaluate(X, np.zeros((0, 3), dtype=np.int32), u, 1e4, 1e3)
     assert all(v.shape[0] == 0 for v in empty.values())
+
+
+@pytest.mark.parametrize("model", ["stable_neohookean", "linear"])
+def test_projection_at_special_deformations(model):
+    """The clamp at F = I, a rotation, F = 0, a reflection and a uniform scaling (where the stable Neo-Hookean
+    closed form meets vanishing or repeated components) against the oracle's Jacobi route."""
+    X = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    th = 0.3
+    Fs = [np.eye(2), np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]]), np.zeros((2, 2)),
+          np.diag([1.0, -1.0]), 2.0 * np.eye(2), np.array([[0.0, 1.0], [1.0, 0.0]])]
+    for F in Fs:
+        u = X @ F.T - X  # Dm = I, so F = Ds
+        g = tri.evaluate(X, [[0, 1, 2]], u, 2e4, 3e3, model=model, make_spsd=True, want=("F", "stiff"))
+        o = oracle.tri_eval(X, [[0, 1, 2]], u, 2e4, 3e3, model=model, make_spsd=True)
+        assert np.abs(g["F"][0] - F).max() <= 1e-15
+        assert rel_err(g["stiff"], o["stiff"]) <= 1e-12, (model, F)
+        assert np.linalg.eigvalsh(g["stiff"][0]).min() >= -1e-9 * np.abs(g["stiff"]).max()
