@@ -134,7 +134,8 @@ ssim_status_t ssim_get_material_model(const ssim_ctx* ctx, ssim_material_t* mode
  * hessian_op() (elast/basic_elast.hpp:144-219, tests/fem_intro/main.cpp:302-309), in 3D or 2D (the Ndim = 2
  * models, elast/stable_neohookean.hpp:96-107 and elast/linear.hpp).  F: n row-major dim×dim matrices; lambda,
  * mu: n Lamé pairs; outputs (each may be NULL) Ψ [n], P [n][dim²] and the Hessian [n][dim⁴] with
- * H[(a·dim+c)·dim² + (b·dim+d)] = ∂²Ψ/∂F_ac∂F_bd; make_spsd: the time step's SPD projection (3D only).  All
+ * H[(a·dim+c)·dim² + (b·dim+d)] = ∂²Ψ/∂F_ac∂F_bd; make_spsd: the time step's SPD projection (eigenvalues of H
+ * clamped from below at 1e-3, time_step.hpp:41-59, in either dimension).  All
  * arrays in device memory; launched on `stream` (NULL: the null stream), not synchronised. */
 ssim_status_t ssim_material_eval(int dim, ssim_material_t model, long long n, const double* F, const double* lambda,
                                  const double* mu, int make_spsd, double* psi, double* P, double* H, void* stream);

@@ -252,7 +252,6 @@ ssim_status_t ssim_material_eval(int dim, ssim_material_t model, long long n, co
       fail(SSIM_ERR_INVALID_ARGUMENT, "unknown material model");
     if (n < 0) fail(SSIM_ERR_INVALID_ARGUMENT, "n must be non-negative");
     if (n > 0 && (!F || !lambda || !mu)) fail(SSIM_ERR_INVALID_ARGUMENT, "null input array");
-    if (make_spsd && dim != 3) fail(SSIM_ERR_INVALID_ARGUMENT, "the SPD projection is defined for dim 3 only");
     if (n == 0) return;
     launch_material_eval(dim, model, n, F, lambda, mu, make_spsd, psi, P, H, static_cast<hipStream_t>(stream));
     HIPCHK(hipGetLastError());

@@ -277,10 +277,14 @@ __global__ __launch_bounds__(kBlock) void k_material_eval(long long n, const dou
     e = Model == kLinearElastic ? linear2_stress(f, l, m, p) : snh2_stress(f, l, m, p);
     if (H) {
       double h[16];
-      if (Model == kLinearElastic)
+      if (Model == kLinearElastic) {
         linear2_hessian(l, m, h);
-      else
+        if (make_spsd) sym4_project_spd(h, kSpdFloor);
+      } else if (make_spsd) {
+        snh2_project_spd(f, l, m, kSpdFloor, h);
+      } else {
         snh2_hessian(f, l, m, h);
+      }
       store_items<16>(H, first, count, lds, [&](int k) { return h[k]; });
     }
   } else {

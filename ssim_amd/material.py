@@ -16,7 +16,8 @@ MODELS = {"stable_neohookean": 0, "linear": 1}
 
 def evaluate(F, lam, mu, model: str = "stable_neohookean", make_spsd: bool = False, want_H: bool = True):
     """F: (n, d, d) with d = 2 or 3, row-major matrices; lam, mu: scalars or (n,) arrays.  Returns a dict with
-    "psi" (n,), "P" (n, d, d) and, if requested, "H" (n, d², d²) with H[:, a·d+c, b·d+e] = ∂²Ψ/∂F_ac∂F_be."""
+    "psi" (n,), "P" (n, d, d) and, if requested, "H" (n, d², d²) with H[:, a·d+c, b·d+e] = ∂²Ψ/∂F_ac∂F_be; with
+    make_spsd the Hessians have their eigenvalues raised to at least 1e-3 (the time step's SPD projection)."""
     import torch
 
     if model not in MODELS:

@@ -1,8 +1,9 @@
 """Batched material evaluation on MI355X (``ssim_material_eval``, the vmap over a model's stress and Hessian
 operators) against the oracle, in 3D and 2D, for both models.
 
-Bars: Ψ, P and H within 1e-12 relative per item (norm-wise over the item); the SPD-projected 3D Hessian within
-1e-10 (the device projects in closed form in the SVD frame, the oracle by a 9x9 Jacobi eigen-decomposition).
+Bars: Ψ, P and H within 1e-12 relative per item (norm-wise over the item); the SPD-projected Hessians within
+1e-10 (3D: the device projects in closed form in the SVD frame, the oracle by a 9x9 Jacobi eigen-decomposition;
+2D: against numpy.linalg.eigh).
 """
 import numpy as np
 import pytest
@@ -54,10 +55,24 @@ def test_3d_models_match_oracle(model):
         assert "H" not in lean and np.array_equal(lean["P"], got["P"]) and np.array_equal(lean["psi"], got["psi"])
 
 
+@pytest.mark.parametrize("model", ["stable_neohookean", "linear"])
+def test_2d_projection_matches_numpy_eigh(model):
+    """The 2D clamp (closed form for stable Neo-Hookean, Jacobi for linear) against numpy.linalg.eigh of the
+    oracle's Hessian with eigenvalues raised to 1e-3, including F = I, a rotation, F = 0 and a reflection."""
+    F, lam, mu = _inputs(2, 500, 4)
+    F[:4] = [np.eye(2), [[np.cos(0.4), -np.sin(0.4)], [np.sin(0.4), np.cos(0.4)]], np.zeros((2, 2)), np.diag([1, -1])]
+    got = material.evaluate(F, lam, mu, model, make_spsd=True)
+    ref = []
+    for i in range(len(F)):
+        w, Q = np.linalg.eigh(oracle.material2(model, F[i], lam[i], mu[i])[2])
+        ref.append(Q @ np.diag(np.maximum(w, 1e-3)) @ Q.T)
+    assert per_item_rel_err(got["H"], np.array(ref)) <= 1e-10
+    plain = material.evaluate(F, lam, mu, model)
+    assert np.array_equal(got["P"], plain["P"]) and np.array_equal(got["psi"], plain["psi"])
+
+
 def test_invalid_requests_are_rejected():
     F, lam, mu = _inputs(2, 4, 3)
-    with pytest.raises(ValueError):
-        material.evaluate(F, lam, mu, make_spsd=True)
     with pytest.raises(ValueError):
         material.evaluate(F, lam, mu, model="corotated")
     with pytest.raises(ValueError):
