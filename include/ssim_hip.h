@@ -81,6 +81,9 @@ typedef struct {
                           Armijo test can no longer resolve energy decreases, as finite-difference checks of
                           gradients need.  A Hessian that is not positive definite there fails the PCG
                           (SSIM_ERR_NUMERIC) */
+  int pcg_precond;     /* preconditioner of the CG solves with the system matrix: 0 = Jacobi (default; mathprim
+                          diagonal_preconditioner, tests/fem_cu/main.cu:52-54), 1 = none (plain CG, as the
+                          reference's Python bindings, bindings/src/fem_host.cpp:50-51).  Batches take 0 only. */
 } ssim_solver_params;
 
 /* Counters of the last compute_step (mathprim optim_result of stepper/backward_euler.hpp:28-32, plus
@@ -94,6 +97,12 @@ typedef struct {
   int ls_failures;       /* line searches that exhausted ls_max trials */
   double grad_norm;      /* ‖forces‖₂ after the step */
   double energy;         /* incremental potential at the result */
+  double last_change;    /* |E_k − E_{k−1}|: change of the incremental potential over the last accepted iteration
+                            (optim_result::last_change_, stepper/backward_euler.hpp:31; 0 when no iteration ran) */
+  int ls_rejected;       /* Newton: 1 when the last line search exhausted its trials and its smallest step raised
+                            the energy; that step was not taken (u and forces stay at the last accepted iterate)
+                            and the solve stopped unconverged */
+  int reserved;
 } ssim_step_stats;
 
 const char* ssim_last_error(void);
@@ -193,6 +202,11 @@ ssim_status_t ssim_set_array_device(ssim_ctx* ctx, ssim_array_t which, const dou
 ssim_status_t ssim_device_array(ssim_ctx* ctx, ssim_array_t which, double** device_ptr);
 /* Dirichlet flags per dof (dof_type_, 0 = general, 1 = dirichlet; boundary.hpp:8-11) */
 ssim_status_t ssim_get_dof_type(const ssim_ctx* ctx, unsigned char* host_out);
+/* device pointer of the Dirichlet flags (3·nv bytes, owned by ctx, valid until destroy): dof_type()
+ * (time_step.hpp:279-280), which the reference's drivers write from device code (tests/fem_cu/main.cu:41-45).
+ * Flags written there take effect like ssim_mark_dirichlet / ssim_mark_general: at once for the kernels, and
+ * for the filtered mass matrix at the next ssim_reset. */
+ssim_status_t ssim_device_dof_type(ssim_ctx* ctx, unsigned char** device_ptr);
 
 ssim_status_t ssim_num_vertices(const ssim_ctx* ctx, int* nv);
 ssim_status_t ssim_num_cells(const ssim_ctx* ctx, int* nt);

@@ -86,6 +86,7 @@ def lib():
             "orc_lin2_stress": (ctypes.c_double, [_D, ctypes.c_double, ctypes.c_double, _D]),
             "orc_lin2_hessian": (None, [ctypes.c_double, ctypes.c_double, _D]),
             "orc_ts_step_next": (None, [_V]),
+            "orc_ts_set_damping": (ctypes.c_int, [_V, ctypes.c_double]),
             "orc_ts_rest_hessian_diag": (None, [_V, _D]),
             "orc_ts_spmv": (None, [_V, _D, _D, _D]),
             "orc_tri_eval": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I, _D, _D, _D, ctypes.c_double,
@@ -465,13 +466,19 @@ class TimeStep:
         lib().orc_ts_prepare_step(self._h)
 
     def compute_step(self, method="newton"):
-        st = np.zeros(6)
+        st = np.zeros(8)
         fn = {"newton": lib().orc_ts_compute_step, "lbfgs": lib().orc_ts_compute_step_lbfgs,
               "ncg": lib().orc_ts_compute_step_ncg, "forward_euler": lib().orc_ts_compute_step_fe,
               "lbfgs_pd": lib().orc_ts_compute_step_lbfgs_pd, "lbfgs_h0": lib().orc_ts_compute_step_lbfgs_h0}[method]
         gn = fn(self._h, _dp(st))
         return gn, dict(newton=int(st[0]), pcg=int(st[1]), evals=int(st[2]), converged=bool(st[3]),
-                        grad_norm=float(st[4]), ls_fail=int(st[5]))
+                        grad_norm=float(st[4]), ls_fail=int(st[5]), last_change=float(st[6]),
+                        ls_rejected=int(st[7]))
+
+    def set_damping(self, damping):
+        """Velocity damping of step_next (time_step.hpp:337-341, 541-542); ValueError outside [0, 1)."""
+        if lib().orc_ts_set_damping(self._h, float(damping)) != 0:
+            raise ValueError("velocity damping must be in [0, 1).")
 
     def step_next(self):
         lib().orc_ts_step_next(self._h)

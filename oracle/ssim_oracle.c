@@ -411,6 +411,7 @@ struct orc_ts {
   double *X, *lam, *mu, *B, *V;
   int* cells;
   double rho, dt;
+  double damping; /* velocity damping of step_next, in [0, 1) (0 after create, as the reference's default) */
   /* scalar CSR with the full 3x3 block pattern (finite_elements/global_composer.hpp:80-92) */
   int *rowptr, *colidx;
   int *brow, *bcol; /* block adjacency: brow[nv+1], bcol[nblocks] (sorted neighbour ids) */
@@ -871,7 +872,8 @@ double orc_ts_compute_step(orc_ts* ts, double* stats) {
   double* mg = (double*)malloc(sizeof(double) * n);
   double* ut = (double*)malloc(sizeof(double) * n);
   double* gt = (double*)malloc(sizeof(double) * n);
-  int newton = 0, pcg_total = 0, evals = 0, converged = 0, ls_fail = 0;
+  int newton = 0, pcg_total = 0, evals = 0, converged = 0, ls_fail = 0, ls_rejected = 0;
+  double last_change = 0.0;
   double E = orc_ts_energy_grad(ts, u, ts->g);
   evals++;
   for (; newton < ts->max_newton;) {
@@ -896,11 +898,20 @@ double orc_ts_compute_step(orc_ts* ts, double* stats) {
       }
       alpha *= ts->ls_beta;
     }
-    if (!ok) ls_fail++;
+    newton++;
+    if (!ok) {
+      ls_fail++;
+      /* every trial failed and the smallest still raises the energy: keep the iterate and stop (the same rule
+       * as the GPU solver; the behaviour of mathprim's backtracking_linesearcher on failure is not known) */
+      if (!(Et <= E)) {
+        ls_rejected = 1;
+        break;
+      }
+    }
     memcpy(u, ut, sizeof(double) * n);
     memcpy(ts->g, gt, sizeof(double) * n);
+    last_change = fabs(E - Et);
     E = Et;
-    newton++;
   }
   /* optional polish: full Newton steps (α = 1) with the unprojected Hessian, positive definite at the minimum
    * the loop has converged to, so they converge quadratically and take the solution to roundoff accuracy where
@@ -924,6 +935,8 @@ double orc_ts_compute_step(orc_ts* ts, double* stats) {
     stats[3] = converged;
     stats[4] = gn;
     stats[5] = ls_fail;
+    stats[6] = last_change;
+    stats[7] = ls_rejected;
   }
   free(d);
   free(mg);
@@ -1233,8 +1246,8 @@ double orc_ts_compute_step_fe(orc_ts* ts, double* stats) {
   return gn;
 }
 
-/* step_next: u_prev ← u; u ← u_next; v = (u − u_prev)/dt·(1 − damping), damping 0
- * (finite_elements/time_step.hpp:530-543) */
+/* step_next: u_prev ← u; u ← u_next; v = (u − u_prev)/dt, then v ← (1 − damping)·v
+ * (finite_elements/time_step.hpp:530-543: copy, axpy(−1, prev), scal(1/dt), scal(1 − damping)) */
 void orc_ts_step_next(orc_ts* ts) {
   for (int i = 0; i < ts->n; ++i) {
     ts->u_prev[i] = ts->u[i];
@@ -1242,8 +1255,16 @@ void orc_ts_step_next(orc_ts* ts) {
     double v = ts->u[i];
     v += -1.0 * ts->u_prev[i];
     v *= 1.0 / ts->dt;
+    v *= 1.0 - ts->damping;
     ts->vel[i] = v;
   }
+}
+
+/* set_damping (time_step.hpp:337-341; prepare_step checks 0 ≤ damping < 1, :499-500) */
+int orc_ts_set_damping(orc_ts* ts, double damping) {
+  if (!(damping >= 0.0 && damping < 1.0)) return 1;
+  ts->damping = damping;
+  return 0;
 }
 
 /* ---- P1 triangles in 2D: basic_time_step<..., PhysicalDim 2, TopologyDim 3>, element by element ----

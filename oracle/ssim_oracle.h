@@ -103,6 +103,8 @@ double orc_ts_compute_step_lbfgs(orc_ts* ts, double* stats);
 /* nonlinear CG (Dai-Kou DK+ β, backtracking Armijo with the L-BFGS constants); stats as compute_step */
 double orc_ts_compute_step_ncg(orc_ts* ts, double* stats);
 void orc_ts_step_next(orc_ts* ts);
+/* velocity damping in [0, 1) (returns 1 and keeps the old value otherwise) */
+int orc_ts_set_damping(orc_ts* ts, double damping);
 void orc_ts_rest_hessian_diag(orc_ts* ts, double* diag);
 
 /* timing helpers for the CPU baseline: one PCG iteration-equivalent (SpMV + vector work) etc. */

@@ -29,6 +29,7 @@ class SolverParams(ctypes.Structure):
         ("ls_c", ctypes.c_double), ("ls_beta", ctypes.c_double), ("ls_max", ctypes.c_int),
         ("lbfgs_m", ctypes.c_int), ("lbfgs_maxit", ctypes.c_int), ("lbfgs_c", ctypes.c_double),
         ("lbfgs_ls_max", ctypes.c_int), ("fp32_elements", ctypes.c_int), ("newton_polish", ctypes.c_int),
+        ("pcg_precond", ctypes.c_int),
     ]
 
 
@@ -36,11 +37,12 @@ class StepStats(ctypes.Structure):
     _fields_ = [
         ("converged", ctypes.c_int), ("newton_iterations", ctypes.c_int), ("pcg_iterations", ctypes.c_int),
         ("energy_evals", ctypes.c_int), ("hessian_evals", ctypes.c_int), ("ls_failures", ctypes.c_int),
-        ("grad_norm", ctypes.c_double), ("energy", ctypes.c_double),
+        ("grad_norm", ctypes.c_double), ("energy", ctypes.c_double), ("last_change", ctypes.c_double),
+        ("ls_rejected", ctypes.c_int), ("reserved", ctypes.c_int),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class LayoutInfo(ctypes.Structure):
@@ -94,6 +96,7 @@ SIGNATURES = {
     "ssim_set_array_device": [_V, ctypes.c_int, _V],
     "ssim_adjoint_step": [_V, _V, _V, _V, _V, _V, _I],
     "ssim_get_dof_type": [_V, _U8],
+    "ssim_device_dof_type": [_V, ctypes.POINTER(ctypes.POINTER(ctypes.c_ubyte))],
     "ssim_num_vertices": [_V, _I],
     "ssim_num_cells": [_V, _I],
     "ssim_total_mass": [_V, _D],

@@ -375,13 +375,28 @@ void bnewton(ssim_batch* b) {
       }
       if (!left) break;
     }
+    // As in the single-mesh solver: a mesh whose line search exhausted its trials takes the last trial if that
+    // did not raise its energy; otherwise it keeps its iterate and leaves the Newton loop unconverged.
+    bool any_rejected = false;
     for (int m = 0; m < M; ++m) {
+      search[m] = act[m] && search[m];
       if (!act[m]) continue;
-      if (search[m]) b->stats[m].ls_failures++;  // the last trial is taken, as in the single-mesh solver
+      if (search[m]) {
+        b->stats[m].ls_failures++;
+        if (!(Et[m] <= E[m])) {
+          b->stats[m].ls_rejected = 1;
+          b->stats[m].newton_iterations = it + 1;
+          act[m] = 0;
+          any_rejected = true;
+          continue;
+        }
+      }
+      b->stats[m].last_change = std::fabs(E[m] - Et[m]);
       E[m] = Et[m];
       gg[m] = ggt[m];
     }
-    launch_bcopy2(b->view(b->mask_newton), b->nvwg, c->utry.p, u, c->gtry.p, g, s);
+    if (any_rejected) upload_mask(b, b->mask_ls, act);  // copy back only the meshes that took their trial
+    launch_bcopy2(b->view(any_rejected ? b->mask_ls : b->mask_newton), b->nvwg, c->utry.p, u, c->gtry.p, g, s);
     HIPCHK(hipGetLastError());
   }
   for (int m = 0; m < M; ++m)
@@ -572,6 +587,7 @@ ssim_status_t ssim_batch_set_solver_params(ssim_batch* b, const ssim_solver_para
     if (p->max_newton < 0 || !(p->pcg_rtol > 0) || p->pcg_maxit < 0 || !(p->ls_c > 0 && p->ls_c < 1) ||
         !(p->ls_beta > 0 && p->ls_beta < 1) || p->ls_max < 1 || p->newton_polish < 0 || p->newton_polish > 16)
       fail(SSIM_ERR_INVALID_ARGUMENT, "solver parameters out of range");
+    if (p->pcg_precond != 0) fail(SSIM_ERR_INVALID_ARGUMENT, "batches solve with Jacobi-PCG only (pcg_precond = 0)");
     b->c.prm = *p;
   });
 }

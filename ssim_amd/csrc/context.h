@@ -100,6 +100,7 @@ struct ssim_ctx {
   ssim_hip::DBuf<double> arr[SSIM_ARRAY_COUNT];
   ssim_hip::DBuf<unsigned char> dof;
   ssim_hip::DBuf<double> utry, gtry, dir, r, z, p, q, tmp, dinv, hrest;
+  ssim_hip::DBuf<double> ones;  // unit diagonal: the preconditioner of plain CG (pcg_precond = 1)
   // device: assembly maps
   ssim_hip::DBuf<int> gperm, gseg, hperm, hseg, wu, offu, ucol, wl, offl, ub_pos;
   ssim_hip::DBuf<int2> ub_rc, lref;

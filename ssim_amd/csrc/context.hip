@@ -36,6 +36,7 @@ void ssim_hip::default_params(ssim_solver_params& p) {
   p.lbfgs_ls_max = 30;
   p.fp32_elements = 0;
   p.newton_polish = 0;
+  p.pcg_precond = 0;
 }
 
 void ssim_hip::check_material(double youngs, double poisson, double density) {
@@ -299,7 +300,8 @@ ssim_status_t ssim_set_solver_params(ssim_ctx* ctx, const ssim_solver_params* p)
     if (!p) fail(SSIM_ERR_INVALID_ARGUMENT, "params is null");
     if (p->max_newton < 0 || p->pcg_maxit < 0 || !(p->pcg_rtol >= 0) || !(p->ls_c > 0 && p->ls_c < 1) ||
         !(p->ls_beta > 0 && p->ls_beta < 1) || p->ls_max < 1 || p->lbfgs_m < 1 || p->lbfgs_maxit < 0 ||
-        !(p->lbfgs_c > 0 && p->lbfgs_c < 1) || p->lbfgs_ls_max < 1 || p->newton_polish < 0 || p->newton_polish > 16)
+        !(p->lbfgs_c > 0 && p->lbfgs_c < 1) || p->lbfgs_ls_max < 1 || p->newton_polish < 0 || p->newton_polish > 16 ||
+        p->pcg_precond < 0 || p->pcg_precond > 1)
       fail(SSIM_ERR_INVALID_ARGUMENT, "solver parameter out of range");
     const bool fp32_changed = (p->fp32_elements != 0) != (ctx->prm.fp32_elements != 0);
     ctx->prm = *p;
@@ -540,6 +542,16 @@ ssim_status_t ssim_device_array(ssim_ctx* ctx, ssim_array_t which, double** devi
     return SSIM_ERR_INVALID_ARGUMENT;
   }
   *device_ptr = ctx->a(which);
+  return SSIM_OK;
+}
+
+ssim_status_t ssim_device_dof_type(ssim_ctx* ctx, unsigned char** device_ptr) {
+  if (guard(ctx)) return SSIM_ERR_INVALID_ARGUMENT;
+  if (!device_ptr) {
+    g_err = "device_ptr is null";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
+  *device_ptr = ctx->dof.p;
   return SSIM_OK;
 }
 

@@ -53,6 +53,10 @@ struct PcgScalars {
   double rz, pq, alpha, beta, rr, bb, tol2, pad;
   double rz_hist[2];  // ρ_k in slot k % 2, written by kernel A of iteration k (ρ_0 by the initialisation)
   int done, iters, maxit, breakdown;  // read back together by the host
+  // Iteration at which the single-mesh PCG stopped (INT_MAX while it runs, −1 if it stops before iterating).
+  // Kernel A returns at entry only when its iteration is past stop_k, i.e. when an earlier launch stopped the
+  // solve; a stop published during the same launch (by its workgroup 0) never makes a workgroup skip its rows.
+  int stop_k, pad_i;
 };
 
 // Reduction results of element / vector kernels.

@@ -5,6 +5,8 @@
 // (wavefront shuffles, a fixed-order workgroup tree and a last-workgroup final pass) and atomic-free
 // assembly (every element writes its contributions to precomputed, target-sorted slots; a segmented reduce
 // sums each target's contiguous run).
+#include <climits>
+
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -653,6 +655,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fin(const double* __restric
     sc->maxit = maxit;
     sc->breakdown = 0;
     sc->done = (t[1] <= sc->tol2 || maxit <= 0 || !(t[1] == t[1])) ? 1 : 0;
+    sc->stop_k = sc->done ? -1 : INT_MAX;
   }
 }
 
@@ -681,7 +684,10 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
     double* __restrict__ pa, double* __restrict__ x) {
   const int k = it->ka;
   if (blockIdx.x == 0 && threadIdx.x == 0) it->kb = k;
-  if (sc->done) return;
+  // Only a stop from an earlier launch ends the work here: workgroup 0 of this launch may publish a stop at
+  // iteration k while other workgroups have yet to start, and those must still apply x_k = x_{k−1} + α p for
+  // their rows (below) -- they reach the same stop decision from the same partial sums.
+  if (k > sc->stop_k) return;
   const double* __restrict__ pb_cur = (k & 1) ? pb1 : pb0;
   const int lb = logical_block(RowMap<LPR>::blocks(A.nrows), A.xcd_remap);
   int v = 0, j = 0;
@@ -703,7 +709,10 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
       sc->rz = t_cur[0];
       sc->rz_hist[k & 1] = t_cur[0];
       if (nan) sc->breakdown = 2;
-      if (done) sc->done = 1;  // only ever 0 → 1: a workgroup that reads it early skips work that is not needed
+      if (done) {
+        sc->stop_k = k;  // launches of later iterations return at entry
+        sc->done = 1;    // read by kernel B of this iteration (the next launch) and by the host
+      }
     }
     // x_k = x_{k−1} + α_{k−1}·p_{k−1} for the own row (kernel B published α_{k−1})
     if (active) {
@@ -770,6 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
   if (!(pq[0] > 0.0)) {  // not positive definite (or NaN): every workgroup sees the same pq
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->breakdown = 1;
+      sc->stop_k = k;
       sc->done = 1;
     }
     return;

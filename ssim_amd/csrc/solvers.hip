@@ -98,13 +98,24 @@ static hipGraphExec_t pcg_chunk_graph(ssim_ctx* c, const BellView& A, const doub
   return exec;
 }
 
+// The preconditioner of plain CG (pcg_precond = 1): z = r, as a diagonal of ones in Jacobi's place.
+static const double* unit_diagonal(ssim_ctx* c) {
+  if (c->ones.count != (size_t)std::max(c->n, 1)) {
+    c->ones.alloc(std::max(c->n, 1));
+    std::vector<double> h(c->ones.count, 1.0);
+    HIPCHK(hipMemcpyAsync(c->ones.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+    sync(c);
+  }
+  return c->ones.p;
+}
+
 // Jacobi-PCG: x = A⁻¹ (sign·b) for the blocked-ELL values `val` with inverse diagonal `dinv` (default: the
 // current Hessian).  Returns iterations.
 int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit, const double* val,
               const double* dinv) {
   hipStream_t s = c->stream;
   if (!val) val = c->val.p;
-  if (!dinv) dinv = c->dinv.p;
+  if (!dinv) dinv = c->prm.pcg_precond == 1 ? unit_diagonal(c) : c->dinv.p;
   const BellView A = c->bell(val);
   const int nb = launch_pcg_init(c->n, b, sign, dinv, x, c->r.p, c->z.p, rtol, maxit, c->pcg.p, c->pcg_it.p,
                                  c->part_pb(0), s);
@@ -186,9 +197,17 @@ void newton_solve(ssim_ctx* c) {
       }
       alpha *= c->prm.ls_beta;
     }
-    if (!ok) c->stats.ls_failures++;
+    if (!ok) {
+      c->stats.ls_failures++;
+      if (!(trial.E <= cur.E)) {  // even the smallest trial raises the energy: keep the iterate and stop
+        c->stats.ls_rejected = 1;
+        ++it;
+        break;
+      }
+    }
     HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    c->stats.last_change = std::fabs(cur.E - trial.E);
     cur = trial;
   }
   c->stats.newton_iterations = it;
@@ -382,6 +401,7 @@ static void lbfgs_solve_gram(ssim_ctx* c) {
     launch_lbfgs_shift(n, c->utry.p, c->gtry.p, u, g, c->lb_S.p + (size_t)head * n, c->lb_Y.p + (size_t)head * n, s);
     HIPCHK(hipGetLastError());
     fresh = head;
+    c->stats.last_change = std::fabs(cur.E - trial.E);
     cur = trial;
   }
   c->stats.newton_iterations = it;
@@ -474,6 +494,7 @@ void lbfgs_solve(ssim_ctx* c, bool pd, bool h0) {
     }
     HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    c->stats.last_change = std::fabs(cur.E - trial.E);
     cur = trial;
   }
   c->stats.newton_iterations = it;
@@ -593,6 +614,7 @@ void ncg_solve(ssim_ctx* c, bool precond) {
     HIPCHK(hipMemcpyAsync(u, c->utry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipMemcpyAsync(g, c->gtry.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     if (precond) std::swap(pa, pb);
+    c->stats.last_change = std::fabs(cur.E - trial.E);
     cur = trial;
     alpha_prev = alpha;
     gd_prev = gd;

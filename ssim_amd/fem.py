@@ -141,6 +141,12 @@ class TetFiniteElementSolver_Hip:
     def set_rtol(self, rtol: float):
         _lib.call("ssim_set_threshold", self._h, float(rtol))
 
+    def set_damping(self, damping: float):
+        """Velocity damping of step_next, v ← (1 − damping)·v (basic_time_step::set_damping,
+        finite_elements/time_step.hpp:337-341, 541-542; not bound by the reference's Python module).  ValueError
+        outside [0, 1), as the reference's std::invalid_argument."""
+        _lib.call("ssim_set_damping", self._h, float(damping))
+
     def add_gravity(self, gravity: np.ndarray):
         g = np.asarray(gravity, dtype=np.float64).reshape(3)
         for d in range(3):

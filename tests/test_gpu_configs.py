@@ -3,15 +3,13 @@
 * FP32 element variant at the 786k-tet bar: max position error after 60 steps against the FP64 run of the same
   trajectory (reported; bounded at 1e-5 relative to the positions).
 * Batched stepping of the 256-bar batch: every mesh against the single-mesh path after one step (≤1e-8).
-* The differentiable rollout of configuration 5 (reduced bars by default; full size with SSIM_CONFIG5_FULL=1):
-  gradients against central differences (≤1e-5) and the oracle (≤1e-6).
+* The differentiable rollout of configuration 5 (4 reduced bars, and the full 32 bars of 16×16×64 cubes with 64
+  directions): gradients against central differences (≤1e-5) and the oracle (≤1e-6).
 * L-BFGS on a twisted cube with per-tet E (log-uniform in [1e4, 1e6]) and ν (uniform in [0.3, 0.48]), z = 0 face
   clamped: positions after 3 steps within 1e-8 of the oracle.  L-BFGS stops anywhere inside ‖g‖ < tol and its
   iterates depend on rounding (history, line-search decisions), so GPU and oracle are compared on tightly
   converged steps (threshold 1e-10 × mass), where both sit at the same minimiser.
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -151,8 +149,6 @@ def test_config5_structure_rollout_gradient_reduced():
     assert fd <= 1e-5 and orc <= 1e-6
 
 
-@pytest.mark.skipif(os.environ.get("SSIM_CONFIG5_FULL") != "1",
-                    reason="full-size configuration 5 check (minutes of GPU and CPU time): set SSIM_CONFIG5_FULL=1")
 def test_config5_full_rollout_gradient():
     """Configuration 5 at full size: 32 bars of 16×16×64 cubes, 20 steps; 64 finite-difference directions on bar 0.
     The oracle's gradient (CPU, minutes per step at this size) is checked on bar 0 over the first 2 steps."""
