@@ -62,6 +62,7 @@ struct DBuf {
 
 struct ssim_ctx {
   int nv = 0, nt = 0, n = 0;
+  int device = 0;  // the HIP device current at ssim_create
   double dt = 1e-3, youngs = 1e6, poisson = 0.33, density = 1e3, damping = 0.0;
   double rtol = 1e-2, tol_abs = 1e-2, total_mass = 0.0;
   double rho_reset = 0.0;  // density the mass terms use: captured at reset, as the reference's mass matrix
@@ -247,8 +248,18 @@ struct EvalOut {
   double E, gg;
 };
 EvalOut eval_energy_grad(ssim_ctx* c, const double* u, double* g);
+// work buffers of one PCG solve: residual r, preconditioned residual z, direction p, q = A p, the partial sums of
+// kernel A (pa) and of the residual ping-pong (pb0, pb1), and the device-side scalars and iteration counter
+struct PcgWork {
+  double *r, *z, *p, *q, *pa, *pb0, *pb1;
+  PcgScalars* sc;
+  PcgIter* it;
+};
+PcgWork ctx_pcg_work(ssim_ctx* c);
+void create_poll_events(ssim_ctx* c);  // the two events pcg_solve polls (created once, by reset)
+void ensure_unit_diagonal(ssim_ctx* c);  // the all-ones diagonal of plain CG (pcg_precond = 1)
 int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit,
-              const double* val = nullptr, const double* dinv = nullptr);
+              const double* val = nullptr, const double* dinv = nullptr, const PcgWork* work = nullptr);
 void newton_solve(ssim_ctx* c);
 void lbfgs_solve(ssim_ctx* c, bool pd, bool h0 = false);
 void ncg_solve(ssim_ctx* c, bool precond);

@@ -130,6 +130,7 @@ ssim_status_t ssim_create(ssim_ctx** out, const double* vertices, int nv, const 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) fail(SSIM_ERR_HIP, "no HIP device available");
     c = new ssim_ctx();
+    HIPCHK(hipGetDevice(&c->device));
     c->nv = nv;
     c->nt = nt;
     c->n = 3 * nv;
@@ -305,6 +306,7 @@ ssim_status_t ssim_set_solver_params(ssim_ctx* ctx, const ssim_solver_params* p)
       fail(SSIM_ERR_INVALID_ARGUMENT, "solver parameter out of range");
     const bool fp32_changed = (p->fp32_elements != 0) != (ctx->prm.fp32_elements != 0);
     ctx->prm = *p;
+    if (p->pcg_precond == 1) ensure_unit_diagonal(ctx);  // allocated here, so that no solve allocates
     if (fp32_changed) ctx->hrest.release();
   });
 }

@@ -62,19 +62,27 @@ void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_db
 }
 
 
+void create_poll_events(ssim_ctx* c) {
+  for (auto& e : c->poll_ev)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+// The context's own PCG work buffers.
+PcgWork ctx_pcg_work(ssim_ctx* c) {
+  return PcgWork{c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), c->part_pb(0), c->part_pb(1), c->pcg.p, c->pcg_it.p};
+}
+
 // Enqueues `count` PCG iterations (kernels A and B) on stream s.
-static void enqueue_pcg_iterations(ssim_ctx* c, const BellView& A, const double* dinv, double* x, int nb, int na, int count,
-                            hipStream_t s) {
+static void enqueue_pcg_iterations(ssim_ctx* c, const PcgWork& w, const BellView& A, const double* dinv, double* x,
+                                   int nb, int na, int count, hipStream_t s) {
   for (int j = 0; j < count; ++j) {
-    launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
-                    c->part_pa(), x, s);
-    launch_pcg_update(c->n, dinv, c->r.p, c->z.p, c->q.p, c->part_pa(), na, c->part_pb(0),
-                      c->part_pb(1), c->pcg_it.p, c->pcg.p, s);
+    launch_pcg_spmv(A, w.z, w.p, w.q, w.pb0, w.pb1, nb, w.it, w.sc, w.pa, x, s);
+    launch_pcg_update(c->n, dinv, w.r, w.z, w.q, w.pa, na, w.pb0, w.pb1, w.it, w.sc, s);
   }
 }
 
-// The graph of `count` iterations on matrix A writing x, captured on first use (buffers are fixed until the
-// next reset, which drops the cache).
+// The graph of `count` iterations on matrix A writing x with the context's work buffers, captured on first use
+// (buffers are fixed until the next reset, which drops the cache).
 static hipGraphExec_t pcg_chunk_graph(ssim_ctx* c, const BellView& A, const double* dinv, double* x, int nb, int na,
                                int count) {
   for (const auto& g : c->pcg_graphs)
@@ -83,7 +91,7 @@ static hipGraphExec_t pcg_chunk_graph(ssim_ctx* c, const BellView& A, const doub
   HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
   hipGraph_t graph = nullptr;
   try {
-    enqueue_pcg_iterations(c, A, dinv, x, nb, na, count, c->cap_stream);
+    enqueue_pcg_iterations(c, ctx_pcg_work(c), A, dinv, x, nb, na, count, c->cap_stream);
   } catch (...) {
     (void)hipStreamEndCapture(c->cap_stream, &graph);
     if (graph) (void)hipGraphDestroy(graph);
@@ -98,30 +106,35 @@ static hipGraphExec_t pcg_chunk_graph(ssim_ctx* c, const BellView& A, const doub
   return exec;
 }
 
-// The preconditioner of plain CG (pcg_precond = 1): z = r, as a diagonal of ones in Jacobi's place.
-static const double* unit_diagonal(ssim_ctx* c) {
+// The preconditioner of plain CG (pcg_precond = 1): z = r, as a diagonal of ones in Jacobi's place.  Made when
+// the parameter is set and at reset, never inside a solve.
+void ensure_unit_diagonal(ssim_ctx* c) {
   if (c->ones.count != (size_t)std::max(c->n, 1)) {
     c->ones.alloc(std::max(c->n, 1));
     std::vector<double> h(c->ones.count, 1.0);
     HIPCHK(hipMemcpyAsync(c->ones.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
     sync(c);
   }
+}
+static const double* unit_diagonal(ssim_ctx* c) {
+  if (!c->ones.p || c->ones.count != (size_t)std::max(c->n, 1)) fail(SSIM_ERR_STATE, "internal: plain-CG diagonal missing");
   return c->ones.p;
 }
 
 // Jacobi-PCG: x = A⁻¹ (sign·b) for the blocked-ELL values `val` with inverse diagonal `dinv` (default: the
-// current Hessian).  Returns iterations.
+// current Hessian).  Returns iterations.  `work` (default: the context's buffers) holds r, z, p, q, the partial
+// sums and the PCG scalars; with a caller's work the chunks are always launched kernel by kernel (no graphs).
 int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit, const double* val,
-              const double* dinv) {
+              const double* dinv, const PcgWork* work) {
   hipStream_t s = c->stream;
   if (!val) val = c->val.p;
   if (!dinv) dinv = c->prm.pcg_precond == 1 ? unit_diagonal(c) : c->dinv.p;
+  const PcgWork w = work ? *work : ctx_pcg_work(c);
+  const bool graphs = c->pcg_graph && !work;
   const BellView A = c->bell(val);
-  const int nb = launch_pcg_init(c->n, b, sign, dinv, x, c->r.p, c->z.p, rtol, maxit, c->pcg.p, c->pcg_it.p,
-                                 c->part_pb(0), s);
+  const int nb = launch_pcg_init(c->n, b, sign, dinv, x, w.r, w.z, rtol, maxit, w.sc, w.it, w.pb0, s);
   HIPCHK(hipGetLastError());
-  for (auto& e : c->poll_ev)
-    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  create_poll_events(c);
   const int na = pcg_spmv_grid(A);
   // Chunks of iterations are enqueued one ahead of the done-flag poll, so the GPU never drains while the host
   // reads a flag; kernels enqueued after convergence return at entry.  The first iteration of a chunk is launched
@@ -129,21 +142,19 @@ int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol,
   int* flags = reinterpret_cast<int*>(c->pin);  // two slots of 4 ints: done, iters, maxit, breakdown
   auto enqueue = [&](int count, int slot) {
     timed_ext(c, SSIM_KERNEL_PCG_SPMV, [&](hipEvent_t e0, hipEvent_t e1) {
-      launch_pcg_spmv(A, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), nb, c->pcg_it.p, c->pcg.p,
-                      c->part_pa(), x, s, e0, e1);
+      launch_pcg_spmv(A, w.z, w.p, w.q, w.pb0, w.pb1, nb, w.it, w.sc, w.pa, x, s, e0, e1);
     });
     timed_ext(c, SSIM_KERNEL_PCG_UPDATE, [&](hipEvent_t e0, hipEvent_t e1) {
-      launch_pcg_update(c->n, dinv, c->r.p, c->z.p, c->q.p, c->part_pa(), na, c->part_pb(0),
-                        c->part_pb(1), c->pcg_it.p, c->pcg.p, s, e0, e1);
+      launch_pcg_update(c->n, dinv, w.r, w.z, w.q, w.pa, na, w.pb0, w.pb1, w.it, w.sc, s, e0, e1);
     });
     if (count > 1) {
-      if (c->pcg_graph)
+      if (graphs)
         HIPCHK(hipGraphLaunch(pcg_chunk_graph(c, A, dinv, x, nb, na, count - 1), s));
       else
-        enqueue_pcg_iterations(c, A, dinv, x, nb, na, count - 1, s);
+        enqueue_pcg_iterations(c, w, A, dinv, x, nb, na, count - 1, s);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(flags + 4 * slot, &c->pcg.p->done, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(flags + 4 * slot, &w.sc->done, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(c->poll_ev[slot], s));
   };
   int chunk = 8, launched = 0, slot = 0;

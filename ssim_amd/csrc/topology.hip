@@ -348,6 +348,8 @@ void ssim_hip::do_reset(ssim_ctx* c) {
   c->pcg.alloc(1);
   c->pcg_it.alloc(1);
   c->ev.alloc(1);
+  create_poll_events(c);
+  if (c->prm.pcg_precond == 1) ensure_unit_diagonal(c);
   HIPCHK(hipMemsetAsync(c->pcg.p, 0, sizeof(PcgScalars), s));
   HIPCHK(hipMemsetAsync(c->pcg_it.p, 0, sizeof(PcgIter), s));
   HIPCHK(hipMemsetAsync(c->ev.p, 0, sizeof(EvalScalars), s));
