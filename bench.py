@@ -115,10 +115,33 @@ def host_cpu():
             "usable_cpus": usable}
 
 
+def cpu_steps(case, state, threads, rtol, nsteps=3):
+    """Times whole oracle time steps (prepare + Newton + step_next, the reference's step(), fem_host.cpp:102-111)
+    of the same bar on `threads` OpenMP threads, starting from the GPU solver's state at the start of its timed
+    window, so that the CPU takes the first `nsteps` of the steps the GPU timed; the median of steps 2..nsteps
+    after the first (warm-up) one, as BASELINE.md's CPU plan asks.  Returns (median seconds, per-step records)."""
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    ts = oracle.TimeStep.from_case(case)
+    ts.set_threshold(rtol)
+    ts.set_state(*state)
+    rec = []
+    for k in range(nsteps):
+        t0 = time.perf_counter()
+        _, st = ts.step()
+        rec.append({"s": time.perf_counter() - t0, "newton": st["newton"], "pcg": st["pcg"]})
+        print(f"[bench] CPU oracle step {k + 1}/{nsteps}: {rec[-1]['s']:.2f} s, {st['newton']} Newton, "
+              f"{st['pcg']} PCG", file=sys.stderr, flush=True)
+    return float(np.median([r["s"] for r in rec[1:]])), rec
+
+
 def cpu_sample(case, counts, threads, rtol):
     """Times the oracle's phases on the same mesh (one Hessian assembly, two energy+gradient evaluations, 20 PCG
     iterations) and composes the CPU time of the GPU-timed steps from their Newton / PCG / evaluation counts; the
-    same phases once more on one thread (one evaluation, one assembly, 5 PCG iterations)."""
+    same phases once more on one thread (one evaluation, one assembly, 5 PCG iterations).  A cross-check of the
+    timed whole steps of cpu_steps."""
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -230,6 +253,8 @@ def cpu_adjoint(rc, rtol, threads, adj_pcg_per_step, steps=2):
     ts.set_ext_accel(case.accel)
     ts.set_state(case.u0, None, case.v0)
     ts.set_threshold(rtol)
+    ts.set_solver(pcg_rtol=FD_SETTINGS["pcg_rtol"])
+    ts.set_polish(FD_SETTINGS["newton_polish"])
     t0 = time.perf_counter()
     for _ in range(steps):
         ts.step()
@@ -282,6 +307,7 @@ def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
     tuned = autotune.tune(B, repeats=1, cache=None)
     tune_s = time.perf_counter() - t0
     print(f"[bench] batched: autotuned {tuned} in {tune_s:.1f} s", file=sys.stderr, flush=True)
+    # both paths take the same steps 2..steps+1 from the same state (the autotuner restored the batch's)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tot = dict(newton_iterations=0, pcg_iterations=0)
@@ -292,12 +318,18 @@ def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
         tot["pcg_iterations"] += st["pcg_iterations"]
     torch.cuda.synchronize()
     tb = (time.perf_counter() - t0) / steps
+    seq_newton = seq_pcg = 0
     t0 = time.perf_counter()
-    for s in singles:
-        s.step()
+    for _ in range(steps):
+        for s in singles:
+            s.step()
+            st = s.last_stats()
+            seq_newton += st["newton_iterations"]
+            seq_pcg += st["pcg_iterations"]
     torch.cuda.synchronize()
-    ts = time.perf_counter() - t0
-    print(f"[bench] batched {tb:.3f} s/step, sequential {ts:.3f} s/step", file=sys.stderr, flush=True)
+    ts = (time.perf_counter() - t0) / steps
+    print(f"[bench] batched {tb:.3f} s/step, sequential {ts:.3f} s/step (same {steps} steps)", file=sys.stderr,
+          flush=True)
     return {
         "workload": "256 bars, dims uniform in 4..24, per-bar E log-uniform [1e4,1e6], nu uniform [0.3,0.48]",
         "tuning": tuned,
@@ -309,9 +341,12 @@ def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
         "batched_steps_per_s": 1.0 / tb,
         "sequential_steps_per_s": 1.0 / ts,
         "speedup_vs_sequential": ts / tb,
-        "timed_batched_steps": steps,
+        "timed_steps": steps,
+        "timed_window": f"steps 2-{steps + 1} of every mesh, batched and sequential",
         "lockstep_newton_iterations": tot["newton_iterations"],
         "lockstep_pcg_iterations": tot["pcg_iterations"],
+        "sequential_newton_iterations": seq_newton,
+        "sequential_pcg_iterations": seq_pcg,
         "setup_s": setup_b,
         "per_mesh_max_position_rel_err": worst,
         "newton_count_mismatches": mism,
@@ -391,15 +426,24 @@ def bench_fp32(case, rtol, total=60):
     }
 
 
-def bench_adjoint(rtol, nbar=32, seed=0, cpu_threads=None):
-    """Forward and adjoint (backward) time of a 20-step differentiable rollout of 32 bars of 16x16x64 cubes."""
+# the settings at which the adjoint gradient of configuration 5 is checked against central differences
+# (tests/test_gpu_configs.py): Newton threshold 1e-8·mass, PCG to 1e-12, two polishing Newton steps
+FD_SETTINGS = dict(rtol=1e-8, pcg_rtol=1e-12, newton_polish=2)
+
+
+def bench_adjoint(nbar=32, seed=0, cpu_threads=None, checkpoint_every=1, rc=None):
+    """Forward and adjoint (backward) time of a 20-step differentiable rollout of 32 bars of 16x16x64 cubes at the
+    finite-difference-verified solver settings; with checkpoint_every > 1 the backward pass recomputes the states
+    between checkpoints."""
     import torch
 
     from ssim_amd import adjoint, batch, mesh
 
-    rc = mesh.rollout_batch(nbar, seed=seed)
+    rtol = FD_SETTINGS["rtol"]
+    rc = rc or mesh.rollout_batch(nbar, seed=seed)
     B = batch.TetFiniteElementBatch_Hip.from_cases(rc.cases, rtol=rtol)
-    runner = adjoint.Rollout(B, rc.nu, rc.steps, rc.cases[0].dt)
+    B.set_solver_params(pcg_rtol=FD_SETTINGS["pcg_rtol"], newton_polish=FD_SETTINGS["newton_polish"])
+    runner = adjoint.Rollout(B, rc.nu, rc.steps, rc.cases[0].dt, checkpoint_every=checkpoint_every)
     idx, tgt, off = [], [], 0
     for c, fe, t in zip(rc.cases, rc.free_end, rc.targets):
         idx.append(off + fe)
@@ -422,13 +466,17 @@ def bench_adjoint(rtol, nbar=32, seed=0, cpu_threads=None):
     L.backward()
     torch.cuda.synchronize()
     bwd = time.perf_counter() - t0
-    print(f"[bench] adjoint: forward {fwd:.2f} s, backward {bwd:.2f} s", file=sys.stderr, flush=True)
+    print(f"[bench] adjoint (checkpoint every {checkpoint_every}): forward {fwd:.2f} s, backward {bwd:.2f} s",
+          file=sys.stderr, flush=True)
     return {
-        "workload": (f"{nbar} bars of 16x16x64 cubes ({rc.cases[0].nt} tets each), per-tet log E uniform in "
+        "workload": (f"{len(rc.cases)} bars of 16x16x64 cubes ({rc.cases[0].nt} tets each), per-tet log E uniform in "
                      "[ln 1e4, ln 1e6], nu=0.45, 20 steps, loss sum over free-end vertices of |u - t|^2, |t|=0.02 m"),
         "tets": int(sum(c.nt for c in rc.cases)),
         "steps": rc.steps,
-        "newton_tol": f"{rtol:g}*mass per mesh",
+        "settings": (f"Newton tol {rtol:g}*mass per mesh, PCG rtol {FD_SETTINGS['pcg_rtol']:g}, "
+                     f"{FD_SETTINGS['newton_polish']} polishing Newton steps (the finite-difference-checked settings)"),
+        "checkpoint_every": checkpoint_every,
+        "recomputed_forward_steps": rc.steps - len(range(0, rc.steps, checkpoint_every)),
         "loss": loss,
         "forward_s": fwd,
         "backward_s": bwd,
@@ -451,6 +499,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="whole oracle steps timed for cpu_baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
     ap.add_argument("--no-lbfgs", action="store_true", help="skip the L-BFGS twisted-cube leg")
@@ -475,17 +524,23 @@ def main():
     for k in range(args.warmup):
         solver.step()
         print(f"[bench] warmup step {k}: {solver.last_stats()}", file=sys.stderr, flush=True)
+    # state at the start of the timed window: the CPU baseline steps from here (copied before timing starts)
+    state0 = tuple(solver.get_array(a) for a in ("deformation", "prev_deform", "velocity"))
     solver.set_kernel_timing(True)
     counts = dict(steps=args.steps, newton_iterations=0, pcg_iterations=0, energy_evals=0, hessian_evals=0)
+    step_ends, step_stats = [], []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        solver.step()
+        solver.step()  # returns after the step's last host read-back, so the step's wall time is its own
         st = solver.last_stats()
+        step_ends.append(time.perf_counter())
+        step_stats.append(st)
         for key in ("newton_iterations", "pcg_iterations", "energy_evals", "hessian_evals"):
             counts[key] += st[key]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    step_s = np.diff([t0] + step_ends)
     ktimes = solver.kernel_timing()
     solver.set_kernel_timing(False)
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s; counts {counts}", file=sys.stderr, flush=True)
@@ -509,13 +564,37 @@ def main():
     }
     cpu = None
     if not args.no_cpu:
-        print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
-        cpu = cpu_sample(case, counts, args.cpu_threads, args.rtol)
+        print("[bench] timing the CPU oracle: whole steps from the timed window's start", file=sys.stderr, flush=True)
+        nsteps = min(args.cpu_steps, args.steps)
+        med, rec = cpu_steps(case, state0, args.cpu_threads, args.rtol, nsteps)
+        composed = cpu_sample(case, counts, args.cpu_threads, args.rtol)
+        gpu_med = float(np.median(step_s[1:nsteps]))
+        cpu = {
+            "value": 1.0 / med,
+            "unit": "steps/s",
+            "cores": composed["cores"],
+            "kind": "port",
+            "sample": (f"oracle whole time steps 1-{nsteps} of the GPU's timed window (started from the GPU state "
+                       f"after warm-up), {composed['cores']} OpenMP threads: median of steps 2-{nsteps} = {med:.2f} s "
+                       f"per step; per step {[round(r['s'], 2) for r in rec]} s, Newton {[r['newton'] for r in rec]}, "
+                       f"PCG {[r['pcg'] for r in rec]}"),
+            "gpu_same_steps": {"median_s": gpu_med, "newton": [st["newton_iterations"] for st in step_stats[:nsteps]],
+                               "pcg": [st["pcg_iterations"] for st in step_stats[:nsteps]],
+                               "gpu_over_cpu": med / gpu_med},
+            "composed": {k: composed[k] for k in ("value", "unit", "sample", "one_thread")},
+            "host": composed["host"],
+        }
     del solver
     fp32 = None if args.no_fp32 else bench_fp32(case, args.rtol)
     cpu_threads = None if args.no_cpu else args.cpu_threads
     batched = None if args.no_batched else bench_batched(args.rtol, cpu_threads=cpu_threads)
-    adj = None if args.no_adjoint else bench_adjoint(args.rtol, cpu_threads=cpu_threads)
+    adj = None
+    if not args.no_adjoint:
+        from ssim_amd import mesh as _mesh
+
+        rc = _mesh.rollout_batch(32, seed=args.seed)
+        adj = bench_adjoint(cpu_threads=cpu_threads, rc=rc)
+        adj["with_recompute"] = bench_adjoint(checkpoint_every=5, rc=rc)
     lbfgs = None if args.no_lbfgs else bench_lbfgs(args.rtol, cpu_threads=cpu_threads)
     # GPU / CPU ratios of the extension legs (their cpu_baseline objects are bounded samples, as for the main line)
     if batched and batched.get("cpu_baseline"):
