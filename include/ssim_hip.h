@@ -248,6 +248,13 @@ ssim_status_t ssim_eval_elements(ssim_ctx* ctx, const double* u, int make_spsd, 
 ssim_status_t ssim_pcg_solve(ssim_ctx* ctx, const double* b, double* x, double rtol, int maxit, int* iters_out);
 /* y = H x with the current system matrix (host arrays) */
 ssim_status_t ssim_spmv(ssim_ctx* ctx, const double* x, double* y);
+/* ssim_pcg_solve on DEVICE vectors b and x (3·nv doubles each) that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (HIP graph-capture setups, serving loops, harnesses
+ * that forbid hipMalloc).  Its work vectors, partial sums and PCG scalars live in a static 64 MiB device array
+ * per GPU (up to about 2.0M dofs; larger meshes return SSIM_ERR_INVALID_ARGUMENT).  Same kernels and results as
+ * ssim_pcg_solve.  Calls that overlap on one GPU share that array and race. */
+ssim_status_t ssim_pcg_solve_noalloc(ssim_ctx* ctx, const double* b, double* x, double rtol, int maxit,
+                                     int* iters_out);
 
 /* ---- adjoint of one implicit step (no reference counterpart: the reference is forward-only) ----
  * At the converged u* = next_deform of the last compute_step, the step solves g(u; û, λ, μ) = 0 on the free dofs

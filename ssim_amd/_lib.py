@@ -111,6 +111,7 @@ SIGNATURES = {
     "ssim_device_mesh": [_V, ctypes.POINTER(_D), ctypes.POINTER(_I)],
     "ssim_eval_elements": [_V, _D, ctypes.c_int, _D, _D, _D, _D],
     "ssim_pcg_solve": [_V, _D, _D, ctypes.c_double, ctypes.c_int, _I],
+    "ssim_pcg_solve_noalloc": [_V, _V, _V, ctypes.c_double, ctypes.c_int, _I],
     "ssim_spmv": [_V, _D, _D],
     "ssim_set_kernel_timing": [_V, ctypes.c_int],
     "ssim_get_kernel_timing": [_V, ctypes.c_int, _D, ctypes.POINTER(ctypes.c_longlong)],

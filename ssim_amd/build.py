@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libssim_hip.so")
-SOURCES = ["kernels.hip", "batch_kernels.hip", "context.hip", "topology.hip", "solvers.hip", "batch.hip"]
+SOURCES = ["kernels.hip", "batch_kernels.hip", "context.hip", "topology.hip", "solvers.hip", "batch.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

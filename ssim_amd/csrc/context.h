@@ -258,6 +258,7 @@ struct PcgWork {
 PcgWork ctx_pcg_work(ssim_ctx* c);
 void create_poll_events(ssim_ctx* c);  // the two events pcg_solve polls (created once, by reset)
 void ensure_unit_diagonal(ssim_ctx* c);  // the all-ones diagonal of plain CG (pcg_precond = 1)
+void noalloc_scratch_init();             // noalloc.hip: the current device's address of the no-alloc scratch
 int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol, int maxit,
               const double* val = nullptr, const double* dinv = nullptr, const PcgWork* work = nullptr);
 void newton_solve(ssim_ctx* c);

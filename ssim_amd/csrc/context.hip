@@ -131,6 +131,7 @@ ssim_status_t ssim_create(ssim_ctx** out, const double* vertices, int nv, const 
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) fail(SSIM_ERR_HIP, "no HIP device available");
     c = new ssim_ctx();
     HIPCHK(hipGetDevice(&c->device));
+    noalloc_scratch_init();
     c->nv = nv;
     c->nt = nt;
     c->n = 3 * nv;
