@@ -27,8 +27,14 @@
  *   orc_tri_eval         (tests/test_oracle_tri.py) area / Dm⁻¹ against numpy, affine F, closed-form mass, the
  *                        cotangent Laplacian, central FD of the energy and gradient, numpy.linalg.eigh for the
  *                        clamp, rotation invariance.
- * The solver trajectories of mathprim (the reference's external solver library, not vendored) are
- * parity unpinned: iteration order, stopping norm and line-search constants follow the benchmark configuration.
+ *   prepare_step,        (tests/test_oracle_stepping.py) free fall from rest: u_n = ½dt²g·n(n+1), v_n = n·dt·g and
+ *   step_next, damping   the damped closed form; a clamped, damped linear bar settles at K_ff u_f = M_ff g_f with
+ *                        K and M assembled independently (pins the dt² scaling of the elastic energy).
+ * Parity unpinned:
+ *   - the solver trajectories of mathprim (the reference's external solver library, not vendored): iteration
+ *     order, stopping norm and line-search constants follow the benchmark configuration;
+ *   - snh2_delta, the 2D stable Neo-Hookean δ: the 2D checks hold for any δ, and the reference states no property
+ *     its quadratic encodes.
  */
 #include "ssim_oracle.h"
 

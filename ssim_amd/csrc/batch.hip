@@ -137,11 +137,11 @@ void beval(ssim_batch* b, const DBuf<int>& mask, const double* u, double* g) {
   hipStream_t s = c->stream;
   const BatchView bt = b->view(mask);
   timed(c, SSIM_KERNEL_ELEM_GRAD, [&] {
-    launch_belem_grad(c->mesh(), bt, b->ntwg, u, c->a(SSIM_ARRAY_INERTIA), c->dt * c->dt, c->gperm.p,
-                      c->gcontrib.p, c->part_e(), c->prm.fp32_elements != 0, s);
+    launch_belem_grad_tile(c->mesh(), bt, b->ntwg, u, c->a(SSIM_ARRAY_INERTIA), c->dt * c->dt, c->gt_slot.p,
+                           c->gt_off.p, c->gt_run.p, c->gpart.p, c->part_e(), c->prm.fp32_elements != 0, s);
   });
   timed(c, SSIM_KERNEL_GRAD_REDUCE, [&] {
-    launch_bgrad_reduce(bt, b->nvwg, c->gseg.p, c->gcontrib.p, c->dof.p, g, c->part_g(), s);
+    launch_bgrad_gather(bt, b->nvwg, c->gv_seg.p, c->gv_pos.p, c->gpart.p, c->dof.p, g, c->part_g(), s);
   });
   launch_bfinalize_eval(bt, c->part_e(), c->part_g(), b->ev.p, s);
   HIPCHK(hipGetLastError());
@@ -771,9 +771,7 @@ ssim_status_t ssim_batch_set_material(ssim_batch* b, const double* lambda, const
       }
       off += b->nt_m[m];
     }
-    c->lam.upload(c->hlam, c->stream);
-    c->mu.upload(c->hmu, c->stream);
-    sync(c);
+    upload_material(c);  // also decides whether the material is uniform
   });
 }
 

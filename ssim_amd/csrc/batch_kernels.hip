@@ -28,19 +28,6 @@ __device__ __forceinline__ size_t bdof_w(int w, int j) {
 // ------------------------------------------------------------------------------------------------------
 // energy + gradient
 
-template <typename T, int Model>
-__global__ __launch_bounds__(kBlock) void k_belem_grad(MeshView m, BatchView bt, const double* __restrict__ u,
-                                                       const double* __restrict__ uhat, double dt2,
-                                                       const int* __restrict__ gperm, double* __restrict__ gcontrib,
-                                                       double* __restrict__ part_e) {
-  const int mesh = bt.twg_mesh[blockIdx.x];
-  if (!bt.mask[mesh]) return;
-  const int e = blockIdx.x * kBlock + threadIdx.x;
-  double energy[1] = {0.0};
-  if (e < bt.tend[mesh]) energy[0] = elem_grad_tet<T, Model>(m, e, u, uhat, dt2, gperm, gcontrib);
-  store_partials<1>(energy, part_e);
-}
-
 __global__ __launch_bounds__(kBlock) void k_bgrad_reduce(BatchView bt, const int* __restrict__ gseg,
                                                          const double* __restrict__ gcontrib,
                                                          const unsigned char* __restrict__ dof, double* __restrict__ g,
@@ -48,6 +35,33 @@ __global__ __launch_bounds__(kBlock) void k_bgrad_reduce(BatchView bt, const int
   if (!bt.mask[bt.vwg_mesh[blockIdx.x]]) return;  // uniform over the workgroup
   const int v0 = blockIdx.x * kBlock;
   double gg[1] = {grad_reduce_vertices(v0, v0 + kBlock, gseg, gcontrib, dof, g)};
+  store_partials<1>(gg, part_g);
+}
+
+// the tile-reduced evaluation (single-mesh k_elem_grad_tile): tet workgroup b is tile b of the concatenated mesh,
+// whose tets and vertices all belong to one mesh
+template <typename T, int Model>
+__global__ __launch_bounds__(kBlock) void k_belem_grad_tile(MeshView m, BatchView bt, const double* __restrict__ u,
+                                                            const double* __restrict__ uhat, double dt2,
+                                                            const ushort4* __restrict__ tslot,
+                                                            const int* __restrict__ gt_off,
+                                                            const int* __restrict__ gt_run, double* __restrict__ gpart,
+                                                            double* __restrict__ part_e) {
+  const int mesh = bt.twg_mesh[blockIdx.x];
+  if (!bt.mask[mesh]) return;  // uniform over the workgroup
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  double energy[1] = {elem_grad_tile<T, Model>(m, blockIdx.x, e < bt.tend[mesh], u, uhat, dt2, tslot, gt_off, gt_run,
+                                               gpart)};
+  store_partials<1>(energy, part_e);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bgrad_gather(BatchView bt, const int* __restrict__ gv_seg,
+                                                         const int* __restrict__ gv_pos,
+                                                         const double* __restrict__ gpart,
+                                                         const unsigned char* __restrict__ dof, double* __restrict__ g,
+                                                         double* __restrict__ part_g) {
+  if (!bt.mask[bt.vwg_mesh[blockIdx.x]]) return;  // uniform over the workgroup
+  double gg[1] = {grad_gather_vertex(blockIdx.x * kBlock + threadIdx.x, gv_seg, gv_pos, gpart, dof, g)};
   store_partials<1>(gg, part_g);
 }
 
@@ -366,13 +380,21 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const int*
 // ------------------------------------------------------------------------------------------------------
 // launchers
 
-void launch_belem_grad(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* uhat,
-                       double dt2, const int* gperm, double* gcontrib, double* part_e, bool fp32, hipStream_t s) {
-  if (ntwg <= 0) return;
+
+int launch_belem_grad_tile(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* uhat,
+                           double dt2, const ushort4* tslot, const int* gt_off, const int* gt_run, double* gpart,
+                           double* part_e, bool fp32, hipStream_t s) {
   dispatch_elem(m.model, fp32, [&](auto k) {
     using K = decltype(k);
-    k_belem_grad<typename K::type, K::model><<<ntwg, kBlock, 0, s>>>(m, bt, u, uhat, dt2, gperm, gcontrib, part_e);
+    k_belem_grad_tile<typename K::type, K::model>
+        <<<ntwg, kBlock, 0, s>>>(m, bt, u, uhat, dt2, tslot, gt_off, gt_run, gpart, part_e);
   });
+  return ntwg;
+}
+
+void launch_bgrad_gather(const BatchView& bt, int nvwg, const int* gv_seg, const int* gv_pos, const double* gpart,
+                         const unsigned char* dof, double* g, double* part_g, hipStream_t s) {
+  k_bgrad_gather<<<nvwg, kBlock, 0, s>>>(bt, gv_seg, gv_pos, gpart, dof, g, part_g);
 }
 
 void launch_bgrad_reduce(const BatchView& bt, int nvwg, const int* gseg, const double* gcontrib,

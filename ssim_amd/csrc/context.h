@@ -67,6 +67,8 @@ struct ssim_ctx {
   double rtol = 1e-2, tol_abs = 1e-2, total_mass = 0.0;
   double rho_reset = 0.0;  // density the mass terms use: captured at reset, as the reference's mass matrix
   bool per_tet = false;
+  bool uniform_material = true;  // every tet has (lam0, mu0), bitwise (set by upload_material)
+  double lam0 = 0.0, mu0 = 0.0;
   bool has_reset = false;
   ssim_stepper_t stepper = SSIM_STEPPER_NEWTON;
   ssim_material_t model = SSIM_MATERIAL_STABLE_NEOHOOKEAN;
@@ -104,6 +106,11 @@ struct ssim_ctx {
   ssim_hip::DBuf<double> ones;  // unit diagonal: the preconditioner of plain CG (pcg_precond = 1)
   // device: assembly maps
   ssim_hip::DBuf<int> gperm, gseg, hperm, hseg, wu, offu, ucol, wl, offl, ub_pos;
+  // tile-reduced gradient: tile-local LDS slots of every tet corner, per tile the offsets of its vertices and their
+  // run boundaries, per vertex the positions of its (tile, vertex) partials, and the partials themselves
+  ssim_hip::DBuf<ushort4> gt_slot;
+  ssim_hip::DBuf<int> gt_off, gt_run, gv_seg, gv_pos;
+  ssim_hip::DBuf<double> gpart;
   ssim_hip::DBuf<int2> ub_rc, lref;
   ssim_hip::DBuf<double> gcontrib, hcontrib, val, massval;
   ssim_hip::DBuf<double> adj_lam, adj_mu;  // per-tet material gradients when the caller passes no buffer
@@ -171,6 +178,9 @@ struct ssim_ctx {
     m.V = V.p;
     m.lam = lam.p;
     m.mu = mu.p;
+    m.lam0 = lam0;
+    m.mu0 = mu0;
+    m.uniform = uniform_material ? 1 : 0;
     m.nv = nv;
     m.nt = nt;
     m.rho = rho_reset;

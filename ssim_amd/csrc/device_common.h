@@ -151,8 +151,8 @@ __device__ __forceinline__ TetLocal load_tet(const MeshView& m, int e) {
 #pragma unroll
   for (int k = 0; k < 9; ++k) t.B[k] = m.B[(size_t)k * m.nt + e];
   t.V = m.V[e];
-  t.lam = m.lam[e];
-  t.mu = m.mu[e];
+  t.lam = m.uniform ? m.lam0 : m.lam[e];  // uniform over the launch: no divergence
+  t.mu = m.uniform ? m.mu0 : m.mu[e];
   return t;
 }
 
@@ -234,11 +234,10 @@ __host__ __device__ constexpr int pair_index(int v, int w) { return w * (w + 1) 
 //   e = ½·(ρV/20)·(Σ|Δ_i|² + |ΣΔ_i|²) + dt²·V·Ψ,   g_i = (ρV/20)·(Δ_i + ΣΔ_j) + dt²·V·f_i,   Δ = u − û;
 // Hessian blocks (update_hessian, time_step.hpp:626-658; wrapped_pfpx :41-59).
 
-// energy of tet e at deformation u (predictor uhat); its 4 gradient contributions go to the slots gperm[4e+i]
+// energy of tet e at deformation u (predictor uhat) and its 4 vertex gradient contributions c[i][a]
 template <typename T, int Model>
-__device__ __forceinline__ double elem_grad_tet(const MeshView& m, int e, const double* __restrict__ u,
-                                                const double* __restrict__ uhat, double dt2,
-                                                const int* __restrict__ gperm, double* __restrict__ gcontrib) {
+__device__ __forceinline__ double elem_grad_contrib(const MeshView& m, int e, const double* __restrict__ u,
+                                                    const double* __restrict__ uhat, double dt2, double c[4][3]) {
   const TetLocal t = load_tet(m, e);
   double x[4][3], d[4][3];
 #pragma unroll
@@ -267,17 +266,74 @@ __device__ __forceinline__ double elem_grad_tet(const MeshView& m, int e, const 
 #pragma unroll
   for (int a = 0; a < 3; ++a) S[a] = d[0][a] + d[1][a] + d[2][a] + d[3][a];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const size_t pos = 3 * (size_t)gperm[4 * (size_t)e + i];
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       dd += d[i][a] * d[i][a];
-      gcontrib[pos + a] = mm * (d[i][a] + S[a]) + sc * (double)f[i][a];
+      c[i][a] = mm * (d[i][a] + S[a]) + sc * (double)f[i][a];
     }
-  }
 #pragma unroll
   for (int a = 0; a < 3; ++a) ss += S[a] * S[a];
   return 0.5 * mm * (dd + ss) + sc * (double)psi;
+}
+
+// Tile phase of the tile-reduced gradient (every thread of the workgroup calls it; it synchronises): thread t
+// evaluates tet e = tile·kBlock + t when `active`, puts its 4 vertex contributions into LDS at the tile-local slots
+// tslot[e] (the tile's corners sorted by (vertex, tet, corner)), and the workgroup then sums each tile vertex's
+// contiguous run in slot order into gpart[gt_off[tile] + j].  Returns the thread's element energy.
+template <typename T, int Model>
+__device__ __forceinline__ double elem_grad_tile(const MeshView& m, int tile, bool active, const double* __restrict__ u,
+                                                 const double* __restrict__ uhat, double dt2,
+                                                 const ushort4* __restrict__ tslot, const int* __restrict__ gt_off,
+                                                 const int* __restrict__ gt_run, double* __restrict__ gpart) {
+  __shared__ double rec[4 * kBlock * 3];
+  const int e = tile * kBlock + (int)threadIdx.x;
+  double energy = 0.0;
+  if (active) {
+    double c[4][3];
+    energy = elem_grad_contrib<T, Model>(m, e, u, uhat, dt2, c);
+    const ushort4 sl = tslot[e];
+    const int slot[4] = {sl.x, sl.y, sl.z, sl.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) rec[3 * slot[i] + a] = c[i][a];
+  }
+  __syncthreads();
+  const int t0 = gt_off[tile], ntv = gt_off[tile + 1] - t0;
+  const int* __restrict__ run = gt_run + t0 + tile;  // ntv + 1 run boundaries of this tile
+  for (int j = threadIdx.x; j < ntv; j += kBlock) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int q1 = run[j + 1];
+    for (int q = run[j]; q < q1; ++q)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) acc[a] += rec[3 * q + a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) gpart[3 * (size_t)(t0 + j) + a] = acc[a];
+  }
+  return energy;
+}
+
+// g[v] = Σ of v's tile partials in tile order, zeroed on Dirichlet dofs (boundary.hpp:45-51); returns v's share
+// of ‖g‖²
+__device__ __forceinline__ double grad_gather_vertex(int v, const int* __restrict__ gv_seg,
+                                                     const int* __restrict__ gv_pos, const double* __restrict__ gpart,
+                                                     const unsigned char* __restrict__ dof, double* __restrict__ g) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int k1 = gv_seg[v + 1];
+  for (int k = gv_seg[v]; k < k1; ++k) {
+    const size_t p = 3 * (size_t)gv_pos[k];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] += gpart[p + a];
+  }
+  double gg = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double val = dof[3 * (size_t)v + a] ? 0.0 : acc[a];
+    g[3 * (size_t)v + a] = val;
+    gg += val * val;
+  }
+  return gg;
 }
 
 // the 10 oriented upper blocks of tet e (dt²V-scaled stiffness plus consistent mass) to the slots hperm[10e+k]

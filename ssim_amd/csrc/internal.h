@@ -21,6 +21,8 @@ struct MeshView {
   const double* V;      // [nt]  rest volumes
   const double* lam;    // [nt]
   const double* mu;     // [nt]
+  double lam0, mu0;     // the material when every tet has the same (λ, μ); the arrays are then not read
+  int uniform;          // 1: use (lam0, mu0)
   int nv, nt;
   double rho;
   int model = 0;      // MaterialModel (elem_math.h)
@@ -91,10 +93,16 @@ struct PcgIter {
 // ---- launchers (kernels.hip) ----
 void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, double* V, hipStream_t s);
 // Reductions: a kernel stores one partial per workgroup; launch_* return the number of partials written.
-// energy + gradient contributions at deformation u (predictor uhat); energy partials to part_e
-int launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
-                     double* gcontrib, double* part_e, bool fp32, hipStream_t s);
-// g[v] = Σ contributions of v, zeroed on Dirichlet dofs; ‖g‖² partials to part_g
+// energy + gradient at deformation u (predictor uhat), reduced per tile of kBlock tets in LDS: one partial per
+// (tile, vertex of the tile); energy partials to part_e
+int launch_elem_grad_tile(const MeshView& m, const double* u, const double* uhat, double dt2, const ushort4* tslot,
+                          const int* gt_off, const int* gt_run, double* gpart, double* part_e, bool fp32,
+                          hipStream_t s);
+// g[v] = Σ of v's tile partials (in tile order), zeroed on Dirichlet dofs; ‖g‖² partials to part_g
+int launch_grad_gather(int nv, const int* gv_seg, const int* gv_pos, const double* gpart, const unsigned char* dof,
+                       double* g, double* part_g, hipStream_t s);
+// g[v] = Σ per-corner contributions of v (vertex-sorted slots gperm, runs gseg; the adjoint's mass term), zeroed
+// on Dirichlet dofs; ‖g‖² partials to part_g
 int launch_grad_reduce(int nv, const int* gseg, const double* gcontrib, const unsigned char* dof, double* g,
                        double* part_g, hipStream_t s);
 // one workgroup: out[0..ka) = Σ pa (width ka ∈ {1,2}); if pb: out[ka] = Σ pb (width 1)
@@ -183,8 +191,12 @@ void launch_sub(int n, const double* a, const double* b, double* out, hipStream_
 int max_partials(int n);  // upper bound on partials any kernel writes for n items
 
 // ---- batched launchers (batch_kernels.hip); nvwg / ntwg = vertex / tet workgroups of the batch ----
-void launch_belem_grad(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* uhat,
-                       double dt2, const int* gperm, double* gcontrib, double* part_e, bool fp32, hipStream_t s);
+// tile-reduced batched evaluation: one partial per (tile, vertex); the gather adds them per vertex
+int launch_belem_grad_tile(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* uhat,
+                           double dt2, const ushort4* tslot, const int* gt_off, const int* gt_run, double* gpart,
+                           double* part_e, bool fp32, hipStream_t s);
+void launch_bgrad_gather(const BatchView& bt, int nvwg, const int* gv_seg, const int* gv_pos, const double* gpart,
+                         const unsigned char* dof, double* g, double* part_g, hipStream_t s);
 void launch_bgrad_reduce(const BatchView& bt, int nvwg, const int* gseg, const double* gcontrib,
                          const unsigned char* dof, double* g, double* part_g, hipStream_t s);
 // per mesh (masked): out[m].energy = Σ part_e over its tet workgroups, out[m].gg = Σ part_g over its vertex

@@ -56,27 +56,54 @@ void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, do
 // The consistent mass M_e = ρV(1+δ_ij)/20 (bilinears.hpp:140-148) is applied element by element instead of
 // through a CSR SpMV.
 
+// Energy + gradient, tile-reduced: workgroup b takes the kBlock consecutive tets of tile b (one per thread),
+// writes each tet's 4 vertex contributions into LDS at the tile-local slots tslot (the tile's corners sorted by
+// (vertex, tet, corner)), and sums every tile vertex's contiguous run in order into one partial per (tile,
+// vertex), gpart[gt_off[b] + j].  Only those partials reach HBM (about 0.6 per tet on a structured bar instead of
+// 4 contributions); k_grad_gather adds each vertex's partials in tile order.
 template <typename T, int Model>
-__global__ __launch_bounds__(kBlock) void k_elem_grad(MeshView m, const double* __restrict__ u,
-                                                      const double* __restrict__ uhat, double dt2,
-                                                      const int* __restrict__ gperm, double* __restrict__ gcontrib,
-                                                      double* __restrict__ part_e) {
+__global__ __launch_bounds__(kBlock) void k_elem_grad_tile(MeshView m, const double* __restrict__ u,
+                                                           const double* __restrict__ uhat, double dt2,
+                                                           const ushort4* __restrict__ tslot,
+                                                           const int* __restrict__ gt_off,
+                                                           const int* __restrict__ gt_run,
+                                                           double* __restrict__ gpart, double* __restrict__ part_e) {
   const int lb = logical_block(nblocks(m.nt > 0 ? m.nt : 1), m.xcd_remap);
-  if (lb < 0) return;
+  if (lb < 0) return;  // uniform over the workgroup
   const int e = lb * kBlock + threadIdx.x;
-  double energy[1] = {0.0};
-  if (e < m.nt) energy[0] = elem_grad_tet<T, Model>(m, e, u, uhat, dt2, gperm, gcontrib);
+  double energy[1] = {elem_grad_tile<T, Model>(m, lb, e < m.nt && m.cells[e].x >= 0, u, uhat, dt2, tslot, gt_off,
+                                               gt_run, gpart)};
   store_partials<1>(energy, part_e, lb);
 }
 
-int launch_elem_grad(const MeshView& m, const double* u, const double* uhat, double dt2, const int* gperm,
-                     double* gcontrib, double* part_e, bool fp32, hipStream_t s) {
+int launch_elem_grad_tile(const MeshView& m, const double* u, const double* uhat, double dt2, const ushort4* tslot,
+                          const int* gt_off, const int* gt_run, double* gpart, double* part_e, bool fp32,
+                          hipStream_t s) {
   const int nb = nblocks(m.nt > 0 ? m.nt : 1);
   dispatch_elem(m.model, fp32, [&](auto k) {
     using K = decltype(k);
-    k_elem_grad<typename K::type, K::model>
-        <<<grid_for(nb, m.xcd_remap), kBlock, 0, s>>>(m, u, uhat, dt2, gperm, gcontrib, part_e);
+    k_elem_grad_tile<typename K::type, K::model>
+        <<<grid_for(nb, m.xcd_remap), kBlock, 0, s>>>(m, u, uhat, dt2, tslot, gt_off, gt_run, gpart, part_e);
   });
+  return nb;
+}
+
+// g[v] = Σ over the tiles that touch v (ascending) of their partials for v; Dirichlet dofs zeroed
+// (boundary.hpp:45-51); partials of ‖g‖²
+__global__ __launch_bounds__(kBlock) void k_grad_gather(int nv, const int* __restrict__ gv_seg,
+                                                        const int* __restrict__ gv_pos,
+                                                        const double* __restrict__ gpart,
+                                                        const unsigned char* __restrict__ dof, double* __restrict__ g,
+                                                        double* __restrict__ part_g) {
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  double gg[1] = {v < nv ? grad_gather_vertex(v, gv_seg, gv_pos, gpart, dof, g) : 0.0};
+  store_partials<1>(gg, part_g);
+}
+
+int launch_grad_gather(int nv, const int* gv_seg, const int* gv_pos, const double* gpart, const unsigned char* dof,
+                       double* g, double* part_g, hipStream_t s) {
+  const int nb = nblocks(nv > 0 ? nv : 1);
+  k_grad_gather<<<nb, kBlock, 0, s>>>(nv, gv_seg, gv_pos, gpart, dof, g, part_g);
   return nb;
 }
 

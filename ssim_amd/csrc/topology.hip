@@ -261,7 +261,56 @@ void build_topology(ssim_ctx* c) {
       if (cells[4 * e] >= 0)
         for (int i = 0; i < 4; ++i) gperm[4 * e + i] = pos[cells[4 * e + i]]++;
   }
+  // tiles of kBlock consecutive tets: per tile its vertices ascending, the tile's corners sorted by (vertex, tet,
+  // corner) into LDS slots, and the run boundaries of every tile vertex; per vertex the positions of its tile
+  // partials in tile order
+  const int ntiles = (nt + kBlock - 1) / kBlock;
+  std::vector<int> gt_off(ntiles + 1, 0), gt_run, gt_vert;
+  std::vector<ushort4> gt_slot(std::max(nt, 1), make_ushort4(0, 0, 0, 0));
+  {
+    std::vector<int> tv, cnt, fill;
+    for (int t = 0; t < ntiles; ++t) {
+      const int e0 = t * kBlock, e1 = std::min(nt, e0 + kBlock);
+      tv.clear();
+      for (int e = e0; e < e1; ++e)
+        if (cells[4 * e] >= 0)
+          for (int i = 0; i < 4; ++i) tv.push_back(cells[4 * e + i]);
+      std::sort(tv.begin(), tv.end());
+      tv.erase(std::unique(tv.begin(), tv.end()), tv.end());
+      const int ntv = (int)tv.size();
+      cnt.assign(ntv + 1, 0);
+      for (int e = e0; e < e1; ++e)
+        if (cells[4 * e] >= 0)
+          for (int i = 0; i < 4; ++i) cnt[std::lower_bound(tv.begin(), tv.end(), cells[4 * e + i]) - tv.begin() + 1]++;
+      for (int j = 0; j < ntv; ++j) cnt[j + 1] += cnt[j];
+      gt_run.insert(gt_run.end(), cnt.begin(), cnt.end());  // ntv + 1 boundaries
+      fill.assign(cnt.begin(), cnt.end() - 1);
+      for (int e = e0; e < e1; ++e) {
+        if (cells[4 * e] < 0) continue;
+        unsigned short sl[4];
+        for (int i = 0; i < 4; ++i)
+          sl[i] = (unsigned short)fill[std::lower_bound(tv.begin(), tv.end(), cells[4 * e + i]) - tv.begin()]++;
+        gt_slot[e] = make_ushort4(sl[0], sl[1], sl[2], sl[3]);
+      }
+      gt_off[t + 1] = gt_off[t] + ntv;
+      gt_vert.insert(gt_vert.end(), tv.begin(), tv.end());
+    }
+  }
+  std::vector<int> gv_seg(nv + 1, 0), gv_pos(std::max<size_t>(gt_vert.size(), 1), 0);
+  for (int v : gt_vert) gv_seg[v + 1]++;
+  for (int v = 0; v < nv; ++v) gv_seg[v + 1] += gv_seg[v];
+  {
+    std::vector<int> pos(gv_seg.begin(), gv_seg.end() - 1);
+    for (size_t k = 0; k < gt_vert.size(); ++k) gv_pos[pos[gt_vert[k]]++] = (int)k;  // k ascends with the tile
+  }
   hipStream_t s = c->stream;
+  c->gt_slot.upload(gt_slot, s);
+  c->gt_off.upload(gt_off, s);
+  if (gt_run.empty()) gt_run.push_back(0);
+  c->gt_run.upload(gt_run, s);
+  c->gv_seg.upload(gv_seg, s);
+  c->gv_pos.upload(gv_pos, s);
+  c->gpart.alloc(3 * std::max<size_t>(gt_vert.size(), 1));
   c->wu.upload(c->hwu.data(), ns, s);
   c->offu.upload(c->hoffu.data(), ns, s);
   c->ucol.upload(hucol, s);
@@ -295,6 +344,13 @@ void ssim_hip::upload_material(ssim_ctx* c) {
     std::fill(lam.begin(), lam.end(), l);
     std::fill(mu.begin(), mu.end(), m);
   }
+  // a material that is the same (λ, μ) bitwise in every tet is passed to the element kernels as two scalars, so
+  // that they read no per-tet material arrays
+  c->uniform_material = true;
+  for (int e = 1; e < c->nt && c->uniform_material; ++e)
+    c->uniform_material = lam[e] == lam[0] && mu[e] == mu[0];
+  c->lam0 = c->nt ? lam[0] : 0.0;
+  c->mu0 = c->nt ? mu[0] : 0.0;
   c->lam.upload(lam, c->stream);
   c->mu.upload(mu, c->stream);
   sync(c);

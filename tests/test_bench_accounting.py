@@ -1,7 +1,8 @@
 """The bench's byte accounting at the 786k-tet bar (32×32×128 cubes), computed from the mesh on the host: the
 SpMV kernel's algorithmic bytes (76 B per stored upper block, 8 B per lower reference, 7·n·8 B of vectors) and the
-per-step terms K1 (energy + gradient, per-tet material: 112·nt + 4·n·8) and K2 (SPD Hessian assembly:
-112·nt + 2·n·8 + nnz·8 + n), which reproduce the per-unit figures 101.6 MB and 241.5 MB of the accounting."""
+per-step terms K1 (energy + gradient: 112·nt + 4·n·8 with per-tet material, 96·nt + 4·n·8 with uniform material) and
+K2 (SPD Hessian assembly: 112·nt or 96·nt + 2·n·8 + nnz·8 + n), which reproduce the per-unit figures 101.6 / 89.0 MB
+and 241.5 / 228.9 MB of the accounting."""
 import numpy as np
 
 import bench
@@ -17,7 +18,9 @@ def test_config2_byte_accounting():
     assert (nv, nt, edges) == (140481, 786432, 945344)
     assert bench.spmv_algorithmic_bytes(info) == 113_686_260
     counts = dict(steps=1, pcg_iterations=1, hessian_evals=0, energy_evals=0)
-    acc = bench.step_accounting(info, counts, 1.0)
+    acc = bench.step_accounting(info, counts, 1.0, uniform=False)
     assert round(acc["k1"] / 1e6, 1) == 101.6 and round(acc["k2"] / 1e6, 1) == 241.5
+    k1u, k2u = bench.element_bytes(info, uniform=True)
+    assert round(k1u / 1e6, 1) == 89.0 and round(k2u / 1e6, 1) == 228.9
     assert acc["k3_k4"] == 113_686_260 + 5 * 3 * nv * 8
     assert acc["bytes_per_step"] == acc["k3_k4"]
