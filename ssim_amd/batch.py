@@ -201,3 +201,89 @@ class TetFiniteElementBatch_Hip:
             b.set_rtol(rtol)
         b.reset()
         return b
+
+
+def size_groups(nts, groups: int):
+    """Splits meshes with tet counts ``nts`` into at most ``groups`` size classes: meshes sorted by tet count
+    (ties by index) are cut into contiguous runs of about equal total tet count.  Returns the index lists, each
+    ascending."""
+    groups = max(1, min(int(groups), len(nts)))
+    order = sorted(range(len(nts)), key=lambda m: (nts[m], m))
+    total = float(sum(nts))
+    members, acc, g = [[] for _ in range(groups)], 0.0, 0
+    for m in order:
+        if g < groups - 1 and members[g] and acc >= total * (g + 1) / groups:
+            g += 1
+        members[g].append(m)
+        acc += nts[m]
+    return [sorted(ms) for ms in members if ms]
+
+
+class BatchGroups:
+    """A batch split into ``groups`` sub-batches stepped concurrently, each on its own HIP stream from its own host
+    thread (no reference counterpart: an extension of batching).
+
+    Lock-step batching runs every PCG iteration as launches over all meshes still iterating, so the last
+    iterations of the slowest meshes are small launches whose dispatch latency the GPU cannot hide.  Grouping
+    meshes of similar size (tet count) into sub-batches and stepping the sub-batches concurrently lets one group's
+    long tail overlap the others' work.  Each mesh still follows the single-mesh Newton step exactly, and a mesh's
+    arithmetic does not depend on the group it is in.  Mesh indices are those of the case list.
+    """
+
+    def __init__(self, cases, groups: int = 2, rtol=None, material: str = "stable_neohookean"):
+        import torch
+
+        if groups < 1:
+            raise ValueError("groups must be positive")
+        self.members = size_groups([c.nt for c in cases], groups)
+        self.batches = [TetFiniteElementBatch_Hip.from_cases([cases[m] for m in ms], rtol=rtol, material=material)
+                        for ms in self.members]
+        self.where = {}
+        for gi, ms in enumerate(self.members):
+            for li, m in enumerate(ms):
+                self.where[m] = (gi, li)
+        self._streams = [torch.cuda.Stream() for _ in self.batches]  # default priority
+        for b, st in zip(self.batches, self._streams):
+            _lib.call("ssim_batch_set_stream", b.handle, ctypes.c_void_p(st.cuda_stream))
+        self._pool = None
+        self._ncases = len(cases)
+
+    @property
+    def num_meshes(self) -> int:
+        return self._ncases
+
+    def _each(self, fn):
+        if len(self.batches) == 1:
+            return [fn(self.batches[0])]
+        if self._pool is None:
+            from concurrent.futures import ThreadPoolExecutor
+
+            self._pool = ThreadPoolExecutor(max_workers=len(self.batches))
+        return [f.result() for f in [self._pool.submit(fn, b) for b in self.batches]]
+
+    def step(self):
+        self._each(lambda b: b.step())
+
+    def set_solver_params(self, **kw):
+        for b in self.batches:
+            b.set_solver_params(**kw)
+
+    def set_tuning(self, **kw):
+        for b in self.batches:
+            b.set_tuning(**kw)
+
+    def get_array(self, name: str, mesh: int) -> np.ndarray:
+        g, m = self.where[mesh]
+        return self.batches[g].get_array(name, m)
+
+    def last_stats(self, mesh: int) -> dict:
+        g, m = self.where[mesh]
+        return self.batches[g].last_stats(m)
+
+    def totals(self) -> dict:
+        """Lock-step Newton and PCG iterations of the last step, per group."""
+        return [{k: b.last_stats()[k] for k in ("newton_iterations", "pcg_iterations")} for b in self.batches]
+
+    def __del__(self):
+        if getattr(self, "_pool", None) is not None:
+            self._pool.shutdown(wait=True)

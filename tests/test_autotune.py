@@ -57,3 +57,18 @@ def test_tuning_a_solver_applies_the_setting_restores_the_state_and_caches(tmp_p
     assert {k: B.tuning()[k] for k in bb} == bb
     assert np.array_equal(B.get_array("deformation"), ub)
     print(f"[gpu] tuned batch setting {bb}", flush=True)
+
+
+def test_size_groups_partition_by_tet_count():
+    from ssim_amd.batch import size_groups
+
+    nts = [5, 100, 7, 50, 50, 1, 300, 8]
+    for g in (1, 2, 3, 4, 8, 20):
+        parts = size_groups(nts, g)
+        flat = sorted(m for p in parts for m in p)
+        assert flat == list(range(len(nts)))  # every mesh once
+        assert len(parts) <= min(g, len(nts))
+        # size classes: every mesh of an earlier group is no larger than every mesh of a later one
+        for a, b in zip(parts, parts[1:]):
+            assert max(nts[m] for m in a) <= min(nts[m] for m in b)
+    assert size_groups(nts, 1) == [list(range(len(nts)))]
