@@ -50,13 +50,18 @@ def build_case(nx, ny, nz, seed):
 
 def spmv_algorithmic_bytes(info):
     """Bytes one launch of the PCG SpMV kernel must move, for the format it reads (symmetric BSR3: only blocks
-    with row <= column are stored).  76 B per stored block (72 B of values + a 4 B column index), 8 B per
-    lower-block reference (column + value offset), the gathered input z once (n·8 B), and the lane's own rows
-    of p, q and x read and written (6·n·8 B; the kernel applies the previous iteration's x += α·p).  Padding
-    slots are not counted."""
-    n = 3 * info["nv"]
+    with row <= column are stored).  The fixed layout (``info["fixed_layout"]``, the one regular meshes get): 72 B
+    of values + a 2 B column offset per stored off-diagonal block, 48 B per diagonal block (its upper triangle),
+    2 B per lower-block reference and a 4 B word of the references' partner slots per row.  The general layout:
+    76 B per stored block (72 B of values + a 4 B column index) and 8 B per lower-block reference (column +
+    value offset).  Both: the gathered input z once (n·8 B), and the lane's own rows of p, q and x read and
+    written (6·n·8 B; the kernel applies the previous iteration's x += α·p).  Padding slots are not counted."""
+    nv = info["nv"]
+    n = 3 * nv
     upper = info["upper_blocks"]
     lower = info["nonzero_blocks"] - upper
+    if info.get("fixed_layout"):
+        return 74 * (upper - nv) + 48 * nv + 2 * lower + 4 * nv + 7 * n * 8
     return 76 * upper + 8 * lower + 7 * n * 8
 
 

@@ -296,6 +296,10 @@ typedef struct {
   long long lower_slots;     /* lower-part reference slots including padding */
   int upper_blocks;          /* blocks with row ≤ column */
   int slices;                /* 64-row slices */
+  int fixed_layout;          /* 1: every row padded to 7 upper and 7 lower off-diagonal blocks, diagonal blocks
+                                stored as their 6-value upper triangle, 16-bit column offsets (32 B of indices per
+                                row); 0: the general layout (32-bit columns and references, 9-value blocks) */
+  int reserved;
 } ssim_layout_info;
 ssim_status_t ssim_get_layout_info(const ssim_ctx* ctx, ssim_layout_info* info);
 
@@ -303,9 +307,9 @@ ssim_status_t ssim_get_layout_info(const ssim_ctx* ctx, ssim_layout_info* info);
 typedef enum {
   SSIM_TUNE_XCD_REMAP = 0,     /* 1 (default): each of the 8 XCDs takes a contiguous range of SpMV row slices and of tets */
   SSIM_TUNE_PCG_CHUNK_MAX = 1, /* PCG iterations enqueued per done-flag poll (default 64) */
-  SSIM_TUNE_SPMV_LANES = 2,    /* SpMV variant: 0 = one lane per row, fully unrolled over a layout padded to 8
-                                  upper / 7 lower blocks (default when every row fits), 1 = one lane per row with
-                                  loops (default otherwise), 4 = four lanes per row */
+  SSIM_TUNE_SPMV_LANES = 2,    /* SpMV variant: 0 = one lane per row, fully unrolled over the fixed layout (the
+                                  only variant, and the one set, when the mesh has it: ssim_layout_info), 1 = one
+                                  lane per row with loops (default otherwise), 4 = four lanes per row */
   SSIM_TUNE_PCG_GRAPH = 3,     /* 1: replay PCG chunks from captured hipGraphs; 0 (default): launch each kernel
                                   (measured the same speed for one 786k-tet mesh and for 256 batched meshes: the gap
                                   between dependent kernels is not the host's launch cost) */

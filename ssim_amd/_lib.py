@@ -48,7 +48,8 @@ class StepStats(ctypes.Structure):
 class LayoutInfo(ctypes.Structure):
     _fields_ = [("nv", ctypes.c_int), ("nt", ctypes.c_int), ("nonzero_blocks", ctypes.c_longlong),
                 ("padded_slots", ctypes.c_longlong), ("lower_slots", ctypes.c_longlong),
-                ("upper_blocks", ctypes.c_int), ("slices", ctypes.c_int)]
+                ("upper_blocks", ctypes.c_int), ("slices", ctypes.c_int), ("fixed_layout", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
 
 
 KERNELS = {"elem_grad": 0, "grad_reduce": 1, "elem_hess": 2, "hess_reduce": 3, "pcg_spmv": 4, "pcg_update": 5,

@@ -93,7 +93,7 @@ struct ssim_ctx {
   int pcg_chunk_max = 64;
   int spmv_lanes = 1;
   bool spmv_lanes_default = true;  // pick the variant from the layout at reset
-  bool fixed_layout = false;       // every slice padded to kFixedWU / kFixedWL slots
+  bool fixed_layout = false;       // the fixed layout: 7 + 7 off-diagonal slots, 6-value diagonals, 16-bit indices
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
 
   // device: mesh and rest data
@@ -112,6 +112,7 @@ struct ssim_ctx {
   ssim_hip::DBuf<int> gt_off, gt_run, gv_seg, gv_pos;
   ssim_hip::DBuf<double> gpart;
   ssim_hip::DBuf<int2> ub_rc, lref;
+  ssim_hip::DBuf<uint4> idx;  // fixed layout: two 16-byte index words per row
   ssim_hip::DBuf<double> gcontrib, hcontrib, val, massval;
   ssim_hip::DBuf<double> adj_lam, adj_mu;  // per-tet material gradients when the caller passes no buffer
   // preconditioner factor L of the externally preconditioned NCG: block CSR on the block adjacency
@@ -197,6 +198,7 @@ struct ssim_ctx {
     A.wl = wl.p;
     A.offl = offl.p;
     A.lref = lref.p;
+    A.idx = idx.p;
     A.nrows = nv;
     A.nslices = nslices;
     A.xcd_remap = xcd_remap;
@@ -204,6 +206,22 @@ struct ssim_ctx {
     return A;
   }
   double* a(ssim_array_t w) { return arr[w].p; }
+  // value base of upper slot ku of row v (ku = 0: the diagonal block; 6 values in the fixed layout, 9 otherwise)
+  size_t valpos(int v, int ku) const {
+    const int s = v / ssim_hip::kSlice, lane = v % ssim_hip::kSlice;
+    if (fixed_layout)
+      return ssim_hip::kBellZeroLead + (size_t)s * ssim_hip::kBellSliceVals +
+             (ku ? (size_t)(6 + 9 * (ku - 1)) * ssim_hip::kSlice : 0) + lane;
+    return 9 * (size_t)hoffu[s] + (size_t)ku * 9 * ssim_hip::kSlice + lane;
+  }
+  // offset of entry (b, d) from the value base of a block (a diagonal block of the fixed layout keeps its upper
+  // triangle: entry (b, d) and (d, b) share one value)
+  static size_t val_entry(bool sym6, int b, int d) {
+    if (!sym6) return (size_t)(3 * b + d) * ssim_hip::kSlice;
+    const int lo = b < d ? b : d, hi = b < d ? d : b;
+    static const int m[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    return (size_t)m[lo][hi] * ssim_hip::kSlice;
+  }
 };
 
 

@@ -642,13 +642,9 @@ static void export_csr(ssim_ctx* c, const double* dval, int* rowptr, int* colidx
           for (int d = 0; d < 3; ++d) colidx[p++] = 3 * c->bcol[k] + d;
   }
   if (values) {
-    std::vector<double> hv(9 * (size_t)std::max<long long>(c->ncolslots, 1));
+    std::vector<double> hv(c->val.count);
     HIPCHK(hipMemcpyAsync(hv.data(), dval, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, c->stream));
     sync(c);
-    auto base = [&](int v, int ku) -> size_t {
-      const int s = v / kSlice, lane = v % kSlice;
-      return 9 * (size_t)c->hoffu[s] + (size_t)ku * 9 * kSlice + lane;
-    };
     long long p = 0;
     for (int v = 0; v < nv; ++v) {
       const int len = c->brow[v + 1] - c->brow[v];
@@ -657,13 +653,14 @@ static void export_csr(ssim_ctx* c, const double* dval, int* rowptr, int* colidx
         for (int k = 0; k < len; ++k) {
           const int w = c->bcol[c->brow[v] + k];
           if (k >= kd) {
-            const size_t bs = base(v, c->bslot[c->brow[v] + k]);
-            for (int d = 0; d < 3; ++d) values[p++] = hv[bs + (size_t)(3 * b + d) * kSlice];
+            const size_t bs = c->valpos(v, c->bslot[c->brow[v] + k]);
+            const bool sym6 = c->fixed_layout && k == kd;
+            for (int d = 0; d < 3; ++d) values[p++] = hv[bs + ssim_ctx::val_entry(sym6, b, d)];
           } else {  // lower block (v, w) = transpose of the stored (w, v)
             const int* bw = c->bcol.data() + c->brow[w];
             const int kw = (int)(std::lower_bound(bw, bw + (c->brow[w + 1] - c->brow[w]), v) - bw);
-            const size_t bs = base(w, c->bslot[c->brow[w] + kw]);
-            for (int d = 0; d < 3; ++d) values[p++] = hv[bs + (size_t)(3 * d + b) * kSlice];
+            const size_t bs = c->valpos(w, c->bslot[c->brow[w] + kw]);
+            for (int d = 0; d < 3; ++d) values[p++] = hv[bs + ssim_ctx::val_entry(false, d, b)];
           }
         }
     }
@@ -831,6 +828,8 @@ ssim_status_t ssim_get_layout_info(const ssim_ctx* ctx, ssim_layout_info* info) 
   info->lower_slots = ctx->nlowslots;
   info->upper_blocks = ctx->nub;
   info->slices = ctx->nslices;
+  info->fixed_layout = ctx->fixed_layout ? 1 : 0;
+  info->reserved = 0;
   return SSIM_OK;
 }
 
@@ -851,6 +850,8 @@ ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value) {
         if (value != 0 && value != 1 && value != 4) fail(SSIM_ERR_INVALID_ARGUMENT, "SpMV variant must be 0, 1 or 4");
         if (value == 0 && !ctx->fixed_layout)
           fail(SSIM_ERR_INVALID_ARGUMENT, "the fixed-width SpMV needs at most 8 upper and 7 lower blocks per row");
+        if (value != 0 && ctx->fixed_layout)
+          fail(SSIM_ERR_INVALID_ARGUMENT, "this mesh has the fixed layout, which only the fixed-width SpMV (0) reads");
         ctx->spmv_lanes = value;
         ctx->spmv_lanes_default = false;
         ctx->drop_graphs();

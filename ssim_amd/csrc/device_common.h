@@ -511,7 +511,9 @@ __device__ __forceinline__ double grad_reduce_vertices(int v0, int v1, const int
 }
 
 // upper block ub = (rc.x, rc.y) with its summed contributions S: Dirichlet identity filter, store into the
-// blocked-ELL values; Jacobi inverse diagonal (and optionally the diagonal) for a diagonal block
+// blocked-ELL values; Jacobi inverse diagonal (and optionally the diagonal) for a diagonal block.
+// ub_pos[ub] ≥ 0 is the value base of a 9-value block; a negative entry −(base + 1) marks a diagonal block of
+// the fixed layout, which keeps the upper triangle (00, 01, 02, 11, 12, 22) only.
 __device__ __forceinline__ void hess_store_block(int ub, int2 rc, double (&S)[9], const int* __restrict__ ub_pos,
                                                  const unsigned char* __restrict__ dof, int filter_dbc,
                                                  double* __restrict__ val, double* __restrict__ dinv,
@@ -523,9 +525,17 @@ __device__ __forceinline__ void hess_store_block(int ub, int2 rc, double (&S)[9]
       for (int dd = 0; dd < 3; ++dd)
         if (dof[3 * (size_t)rc.x + b] || dof[3 * (size_t)rc.y + dd]) S[b * 3 + dd] = (rc.x == rc.y && b == dd) ? 1.0 : 0.0;
   }
-  const size_t pos = (size_t)ub_pos[ub];
+  const int praw = ub_pos[ub];
+  if (praw >= 0) {
+    const size_t pos = (size_t)praw;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) val[pos + (size_t)k * kSlice] = S[k];
+    for (int k = 0; k < 9; ++k) val[pos + (size_t)k * kSlice] = S[k];
+  } else {
+    const size_t pos = (size_t)(-(praw + 1));
+    constexpr int kUpper[6] = {0, 1, 2, 4, 5, 8};
+#pragma unroll
+    for (int m = 0; m < 6; ++m) val[pos + (size_t)m * kSlice] = S[kUpper[m]];
+  }
   if (rc.x == rc.y) {
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
@@ -611,58 +621,70 @@ __device__ __forceinline__ void bell_row_part(const BellView& A, int v, int j, c
   y[2] = y2;
 }
 
-// Fixed-width variant for layouts padded to exactly WU upper and WL = WU − 1 lower slots in every slice (used when
-// the mesh allows it; the Freudenthal bar has 8 and 7): straight-line code, all column / reference loads of the
-// row first.  The slots are aligned by direction (lower slot WU−1−k refers to blocks stored in upper slot k of
-// the neighbour rows), and each pair (lower WU−1−k, upper k) is issued together, the far directions first: the
-// lower reads of a slice then reach L2 at about the time the slice that stores those blocks reads them itself.
-template <int WU, int WL>
+// Row product over the fixed layout (every row padded to 7 upper and 7 lower off-diagonal slots plus the 6-value
+// diagonal, 16-bit indices): straight-line code, the row's two 16-byte index words loaded first.  The slots are
+// aligned by direction where the mesh allows (lower slot 7 − k refers to blocks stored in upper slot k of the
+// neighbour rows), and each pair (lower 7 − k, upper k) is issued together, the far directions first: the lower
+// reads of a slice then reach L2 at about the time the slice that stores those blocks reads them itself.
 __device__ __forceinline__ void bell_row_fixed(const BellView& A, int v, const double* __restrict__ x, double y[3]) {
-  static_assert(WL == WU - 1, "lower slot WU-1-k pairs with upper slot k");
+  static_assert(kFixedWU == 8 && kFixedWL == 7, "two 16-byte index words per row");
   const int s = v >> 6, lane = v & 63;
-  const int offu = s * WU * kSlice;  // uniform widths: the offsets are implied
-  const int* col = A.ucol + offu + lane;
-  const double* val = A.val + 9 * (size_t)offu + lane;
-  const int2* ref = A.lref + s * WL * kSlice + lane;
-  int cu[WU];
-  int2 cl[WL];
-#pragma unroll
-  for (int k = 0; k < WU; ++k) cu[k] = col[k * kSlice];
-#pragma unroll
-  for (int k = 0; k < WL; ++k) cl[k] = ref[k * kSlice];
+  const uint4 i0 = A.idx[(2 * s) * kSlice + lane];
+  const uint4 i1 = A.idx[(2 * s + 1) * kSlice + lane];
+  const unsigned up[7] = {i0.x & 0xffffu, i0.x >> 16, i0.y & 0xffffu, i0.y >> 16,
+                          i0.z & 0xffffu, i0.z >> 16, i0.w & 0xffffu};
+  const unsigned lo[7] = {i0.w >> 16, i1.x & 0xffffu, i1.x >> 16, i1.y & 0xffffu,
+                          i1.y >> 16,  i1.z & 0xffffu, i1.z >> 16};
+  const unsigned slots = i1.w;
+  const double* val = A.val + kBellZeroLead + (size_t)s * kBellSliceVals + lane;
   double y0 = 0, y1 = 0, y2 = 0;
-  auto upper = [&](int k) {
-    const size_t w3 = 3 * (size_t)cu[k];
+  auto upper = [&](int k) {  // k = 1..7
+    const size_t w3 = 3 * (size_t)(v + (int)up[k - 1]);
     const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
-    const double* vk = val + (size_t)k * 9 * kSlice;
+    const double* vk = val + (size_t)(6 + 9 * (k - 1)) * kSlice;
     y0 += vk[0 * kSlice] * x0 + vk[1 * kSlice] * x1 + vk[2 * kSlice] * x2;
     y1 += vk[3 * kSlice] * x0 + vk[4 * kSlice] * x1 + vk[5 * kSlice] * x2;
     y2 += vk[6 * kSlice] * x0 + vk[7 * kSlice] * x1 + vk[8 * kSlice] * x2;
   };
-  auto lower = [&](int k) {
-    const size_t w3 = 3 * (size_t)cl[k].x;
+  auto lower = [&](int j) {  // j = 0..6
+    const int w = v - (int)lo[j];
+    const int kw = (int)((slots >> (3 * j)) & 7u);
+    const size_t w3 = 3 * (size_t)w;
     const double x0 = x[w3], x1 = x[w3 + 1], x2 = x[w3 + 2];
-    const double* b = A.val + (size_t)cl[k].y;
+    // value offsets fit 32 bits (checked when the layout is built); padding (kw = 0) reads the leading zero
+    // block.  Selected with a mask: a conditional expression here becomes a branch per block.
+    const unsigned real = (unsigned)kBellZeroLead + (unsigned)(w >> 6) * (unsigned)kBellSliceVals +
+                          (unsigned)(6 + 9 * (kw - 1)) * kSlice + (unsigned)(w & 63);
+    const unsigned m = 0u - (unsigned)(kw != 0);
+    const double* b = A.val + ((real & m) | ((unsigned)lane & ~m));
     y0 += b[0 * kSlice] * x0 + b[3 * kSlice] * x1 + b[6 * kSlice] * x2;
     y1 += b[1 * kSlice] * x0 + b[4 * kSlice] * x1 + b[7 * kSlice] * x2;
     y2 += b[2 * kSlice] * x0 + b[5 * kSlice] * x1 + b[8 * kSlice] * x2;
   };
 #pragma unroll
-  for (int k = WU - 1; k >= 1; --k) {
-    lower(WU - 1 - k);
+  for (int k = kFixedWU - 1; k >= 1; --k) {
+    lower(kFixedWU - 1 - k);
     upper(k);
   }
-  upper(0);
+  {
+    const size_t v3 = 3 * (size_t)v;
+    const double x0 = x[v3], x1 = x[v3 + 1], x2 = x[v3 + 2];
+    const double d00 = val[0 * kSlice], d01 = val[1 * kSlice], d02 = val[2 * kSlice];
+    const double d11 = val[3 * kSlice], d12 = val[4 * kSlice], d22 = val[5 * kSlice];
+    y0 += d00 * x0 + d01 * x1 + d02 * x2;
+    y1 += d01 * x0 + d11 * x1 + d12 * x2;
+    y2 += d02 * x0 + d12 * x1 + d22 * x2;
+  }
   y[0] = y0;
   y[1] = y1;
   y[2] = y2;
 }
 
-// LPR = 0 selects the fixed-width single-lane variant (layout padded to 8 upper / 7 lower slots)
+// LPR = 0 selects the fixed-layout single-lane row product
 template <int LPR>
 __device__ __forceinline__ void row_part(const BellView& A, int v, int j, const double* __restrict__ x, double y[3]) {
   if (LPR == 0)
-    bell_row_fixed<kFixedWU, kFixedWL>(A, v, x, y);
+    bell_row_fixed(A, v, x, y);
   else
     bell_row_part<LPR>(A, v, j, x, y);
 }

@@ -167,15 +167,24 @@ void build_topology(ssim_ctx* c) {
     max_wu = std::max(max_wu, c->hwu[s]);
     max_wl = std::max(max_wl, hwl[s]);
   }
-  // rows narrow enough for the fixed-width kernel: pad every slice to the same widths
-  c->fixed_layout = max_wu <= kFixedWU && max_wl <= kFixedWL;
+  // rows narrow enough for the fixed layout, with every column within kBellMaxOffset vertices of its row: pad
+  // every slice to the same widths
+  int max_off = 0;
+  for (int v = 0; v < nv; ++v)
+    if (len[v] > 0)
+      max_off = std::max({max_off, c->bcol[c->brow[v + 1] - 1] - v, v - c->bcol[c->brow[v]]});
+  c->fixed_layout = max_wu <= kFixedWU && max_wl <= kFixedWL && max_off <= kBellMaxOffset &&
+                    (long long)kBellZeroLead + (long long)ns * kBellSliceVals < (1LL << 31);
   if (c->fixed_layout)
     for (int s = 0; s < c->nslices; ++s) {
       c->hwu[s] = kFixedWU;
       hwl[s] = kFixedWL;
     }
-  if (c->spmv_lanes == 0 && !c->fixed_layout) c->spmv_lanes = 1;
-  if (c->spmv_lanes_default) c->spmv_lanes = c->fixed_layout ? 0 : 1;
+  // each layout has its own row products: the fixed layout the unrolled one (0), the general one 1 or 4 lanes
+  if (c->fixed_layout)
+    c->spmv_lanes = 0;
+  else if (c->spmv_lanes == 0 || c->spmv_lanes_default)
+    c->spmv_lanes = 1;
   assign_slots(c, len);
   long long offu = 0, offl = 0;
   for (int s = 0; s < c->nslices; ++s) {
@@ -189,29 +198,60 @@ void build_topology(ssim_ctx* c) {
 
   if (9 * offu + 9 * kSlice >= (1LL << 31) || offl >= (1LL << 31))
     fail(SSIM_ERR_INVALID_ARGUMENT, "mesh too large for 32-bit blocked-ELL offsets");
-  auto valpos = [&](int v, int ku) -> int {  // value base of upper slot ku of row v
-    const int s = v / kSlice, lane = v % kSlice;
-    return 9 * c->hoffu[s] + ku * 9 * kSlice + lane;
-  };
-  // padding: upper slots → (v, zero values); lower references → (v, a zero block after the last slice)
-  std::vector<int> hucol(std::max<long long>(offu, 1), 0);
-  std::vector<int2> hlref(std::max<long long>(offl, 1), make_int2(0, 0));
-  const long long zero_block = 9 * offu;
-  for (int s = 0; s < c->nslices; ++s)
-    for (int l = 0; l < kSlice; ++l) {
-      const int v = std::min(s * kSlice + l, nv - 1);
-      for (int k = 0; k < c->hwu[s]; ++k) hucol[c->hoffu[s] + k * kSlice + l] = v;
-      for (int k = 0; k < hwl[s]; ++k) hlref[hoffl[s] + k * kSlice + l] = make_int2(v, (int)(zero_block + l));
+  auto valpos = [&](int v, int ku) -> int { return (int)c->valpos(v, ku); };
+  std::vector<int> hucol;
+  std::vector<int2> hlref;
+  std::vector<uint4> hidx;
+  if (c->fixed_layout) {
+    // two index words per row: 16-bit column offsets of upper slots 1..7, 16-bit row offsets of lower slots
+    // 0..6 and their partners' upper slots (3 bits each); padding stays 0 (column v with zero values; the
+    // leading zero block)
+    std::vector<unsigned> up(7 * (size_t)ns * kSlice, 0), lo(7 * (size_t)ns * kSlice, 0), sw((size_t)ns * kSlice, 0);
+    for (int v = 0; v < nv; ++v) {
+      const int kd = c->kdiag[v];
+      for (int k = kd + 1; k < len[v]; ++k) {
+        const int ku = c->bslot[c->brow[v] + k];
+        up[7 * (size_t)v + ku - 1] = (unsigned)(c->bcol[c->brow[v] + k] - v);
+      }
+      for (int k = 0; k < kd; ++k) {
+        const int w = c->bcol[c->brow[v] + k];
+        const int j = c->bslot[c->brow[v] + k];
+        const int kw = c->bslot[c->brow[w] + slot(w, v)];
+        lo[7 * (size_t)v + j] = (unsigned)(v - w);
+        sw[v] |= (unsigned)kw << (3 * j);
+      }
     }
-  for (int v = 0; v < nv; ++v) {
-    const int s = v / kSlice, lane = v % kSlice;
-    const int kd = c->kdiag[v];
-    for (int k = kd; k < len[v]; ++k)
-      hucol[c->hoffu[s] + c->bslot[c->brow[v] + k] * kSlice + lane] = c->bcol[c->brow[v] + k];
-    for (int k = 0; k < kd; ++k) {
-      const int w = c->bcol[c->brow[v] + k];
-      hlref[hoffl[s] + c->bslot[c->brow[v] + k] * kSlice + lane] =
-          make_int2(w, valpos(w, c->bslot[c->brow[w] + slot(w, v)]));
+    hidx.assign(2 * (size_t)ns * kSlice, make_uint4(0, 0, 0, 0));
+    for (int v = 0; v < nv; ++v) {
+      const unsigned* u = &up[7 * (size_t)v];
+      const unsigned* l = &lo[7 * (size_t)v];
+      const int s = v / kSlice, lane = v % kSlice;
+      hidx[(2 * (size_t)s) * kSlice + lane] =
+          make_uint4(u[0] | u[1] << 16, u[2] | u[3] << 16, u[4] | u[5] << 16, u[6] | l[0] << 16);
+      hidx[(2 * (size_t)s + 1) * kSlice + lane] =
+          make_uint4(l[1] | l[2] << 16, l[3] | l[4] << 16, l[5] | l[6] << 16, sw[v]);
+    }
+  } else {
+    // padding: upper slots → (v, zero values); lower references → (v, a zero block after the last slice)
+    hucol.assign(std::max<long long>(offu, 1), 0);
+    hlref.assign(std::max<long long>(offl, 1), make_int2(0, 0));
+    const long long zero_block = 9 * offu;
+    for (int s = 0; s < c->nslices; ++s)
+      for (int l = 0; l < kSlice; ++l) {
+        const int v = std::min(s * kSlice + l, nv - 1);
+        for (int k = 0; k < c->hwu[s]; ++k) hucol[c->hoffu[s] + k * kSlice + l] = v;
+        for (int k = 0; k < hwl[s]; ++k) hlref[hoffl[s] + k * kSlice + l] = make_int2(v, (int)(zero_block + l));
+      }
+    for (int v = 0; v < nv; ++v) {
+      const int s = v / kSlice, lane = v % kSlice;
+      const int kd = c->kdiag[v];
+      for (int k = kd; k < len[v]; ++k)
+        hucol[c->hoffu[s] + c->bslot[c->brow[v] + k] * kSlice + lane] = c->bcol[c->brow[v] + k];
+      for (int k = 0; k < kd; ++k) {
+        const int w = c->bcol[c->brow[v] + k];
+        hlref[hoffl[s] + c->bslot[c->brow[v] + k] * kSlice + lane] =
+            make_int2(w, valpos(w, c->bslot[c->brow[w] + slot(w, v)]));
+      }
     }
   }
 
@@ -225,7 +265,8 @@ void build_topology(ssim_ctx* c) {
     for (int k = c->kdiag[v]; k < len[v]; ++k) {
       const int ub = ubptr[v] + (k - c->kdiag[v]);
       hub_rc[ub] = make_int2(v, c->bcol[c->brow[v] + k]);
-      hub_pos[ub] = valpos(v, c->bslot[c->brow[v] + k]);
+      const int pos = valpos(v, c->bslot[c->brow[v] + k]);
+      hub_pos[ub] = (c->fixed_layout && k == c->kdiag[v]) ? -(pos + 1) : pos;  // 6-value diagonal
     }
 
   // Hessian contributions: 10 local pairs (v ≤ w) per tet, sorted by (upper block, tet, pair)
@@ -313,17 +354,25 @@ void build_topology(ssim_ctx* c) {
   c->gpart.alloc(3 * std::max<size_t>(gt_vert.size(), 1));
   c->wu.upload(c->hwu.data(), ns, s);
   c->offu.upload(c->hoffu.data(), ns, s);
-  c->ucol.upload(hucol, s);
   c->wl.upload(hwl.data(), ns, s);
   c->offl.upload(hoffl.data(), ns, s);
-  c->lref.upload(hlref, s);
+  if (c->fixed_layout) {
+    c->ucol.release();
+    c->lref.release();
+    c->idx.upload(hidx, s);
+  } else {
+    c->idx.release();
+    c->ucol.upload(hucol, s);
+    c->lref.upload(hlref, s);
+  }
   c->ub_rc.upload(hub_rc, s);
   c->ub_pos.upload(hub_pos, s);
   c->hseg.upload(hcount, s);
   c->hperm.upload(hperm, s);
   c->gseg.upload(gseg, s);
   c->gperm.upload(gperm, s);
-  c->val.alloc(9 * (size_t)offu + 9 * kSlice);  // + the zero block of the padded lower references
+  // + the zero block of the padded lower references (after the last slice, or leading in the fixed layout)
+  c->val.alloc(c->fixed_layout ? kBellZeroLead + (size_t)ns * kBellSliceVals : 9 * (size_t)offu + 9 * kSlice);
   HIPCHK(hipMemsetAsync(c->val.p, 0, c->val.count * sizeof(double), s));
   c->gcontrib.alloc(12 * (size_t)std::max(nt, 1));
   c->hcontrib.alloc(90 * (size_t)std::max(nt, 1));

@@ -1,5 +1,7 @@
 """The bench's byte accounting at the 786k-tet bar (32×32×128 cubes), computed from the mesh on the host: the
-SpMV kernel's algorithmic bytes (76 B per stored upper block, 8 B per lower reference, 7·n·8 B of vectors) and the
+SpMV kernel's algorithmic bytes (fixed layout: 74 B per stored off-diagonal upper block, 48 B per diagonal block,
+2 B per lower reference, 4 B per row of partner slots; general layout: 76 B per stored upper block, 8 B per lower
+reference; both 7·n·8 B of vectors) and the
 per-step terms K1 (energy + gradient: 112·nt + 4·n·8 with per-tet material, 96·nt + 4·n·8 with uniform material) and
 K2 (SPD Hessian assembly: 112·nt or 96·nt + 2·n·8 + nnz·8 + n), which reproduce the per-unit figures 101.6 / 89.0 MB
 and 241.5 / 228.9 MB of the accounting."""
@@ -17,6 +19,11 @@ def test_config2_byte_accounting():
     info = dict(nv=nv, nt=nt, upper_blocks=nv + edges, nonzero_blocks=nv + 2 * edges)
     assert (nv, nt, edges) == (140481, 786432, 945344)
     assert bench.spmv_algorithmic_bytes(info) == 113_686_260
+    # the fixed layout: per row 24 B of diagonal values and its 4 B column less, a 4 B slot word more; per edge
+    # 2 B less of upper column and 6 B less of lower reference
+    assert bench.spmv_algorithmic_bytes(dict(info, fixed_layout=1)) == 102_751_964
+    assert bench.spmv_algorithmic_bytes(info) - bench.spmv_algorithmic_bytes(dict(info, fixed_layout=1)) == (
+        24 * nv + 8 * edges)
     counts = dict(steps=1, pcg_iterations=1, hessian_evals=0, energy_evals=0)
     acc = bench.step_accounting(info, counts, 1.0, uniform=False)
     assert round(acc["k1"] / 1e6, 1) == 101.6 and round(acc["k2"] / 1e6, 1) == 241.5

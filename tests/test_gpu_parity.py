@@ -303,6 +303,65 @@ def test_irregular_mesh_spmv_variants_match_oracle(lanes):
         assert rel_err(_positions(case, g2.deformation().T.reshape(-1)), _positions(case, o2.get_state()[0])) <= 1e-8
 
 
+def _spread_case(case, gap=70_000):
+    """The same bar with `gap` Dirichlet-fixed vertices in no tet inserted in the middle of the numbering: the
+    bar's middle edges then join vertices more than 65,535 apart, which the fixed layout's 16-bit offsets cannot
+    hold."""
+    nv = case.nv
+    half = nv // 2
+    new = np.arange(nv) + np.where(np.arange(nv) < half, 0, gap)
+    N = nv + gap
+
+    def spread(a, fill=0.0):
+        out = np.full((N, 3), fill, dtype=a.dtype)
+        out[new] = a.reshape(nv, 3)
+        return out.reshape(-1)
+
+    X = spread(case.X.reshape(-1)).reshape(N, 3)
+    dof = spread(case.dof_type, 1).astype(np.uint8)
+    return mesh.BarCase(X, new[case.cells].astype(case.cells.dtype), case.lam, case.mu, case.rho, case.dt, dof,
+                        spread(case.dbc), spread(case.accel), spread(case.u0), spread(case.v0),
+                        name=case.name + "-spread"), new
+
+
+def test_fixed_and_general_layouts_agree(bar):
+    """A Freudenthal bar gets the fixed layout (7 + 7 off-diagonal slots, 6-value diagonals, 16-bit offsets);
+    the same bar renumbered with column offsets beyond 65,535 gets the general layout.  The two SpMVs agree to
+    rounding on the bar's rows (identity rows elsewhere), so do the exported Hessians (the fixed layout keeps the
+    upper triangle of each diagonal block), and three Newton steps take the same iterations and
+    agree within 1e-10 (positions)."""
+    big, new = _spread_case(bar)
+    gs = fem.TetFiniteElementSolver_Hip.from_case(bar)
+    gb = fem.TetFiniteElementSolver_Hip.from_case(big)
+    print(f"[gpu] layouts: bar {bar.nv} vertices, spread {big.nv} vertices", flush=True)
+    assert gs.layout_info()["fixed_layout"] == 1 and gs.tuning()["spmv_lanes"] == 0
+    assert gb.layout_info()["fixed_layout"] == 0 and gb.tuning()["spmv_lanes"] == 1
+    with pytest.raises(ValueError):
+        gs.set_tuning(spmv_lanes=1)
+    u = _strained(bar, 3, 0.3)
+    ub = np.zeros(3 * big.nv)
+    ub.reshape(-1, 3)[new] = u.reshape(-1, 3)
+    gs.set_array("next_deform", u)
+    gb.set_array("next_deform", ub)
+    gs.update_hessian(True)
+    gb.update_hessian(True)
+    x = np.random.default_rng(2).standard_normal(3 * big.nv)
+    ys, yb = gs.spmv(x.reshape(-1, 3)[new].reshape(-1)), gb.spmv(x)
+    assert rel_err(yb.reshape(-1, 3)[new].reshape(-1), ys) <= 1e-13
+    rest = np.setdiff1d(np.arange(big.nv), new)
+    assert np.array_equal(yb.reshape(-1, 3)[rest], x.reshape(-1, 3)[rest])
+    Hs, Hb = gs.hessian(), gb.hessian()
+    perm = (3 * new[:, None] + np.arange(3)).reshape(-1)
+    assert rel_err(Hb[perm][:, perm].toarray().reshape(-1), Hs.toarray().reshape(-1)) <= 1e-13
+    for step in range(3):
+        gs.step()
+        gb.step()
+        assert gs.last_stats()["newton_iterations"] == gb.last_stats()["newton_iterations"], step
+    ds = gs.deformation().T.reshape(-1)
+    db = gb.deformation().T.reshape(-1, 3)[new].reshape(-1)
+    assert rel_err(_positions(bar, db), _positions(bar, ds)) <= 1e-10
+
+
 def test_lbfgs_gram_and_vector_recursions_agree():
     """The two-loop recursion on Gram matrices (the default) and vector by vector are the same arithmetic up to
     the order of the sums: at threshold 1e-8 they take the same iterations and agree within 1e-9 (positions).
