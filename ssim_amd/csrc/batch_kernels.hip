@@ -277,6 +277,10 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
   const int li = logical_block(nlist, A.xcd_remap);
   if (li < 0) return;
   const int lb = slist[li];
+  const int v = lb * kSlice + threadIdx.x;
+  // the fixed layout's index words go out with the mesh's scalars and partial sums, which they do not depend on
+  FixedRowIndex ix{};
+  if (LPR == 0) ix = bell_fixed_index(A, v);
   const int vw = lb / kBatchSlices;
   const int mesh = bt.vwg_mesh[vw];
   PcgScalars* S = sc + mesh;
@@ -301,19 +305,32 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
     if (done) return;
     beta = t_cur[0] / t_prev[0];
   }
-  const int v = lb * kSlice + threadIdx.x;
   double sv[3] = {0.0, 0.0, 0.0};
   // v < nrows always holds here; keeping the row product under a branch, as in the single-mesh kernel, gives it a
   // scheduling region of its own -- without it the compiler spills the fixed-width row at this register budget
-  if (v < A.nrows) row_part<LPR>(A, v, 0, z, sv);
+  if (v < A.nrows) {
+    if (LPR == 0)
+      bell_row_fixed(A, v, ix, z, sv);
+    else
+      row_part<LPR>(A, v, 0, z, sv);
+  }
+  // the row's z, p and q in one round of loads (no branch around them), as in the single-mesh kernel
+  double zo[3], po[3], qo[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const size_t i = 3 * (size_t)v + a;
+    zo[a] = z[i];
+    po[a] = p[i];
+    qo[a] = q[i];
+  }
   double acc[1] = {0.0};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const size_t i = 3 * (size_t)v + a;
-    double pv = z[i], qv = sv[a];
+    double pv = zo[a], qv = sv[a];
     if (k > 0) {
-      pv += beta * p[i];
-      qv += beta * q[i];
+      pv += beta * po[a];
+      qv += beta * qo[a];
     }
     p[i] = pv;
     q[i] = qv;

@@ -74,7 +74,10 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, in
     for (int u = 0; u < U; ++u) {
       const int b = b0 + u * NT;
 #pragma unroll
-      for (int k = 0; k < K; ++k) buf[u][k] = (b < count) ? part[(size_t)b * K + k] : 0.0;
+      for (int k = 0; k < K; ++k) {  // a clamped index and a select: no branch around the load
+        const double pv = part[(size_t)(b < count ? b : count - 1) * K + k];
+        buf[u][k] = (b < count) ? pv : 0.0;
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -626,11 +629,21 @@ __device__ __forceinline__ void bell_row_part(const BellView& A, int v, int j, c
 // aligned by direction where the mesh allows (lower slot 7 − k refers to blocks stored in upper slot k of the
 // neighbour rows), and each pair (lower 7 − k, upper k) is issued together, the far directions first: the lower
 // reads of a slice then reach L2 at about the time the slice that stores those blocks reads them itself.
-__device__ __forceinline__ void bell_row_fixed(const BellView& A, int v, const double* __restrict__ x, double y[3]) {
+struct FixedRowIndex {
+  uint4 i0, i1;
+};
+
+__device__ __forceinline__ FixedRowIndex bell_fixed_index(const BellView& A, int v) {
   static_assert(kFixedWU == 8 && kFixedWL == 7, "two 16-byte index words per row");
   const int s = v >> 6, lane = v & 63;
-  const uint4 i0 = A.idx[(2 * s) * kSlice + lane];
-  const uint4 i1 = A.idx[(2 * s + 1) * kSlice + lane];
+  return {A.idx[(2 * s) * kSlice + lane], A.idx[(2 * s + 1) * kSlice + lane]};
+}
+
+// the row product given the row's index words (loaded earlier by the caller, e.g. before a wait on scalars)
+__device__ __forceinline__ void bell_row_fixed(const BellView& A, int v, const FixedRowIndex& ix,
+                                               const double* __restrict__ x, double y[3]) {
+  const int s = v >> 6, lane = v & 63;
+  const uint4 i0 = ix.i0, i1 = ix.i1;
   const unsigned up[7] = {i0.x & 0xffffu, i0.x >> 16, i0.y & 0xffffu, i0.y >> 16,
                           i0.z & 0xffffu, i0.z >> 16, i0.w & 0xffffu};
   const unsigned lo[7] = {i0.w >> 16, i1.x & 0xffffu, i1.x >> 16, i1.y & 0xffffu,
@@ -684,7 +697,7 @@ __device__ __forceinline__ void bell_row_fixed(const BellView& A, int v, const d
 template <int LPR>
 __device__ __forceinline__ void row_part(const BellView& A, int v, int j, const double* __restrict__ x, double y[3]) {
   if (LPR == 0)
-    bell_row_fixed(A, v, x, y);
+    bell_row_fixed(A, v, bell_fixed_index(A, v), x, y);
   else
     bell_row_part<LPR>(A, v, j, x, y);
 }

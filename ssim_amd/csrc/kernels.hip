@@ -707,29 +707,58 @@ int launch_pcg_init(int n, const double* b, double sign, const double* dinv, dou
 template <int LPR>
 __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_pcg_spmv(
     BellView A, const double* __restrict__ z, double* __restrict__ p, double* __restrict__ q,
-    const double* __restrict__ pb0, const double* __restrict__ pb1, int nb, PcgIter* it, PcgScalars* sc,
-    double* __restrict__ pa, double* __restrict__ x) {
-  const int k = it->ka;
-  if (blockIdx.x == 0 && threadIdx.x == 0) it->kb = k;
-  // Only a stop from an earlier launch ends the work here: workgroup 0 of this launch may publish a stop at
-  // iteration k while other workgroups have yet to start, and those must still apply x_k = x_{k−1} + α p for
-  // their rows (below) -- they reach the same stop decision from the same partial sums.
-  if (k > sc->stop_k) return;
-  const double* __restrict__ pb_cur = (k & 1) ? pb1 : pb0;
+    const double* __restrict__ pb0, const double* __restrict__ pb1, int nb, PcgIter* __restrict__ it,
+    PcgScalars* __restrict__ sc, double* __restrict__ pa, double* __restrict__ x) {
+  // The iteration counter and every scalar this launch reads, in one round of loads (none depends on another;
+  // ρ_{k−1} is selected from both slots afterwards).  The empty asm statement takes them as inputs, which keeps
+  // the compiler from sinking the loads below the row product.
   const int lb = logical_block(RowMap<LPR>::blocks(A.nrows), A.xcd_remap);
   int v = 0, j = 0;
   if (lb >= 0) RowMap<LPR>::map(lb, v, j);
   const bool active = lb >= 0 && v < A.nrows;
+  // the fixed layout's index words do not depend on the scalars: their loads go out with them
+  FixedRowIndex ix{};
+  if (LPR == 0 && active) ix = bell_fixed_index(A, v);
+  const int k = it->ka;
+  const int stop_k = sc->stop_k, maxit = sc->maxit;
+  const double tol2 = sc->tol2, al = sc->alpha;  // α_{k−1} (used only for k > 0)
+  const double rz0 = sc->rz_hist[0], rz1 = sc->rz_hist[1];
+  asm volatile("" ::"s"(k), "s"(stop_k), "s"(maxit), "s"(tol2), "s"(al), "s"(rz0), "s"(rz1));
+  const double rz_prev = (k & 1) ? rz0 : rz1;  // ρ_{k−1} (slot (k − 1) % 2), published by the previous launch
+  if (blockIdx.x == 0 && threadIdx.x == 0) it->kb = k;
+  // Only a stop from an earlier launch ends the work here: workgroup 0 of this launch may publish a stop at
+  // iteration k while other workgroups have yet to start, and those must still apply x_k = x_{k−1} + α p for
+  // their rows (below) -- they reach the same stop decision from the same partial sums.
+  if (k > stop_k) return;
+  const double* __restrict__ pb_cur = (k & 1) ? pb1 : pb0;
   double sv[3] = {0.0, 0.0, 0.0};
-  if (active) row_part<LPR>(A, v, j, z, sv);
+  if (active) {
+    if (LPR == 0)
+      bell_row_fixed(A, v, ix, z, sv);
+    else
+      row_part<LPR>(A, v, j, z, sv);
+  }
   row_combine<LPR>(sv);
+  // The lane's own entries of z, p_{k−1}, q_{k−1} and x and the partial sums of residual k are loaded in one
+  // round, so the rest of the kernel waits for one memory latency instead of one per phase.
+  double zo[3] = {0.0, 0.0, 0.0}, po[3] = {0.0, 0.0, 0.0}, qo[3] = {0.0, 0.0, 0.0}, xo[3] = {0.0, 0.0, 0.0};
+  if (active) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      if (LPR <= 1 || a == j) {  // with LPR = 4, lane j of the row owns component j
+        const size_t i = 3 * (size_t)v + a;
+        zo[a] = z[i];
+        po[a] = p[i];
+        qo[a] = q[i];
+        xo[a] = x[i];
+      }
+  }
+  double t_cur[2] = {0.0, 0.0};
+  if (k > 0) sum_partials<2, kSpmvBlock>(pb_cur, nb, t_cur);
   double beta = 0.0;
   if (k > 0) {
-    double t_cur[2];
-    sum_partials<2, kSpmvBlock>(pb_cur, nb, t_cur);
-    const double rz_prev = sc->rz_hist[(k - 1) & 1];  // ρ_{k−1}, published by this kernel's previous launch
     const bool nan = !(t_cur[1] == t_cur[1]);
-    const bool done = t_cur[1] <= sc->tol2 || k >= sc->maxit || nan;
+    const bool done = t_cur[1] <= tol2 || k >= maxit || nan;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->iters = k;
       sc->rr = t_cur[1];
@@ -743,13 +772,9 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
     }
     // x_k = x_{k−1} + α_{k−1}·p_{k−1} for the own row (kernel B published α_{k−1})
     if (active) {
-      const double al = sc->alpha;
 #pragma unroll
       for (int a = 0; a < 3; ++a)
-        if (LPR <= 1 || a == j) {
-          const size_t i = 3 * (size_t)v + a;
-          x[i] += al * p[i];
-        }
+        if (LPR <= 1 || a == j) x[3 * (size_t)v + a] = xo[a] + al * po[a];
     }
     if (done) return;
     beta = t_cur[0] / rz_prev;
@@ -758,12 +783,12 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
   if (active) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      if (LPR <= 1 || a == j) {  // with LPR = 4, lane j of the row owns component j
+      if (LPR <= 1 || a == j) {
         const size_t i = 3 * (size_t)v + a;
-        double pv = z[i], qv = sv[a];
+        double pv = zo[a], qv = sv[a];
         if (k > 0) {
-          pv += beta * p[i];
-          qv += beta * q[i];
+          pv += beta * po[a];
+          qv += beta * qo[a];
         }
         p[i] = pv;
         q[i] = qv;
@@ -783,11 +808,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
                                                        double* __restrict__ r, double* __restrict__ z,
                                                        const double* __restrict__ q,
                                                        const double* __restrict__ pa, int na, double* __restrict__ pb0,
-                                                       double* __restrict__ pb1, int nb, PcgIter* it,
-                                                       PcgScalars* sc) {
+                                                       double* __restrict__ pb1, int nb,
+                                                       PcgIter* __restrict__ it, PcgScalars* __restrict__ sc) {
+  // the counter, done and both ρ slots in one round of loads; ρ_k is the slot kernel A wrote
   const int k = it->kb;
+  const int done = sc->done;
+  const double rz0 = sc->rz_hist[0], rz1 = sc->rz_hist[1];
+  asm volatile("" ::"s"(k), "s"(done), "s"(rz0), "s"(rz1));
+  const double rz_k = (k & 1) ? rz1 : rz0;
   if (blockIdx.x == 0 && threadIdx.x == 0) it->ka = k + 1;
-  if (sc->done) return;
+  if (done) return;
   const double* __restrict__ pb_cur = (k & 1) ? pb1 : pb0;
   double* __restrict__ pb_next = (k & 1) ? pb0 : pb1;
   double rv[kVec], qv[kVec], dv[kVec];
@@ -800,9 +830,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
       dv[j] = dinv[i];
     }
   }
-  double pq[1], rz[1];
+  double pq[1];
   sum_partials<1, kBlock, 16>(pa, na, pq);  // one pass over up to 4,096 partials
-  rz[0] = sc->rz_hist[k & 1];
   if (!(pq[0] > 0.0)) {  // not positive definite (or NaN): every workgroup sees the same pq
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->breakdown = 1;
@@ -811,7 +840,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
     }
     return;
   }
-  const double alpha = rz[0] / pq[0];
+  const double alpha = rz_k / pq[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->alpha = alpha;
   double v[2] = {0.0, 0.0};
 #pragma unroll
