@@ -11,10 +11,32 @@ namespace ssim_hip {
 // ------------------------------------------------------------------------------------------------------
 // reductions
 
+// v of lane (lane pattern Ctrl) in the enabled rows of 16 lanes (RowMask), 0 elsewhere: DPP moves of the two
+// 32-bit halves.  Every lane of the wave must be active.
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, Ctrl, RowMask, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), Ctrl, RowMask, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes, returned to every lane, in a fixed order: lane pairs, quads, half rows and rows by DPP
+// lane permutations (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror, row_mirror), then rows 0+1 and 2+3
+// (row_bcast:15 into rows 1 and 3) and the two halves (row_bcast:31 into row 3); lane 63 holds the total, which
+// a read of that lane broadcasts.  DPP moves take a few cycles where a shuffle through ds_bpermute waits for
+// the LDS pipeline, and the PCG kernels sit on several of these reductions per iteration.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
+  v += dpp_move<0xB1>(v);       // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);       // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);      // row_half_mirror
+  v += dpp_move<0x140>(v);      // row_mirror
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // Reductions never synchronise workgroups inside a kernel.  A producing kernel stores one partial per
@@ -27,12 +49,8 @@ template <int K, int NT = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[K]) {
   static_assert(NT % 64 == 0, "workgroups are whole waves");
 #pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
-  if (NT == 64) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = __shfl(v[k], 0, 64);
-    return;
-  }
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);  // in every lane
+  if (NT == 64) return;
   __shared__ double sm[K][NT / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) {
@@ -83,6 +101,29 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ part, in
     for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int k = 0; k < K; ++k) tot[k] += buf[u][k];
+  }
+  block_sum<K, NT>(tot);
+}
+
+// sum_partials for count ≤ U·NT, as straight-line code: all U loads of a thread are issued before the first add
+// (inside the general version's loop the scheduler may interleave them with waits).  Same order of additions.
+template <int K, int NT, int U>
+__device__ __forceinline__ void sum_partials_1pass(const double* __restrict__ part, int count, double (&tot)[K]) {
+  double buf[U][K];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int b = (int)threadIdx.x + u * NT;
+    const int bc = b < count ? b : count - 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) buf[u][k] = part[(size_t)bc * K + k];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) tot[k] = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const bool in = (int)threadIdx.x + u * NT < count;
+#pragma unroll
+    for (int k = 0; k < K; ++k) tot[k] += in ? buf[u][k] : 0.0;
   }
   block_sum<K, NT>(tot);
 }

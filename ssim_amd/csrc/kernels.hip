@@ -810,7 +810,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
                                                        const double* __restrict__ pa, int na, double* __restrict__ pb0,
                                                        double* __restrict__ pb1, int nb,
                                                        PcgIter* __restrict__ it, PcgScalars* __restrict__ sc) {
-  // the counter, done and both ρ slots in one round of loads; ρ_k is the slot kernel A wrote
+  // The streaming loads go out first, with the counter, done and both ρ slots (ρ_k is the slot kernel A wrote):
+  // the kernel then waits for one memory latency before its partial sums instead of two.  A launch after the
+  // solve stopped reads its share of r, q and D⁻¹ for nothing (the host's polling bounds those launches).
+  double rv[kVec], qv[kVec], dv[kVec];
+#pragma unroll
+  for (int j = 0; j < kVec; ++j) {  // clamped indices: no branch between the loads (the tail's are not used)
+    const int i0 = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
+    const int i = i0 < n ? i0 : n - 1;
+    rv[j] = r[i];
+    qv[j] = q[i];
+    dv[j] = dinv[i];
+  }
   const int k = it->kb;
   const int done = sc->done;
   const double rz0 = sc->rz_hist[0], rz1 = sc->rz_hist[1];
@@ -820,18 +831,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
   if (done) return;
   const double* __restrict__ pb_cur = (k & 1) ? pb1 : pb0;
   double* __restrict__ pb_next = (k & 1) ? pb0 : pb1;
-  double rv[kVec], qv[kVec], dv[kVec];
-#pragma unroll
-  for (int j = 0; j < kVec; ++j) {
-    const int i = (blockIdx.x * kVec + j) * kBlock + threadIdx.x;
-    if (i < n) {
-      rv[j] = r[i];
-      qv[j] = q[i];
-      dv[j] = dinv[i];
-    }
-  }
   double pq[1];
-  sum_partials<1, kBlock, 16>(pa, na, pq);  // one pass over up to 4,096 partials
+  if (na <= 16 * kBlock)
+    sum_partials_1pass<1, kBlock, 16>(pa, na, pq);  // up to 4,096 partials (every mesh up to ~260k vertices)
+  else
+    sum_partials<1, kBlock, 16>(pa, na, pq);
   if (!(pq[0] > 0.0)) {  // not positive definite (or NaN): every workgroup sees the same pq
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->breakdown = 1;
