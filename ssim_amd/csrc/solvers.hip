@@ -7,6 +7,7 @@
 #include "context.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -139,7 +140,14 @@ int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol,
   // Chunks of iterations are enqueued one ahead of the done-flag poll, so the GPU never drains while the host
   // reads a flag; kernels enqueued after convergence return at entry.  The first iteration of a chunk is launched
   // on its own (it carries the sampled kernel timers); the others are replayed from a captured graph.
-  int* flags = reinterpret_cast<int*>(c->pin);  // two slots of 4 ints: done, iters, maxit, breakdown
+  // each poll reads rr, bb, tol2, the ρ slots and done, iters, maxit, breakdown: 64 bytes of PcgScalars from rr on
+  struct Poll {
+    double rr, bb, tol2, pad, rz_hist[2];
+    int done, iters, maxit, breakdown;
+  };
+  static_assert(sizeof(Poll) == 64 && offsetof(PcgScalars, breakdown) - offsetof(PcgScalars, rr) ==
+                                          offsetof(Poll, breakdown), "the poll mirrors PcgScalars from rr on");
+  Poll* polls = reinterpret_cast<Poll*>(c->pin);  // two slots
   auto enqueue = [&](int count, int slot) {
     timed_ext(c, SSIM_KERNEL_PCG_SPMV, [&](hipEvent_t e0, hipEvent_t e1) {
       launch_pcg_spmv(A, w.z, w.p, w.q, w.pb0, w.pb1, nb, w.it, w.sc, w.pa, x, s, e0, e1);
@@ -154,26 +162,45 @@ int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol,
         enqueue_pcg_iterations(c, w, A, dinv, x, nb, na, count - 1, s);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(flags + 4 * slot, &w.sc->done, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(polls + slot, &w.sc->rr, sizeof(Poll), hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(c->poll_ev[slot], s));
   };
-  int chunk = 8, launched = 0, slot = 0;
-  enqueue(chunk, slot);
-  launched += chunk;
+  // Chunk sizes double from 8 up to pcg_chunk_max while the solve is far from converged.  Near the end the
+  // residual history sizes them: from the log-linear decrease of ‖r‖² between the last two polls, the chunk
+  // after the one in flight covers the predicted remainder with a 10 % + 4 margin (at least 4 iterations, so
+  // the GPU stays fed while the host polls).  Launches after the solve stopped return early, but each still
+  // costs a few microseconds; this keeps their number to about the margin instead of up to two full chunks.
+  const int cmax = std::max(8, c->pcg_chunk_max);
+  int launched = 0, slot = 0, inflight = 8;
+  enqueue(inflight, slot);
+  launched += inflight;
+  int next = std::min(16, cmax);
+  int k_prev = -1;
+  double rr_prev = 0.0;
   for (;;) {
-    const int next = std::min(chunk * 2, std::max(8, c->pcg_chunk_max));
     enqueue(next, slot ^ 1);
     launched += next;
     HIPCHK(hipEventSynchronize(c->poll_ev[slot]));
     if (c->timing) harvest_timers(c);
-    const int* f = flags + 4 * slot;
-    if (f[0]) {
-      if (f[3] == 1) fail(SSIM_ERR_NUMERIC, "PCG breakdown: pᵀAp ≤ 0 (matrix not positive definite)");
-      if (f[3] == 2) fail(SSIM_ERR_NUMERIC, "PCG breakdown: NaN residual");
-      return f[1];
+    const Poll& f = polls[slot];
+    if (f.done) {
+      if (f.breakdown == 1) fail(SSIM_ERR_NUMERIC, "PCG breakdown: pᵀAp ≤ 0 (matrix not positive definite)");
+      if (f.breakdown == 2) fail(SSIM_ERR_NUMERIC, "PCG breakdown: NaN residual");
+      return f.iters;
     }
-    if (launched > maxit + 2 * std::max(8, c->pcg_chunk_max) + 16) fail(SSIM_ERR_STATE, "internal: PCG did not stop");
-    chunk = next;
+    if (launched > maxit + 2 * cmax + 16) fail(SSIM_ERR_STATE, "internal: PCG did not stop");
+    // f describes the state after the chunk that just completed (iteration f.iters); `next` is now in flight
+    int want = std::min(next * 2, cmax);
+    if (k_prev >= 0 && f.iters > k_prev && f.rr > 0.0 && rr_prev > 0.0 && f.tol2 > 0.0) {
+      const double slope = std::log(f.rr / rr_prev) / (f.iters - k_prev);  // log ‖r‖² per iteration
+      if (slope < 0.0) {
+        const double remaining = std::log(f.tol2 / f.rr) / slope - next;  // after the chunk in flight
+        if (remaining < want) want = std::max(4, (int)std::ceil(1.1 * remaining + 4.0));
+      }
+    }
+    k_prev = f.iters;
+    rr_prev = f.rr;
+    next = std::min(want, cmax);
     slot ^= 1;
   }
 }
