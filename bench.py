@@ -554,6 +554,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3, help="whole oracle steps timed for cpu_baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-mesh batched leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout leg")
+    ap.add_argument("--adjoint-checkpoints", type=str, default="1,5",
+                    help="checkpoint intervals of the adjoint leg: the first is the reported run, a second one is "
+                         "reported as with_recompute")
     ap.add_argument("--no-lbfgs", action="store_true", help="skip the L-BFGS twisted-cube leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 element-variant leg")
     args = ap.parse_args()
@@ -646,8 +649,10 @@ def main():
         from ssim_amd import mesh as _mesh
 
         rc = _mesh.rollout_batch(32, seed=args.seed)
-        adj = bench_adjoint(cpu_threads=cpu_threads, rc=rc)
-        adj["with_recompute"] = bench_adjoint(checkpoint_every=5, rc=rc)
+        every = [int(v) for v in args.adjoint_checkpoints.split(",") if v]
+        adj = bench_adjoint(cpu_threads=cpu_threads, rc=rc, checkpoint_every=every[0])
+        if len(every) > 1:
+            adj["with_recompute"] = bench_adjoint(checkpoint_every=every[1], rc=rc)
     lbfgs = None if args.no_lbfgs else bench_lbfgs(args.rtol, cpu_threads=cpu_threads)
     # GPU / CPU ratios of the extension legs (their cpu_baseline objects are bounded samples, as for the main line)
     if batched and batched.get("cpu_baseline"):
