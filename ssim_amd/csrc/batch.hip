@@ -263,20 +263,36 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
     for (int m = 0; m < b->nmesh; ++m) iters[m] = 0;
     return;
   }
-  int chunk = 8, launched = 0, slot = 0;
-  enqueue(chunk, slot);
-  launched += chunk;
+  // Chunk sizes double from 8 up to pcg_chunk_max; near the end the chunk after the one in flight covers the
+  // remainder predicted for the slowest mesh still iterating, from the decrease of its ‖r‖² between its last two
+  // polls (10 % + 4 margin, at least 4), as in the single-mesh solve.
+  const int cmax = std::max(8, c->pcg_chunk_max);
+  int launched = 0, slot = 0;
+  enqueue(8, slot);
+  launched += 8;
+  int next = std::min(16, cmax);
+  std::vector<int> k_prev(b->nmesh, -1);
+  std::vector<double> rr_prev(b->nmesh, 0.0);
   for (;;) {
-    const int next = std::min(chunk * 2, std::max(8, c->pcg_chunk_max));
     enqueue(next, slot ^ 1);
     launched += next;
     HIPCHK(hipEventSynchronize(c->poll_ev[slot]));
     if (c->timing) harvest_timers(c);
     const PcgScalars* f = b->pin_pcg(slot);
     int left_slices = 0;
+    double remaining = 0.0;  // predicted iterations after the chunk in flight, over the meshes still iterating
     for (int m = 0; m < b->nmesh; ++m) {
       active[m] = f[m].done ? 0 : 1;
-      if (active[m]) left_slices += kBatchSlices * (b->hvwg_off[m + 1] - b->hvwg_off[m]);
+      if (!active[m]) continue;
+      left_slices += kBatchSlices * (b->hvwg_off[m + 1] - b->hvwg_off[m]);
+      double rem = 1e300;
+      if (k_prev[m] >= 0 && f[m].iters > k_prev[m] && f[m].rr > 0.0 && rr_prev[m] > 0.0 && f[m].tol2 > 0.0) {
+        const double slope = std::log(f[m].rr / rr_prev[m]) / (f[m].iters - k_prev[m]);
+        if (slope < 0.0) rem = std::log(f[m].tol2 / f[m].rr) / slope - next;
+      }
+      remaining = std::max(remaining, rem);
+      k_prev[m] = f[m].iters;
+      rr_prev[m] = f[m].rr;
     }
     if (left_slices == 0) {
       for (int m = 0; m < b->nmesh; ++m) {
@@ -293,7 +309,9 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
       cur ^= 1;
       build_lists(b, active, cur);
     }
-    chunk = next;
+    int want = std::min(next * 2, cmax);
+    if (remaining < want) want = std::max(4, (int)std::ceil(1.1 * remaining + 4.0));
+    next = std::min(want, cmax);
     slot ^= 1;
   }
 }
