@@ -315,9 +315,12 @@ typedef enum {
                                   between dependent kernels is not the host's launch cost) */
   SSIM_TUNE_COMPACT_PCT = 4,   /* batches: compact the PCG grids to the meshes still iterating once those hold at
                                   most this percentage of the current grid (default 70; 0 disables) */
-  SSIM_TUNE_LBFGS_GRAM = 5     /* L-BFGS with the diagonal H0 and m ≤ 8: 1 (default) runs the two-loop recursion on
+  SSIM_TUNE_LBFGS_GRAM = 5,    /* L-BFGS with the diagonal H0 and m ≤ 8: 1 (default) runs the two-loop recursion on
                                   Gram matrices of the stored pairs (one fused dot-product pass and one fused
                                   direction pass per iteration), 0 runs it vector by vector */
+  SSIM_TUNE_GRAD_TILE = 6      /* tets per tile of the tile-reduced energy + gradient: 64, 128 or 256 (default);
+                                  the tile maps are rebuilt on change.  Batches keep 256 (their per-mesh tet ranges
+                                  are padded to 256-tet workgroups) */
 } ssim_tune_t;
 ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value);
 ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value);

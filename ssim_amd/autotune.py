@@ -1,8 +1,9 @@
 """Autotuner for the launch knobs of a solver or a batch (``SSIM_TUNE_*`` in ssim_hip.h).
 
 Knobs: XCD remapping of the SpMV rows, the SpMV variant (fixed-width unrolled / looped / four lanes per row),
-PCG iterations per done-flag poll, graph replay of PCG chunks and, for batches, the threshold at which the
-batched PCG grids are compacted to the meshes still iterating.  None of them changes the algorithm; the
+PCG iterations per done-flag poll, graph replay of PCG chunks, the tile size of the tile-reduced energy + gradient
+(single meshes) and, for batches, the threshold at which the batched PCG grids are compacted to the meshes still
+iterating.  None of them changes the algorithm; the
 variants sum in different orders, so results can differ in the last bits.
 
 The tuner snapshots the state, times one full time step per candidate from that same state (so every
@@ -27,6 +28,7 @@ SINGLE_SPACE = {
     "spmv_lanes": [0, 1, 4],
     "pcg_chunk_max": [64, 32, 128, 256],
     "pcg_graph": [0, 1],
+    "grad_tile": [256, 128, 64],
 }
 BATCH_SPACE = {
     "xcd_remap": [1, 0],

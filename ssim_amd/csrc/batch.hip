@@ -853,6 +853,10 @@ ssim_status_t ssim_batch_set_tuning(ssim_batch* b, ssim_tune_t key, int value) {
     g_err = "batched PCG runs one row per lane (SpMV variant 0 or 1)";
     return SSIM_ERR_INVALID_ARGUMENT;
   }
+  if (key == SSIM_TUNE_GRAD_TILE && value != kBlock) {
+    g_err = "batches evaluate the gradient in 256-tet tiles (their per-mesh tet ranges are padded to 256)";
+    return SSIM_ERR_INVALID_ARGUMENT;
+  }
   if (key == SSIM_TUNE_COMPACT_PCT) {
     if (value < 0 || value > 100) {
       g_err = "compaction threshold must be a percentage in [0, 100]";

@@ -91,6 +91,7 @@ struct ssim_ctx {
   int nslices = 0, nub = 0;
   int xcd_remap = 1;
   int pcg_chunk_max = 64;
+  int grad_tile = ssim_hip::kBlock;  // tets per tile (= threads per workgroup) of the tile-reduced gradient
   int spmv_lanes = 1;
   bool spmv_lanes_default = true;  // pick the variant from the layout at reset
   bool fixed_layout = false;       // the fixed layout: 7 + 7 off-diagonal slots, 6-value diagonals, 16-bit indices
@@ -270,6 +271,7 @@ void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_db
 void check_material(double youngs, double poisson, double density);  // time_step.hpp:369-374
 void require_reset(ssim_ctx* c);
 void upload_material(ssim_ctx* c);
+void build_grad_tiles(ssim_ctx* c);  // the tile maps of the tile-reduced gradient for grad_tile tets per tile
 
 // stepper drivers (solvers.hip)
 struct EvalOut {

@@ -862,6 +862,14 @@ ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value) {
       case SSIM_TUNE_LBFGS_GRAM:
         ctx->lbfgs_gram = value != 0;
         break;
+      case SSIM_TUNE_GRAD_TILE:
+        if (value != 64 && value != 128 && value != kBlock)
+          fail(SSIM_ERR_INVALID_ARGUMENT, "gradient tile must be 64, 128 or 256 tets");
+        if (value != ctx->grad_tile) {
+          ctx->grad_tile = value;
+          if (ctx->nt > 0 && !ctx->hcells.empty()) build_grad_tiles(ctx);
+        }
+        break;
       default:
         fail(SSIM_ERR_INVALID_ARGUMENT, "unknown tuning key");
     }
@@ -888,6 +896,9 @@ ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value) 
       return SSIM_OK;
     case SSIM_TUNE_LBFGS_GRAM:
       *value = ctx->lbfgs_gram;
+      return SSIM_OK;
+    case SSIM_TUNE_GRAD_TILE:
+      *value = ctx->grad_tile;
       return SSIM_OK;
     default:
       g_err = "unknown tuning key";

@@ -321,17 +321,17 @@ __device__ __forceinline__ double elem_grad_contrib(const MeshView& m, int e, co
   return 0.5 * mm * (dd + ss) + sc * (double)psi;
 }
 
-// Tile phase of the tile-reduced gradient (every thread of the workgroup calls it; it synchronises): thread t
-// evaluates tet e = tile·kBlock + t when `active`, puts its 4 vertex contributions into LDS at the tile-local slots
+// Tile phase of the tile-reduced gradient (every thread of the TILE-thread workgroup calls it; it synchronises):
+// thread t evaluates tet e = tile·TILE + t when `active`, puts its 4 vertex contributions into LDS at the tile-local slots
 // tslot[e] (the tile's corners sorted by (vertex, tet, corner)), and the workgroup then sums each tile vertex's
 // contiguous run in slot order into gpart[gt_off[tile] + j].  Returns the thread's element energy.
-template <typename T, int Model>
+template <typename T, int Model, int TILE = kBlock>
 __device__ __forceinline__ double elem_grad_tile(const MeshView& m, int tile, bool active, const double* __restrict__ u,
                                                  const double* __restrict__ uhat, double dt2,
                                                  const ushort4* __restrict__ tslot, const int* __restrict__ gt_off,
                                                  const int* __restrict__ gt_run, double* __restrict__ gpart) {
-  __shared__ double rec[4 * kBlock * 3];
-  const int e = tile * kBlock + (int)threadIdx.x;
+  __shared__ double rec[4 * TILE * 3];
+  const int e = tile * TILE + (int)threadIdx.x;
   double energy = 0.0;
   if (active) {
     double c[4][3];
@@ -346,7 +346,7 @@ __device__ __forceinline__ double elem_grad_tile(const MeshView& m, int tile, bo
   __syncthreads();
   const int t0 = gt_off[tile], ntv = gt_off[tile + 1] - t0;
   const int* __restrict__ run = gt_run + t0 + tile;  // ntv + 1 run boundaries of this tile
-  for (int j = threadIdx.x; j < ntv; j += kBlock) {
+  for (int j = threadIdx.x; j < ntv; j += TILE) {
     double acc[3] = {0.0, 0.0, 0.0};
     const int q1 = run[j + 1];
     for (int q = run[j]; q < q1; ++q)

@@ -9,6 +9,7 @@ namespace ssim_hip {
 constexpr int kBlock = 256;      // threads per workgroup for every kernel (4 wave64)
 constexpr int kSlice = 64;       // rows per blocked-ELL slice (one wavefront)
 constexpr double kSpdFloor = 1e-3;  // eigenvalue floor of the SPD projection (time_step.hpp:52)
+constexpr int kMinTile = 64;  // the smallest tile of the tile-reduced gradient (workgroup partials are sized for it)
 constexpr int kFixedWU = 8;  // uniform upper / lower slot counts of the fixed-width SpMV layout
 constexpr int kFixedWL = 7;
 
@@ -116,9 +117,9 @@ void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, do
 // Reductions: a kernel stores one partial per workgroup; launch_* return the number of partials written.
 // energy + gradient at deformation u (predictor uhat), reduced per tile of kBlock tets in LDS: one partial per
 // (tile, vertex of the tile); energy partials to part_e
-int launch_elem_grad_tile(const MeshView& m, const double* u, const double* uhat, double dt2, const ushort4* tslot,
-                          const int* gt_off, const int* gt_run, double* gpart, double* part_e, bool fp32,
-                          hipStream_t s);
+int launch_elem_grad_tile(const MeshView& m, int tile, const double* u, const double* uhat, double dt2,
+                          const ushort4* tslot, const int* gt_off, const int* gt_run, double* gpart, double* part_e,
+                          bool fp32, hipStream_t s);
 // g[v] = Σ of v's tile partials (in tile order), zeroed on Dirichlet dofs; ‖g‖² partials to part_g
 int launch_grad_gather(int nv, const int* gv_seg, const int* gv_pos, const double* gpart, const unsigned char* dof,
                        double* g, double* part_g, hipStream_t s);

@@ -14,7 +14,7 @@
 
 namespace ssim_hip {
 
-int max_partials(int n) { return (n + kBlock - 1) / kBlock + 1; }
+int max_partials(int n) { return (n + kMinTile - 1) / kMinTile + 1; }
 
 __host__ __device__ static inline int nblocks(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -56,34 +56,45 @@ void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, do
 // The consistent mass M_e = ρV(1+δ_ij)/20 (bilinears.hpp:140-148) is applied element by element instead of
 // through a CSR SpMV.
 
-// Energy + gradient, tile-reduced: workgroup b takes the kBlock consecutive tets of tile b (one per thread),
-// writes each tet's 4 vertex contributions into LDS at the tile-local slots tslot (the tile's corners sorted by
-// (vertex, tet, corner)), and sums every tile vertex's contiguous run in order into one partial per (tile,
-// vertex), gpart[gt_off[b] + j].  Only those partials reach HBM (about 0.6 per tet on a structured bar instead of
-// 4 contributions); k_grad_gather adds each vertex's partials in tile order.
-template <typename T, int Model>
-__global__ __launch_bounds__(kBlock) void k_elem_grad_tile(MeshView m, const double* __restrict__ u,
-                                                           const double* __restrict__ uhat, double dt2,
-                                                           const ushort4* __restrict__ tslot,
-                                                           const int* __restrict__ gt_off,
-                                                           const int* __restrict__ gt_run,
-                                                           double* __restrict__ gpart, double* __restrict__ part_e) {
-  const int lb = logical_block(nblocks(m.nt > 0 ? m.nt : 1), m.xcd_remap);
+// Energy + gradient, tile-reduced: workgroup b takes the TILE consecutive tets of tile b (one per thread; TILE =
+// 64, 128 or 256, a launch knob), writes each tet's 4 vertex contributions into LDS at the tile-local slots tslot
+// (the tile's corners sorted by (vertex, tet, corner)), and sums every tile vertex's contiguous run in order into
+// one partial per (tile, vertex), gpart[gt_off[b] + j].  Only those partials reach HBM (about 0.6 per tet on a
+// structured bar with 256-tet tiles instead of 4 contributions); k_grad_gather adds each vertex's partials in tile
+// order.
+template <typename T, int Model, int TILE>
+__global__ __launch_bounds__(TILE) void k_elem_grad_tile(MeshView m, const double* __restrict__ u,
+                                                         const double* __restrict__ uhat, double dt2,
+                                                         const ushort4* __restrict__ tslot,
+                                                         const int* __restrict__ gt_off,
+                                                         const int* __restrict__ gt_run,
+                                                         double* __restrict__ gpart, double* __restrict__ part_e) {
+  const int nt = m.nt > 0 ? m.nt : 1;
+  const int lb = logical_block((nt + TILE - 1) / TILE, m.xcd_remap);
   if (lb < 0) return;  // uniform over the workgroup
-  const int e = lb * kBlock + threadIdx.x;
-  double energy[1] = {elem_grad_tile<T, Model>(m, lb, e < m.nt && m.cells[e].x >= 0, u, uhat, dt2, tslot, gt_off,
-                                               gt_run, gpart)};
-  store_partials<1>(energy, part_e, lb);
+  const int e = lb * TILE + threadIdx.x;
+  double energy[1] = {elem_grad_tile<T, Model, TILE>(m, lb, e < m.nt && m.cells[e].x >= 0, u, uhat, dt2, tslot,
+                                                     gt_off, gt_run, gpart)};
+  store_partials<1, TILE>(energy, part_e, lb);
 }
 
-int launch_elem_grad_tile(const MeshView& m, const double* u, const double* uhat, double dt2, const ushort4* tslot,
-                          const int* gt_off, const int* gt_run, double* gpart, double* part_e, bool fp32,
-                          hipStream_t s) {
-  const int nb = nblocks(m.nt > 0 ? m.nt : 1);
+int launch_elem_grad_tile(const MeshView& m, int tile, const double* u, const double* uhat, double dt2,
+                          const ushort4* tslot, const int* gt_off, const int* gt_run, double* gpart, double* part_e,
+                          bool fp32, hipStream_t s) {
+  const int nt = m.nt > 0 ? m.nt : 1;
+  const int nb = (nt + tile - 1) / tile;
   dispatch_elem(m.model, fp32, [&](auto k) {
     using K = decltype(k);
-    k_elem_grad_tile<typename K::type, K::model>
-        <<<grid_for(nb, m.xcd_remap), kBlock, 0, s>>>(m, u, uhat, dt2, tslot, gt_off, gt_run, gpart, part_e);
+    const int grid = grid_for(nb, m.xcd_remap);
+    if (tile == 64)
+      k_elem_grad_tile<typename K::type, K::model, 64>
+          <<<grid, 64, 0, s>>>(m, u, uhat, dt2, tslot, gt_off, gt_run, gpart, part_e);
+    else if (tile == 128)
+      k_elem_grad_tile<typename K::type, K::model, 128>
+          <<<grid, 128, 0, s>>>(m, u, uhat, dt2, tslot, gt_off, gt_run, gpart, part_e);
+    else
+      k_elem_grad_tile<typename K::type, K::model, kBlock>
+          <<<grid, kBlock, 0, s>>>(m, u, uhat, dt2, tslot, gt_off, gt_run, gpart, part_e);
   });
   return nb;
 }

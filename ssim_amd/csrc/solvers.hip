@@ -24,8 +24,8 @@ EvalOut eval_energy_grad(ssim_ctx* c, const double* u, double* g) {
   const double dt2 = c->dt * c->dt;
   int ne = 0, ng = 0;
   timed(c, SSIM_KERNEL_ELEM_GRAD, [&] {
-    ne = launch_elem_grad_tile(c->mesh(), u, c->a(SSIM_ARRAY_INERTIA), dt2, c->gt_slot.p, c->gt_off.p, c->gt_run.p,
-                               c->gpart.p, c->part_e(), c->prm.fp32_elements != 0, s);
+    ne = launch_elem_grad_tile(c->mesh(), c->grad_tile, u, c->a(SSIM_ARRAY_INERTIA), dt2, c->gt_slot.p, c->gt_off.p,
+                               c->gt_run.p, c->gpart.p, c->part_e(), c->prm.fp32_elements != 0, s);
   });
   timed(c, SSIM_KERNEL_GRAD_REDUCE, [&] {
     ng = launch_grad_gather(c->nv, c->gv_seg.p, c->gv_pos.p, c->gpart.p, c->dof.p, g, c->part_g(), s);

@@ -302,16 +302,50 @@ void build_topology(ssim_ctx* c) {
       if (cells[4 * e] >= 0)
         for (int i = 0; i < 4; ++i) gperm[4 * e + i] = pos[cells[4 * e + i]]++;
   }
-  // tiles of kBlock consecutive tets: per tile its vertices ascending, the tile's corners sorted by (vertex, tet,
-  // corner) into LDS slots, and the run boundaries of every tile vertex; per vertex the positions of its tile
-  // partials in tile order
-  const int ntiles = (nt + kBlock - 1) / kBlock;
+  hipStream_t s = c->stream;
+  build_grad_tiles(c);
+  c->wu.upload(c->hwu.data(), ns, s);
+  c->offu.upload(c->hoffu.data(), ns, s);
+  c->wl.upload(hwl.data(), ns, s);
+  c->offl.upload(hoffl.data(), ns, s);
+  if (c->fixed_layout) {
+    c->ucol.release();
+    c->lref.release();
+    c->idx.upload(hidx, s);
+  } else {
+    c->idx.release();
+    c->ucol.upload(hucol, s);
+    c->lref.upload(hlref, s);
+  }
+  c->ub_rc.upload(hub_rc, s);
+  c->ub_pos.upload(hub_pos, s);
+  c->hseg.upload(hcount, s);
+  c->hperm.upload(hperm, s);
+  c->gseg.upload(gseg, s);
+  c->gperm.upload(gperm, s);
+  // + the zero block of the padded lower references (after the last slice, or leading in the fixed layout)
+  c->val.alloc(c->fixed_layout ? kBellZeroLead + (size_t)ns * kBellSliceVals : 9 * (size_t)offu + 9 * kSlice);
+  HIPCHK(hipMemsetAsync(c->val.p, 0, c->val.count * sizeof(double), s));
+  c->gcontrib.alloc(12 * (size_t)std::max(nt, 1));
+  c->hcontrib.alloc(90 * (size_t)std::max(nt, 1));
+  sync(c);  // host staging vectors go out of scope
+}
+
+}  // namespace
+
+// Tiles of grad_tile consecutive tets for the tile-reduced gradient: per tile its vertices ascending, the tile's
+// corners sorted by (vertex, tet, corner) into LDS slots, and the run boundaries of every tile vertex; per vertex
+// the positions of its tile partials in tile order.
+void ssim_hip::build_grad_tiles(ssim_ctx* c) {
+  const int nv = c->nv, nt = c->nt, T = c->grad_tile;
+  const int* cells = c->hcells.data();
+  const int ntiles = (nt + T - 1) / T;
   std::vector<int> gt_off(ntiles + 1, 0), gt_run, gt_vert;
   std::vector<ushort4> gt_slot(std::max(nt, 1), make_ushort4(0, 0, 0, 0));
   {
     std::vector<int> tv, cnt, fill;
     for (int t = 0; t < ntiles; ++t) {
-      const int e0 = t * kBlock, e1 = std::min(nt, e0 + kBlock);
+      const int e0 = t * T, e1 = std::min(nt, e0 + T);
       tv.clear();
       for (int e = e0; e < e1; ++e)
         if (cells[4 * e] >= 0)
@@ -352,34 +386,8 @@ void build_topology(ssim_ctx* c) {
   c->gv_seg.upload(gv_seg, s);
   c->gv_pos.upload(gv_pos, s);
   c->gpart.alloc(3 * std::max<size_t>(gt_vert.size(), 1));
-  c->wu.upload(c->hwu.data(), ns, s);
-  c->offu.upload(c->hoffu.data(), ns, s);
-  c->wl.upload(hwl.data(), ns, s);
-  c->offl.upload(hoffl.data(), ns, s);
-  if (c->fixed_layout) {
-    c->ucol.release();
-    c->lref.release();
-    c->idx.upload(hidx, s);
-  } else {
-    c->idx.release();
-    c->ucol.upload(hucol, s);
-    c->lref.upload(hlref, s);
-  }
-  c->ub_rc.upload(hub_rc, s);
-  c->ub_pos.upload(hub_pos, s);
-  c->hseg.upload(hcount, s);
-  c->hperm.upload(hperm, s);
-  c->gseg.upload(gseg, s);
-  c->gperm.upload(gperm, s);
-  // + the zero block of the padded lower references (after the last slice, or leading in the fixed layout)
-  c->val.alloc(c->fixed_layout ? kBellZeroLead + (size_t)ns * kBellSliceVals : 9 * (size_t)offu + 9 * kSlice);
-  HIPCHK(hipMemsetAsync(c->val.p, 0, c->val.count * sizeof(double), s));
-  c->gcontrib.alloc(12 * (size_t)std::max(nt, 1));
-  c->hcontrib.alloc(90 * (size_t)std::max(nt, 1));
   sync(c);  // host staging vectors go out of scope
 }
-
-}  // namespace
 
 void ssim_hip::upload_material(ssim_ctx* c) {
   std::vector<double> lam(c->nt), mu(c->nt);

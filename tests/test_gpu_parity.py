@@ -109,6 +109,42 @@ def test_energy_gradient_hessian_mass_match_oracle(bar):
     assert abs(g.total_mass() - o.total_mass()) <= 1e-12 * o.total_mass()
 
 
+def test_gradient_tile_sizes_match_oracle():
+    """The tile-reduced energy + gradient with 64-, 128- and 256-tet tiles (SSIM_TUNE_GRAD_TILE) against the oracle
+    (≤1e-10) and against each other (≤1e-13: the same contributions summed in other groupings), on a bar whose tets
+    fill several tiles of every size; then three Newton steps with 64-tet tiles against the oracle's positions."""
+    case = mesh.cantilever_bar(12, 4, 4, seed=2)
+    g, o = _pair(case)
+    assert case.nt > 4 * 256 and g.tuning()["grad_tile"] == 256
+    rng = np.random.default_rng(4)
+    u = _strained(case, 6, 0.4)
+    uhat = u + 1e-3 * rng.standard_normal(3 * case.nv)
+    uhat[case.dof_type == 1] = 0
+    g.set_array("next_deform", u)
+    g.set_array("inertia", uhat)
+    o.set_vectors(u_hat=uhat)
+    Eo, go = o.energy_grad(u)
+    got = {}
+    for tile in (256, 128, 64):
+        g.set_tuning(grad_tile=tile)
+        print(f"[gpu] gradient tiles of {tile} tets", flush=True)
+        got[tile] = (g.update_energy_and_gradients(), g.forces().T.reshape(-1).copy())
+        assert abs(got[tile][0] - Eo) <= 1e-10 * abs(Eo)
+        assert rel_err(got[tile][1], go) <= 1e-10
+    for tile in (128, 64):
+        assert abs(got[tile][0] - got[256][0]) <= 1e-13 * abs(got[256][0])
+        assert rel_err(got[tile][1], got[256][1]) <= 1e-13
+    with pytest.raises(ValueError):
+        g.set_tuning(grad_tile=100)
+    g2, o2 = _pair(case)
+    g2.set_tuning(grad_tile=64)
+    for step in range(3):
+        g2.step()
+        _, st = o2.step()
+        assert g2.last_stats()["newton_iterations"] == st["newton"], step
+        assert rel_err(_positions(case, g2.deformation().T.reshape(-1)), _positions(case, o2.get_state()[0])) <= 1e-8
+
+
 def test_rest_shape_and_filtered_mass_match(bar):
     """dminv / rest_vol against numpy (Dm⁻¹ and |det Dm|/6, ΣV = the bar's volume) and mass_matrix_filtered against
     the oracle's mass matrix with identity Dirichlet rows and columns (time_step.hpp:460-466)."""
