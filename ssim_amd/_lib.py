@@ -151,7 +151,8 @@ SIGNATURES = {
     "ssim_batch_get_kernel_timing": [_V, ctypes.c_int, _D, ctypes.POINTER(ctypes.c_longlong)],
 }
 
-TUNING = {"xcd_remap": 0, "pcg_chunk_max": 1, "spmv_lanes": 2, "pcg_graph": 3, "lbfgs_gram": 5, "grad_tile": 6}
+TUNING = {"xcd_remap": 0, "pcg_chunk_max": 1, "spmv_lanes": 2, "pcg_graph": 3, "lbfgs_gram": 5, "grad_tile": 6,
+          "hess_tile": 7}
 BATCH_TUNING = dict(TUNING, compact_pct=4)
 
 _lib = None

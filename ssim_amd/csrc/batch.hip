@@ -153,14 +153,26 @@ void bassemble(ssim_batch* b, const DBuf<int>& mask, const double* u, int make_s
   ssim_ctx* c = &b->c;
   hipStream_t s = c->stream;
   const BatchView bt = b->view(mask);
-  timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
-    launch_belem_hess(c->mesh(), bt, b->ntwg, u, c->dt * c->dt, make_spsd, c->hperm.p, c->hcontrib.p,
-                      c->prm.fp32_elements != 0, s);
-  });
-  timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
-    launch_bhess_reduce(bt, c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->val.p,
-                        c->dinv.p, s);
-  });
+  if (c->hess_tile) {
+    timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
+      launch_belem_hess_tile(c->mesh(), bt, b->ntwg, u, c->dt * c->dt, make_spsd, c->ht_slot.p, c->ht_off.p,
+                             c->ht_run.p, c->hpart.p, c->hparts, c->prm.fp32_elements != 0, s);
+    });
+    timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
+      launch_bhess_gather(bt, c->nub, c->hb_seg.p, c->hb_pos.p, c->hpart.p, c->hparts, c->ub_rc.p, c->ub_pos.p,
+                          c->dof.p, 1, c->val.p, c->dinv.p, s);
+    });
+  } else {
+    ensure_hcontrib(c);
+    timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
+      launch_belem_hess(c->mesh(), bt, b->ntwg, u, c->dt * c->dt, make_spsd, c->hperm.p, c->hcontrib.p,
+                        c->prm.fp32_elements != 0, s);
+    });
+    timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
+      launch_bhess_reduce(bt, c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->val.p,
+                          c->dinv.p, s);
+    });
+  }
   HIPCHK(hipGetLastError());
 }
 

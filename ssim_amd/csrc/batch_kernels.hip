@@ -116,6 +116,38 @@ __global__ __launch_bounds__(kBlock) void k_belem_hess(MeshView m, BatchView bt,
   elem_hess_tet<T, Model>(m, e, u, dt2, make_spsd, hperm, hcontrib);
 }
 
+// tile-reduced assembly (single-mesh k_elem_hess_tile): tet workgroup b is tile b of the concatenated mesh, whose
+// per-mesh tet ranges are padded to whole tiles, so a tile belongs to one mesh and a masked-out mesh skips it whole
+template <typename T, int Model>
+__global__ __launch_bounds__(kHessTile) void k_belem_hess_tile(MeshView m, BatchView bt, const double* __restrict__ u,
+                                                               double dt2, int make_spsd,
+                                                               const unsigned* __restrict__ hslot,
+                                                               const int* __restrict__ ht_off,
+                                                               const int* __restrict__ ht_run,
+                                                               double* __restrict__ hpart, long long P) {
+  const int mesh = bt.twg_mesh[blockIdx.x];
+  if (!bt.mask[mesh]) return;  // uniform over the workgroup
+  const int e = blockIdx.x * kHessTile + threadIdx.x;
+  elem_hess_tile<T, Model, kHessTile>(m, blockIdx.x, e < bt.tend[mesh], u, dt2, make_spsd, hslot, ht_off, ht_run,
+                                      hpart, P);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bhess_gather(BatchView bt, int nub, const int* __restrict__ hb_seg,
+                                                         const int* __restrict__ hb_pos,
+                                                         const double* __restrict__ hpart, long long P,
+                                                         const int2* __restrict__ ub_rc,
+                                                         const int* __restrict__ ub_pos,
+                                                         const unsigned char* __restrict__ dof, int filter_dbc,
+                                                         double* __restrict__ val, double* __restrict__ dinv) {
+  const int ub = blockIdx.x * kBlock + threadIdx.x;
+  if (ub >= nub) return;
+  const int2 rc = ub_rc[ub];
+  if (!bt.mask[bt.vwg_mesh[rc.x / kBatchRows]]) return;
+  double S[9];
+  hess_gather_block(ub, hb_seg, hb_pos, hpart, P, S);
+  hess_store_block(ub, rc, S, ub_pos, dof, filter_dbc, val, dinv, nullptr);
+}
+
 __global__ __launch_bounds__(kBlock) void k_bhess_reduce(BatchView bt, int nub, const int* __restrict__ hseg,
                                                          const double* __restrict__ hcontrib,
                                                          const int2* __restrict__ ub_rc,
@@ -440,6 +472,25 @@ void launch_belem_hess(const MeshView& m, const BatchView& bt, int ntwg, const d
     using K = decltype(k);
     k_belem_hess<typename K::type, K::model><<<ntwg, kBlock, 0, s>>>(m, bt, u, dt2, make_spsd, hperm, hcontrib);
   });
+}
+
+void launch_belem_hess_tile(const MeshView& m, const BatchView& bt, int ntwg, const double* u, double dt2,
+                            int make_spsd, const unsigned* hslot, const int* ht_off, const int* ht_run,
+                            double* hpart, long long P, bool fp32, hipStream_t s) {
+  if (ntwg <= 0) return;
+  dispatch_elem(m.model, fp32, [&](auto k) {
+    using K = decltype(k);
+    k_belem_hess_tile<typename K::type, K::model>
+        <<<ntwg, kHessTile, 0, s>>>(m, bt, u, dt2, make_spsd, hslot, ht_off, ht_run, hpart, P);
+  });
+}
+
+void launch_bhess_gather(const BatchView& bt, int nub, const int* hb_seg, const int* hb_pos, const double* hpart,
+                         long long P, const int2* ub_rc, const int* ub_pos, const unsigned char* dof, int filter_dbc,
+                         double* bell_val, double* dinv, hipStream_t s) {
+  if (nub <= 0) return;
+  k_bhess_gather<<<(nub + kBlock - 1) / kBlock, kBlock, 0, s>>>(bt, nub, hb_seg, hb_pos, hpart, P, ub_rc, ub_pos,
+                                                                 dof, filter_dbc, bell_val, dinv);
 }
 
 void launch_bhess_reduce(const BatchView& bt, int nub, const int* hseg, const double* hcontrib, const int2* ub_rc,

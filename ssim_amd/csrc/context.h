@@ -112,6 +112,14 @@ struct ssim_ctx {
   ssim_hip::DBuf<ushort4> gt_slot;
   ssim_hip::DBuf<int> gt_off, gt_run, gv_seg, gv_pos;
   ssim_hip::DBuf<double> gpart;
+  // tile-reduced Hessian: tile-local LDS slots of every (tet, pair) contribution (two 16-bit slots per word, 5 words
+  // per tet, word-major), per tile the offsets of its upper blocks and their run boundaries, per upper block the
+  // positions of its (tile, block) partials, and the partials, component-major ([9][hparts])
+  ssim_hip::DBuf<unsigned> ht_slot;
+  ssim_hip::DBuf<int> ht_off, ht_run, hb_seg, hb_pos;
+  ssim_hip::DBuf<double> hpart;
+  long long hparts = 0;
+  int hess_tile = 1;  // 1: tile-reduced Hessian assembly; 0: per-tet contribution slots (k_elem_hess + k_hess_reduce)
   ssim_hip::DBuf<int2> ub_rc, lref;
   ssim_hip::DBuf<uint4> idx;  // fixed layout: two 16-byte index words per row
   ssim_hip::DBuf<double> gcontrib, hcontrib, val, massval;
@@ -268,10 +276,14 @@ void timed_ext(ssim_ctx* c, ssim_kernel_t k, F&& launch) {
 
 void do_reset(ssim_ctx* c);
 void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_dbc, double dt2, double* values);
+void assemble_into(ssim_ctx* c, const double* u, double dt2, int make_spsd, int filter_dbc, double* values,
+                   double* dinv, double* diag, bool fp32);
 void check_material(double youngs, double poisson, double density);  // time_step.hpp:369-374
 void require_reset(ssim_ctx* c);
 void upload_material(ssim_ctx* c);
 void build_grad_tiles(ssim_ctx* c);  // the tile maps of the tile-reduced gradient for grad_tile tets per tile
+void build_hess_tiles(ssim_ctx* c, const std::vector<int>& hub);  // the tile maps of the tile-reduced Hessian
+void ensure_hcontrib(ssim_ctx* c);  // the per-tet contribution slots (projective dynamics, hess_tile = 0)
 
 // stepper drivers (solvers.hip)
 struct EvalOut {

@@ -694,11 +694,7 @@ ssim_status_t export_mass(ssim_ctx* ctx, int filtered, int* rowptr, int* colidx,
     if (values) {
       ctx->massval.alloc(ctx->val.count);
       HIPCHK(hipMemsetAsync(ctx->massval.p, 0, sizeof(double) * ctx->val.count, ctx->stream));
-      launch_elem_hess(ctx->mesh(), ctx->a(SSIM_ARRAY_NEXT_DEFORM), 0.0, 0, ctx->hperm.p, ctx->hcontrib.p, false,
-                       ctx->stream);
-      launch_hess_reduce(ctx->nub, ctx->hseg.p, ctx->hcontrib.p, ctx->ub_rc.p, ctx->ub_pos.p, ctx->dof.p, filtered,
-                         ctx->massval.p, ctx->tmp.p, nullptr, ctx->stream);
-      HIPCHK(hipGetLastError());
+      assemble_into(ctx, ctx->a(SSIM_ARRAY_NEXT_DEFORM), 0.0, 0, filtered, ctx->massval.p, ctx->tmp.p, nullptr, false);
     }
     export_csr(ctx, ctx->massval.p, rowptr, colidx, values);
   });
@@ -870,6 +866,9 @@ ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value) {
           if (ctx->nt > 0 && !ctx->hcells.empty()) build_grad_tiles(ctx);
         }
         break;
+      case SSIM_TUNE_HESS_TILE:
+        ctx->hess_tile = value != 0;
+        break;
       default:
         fail(SSIM_ERR_INVALID_ARGUMENT, "unknown tuning key");
     }
@@ -899,6 +898,9 @@ ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value) 
       return SSIM_OK;
     case SSIM_TUNE_GRAD_TILE:
       *value = ctx->grad_tile;
+      return SSIM_OK;
+    case SSIM_TUNE_HESS_TILE:
+      *value = ctx->hess_tile;
       return SSIM_OK;
     default:
       g_err = "unknown tuning key";

@@ -421,6 +421,146 @@ __device__ __forceinline__ void elem_hess_tet(const MeshView& m, int e, const do
   });
 }
 
+// Compare-exchange of two corners by vertex id (with their shape-gradient rows); compile-time indices only.
+template <typename T, int A, int B>
+__device__ __forceinline__ void corner_cx(int (&id)[4], T (&D)[4][3]) {
+  const bool sw = id[A] > id[B];
+  const int ia = id[A], ib = id[B];
+  id[A] = sw ? ib : ia;
+  id[B] = sw ? ia : ib;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T da = D[A][c], db = D[B][c];
+    D[A][c] = sw ? db : da;
+    D[B][c] = sw ? da : db;
+  }
+}
+
+// Tile phase of the tile-reduced Hessian assembly (every thread of the TILE-thread workgroup calls it; it
+// synchronises).  Thread t takes tet e = tile·TILE + t when `active`: its 9x9 material Hessian H (SPD-projected when
+// make_spsd) and shape gradients D, with the four corners put in ascending vertex order, so that local block (v, w),
+// v ≤ w in that order, is the upper block (min id, max id) of the global matrix (K_wv = K_vwᵀ since H is
+// symmetric).  For each block row i it forms the components (i, 0..2) of its 10 blocks
+// K_vw = dt²V·D_vᵀ·H·D_w plus the consistent mass ρV(1+δ_vw)/20 on the diagonal components, puts them into LDS at
+// its tile-local slots (hslot: the tile's contributions sorted by (upper block, tet, pair), two 16-bit slots per
+// word, word j of tet e at hslot[j·nt + e]), and the workgroup sums every tile block's contiguous run in slot order
+// into one partial per (tile, block) and component: hpart[comp·P + ht_off[tile] + j].  Only those partials reach
+// HBM (about 2.8 blocks per tet on a structured bar with 256-tet tiles instead of 10).
+template <typename T, int Model, int TILE>
+__device__ __forceinline__ void elem_hess_tile(const MeshView& m, int tile, bool active, const double* __restrict__ u,
+                                               double dt2, int make_spsd, const unsigned* __restrict__ hslot,
+                                               const int* __restrict__ ht_off, const int* __restrict__ ht_run,
+                                               double* __restrict__ hpart, long long P) {
+  constexpr int S = 10 * TILE;
+  __shared__ double rec[3 * S];
+  const int e = tile * TILE + (int)threadIdx.x;
+  T H[45], D[4][3];
+  double sc = 0.0, mm = 0.0;
+  unsigned sw[5] = {0u, 0u, 0u, 0u, 0u};
+  if (active) {
+    const TetLocal t = load_tet(m, e);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) sw[j] = hslot[(size_t)j * m.nt + e];
+    double x[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const size_t k = 3 * (size_t)t.id[i] + a;
+        x[i][a] = m.X[k] + u[k];
+      }
+    double Fd[9];
+    def_grad(x, t.B, Fd);
+    T F[9], Bt[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      F[k] = (T)Fd[k];
+      Bt[k] = (T)t.B[k];
+    }
+    material_hessian<Model>(F, (T)t.lam, (T)t.mu, make_spsd, (T)kSpdFloor, H);
+    shape_grads(Bt, D);
+    int id[4] = {t.id[0], t.id[1], t.id[2], t.id[3]};
+    corner_cx<T, 0, 1>(id, D);
+    corner_cx<T, 2, 3>(id, D);
+    corner_cx<T, 0, 2>(id, D);
+    corner_cx<T, 1, 3>(id, D);
+    corner_cx<T, 1, 2>(id, D);
+    sc = dt2 * t.V;
+    mm = m.rho * t.V / 20.0;
+  }
+  const int t0 = ht_off[tile], nb = ht_off[tile + 1] - t0;
+  const int* __restrict__ run = ht_run + t0 + tile;  // nb + 1 run boundaries of this tile
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (active) {
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        T G[4][3];
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            G[w][c] = H[sym9(3 * i + c, 3 * jj + 0)] * D[w][0] + H[sym9(3 * i + c, 3 * jj + 1)] * D[w][1] +
+                      H[sym9(3 * i + c, 3 * jj + 2)] * D[w][2];
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+#pragma unroll
+          for (int v = 0; v <= w; ++v) {
+            const int k = pair_index(v, w);
+            double val = sc * (double)(D[v][0] * G[w][0] + D[v][1] * G[w][1] + D[v][2] * G[w][2]);
+            if (i == jj) val += (v == w) ? 2.0 * mm : mm;
+            rec[jj * S + (int)((sw[k >> 1] >> (16 * (k & 1))) & 0xffffu)] = val;
+          }
+      }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < nb; j += TILE) {
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+      const int q1 = run[j + 1];
+      for (int q = run[j]; q < q1; ++q) {
+        a0 += rec[q];
+        a1 += rec[S + q];
+        a2 += rec[2 * S + q];
+      }
+      double* __restrict__ dst = hpart + (size_t)(3 * i) * P + t0 + j;
+      dst[0] = a0;
+      dst[P] = a1;
+      dst[2 * P] = a2;
+    }
+    __syncthreads();
+  }
+}
+
+// upper block ub: the sum of its tile partials in tile order (hb_pos[hb_seg[ub] .. hb_seg[ub+1])).  The first four
+// partials' loads are issued together (clamped positions, no branch; a block of a structured mesh has at most four),
+// then added in order; longer runs continue one partial at a time.
+__device__ __forceinline__ void hess_gather_block(int ub, const int* __restrict__ hb_seg,
+                                                  const int* __restrict__ hb_pos, const double* __restrict__ hpart,
+                                                  long long P, double (&S)[9]) {
+  constexpr int U = 4;
+  const int k0 = hb_seg[ub], k1 = hb_seg[ub + 1];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) S[c] = 0.0;
+  if (k1 <= k0) return;
+  size_t p[U];
+#pragma unroll
+  for (int i = 0; i < U; ++i) p[i] = (size_t)hb_pos[min(k0 + i, k1 - 1)];
+  double v[U][9];
+#pragma unroll
+  for (int i = 0; i < U; ++i)
+#pragma unroll
+    for (int c = 0; c < 9; ++c) v[i][c] = hpart[(size_t)c * P + p[i]];
+#pragma unroll
+  for (int i = 0; i < U; ++i)
+#pragma unroll
+    for (int c = 0; c < 9; ++c) S[c] += (k0 + i < k1) ? v[i][c] : 0.0;
+  for (int k = k0 + U; k < k1; ++k) {
+    const size_t q = (size_t)hb_pos[k];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) S[c] += hpart[(size_t)c * P + q];
+  }
+}
+
 // Element part of the adjoint of one implicit step, for the adjoint vector l (solution of H l = ∂L/∂u at the
 // converged u):
 //   mass contributions (ρV/20)(l_i + Σ_j l_j) to the gradient slots gperm[4e+i] (reduced per vertex: M l),

@@ -96,6 +96,7 @@ struct EvalScalars {
 // padding tets are absent (first vertex index −1).
 constexpr int kBatchRows = 256;         // vertex and tet granule of a mesh (= kBlock)
 constexpr int kBatchSlices = kBatchRows / kSlice;  // blocked-ELL slices per vertex workgroup
+constexpr int kHessTile = kBatchRows;  // tets per tile of the tile-reduced Hessian (a batch tile is in one mesh)
 struct BatchView {
   int nmesh;
   const int* vwg_mesh;  // [vertex workgroups] mesh of each block of kBatchRows vertices
@@ -135,6 +136,15 @@ void launch_elem_hess(const MeshView& m, const double* u, double dt2, int make_s
 void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const int2* ub_rc, const int* ub_pos,
                         const unsigned char* dof, int filter_dbc, double* bell_val, double* dinv, double* diag,
                         hipStream_t s);
+// tile-reduced Hessian assembly: one partial per (tile of kHessTile tets, upper block) and component into
+// hpart[comp·P + position], then per upper block the sum of its partials in tile order, the Dirichlet identity
+// filter, the blocked-ELL store and the Jacobi inverse diagonal (and the diagonal when diag is not null)
+void launch_elem_hess_tile(const MeshView& m, const double* u, double dt2, int make_spsd, const unsigned* hslot,
+                           const int* ht_off, const int* ht_run, double* hpart, long long P, bool fp32,
+                           hipStream_t s);
+void launch_hess_gather(int nub, const int* hb_seg, const int* hb_pos, const double* hpart, long long P,
+                        const int2* ub_rc, const int* ub_pos, const unsigned char* dof, int filter_dbc,
+                        double* bell_val, double* dinv, double* diag, hipStream_t s);
 void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double* psi, double* P, double* H,
                       double* K, bool fp32, hipStream_t s);
 
@@ -233,6 +243,13 @@ void launch_belem_hess(const MeshView& m, const BatchView& bt, int ntwg, const d
 void launch_bhess_reduce(const BatchView& bt, int nub, const int* hseg, const double* hcontrib, const int2* ub_rc,
                          const int* ub_pos, const unsigned char* dof, int filter_dbc, double* bell_val, double* dinv,
                          hipStream_t s);
+// tile-reduced batched Hessian assembly (masked meshes only; tet workgroup b is tile b)
+void launch_belem_hess_tile(const MeshView& m, const BatchView& bt, int ntwg, const double* u, double dt2,
+                            int make_spsd, const unsigned* hslot, const int* ht_off, const int* ht_run,
+                            double* hpart, long long P, bool fp32, hipStream_t s);
+void launch_bhess_gather(const BatchView& bt, int nub, const int* hb_seg, const int* hb_pos, const double* hpart,
+                         long long P, const int2* ub_rc, const int* ub_pos, const unsigned char* dof, int filter_dbc,
+                         double* bell_val, double* dinv, hipStream_t s);
 void launch_belem_adjoint(const MeshView& m, const BatchView& bt, int ntwg, const double* u, const double* lv,
                           double dt2, const int* gperm, double* gcontrib, double* g_lam, double* g_mu, hipStream_t s);
 // compact (caller's, unpadded, meshes in order) ↔ padded layouts; vcoff / tcoff = compact vertex / tet offset of

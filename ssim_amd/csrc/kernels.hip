@@ -238,6 +238,60 @@ void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const 
                                                 diag);
 }
 
+// Tile-reduced Hessian assembly: workgroup b evaluates the kHessTile tets of tile b and writes one partial per
+// (tile, upper block) and component (device_common.h::elem_hess_tile); k_hess_gather adds each block's partials in
+// tile order.  Replaces the per-tet slots of k_elem_hess + k_hess_reduce (10 blocks of 72 B per tet written and
+// read back) by about 2.8 partial blocks per tet on a structured bar.
+template <typename T, int Model>
+__global__ __launch_bounds__(kHessTile) void k_elem_hess_tile(MeshView m, const double* __restrict__ u, double dt2,
+                                                              int make_spsd, const unsigned* __restrict__ hslot,
+                                                              const int* __restrict__ ht_off,
+                                                              const int* __restrict__ ht_run,
+                                                              double* __restrict__ hpart, long long P) {
+  const int nt = m.nt > 0 ? m.nt : 1;
+  const int lb = logical_block((nt + kHessTile - 1) / kHessTile, m.xcd_remap);
+  if (lb < 0) return;  // uniform over the workgroup
+  const int e = lb * kHessTile + threadIdx.x;
+  elem_hess_tile<T, Model, kHessTile>(m, lb, e < m.nt && m.cells[e].x >= 0, u, dt2, make_spsd, hslot, ht_off, ht_run,
+                                      hpart, P);
+}
+
+void launch_elem_hess_tile(const MeshView& m, const double* u, double dt2, int make_spsd, const unsigned* hslot,
+                           const int* ht_off, const int* ht_run, double* hpart, long long P, bool fp32,
+                           hipStream_t s) {
+  if (m.nt <= 0) return;
+  const int nb = (m.nt + kHessTile - 1) / kHessTile;
+  dispatch_elem(m.model, fp32, [&](auto k) {
+    using K = decltype(k);
+    k_elem_hess_tile<typename K::type, K::model>
+        <<<grid_for(nb, m.xcd_remap), kHessTile, 0, s>>>(m, u, dt2, make_spsd, hslot, ht_off, ht_run, hpart, P);
+  });
+}
+
+// One thread per upper block (r ≤ c): the sum of its tile partials, then as k_hess_reduce (Dirichlet identity
+// filter, blocked-ELL store, Jacobi inverse diagonal).
+__global__ __launch_bounds__(kBlock) void k_hess_gather(int nub, const int* __restrict__ hb_seg,
+                                                        const int* __restrict__ hb_pos,
+                                                        const double* __restrict__ hpart, long long P,
+                                                        const int2* __restrict__ ub_rc, const int* __restrict__ ub_pos,
+                                                        const unsigned char* __restrict__ dof, int filter_dbc,
+                                                        double* __restrict__ val, double* __restrict__ dinv,
+                                                        double* __restrict__ diag) {
+  const int ub = blockIdx.x * kBlock + threadIdx.x;
+  if (ub >= nub) return;
+  double S[9];
+  hess_gather_block(ub, hb_seg, hb_pos, hpart, P, S);
+  hess_store_block(ub, ub_rc[ub], S, ub_pos, dof, filter_dbc, val, dinv, diag);
+}
+
+void launch_hess_gather(int nub, const int* hb_seg, const int* hb_pos, const double* hpart, long long P,
+                        const int2* ub_rc, const int* ub_pos, const unsigned char* dof, int filter_dbc,
+                        double* bell_val, double* dinv, double* diag, hipStream_t s) {
+  if (nub <= 0) return;
+  k_hess_gather<<<nblocks(nub), kBlock, 0, s>>>(nub, hb_seg, hb_pos, hpart, P, ub_rc, ub_pos, dof, filter_dbc,
+                                                bell_val, dinv, diag);
+}
+
 // per-element outputs for parity checks: Ψ, P, H (full 9x9, projected if make_spsd), K (full 12x12, unscaled)
 template <typename T, int Model>
 __global__ __launch_bounds__(kBlock) void k_elem_eval(MeshView m, const double* __restrict__ u, int make_spsd,

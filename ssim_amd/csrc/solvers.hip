@@ -47,18 +47,42 @@ static double device_dot(ssim_ctx* c, const double* a, const double* b) {
 }
 
 
+void ensure_hcontrib(ssim_ctx* c) {
+  if (!c->hcontrib.p) c->hcontrib.alloc(90 * (size_t)std::max(c->nt, 1));
+}
+
+// Hessian assembly at deformation u (dt²-scaled stiffness plus the consistent mass) into the blocked-ELL values,
+// the Jacobi inverse diagonal dinv and, when diag is not null, the diagonal: the tile-reduced kernels, or with
+// hess_tile = 0 the per-tet contribution slots and their per-block reduce
+void assemble_into(ssim_ctx* c, const double* u, double dt2, int make_spsd, int filter_dbc, double* values,
+                   double* dinv, double* diag, bool fp32) {
+  hipStream_t s = c->stream;
+  if (c->hess_tile) {
+    timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
+      launch_elem_hess_tile(c->mesh(), u, dt2, make_spsd, c->ht_slot.p, c->ht_off.p, c->ht_run.p, c->hpart.p,
+                            c->hparts, fp32, s);
+    });
+    timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
+      launch_hess_gather(c->nub, c->hb_seg.p, c->hb_pos.p, c->hpart.p, c->hparts, c->ub_rc.p, c->ub_pos.p, c->dof.p,
+                         filter_dbc, values, dinv, diag, s);
+    });
+  } else {
+    ensure_hcontrib(c);
+    timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
+      launch_elem_hess(c->mesh(), u, dt2, make_spsd, c->hperm.p, c->hcontrib.p, fp32, s);
+    });
+    timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
+      launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, filter_dbc, values,
+                         dinv, diag, s);
+    });
+  }
+  HIPCHK(hipGetLastError());
+}
+
 // update_hessian at deformation u into the blocked-ELL values and the Jacobi diagonal
 void assemble_hessian(ssim_ctx* c, const double* u, int make_spsd, int filter_dbc, double dt2,
                                 double* values) {
-  hipStream_t s = c->stream;
-  timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
-    launch_elem_hess(c->mesh(), u, dt2, make_spsd, c->hperm.p, c->hcontrib.p, c->prm.fp32_elements != 0, s);
-  });
-  timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
-    launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, filter_dbc, values,
-                       c->dinv.p, nullptr, s);
-  });
-  HIPCHK(hipGetLastError());
+  assemble_into(c, u, dt2, make_spsd, filter_dbc, values, c->dinv.p, nullptr, c->prm.fp32_elements != 0);
   c->stats.hessian_evals++;
 }
 
@@ -278,6 +302,7 @@ void build_pd(ssim_ctx* c) {
   c->pd_q.alloc(std::max(c->n, 1));
   HIPCHK(hipMemsetAsync(c->pd_val.p, 0, sizeof(double) * c->pd_val.count, s));  // padding slots stay zero
   const double kappa = pd_diffusivity(c->model, c->youngs, c->poisson);
+  ensure_hcontrib(c);
   launch_elem_pd(c->mesh(), c->dt * c->dt * kappa, c->hperm.p, c->hcontrib.p, s);
   launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->pd_val.p,
                      c->pd_dinv.p, nullptr, s);
@@ -293,10 +318,7 @@ static void ensure_hrest(ssim_ctx* c) {
   c->hrest.alloc(n);
   HIPCHK(hipMemsetAsync(c->tmp.p, 0, sizeof(double) * n, s));
   const double dt2 = c->dt * c->dt;
-  launch_elem_hess(c->mesh(), c->tmp.p, dt2, 1, c->hperm.p, c->hcontrib.p, c->prm.fp32_elements != 0, s);
-  launch_hess_reduce(c->nub, c->hseg.p, c->hcontrib.p, c->ub_rc.p, c->ub_pos.p, c->dof.p, 1, c->val.p, c->dinv.p,
-                     c->hrest.p, s);
-  HIPCHK(hipGetLastError());
+  assemble_into(c, c->tmp.p, dt2, 1, 1, c->val.p, c->dinv.p, c->hrest.p, c->prm.fp32_elements != 0);
 }
 
 // The same L-BFGS iteration with the two-loop recursion carried out on Gram matrices of the stored pairs (the

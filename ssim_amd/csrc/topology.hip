@@ -283,6 +283,19 @@ void build_topology(ssim_ctx* c) {
           hcount[ub + 1]++;
         }
   for (int u = 0; u < c->nub; ++u) hcount[u + 1] += hcount[u];
+  // the same blocks with each tet's corners in ascending vertex order (the tile-reduced assembly's pair order)
+  {
+    std::vector<int> hub_sorted(10 * (size_t)nt, -1);
+    for (int e = 0; e < nt; ++e) {
+      if (cells[4 * e] < 0) continue;
+      int id[4] = {cells[4 * e], cells[4 * e + 1], cells[4 * e + 2], cells[4 * e + 3]};
+      std::sort(id, id + 4);
+      for (int w = 0; w < 4; ++w)
+        for (int v = 0; v <= w; ++v)
+          hub_sorted[10 * (size_t)e + w * (w + 1) / 2 + v] = ubptr[id[v]] + (slot(id[v], id[w]) - c->kdiag[id[v]]);
+    }
+    build_hess_tiles(c, hub_sorted);
+  }
   std::vector<int> hperm(10 * (size_t)nt, -1);
   {
     std::vector<int> pos(hcount.begin(), hcount.end() - 1);
@@ -327,11 +340,64 @@ void build_topology(ssim_ctx* c) {
   c->val.alloc(c->fixed_layout ? kBellZeroLead + (size_t)ns * kBellSliceVals : 9 * (size_t)offu + 9 * kSlice);
   HIPCHK(hipMemsetAsync(c->val.p, 0, c->val.count * sizeof(double), s));
   c->gcontrib.alloc(12 * (size_t)std::max(nt, 1));
-  c->hcontrib.alloc(90 * (size_t)std::max(nt, 1));
+  c->hcontrib.release();  // per-tet contribution slots: allocated on first use (projective dynamics, hess_tile = 0)
   sync(c);  // host staging vectors go out of scope
 }
 
 }  // namespace
+
+// Tiles of kHessTile consecutive tets for the tile-reduced Hessian: per tile its upper blocks ascending, the tile's
+// (tet, pair) contributions sorted by (upper block, tet, pair) into LDS slots, and the run boundaries of every tile
+// block; per upper block the positions of its tile partials in tile order.  hub[10e + k] is the upper block of pair
+// k = w(w+1)/2 + v of tet e with its corners in ascending vertex order (−1 for a padding tet).
+void ssim_hip::build_hess_tiles(ssim_ctx* c, const std::vector<int>& hub) {
+  const int nt = c->nt, T = kHessTile;
+  const int ntiles = (nt + T - 1) / T;
+  std::vector<int> ht_off(ntiles + 1, 0), ht_run, ht_blk;
+  std::vector<unsigned> ht_slot(5 * (size_t)std::max(nt, 1), 0u);
+  std::vector<std::pair<int, int>> keyed;  // (upper block, contribution index 10·(e − e0) + k)
+  std::vector<int> slot_of(10 * T);
+  for (int t = 0; t < ntiles; ++t) {
+    const int e0 = t * T, e1 = std::min(nt, e0 + T);
+    keyed.clear();
+    for (int e = e0; e < e1; ++e)
+      for (int k = 0; k < 10; ++k)
+        if (hub[10 * (size_t)e + k] >= 0) keyed.emplace_back(hub[10 * (size_t)e + k], 10 * (e - e0) + k);
+    std::sort(keyed.begin(), keyed.end());
+    int nb = 0;
+    for (size_t q = 0; q < keyed.size(); ++q) {
+      if (q == 0 || keyed[q].first != keyed[q - 1].first) {
+        ht_blk.push_back(keyed[q].first);
+        ht_run.push_back((int)q);
+        ++nb;
+      }
+      slot_of[keyed[q].second] = (int)q;
+    }
+    ht_run.push_back((int)keyed.size());  // nb + 1 boundaries
+    ht_off[t + 1] = ht_off[t] + nb;
+    for (int e = e0; e < e1; ++e)
+      for (int k = 0; k < 10; ++k)
+        if (hub[10 * (size_t)e + k] >= 0)
+          ht_slot[(size_t)(k >> 1) * nt + e] |= (unsigned)slot_of[10 * (e - e0) + k] << (16 * (k & 1));
+  }
+  std::vector<int> hb_seg(c->nub + 1, 0), hb_pos(std::max<size_t>(ht_blk.size(), 1), 0);
+  for (int b : ht_blk) hb_seg[b + 1]++;
+  for (int u = 0; u < c->nub; ++u) hb_seg[u + 1] += hb_seg[u];
+  {
+    std::vector<int> pos(hb_seg.begin(), hb_seg.end() - 1);
+    for (size_t k = 0; k < ht_blk.size(); ++k) hb_pos[pos[ht_blk[k]]++] = (int)k;  // k ascends with the tile
+  }
+  hipStream_t s = c->stream;
+  c->ht_slot.upload(ht_slot, s);
+  c->ht_off.upload(ht_off, s);
+  if (ht_run.empty()) ht_run.push_back(0);
+  c->ht_run.upload(ht_run, s);
+  c->hb_seg.upload(hb_seg, s);
+  c->hb_pos.upload(hb_pos, s);
+  c->hparts = (long long)ht_blk.size();
+  c->hpart.alloc(9 * (size_t)std::max<long long>(c->hparts, 1));
+  sync(c);  // host staging vectors go out of scope
+}
 
 // Tiles of grad_tile consecutive tets for the tile-reduced gradient: per tile its vertices ascending, the tile's
 // corners sorted by (vertex, tet, corner) into LDS slots, and the run boundaries of every tile vertex; per vertex
