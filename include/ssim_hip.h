@@ -321,9 +321,12 @@ typedef enum {
   SSIM_TUNE_GRAD_TILE = 6,     /* tets per tile of the tile-reduced energy + gradient: 64, 128 or 256 (default);
                                   the tile maps are rebuilt on change.  Batches keep 256 (their per-mesh tet ranges
                                   are padded to 256-tet workgroups) */
-  SSIM_TUNE_HESS_TILE = 7      /* 1 (default): tile-reduced SPD Hessian assembly (one partial per 256-tet tile and
+  SSIM_TUNE_HESS_TILE = 7,     /* 1 (default): tile-reduced SPD Hessian assembly (one partial per 256-tet tile and
                                   upper block); 0: every tet writes its 10 blocks to contribution slots that a
                                   per-block reduce reads back */
+  SSIM_TUNE_GRAD_REST = 8      /* 1 (default): the energy + gradient kernels form Dm⁻¹ and the rest volume from the
+                                  gathered rest positions (the rest-shape kernel's arithmetic); 0: they load the
+                                  stored ones (80 B per tet) */
 } ssim_tune_t;
 ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value);
 ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value);
