@@ -68,7 +68,9 @@ def spmv_algorithmic_bytes(info):
 def element_bytes(info, uniform):
     """Algorithmic bytes of one energy + gradient evaluation (K1) and one SPD Hessian assembly (K2): per tet the
     rest geometry B (72 B), V (8 B) and cells (16 B), plus (λ, μ) (16 B) when the material varies per tet;
-    K1 adds X, u, û read and g written (4·n·8), K2 X and u read (2·n·8), the nnz values written and the dof flags."""
+    K1 adds X, u, û read and g written (4·n·8), K2 X and u read (2·n·8), the nnz values written and the dof flags.
+    These are the accounting's fixed figures: the energy + gradient kernel forms B and V from the gathered rest
+    positions instead of loading them (SSIM_TUNE_GRAD_REST), so it streams 80 B/tet less than K1 counts."""
     n, nt = 3 * info["nv"], info["nt"]
     nnz = 9 * info["nonzero_blocks"]
     per_tet = 96 if uniform else 112
@@ -90,16 +92,19 @@ def step_accounting(info, counts, elapsed, uniform):
 
 
 def element_rooflines(info, ktimes, uniform):
-    """K1 (k_elem_grad_tile + k_grad_gather) and K2 (k_elem_hess + k_hess_reduce) against their algorithmic bytes,
-    from the same hipEvent kernel timers as the SpMV."""
+    """K1 (k_elem_grad_tile + k_grad_gather) and K2 (k_elem_hess_tile + k_hess_gather, the tile-reduced assembly)
+    against their algorithmic bytes, from the same hipEvent kernel timers as the SpMV (timer slots elem_grad /
+    grad_reduce and elem_hess / hess_reduce)."""
     k1, k2 = element_bytes(info, uniform)
     out = {}
-    for name, b, pair in (("k1_energy_gradient", k1, ("elem_grad", "grad_reduce")),
-                          ("k2_hessian_assembly", k2, ("elem_hess", "hess_reduce"))):
+    for name, b, pair, kernels in (
+            ("k1_energy_gradient", k1, ("elem_grad", "grad_reduce"), ["k_elem_grad_tile", "k_grad_gather"]),
+            ("k2_hessian_assembly", k2, ("elem_hess", "hess_reduce"), ["k_elem_hess_tile", "k_hess_gather"])):
         ms = sum(ktimes[k][0] for k in pair)
         gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else None
         out[name] = {"bytes": b, "avg_ms": ms, "achieved": gbs, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS if gbs else None, "kernels": list(pair)}
+                     "frac": gbs / HBM_PEAK_GBS if gbs else None, "kernels": kernels,
+                     "per_kernel_ms": [ktimes[k][0] for k in pair]}
     return out
 
 

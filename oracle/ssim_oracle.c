@@ -30,11 +30,13 @@
  *   prepare_step,        (tests/test_oracle_stepping.py) free fall from rest: u_n = ½dt²g·n(n+1), v_n = n·dt·g and
  *   step_next, damping   the damped closed form; a clamped, damped linear bar settles at K_ff u_f = M_ff g_f with
  *                        K and M assembled independently (pins the dt² scaling of the elastic energy).
+ *   snh2_delta           (tests/test_oracle_2d.py) the property the reference's quadratic encodes: with the
+ *                        stress-free α, the Hessian at F = 0 has zero curvature along pure shear
+ *                        (μ(1 − 1/δ) + λα = 0, which times δ(2+δ)/λ is the quadratic), checked on the oracle's
+ *                        Hessian and on the ε² decay of P(εS)/ε.
  * Parity unpinned:
  *   - the solver trajectories of mathprim (the reference's external solver library, not vendored): iteration
- *     order, stopping norm and line-search constants follow the benchmark configuration;
- *   - snh2_delta, the 2D stable Neo-Hookean δ: the 2D checks hold for any δ, and the reference states no property
- *     its quadratic encodes.
+ *     order, stopping norm and line-search constants follow the benchmark configuration.
  */
 #include "ssim_oracle.h"
 
