@@ -70,7 +70,8 @@ def element_bytes(info, uniform):
     rest geometry B (72 B), V (8 B) and cells (16 B), plus (λ, μ) (16 B) when the material varies per tet;
     K1 adds X, u, û read and g written (4·n·8), K2 X and u read (2·n·8), the nnz values written and the dof flags.
     These are the accounting's fixed figures: the energy + gradient kernel forms B and V from the gathered rest
-    positions instead of loading them (SSIM_TUNE_GRAD_REST), so it streams 80 B/tet less than K1 counts."""
+    positions instead of loading them (SSIM_TUNE_REST_FROM_X), and so does the Hessian tile kernel: they stream
+    80 B/tet less than K1 and K2 count."""
     n, nt = 3 * info["nv"], info["nt"]
     nnz = 9 * info["nonzero_blocks"]
     per_tet = 96 if uniform else 112

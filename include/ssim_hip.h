@@ -324,9 +324,9 @@ typedef enum {
   SSIM_TUNE_HESS_TILE = 7,     /* 1 (default): tile-reduced SPD Hessian assembly (one partial per 256-tet tile and
                                   upper block); 0: every tet writes its 10 blocks to contribution slots that a
                                   per-block reduce reads back */
-  SSIM_TUNE_GRAD_REST = 8      /* 1 (default): the energy + gradient kernels form Dm⁻¹ and the rest volume from the
-                                  gathered rest positions (the rest-shape kernel's arithmetic); 0: they load the
-                                  stored ones (80 B per tet) */
+  SSIM_TUNE_REST_FROM_X = 8    /* 1 (default): the energy + gradient and tile-reduced Hessian kernels form Dm⁻¹ and
+                                  the rest volume from the gathered rest positions (the rest-shape kernel's
+                                  arithmetic, bitwise the stored values); 0: they load the stored ones (80 B/tet) */
 } ssim_tune_t;
 ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value);
 ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value);

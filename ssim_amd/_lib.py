@@ -152,7 +152,7 @@ SIGNATURES = {
 }
 
 TUNING = {"xcd_remap": 0, "pcg_chunk_max": 1, "spmv_lanes": 2, "pcg_graph": 3, "lbfgs_gram": 5, "grad_tile": 6,
-          "hess_tile": 7, "grad_rest": 8}
+          "hess_tile": 7, "rest_from_x": 8}
 BATCH_TUNING = dict(TUNING, compact_pct=4)
 
 _lib = None

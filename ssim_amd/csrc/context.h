@@ -119,7 +119,7 @@ struct ssim_ctx {
   ssim_hip::DBuf<int> ht_off, ht_run, hb_seg, hb_pos;
   ssim_hip::DBuf<double> hpart;
   long long hparts = 0;
-  int grad_rest_x = 1;  // 1: the energy + gradient kernels recompute B, V from X (no 80 B/tet of rest-shape loads)
+  int rest_from_x = 1;  // 1: the energy + gradient and tile Hessian kernels form B, V from X (no 80 B/tet loaded)
   int hess_tile = 1;  // 1: tile-reduced Hessian assembly; 0: per-tet contribution slots (k_elem_hess + k_hess_reduce)
   ssim_hip::DBuf<int2> ub_rc, lref;
   ssim_hip::DBuf<uint4> idx;  // fixed layout: two 16-byte index words per row
@@ -197,7 +197,7 @@ struct ssim_ctx {
     m.rho = rho_reset;
     m.model = static_cast<int>(model);
     m.xcd_remap = xcd_remap;
-    m.rest_x = grad_rest_x;
+    m.rest_x = rest_from_x;
     return m;
   }
   ssim_hip::BellView bell(const double* values) const {

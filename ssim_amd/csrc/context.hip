@@ -869,8 +869,8 @@ ssim_status_t ssim_set_tuning(ssim_ctx* ctx, ssim_tune_t key, int value) {
       case SSIM_TUNE_HESS_TILE:
         ctx->hess_tile = value != 0;
         break;
-      case SSIM_TUNE_GRAD_REST:
-        ctx->grad_rest_x = value != 0;
+      case SSIM_TUNE_REST_FROM_X:
+        ctx->rest_from_x = value != 0;
         break;
       default:
         fail(SSIM_ERR_INVALID_ARGUMENT, "unknown tuning key");
@@ -905,8 +905,8 @@ ssim_status_t ssim_get_tuning(const ssim_ctx* ctx, ssim_tune_t key, int* value) 
     case SSIM_TUNE_HESS_TILE:
       *value = ctx->hess_tile;
       return SSIM_OK;
-    case SSIM_TUNE_GRAD_REST:
-      *value = ctx->grad_rest_x;
+    case SSIM_TUNE_REST_FROM_X:
+      *value = ctx->rest_from_x;
       return SSIM_OK;
     default:
       g_err = "unknown tuning key";

@@ -477,17 +477,33 @@ __device__ __forceinline__ void elem_hess_tile(const MeshView& m, int tile, bool
   double sc = 0.0, mm = 0.0;
   unsigned sw[5] = {0u, 0u, 0u, 0u, 0u};
   if (active) {
-    const TetLocal t = load_tet(m, e);
+    TetLocal t;
+    {
+      const int4 ci = m.cells[e];
+      t.id[0] = ci.x;
+      t.id[1] = ci.y;
+      t.id[2] = ci.z;
+      t.id[3] = ci.w;
+      t.lam = m.uniform ? m.lam0 : m.lam[e];  // uniform over the launch: no divergence
+      t.mu = m.uniform ? m.mu0 : m.mu[e];
+    }
+    if (!m.rest_x) {  // uniform over the launch
+#pragma unroll
+      for (int k = 0; k < 9; ++k) t.B[k] = m.B[(size_t)k * m.nt + e];
+      t.V = m.V[e];
+    }
 #pragma unroll
     for (int j = 0; j < 5; ++j) sw[j] = hslot[(size_t)j * m.nt + e];
-    double x[4][3];
+    double xr[4][3], x[4][3];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         const size_t k = 3 * (size_t)t.id[i] + a;
-        x[i][a] = m.X[k] + u[k];
+        xr[i][a] = m.X[k];
+        x[i][a] = xr[i][a] + u[k];
       }
+    if (m.rest_x) rest_shape(xr[0], xr[1], xr[2], xr[3], t.B, &t.V);
     double Fd[9];
     def_grad(x, t.B, Fd);
     T F[9], Bt[9];

@@ -28,8 +28,8 @@ struct MeshView {
   double rho;
   int model = 0;      // MaterialModel (elem_math.h)
   int xcd_remap = 0;  // 1: the element kernels give each of the 8 XCDs a contiguous range of tets
-  int rest_x = 0;     // 1: the energy + gradient kernels form B and V from the gathered rest positions (the same
-                      // arithmetic as the rest-shape kernel) instead of loading them
+  int rest_x = 0;     // 1: the energy + gradient and tile Hessian kernels form B and V from the gathered rest
+                      // positions (the same arithmetic as the rest-shape kernel) instead of loading them
 };
 
 // Symmetric blocked ELLPACK of 3x3 blocks.  Rows (vertices) are grouped in slices of 64 lanes.  Two layouts:
