@@ -8,6 +8,11 @@
   python -m ssim_amd.profiling evaluators                 the element evaluators (ssim_tri_eval on ~1.0M
                                                           triangles with every output, ssim_material_eval on 2^20
                                                           items with Hessians), 50 timed launches per case
+  python -m ssim_amd.profiling batch-steps                the 256-bar batch of bench.py: one step, the autotuner,
+                                                          two more steps (per-mesh PCG work of the last one)
+  python -m ssim_amd.profiling timeline <rocprofv3 -d dir> the last time step of a kernel trace (from its last
+                                                          k_prepare): busy time per kernel, launch gaps, and the
+                                                          batched PCG launches by grid size
 """
 from __future__ import annotations
 
@@ -16,6 +21,11 @@ import sqlite3
 import statistics
 import sys
 from collections import defaultdict
+
+if __package__ in (None, ""):  # run as a script: make the package importable
+    import os
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 
 def _dbs(path):
@@ -83,7 +93,7 @@ def evaluators(reps=50):
     import numpy as np
     import torch
 
-    from . import _lib
+    from ssim_amd import _lib
 
     _lib.load()
     dev = torch.device("cuda")
@@ -140,6 +150,66 @@ def evaluators(reps=50):
                       f"{mat_bytes / us / 8e4:10.1f}%", flush=True)
 
 
+def batch_steps():
+    """The batch of bench.py's batched leg (256 bars, dims 4..24): step 1, the autotuner's trials, steps 2 and 3;
+    prints the lock-step counts, the per-mesh PCG iterations of step 3 and the batch's PCG work in whole-batch
+    iterations, Σ_m iterations_m · tets_m / Σ_m tets_m."""
+    import time
+
+    import numpy as np
+    import torch
+
+    from ssim_amd import autotune, batch, mesh
+
+    cases = mesh.bar_batch(256, seed=0)
+    B = batch.TetFiniteElementBatch_Hip.from_cases(cases, rtol=1e-6)
+    B.step()
+    print("tuned", autotune.tune(B, repeats=1, cache=None), flush=True)
+    for k in (2, 3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        B.step()
+        torch.cuda.synchronize()
+        st = B.last_stats()
+        print(f"step {k}: {(time.perf_counter() - t0) * 1e3:.1f} ms, lock-step Newton {st['newton_iterations']}, "
+              f"PCG {st['pcg_iterations']}", flush=True)
+    nts = np.array([c.nt for c in cases])
+    pcg = np.array([B.last_stats(m)["pcg_iterations"] for m in range(len(cases))])
+    print(f"step 3 per-mesh PCG iterations min/median/max {pcg.min()} {np.median(pcg):.0f} {pcg.max()}; "
+          f"PCG work in whole-batch iterations {(pcg * nts).sum() / nts.sum():.1f}", flush=True)
+    slow = np.argsort(-pcg)[:5]
+    print("slowest meshes (tets, PCG iterations):", [(int(nts[i]), int(pcg[i])) for i in slow], flush=True)
+
+
+def timeline(path):
+    for db in _dbs(path):
+        cur = sqlite3.connect(db).cursor()
+        rows = [(_short(n), s, e, g) for n, s, e, g in
+                cur.execute("select name, start, end, grid_x from kernels order by start")]
+        starts = [i for i, r in enumerate(rows) if r[0].endswith("k_prepare")]
+        if not starts:
+            continue
+        seg = rows[starts[-1]:]
+        gaps = sum(max(0, b[1] - a[2]) for a, b in zip(seg, seg[1:])) / 1e3
+        print(f"# {db}: last step {len(seg)} launches over {(seg[-1][2] - seg[0][1]) / 1e3:.1f} us, "
+              f"gaps between launches {gaps:.1f} us")
+        busy = defaultdict(list)
+        for n, s0, e0, _ in seg:
+            busy[n].append((e0 - s0) / 1e3)
+        for n, v in sorted(busy.items(), key=lambda x: -sum(x[1])):
+            print(f"{n:40s} {len(v):6d} {sum(v) / 1e3:9.2f} ms")
+        for kern in ("k_bpcg_spmv", "k_bpcg_update"):
+            by = defaultdict(list)
+            for n, s0, e0, g in seg:
+                if kern in n:
+                    by[g].append((e0 - s0) / 1e3)
+            if not by:
+                continue
+            print(f"{kern} by grid (work-items): launches, mean us, total ms")
+            for g in sorted(by, reverse=True):
+                print(f"  {g:9d} {len(by[g]):5d} {statistics.fmean(by[g]):8.2f} {sum(by[g]) / 1e3:8.2f}")
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1] if len(sys.argv) > 1 else ""
     if cmd == "trace":
@@ -150,6 +220,10 @@ if __name__ == "__main__":
         copy(int(sys.argv[2]) if len(sys.argv) > 2 else 512)
     elif cmd == "evaluators":
         evaluators()
+    elif cmd == "batch-steps":
+        batch_steps()
+    elif cmd == "timeline":
+        timeline(sys.argv[2])
     else:
         print(__doc__)
         sys.exit(2)
