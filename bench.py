@@ -17,7 +17,8 @@ Two more objects in the same line cover the batched and differentiable variants 
 ``batched``: 256 bars of uneven size (dims uniform in 4..24, per-bar E log-uniform in [1e4, 1e6] Pa, ν uniform in
 [0.3, 0.48], x = 0 end clamped, gravity) stepped by one set of launches whose knobs the autotuner picks, against
 stepping the same 256 meshes one after another through the single-mesh path on the GPU, with the per-mesh
-agreement of the two after one step;
+agreement of the two after one step, and the same batch split into concurrent size classes and into sequential
+groups of meshes with similar step-1 PCG iterations (``best`` names the fastest of these over the same steps);
 ``adjoint``: a 20-step rollout of 32 bars of 16×16×64 cubes (per-tet log E uniform in [ln 1e4, ln 1e6], ν = 0.45)
 with the loss Σ‖u − t‖² over the free-end vertices, timed forward and backward (adjoint PCG per step);
 ``lbfgs``: L-BFGS steps of the twisted 64³-cube (1,572,864 tets, per-tet material);
@@ -324,6 +325,7 @@ def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
     for s in singles:
         s.step()
     B1 = [B.get_array("deformation", m) for m in range(len(cases))]  # step-1 state, for the grouped runs' check
+    pcg1 = [B.last_stats(m)["pcg_iterations"] for m in range(len(cases))]  # the grouping key of sequential groups
     worst, mism = 0.0, 0
     for m, (c, s) in enumerate(zip(cases, singles)):
         xb = c.X.reshape(-1) + B.get_array("deformation", m)
@@ -363,11 +365,13 @@ def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
     ts = (time.perf_counter() - t0) / steps
     print(f"[bench] batched {tb:.3f} s/step, sequential {ts:.3f} s/step (same {steps} steps)", file=sys.stderr,
           flush=True)
-    # the same batch split into size classes stepped concurrently on their own streams: step 1 untimed, then the
-    # same steps 2..steps+1 as above; each group count is one candidate of the tuning
-    grouped = {}
-    for ngroups in (2, 4):
-        G = batch.BatchGroups(cases, groups=ngroups, rtol=rtol)
+    # the same batch split into size classes stepped concurrently on their own streams, and into groups of meshes
+    # with similar step-1 PCG iterations stepped one after another: step 1 untimed, then the same steps
+    # 2..steps+1 as above; each grouping is one candidate of the tuning
+    grouped, sequential = {}, {}
+    for ngroups, concurrent in ((2, True), (4, True), (3, False), (4, False)):
+        G = (batch.BatchGroups(cases, groups=ngroups, rtol=rtol) if concurrent else
+             batch.BatchGroups(cases, groups=ngroups, rtol=rtol, key=pcg1, concurrent=False))
         G.set_tuning(**tuned)
         G.step()
         gworst = max(float(np.abs(G.get_array("deformation", m) - B1[m]).max() / max(np.abs(B1[m]).max(), 1e-300))
@@ -378,12 +382,17 @@ def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
             G.step()
         torch.cuda.synchronize()
         tg = (time.perf_counter() - t0) / steps
-        grouped[ngroups] = {"s_per_step": tg, "speedup_vs_sequential": ts / tg, "step1_max_rel_diff_vs_batch": gworst,
-                            "group_sizes": [len(ms) for ms in G.members]}
-        print(f"[bench] batched in {ngroups} concurrent groups: {tg:.3f} s/step ({ts / tg:.1f}x sequential), step-1 "
+        rec = {"s_per_step": tg, "speedup_vs_sequential": ts / tg, "step1_max_rel_diff_vs_batch": gworst,
+               "group_sizes": [len(ms) for ms in G.members],
+               "group_tets": [int(sum(cases[m].nt for m in ms)) for ms in G.members]}
+        (grouped if concurrent else sequential)[ngroups] = rec
+        kind = "concurrent size classes" if concurrent else "sequential groups by step-1 PCG iterations"
+        print(f"[bench] batched in {ngroups} {kind}: {tg:.3f} s/step ({ts / tg:.1f}x sequential), step-1 "
               f"difference to the single batch {gworst:.1e}", file=sys.stderr, flush=True)
         del G
-    best_g = min(grouped, key=lambda k: grouped[k]["s_per_step"])
+    cands = [("lock-step", 1, tb)] + [("concurrent size classes", g, r["s_per_step"]) for g, r in grouped.items()] + \
+        [("sequential groups by step-1 PCG iterations", g, r["s_per_step"]) for g, r in sequential.items()]
+    best = min(cands, key=lambda c: c[2])
     return {
         "workload": "256 bars, dims uniform in 4..24, per-bar E log-uniform [1e4,1e6], nu uniform [0.3,0.48]",
         "tuning": tuned,
@@ -396,9 +405,9 @@ def bench_batched(rtol, steps=3, seed=0, cpu_threads=None):
         "sequential_steps_per_s": 1.0 / ts,
         "speedup_vs_sequential": ts / tb,
         "concurrent_groups": grouped,
-        "best": {"groups": best_g if grouped[best_g]["s_per_step"] < tb else 1,
-                 "batched_steps_per_s": 1.0 / min(tb, grouped[best_g]["s_per_step"]),
-                 "speedup_vs_sequential": ts / min(tb, grouped[best_g]["s_per_step"])},
+        "sequential_groups_by_pcg": sequential,
+        "best": {"grouping": best[0], "groups": best[1], "batched_steps_per_s": 1.0 / best[2],
+                 "speedup_vs_sequential": ts / best[2]},
         "timed_steps": steps,
         "timed_window": f"steps 2-{steps + 1} of every mesh, batched and sequential",
         "lockstep_newton_iterations": tot["newton_iterations"],

@@ -72,3 +72,13 @@ def test_size_groups_partition_by_tet_count():
         for a, b in zip(parts, parts[1:]):
             assert max(nts[m] for m in a) <= min(nts[m] for m in b)
     assert size_groups(nts, 1) == [list(range(len(nts)))]
+    # a sort key (e.g. each mesh's PCG iterations of an earlier step): classes follow the key, cut at about equal
+    # total tet count
+    key = [40, 900, 35, 300, 310, 20, 1200, 500]
+    for g in (2, 3, 4):
+        parts = size_groups(nts, g, key=key)
+        assert sorted(m for p in parts for m in p) == list(range(len(nts)))
+        for a, b in zip(parts, parts[1:]):
+            assert max(key[m] for m in a) <= min(key[m] for m in b)
+    with pytest.raises(ValueError):
+        size_groups(nts, 2, key=[1, 2])

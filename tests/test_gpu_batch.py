@@ -111,15 +111,24 @@ def test_graph_replay_and_compaction_give_identical_results():
         assert np.array_equal(r, runs[0])
 
 
-def test_concurrent_groups_match_the_single_batch():
-    """BatchGroups steps size classes of the batch concurrently on their own streams from their own host threads.
-    Every mesh takes the same Newton iterations as in the single batch and lands within 1e-8 of its positions (the
-    bar of the batch tests): the blocked-ELL slot order of a row follows the column offsets that most rows of the
-    whole batch share, so a row's products are summed in an order that depends on the batch's other meshes, and the
-    PCG solutions differ by rounding amplified up to its tolerance."""
+@pytest.mark.parametrize("concurrent", [True, False])
+def test_concurrent_groups_match_the_single_batch(concurrent):
+    """BatchGroups steps size classes of the batch concurrently on their own streams from their own host threads,
+    or groups keyed by each mesh's PCG iterations of a first step one after another.  Every mesh takes the same
+    Newton iterations as in the single batch and lands within 1e-8 of its positions (the bar of the batch tests):
+    the blocked-ELL slot order of a row follows the column offsets that most rows of the whole batch share, so a
+    row's products are summed in an order that depends on the batch's other meshes, and the PCG solutions differ by
+    rounding amplified up to its tolerance."""
     cases = mesh.bar_batch(24, seed=5, lo=3, hi=10)
     B = batch.TetFiniteElementBatch_Hip.from_cases(cases, rtol=1e-4)
-    G = batch.BatchGroups(cases, groups=3, rtol=1e-4)
+    if concurrent:
+        G = batch.BatchGroups(cases, groups=3, rtol=1e-4)
+    else:
+        P = batch.TetFiniteElementBatch_Hip.from_cases(cases, rtol=1e-4)
+        P.step()
+        key = [P.last_stats(m)["pcg_iterations"] for m in range(len(cases))]
+        G = batch.BatchGroups(cases, groups=3, rtol=1e-4, key=key, concurrent=False)
+        assert not hasattr(G, "_streams")
     assert sorted(m for ms in G.members for m in ms) == list(range(len(cases)))
     for step in range(3):
         B.step()
