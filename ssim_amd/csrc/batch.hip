@@ -288,7 +288,7 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
   for (;;) {
     enqueue(next, slot ^ 1);
     launched += next;
-    HIPCHK(hipEventSynchronize(c->poll_ev[slot]));
+    wait_event(c->poll_ev[slot]);
     if (c->timing) harvest_timers(c);
     const PcgScalars* f = b->pin_pcg(slot);
     int left_slices = 0;

@@ -239,6 +239,7 @@ struct ssim_ctx {
 namespace ssim_hip {
 // helpers defined in context.hip, topology.hip and solvers.hip, used across them and by batch.hip
 void sync(ssim_ctx* c);
+void wait_event(hipEvent_t ev);  // bounded polling, then a blocking wait
 void default_params(ssim_solver_params& p);
 // marks (mark) or releases vertices as Dirichlet; targets (3 per vertex) may be null for zero targets
 void edit_dirichlet(ssim_ctx* c, const int* verts, int count, const double* targets, bool mark);

@@ -204,7 +204,7 @@ int pcg_solve(ssim_ctx* c, const double* b, double sign, double* x, double rtol,
   for (;;) {
     enqueue(next, slot ^ 1);
     launched += next;
-    HIPCHK(hipEventSynchronize(c->poll_ev[slot]));
+    wait_event(c->poll_ev[slot]);
     if (c->timing) harvest_timers(c);
     const Poll& f = polls[slot];
     if (f.done) {
