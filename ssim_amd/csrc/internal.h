@@ -202,11 +202,13 @@ struct LbfgsCoef {
   double a[kLbfgsSlots], c[kLbfgsSlots];
   unsigned mask;
 };
-void launch_lbfgs_gram(int n, const double* S, const double* Y, const double* g, const double* h, int k, double* part,
-                       double* out, hipStream_t s);
+// k ≥ 0: the pair of slot k is formed from (u − u_old, g − g_old) and stored there by the same pass
+void launch_lbfgs_gram(int n, double* S, double* Y, const double* g, const double* h, int k, const double* u,
+                       const double* u_old, const double* g_old, double* part, double* out, hipStream_t s);
 int lbfgs_gram_blocks(int n);  // partial rows (of kLbfgsDots doubles) that launch_lbfgs_gram writes
+// also writes the first trial point ut = u + alpha0·r
 void launch_lbfgs_dir(int n, const double* S, const double* Y, const double* g, const double* h, const LbfgsCoef& cf,
-                      double* r, hipStream_t s);
+                      double* r, const double* u, double alpha0, double* ut, hipStream_t s);
 void launch_lbfgs_shift(int n, const double* ut, const double* gt, double* u, double* g, double* sv, double* yv,
                         hipStream_t s);
 void launch_elem_pd(const MeshView& m, double c, const int* hperm, double* hcontrib, hipStream_t s);

@@ -1067,6 +1067,9 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
 // Partials of the 6m + 1 dot products one iteration needs (m = kLbfgsSlots, every slot, used or not):
 //   [0, m)   s_jᵀg        [m, 2m)  y_jᵀDg       [2m]  gᵀDg
 //   [2m+1+j] s_jᵀy_k      [3m+1+j] s_kᵀy_j      [4m+1+j] y_kᵀDy_j      (k = the newest slot; zero when k < 0)
+// The newest pair is formed here, s_k = u − u_old and y_k = g − g_old (the accepted iterate and the one before it),
+// and stored to slot k by the wave role that reads that slot's other vectors (role 0 stores S_k, role 1 Y_k); the
+// other slots are read.  The pair's values are those a separate pass would store, so the sums are unchanged.
 // Two wave roles split the dot products by the stored vectors they read, so that each S_j and Y_j is loaded by
 // one wave only: role 0 reads S_j, g and y_k (s_jᵀg, s_jᵀy_k: 2m accumulators), role 1 reads Y_j, g, H0, s_k and
 // y_k (y_jᵀDg, gᵀDg, s_kᵀy_j, y_kᵀDy_j: 3m + 1 accumulators).  Waves 0 and 1 of a workgroup take one set of
@@ -1076,9 +1079,11 @@ void launch_axpby(int n, double a, const double* x, double b, double* y, hipStre
 constexpr int kGramMaxBlocks = 768;  // 3 workgroups per CU at 130 VGPRs: all resident
 static int gram_grid(int n) { return std::max(1, std::min(kGramMaxBlocks, (n + 255) / 256)); }
 
-__global__ __launch_bounds__(kBlock) void k_lbfgs_gram(int n, const double* __restrict__ S, const double* __restrict__ Y,
+__global__ __launch_bounds__(kBlock) void k_lbfgs_gram(int n, double* __restrict__ S, double* __restrict__ Y,
                                                        const double* __restrict__ g, const double* __restrict__ h,
-                                                       int k, double* __restrict__ part) {
+                                                       int k, const double* __restrict__ u,
+                                                       const double* __restrict__ u_old,
+                                                       const double* __restrict__ g_old, double* __restrict__ part) {
   constexpr int M = kLbfgsSlots;
   static_assert(kBlock == 256, "two element sets of two waves per workgroup");
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1091,20 +1096,22 @@ __global__ __launch_bounds__(kBlock) void k_lbfgs_gram(int n, const double* __re
   const bool pair = k >= 0;
   auto body = [&](int i) {
     const double gi = g[i];
+    const double sk = pair ? u[i] - u_old[i] : 0.0, yk = pair ? gi - g_old[i] : 0.0;
     if (role == 0) {
-      const double yk = pair ? Y[(size_t)k * n + i] : 0.0;
+      if (pair) S[(size_t)k * n + i] = sk;
 #pragma unroll
       for (int j = 0; j < M; ++j) {
-        const double sj = S[(size_t)j * n + i];
+        const double sj = pair && j == k ? sk : S[(size_t)j * n + i];
         acc[j] += sj * gi;
         acc[M + j] += sj * yk;
       }
     } else {
+      if (pair) Y[(size_t)k * n + i] = yk;
       const double di = 1.0 / h[i], dg = gi * di;
-      const double sk = pair ? S[(size_t)k * n + i] : 0.0, dyk = pair ? Y[(size_t)k * n + i] * di : 0.0;
+      const double dyk = yk * di;
 #pragma unroll
       for (int j = 0; j < M; ++j) {
-        const double yj = Y[(size_t)j * n + i];
+        const double yj = pair && j == k ? yk : Y[(size_t)j * n + i];
         acc[j] += yj * dg;
         acc[M + j] += sk * yj;
         acc[2 * M + j] += dyk * yj;
@@ -1148,9 +1155,9 @@ __global__ __launch_bounds__(kBlock) void k_lbfgs_gram_fin(const double* __restr
   if (threadIdx.x == 0) out[d] = t[0];
 }
 
-void launch_lbfgs_gram(int n, const double* S, const double* Y, const double* g, const double* h, int k, double* part,
-                       double* out, hipStream_t s) {
-  k_lbfgs_gram<<<gram_grid(n), kBlock, 0, s>>>(n, S, Y, g, h, k, part);
+void launch_lbfgs_gram(int n, double* S, double* Y, const double* g, const double* h, int k, const double* u,
+                       const double* u_old, const double* g_old, double* part, double* out, hipStream_t s) {
+  k_lbfgs_gram<<<gram_grid(n), kBlock, 0, s>>>(n, S, Y, g, h, k, u, u_old, g_old, part);
   k_lbfgs_gram_fin<<<kLbfgsDots, kBlock, 0, s>>>(part, lbfgs_gram_blocks(n), out);
 }
 
@@ -1177,10 +1184,13 @@ void launch_lbfgs_shift(int n, const double* ut, const double* gt, double* u, do
   k_lbfgs_shift<<<nblocks(n), kBlock, 0, s>>>(n, ut, gt, u, g, sv, yv);
 }
 
-// r = D·(g − Σ_j a_j y_j) + Σ_j c_j s_j over the slots in `mask`
+// r = D·(g − Σ_j a_j y_j) + Σ_j c_j s_j over the slots in `mask`, and the line search's first trial point
+// u_t = u + α₀·r (the expression of k_axpy_to, so the same bits as that pass would store)
 __global__ __launch_bounds__(kBlock) void k_lbfgs_dir(int n, const double* __restrict__ S, const double* __restrict__ Y,
                                                       const double* __restrict__ g, const double* __restrict__ h,
-                                                      LbfgsCoef cf, double* __restrict__ r) {
+                                                      LbfgsCoef cf, double* __restrict__ r,
+                                                      const double* __restrict__ u, double alpha0,
+                                                      double* __restrict__ ut) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   double q = g[i], t = 0.0;
@@ -1190,13 +1200,15 @@ __global__ __launch_bounds__(kBlock) void k_lbfgs_dir(int n, const double* __res
       q -= cf.a[j] * Y[(size_t)j * n + i];
       t += cf.c[j] * S[(size_t)j * n + i];
     }
-  r[i] = q / h[i] + t;
+  const double ri = q / h[i] + t;
+  r[i] = ri;
+  ut[i] = u[i] + alpha0 * ri;
 }
 
 void launch_lbfgs_dir(int n, const double* S, const double* Y, const double* g, const double* h, const LbfgsCoef& cf,
-                      double* r, hipStream_t s) {
+                      double* r, const double* u, double alpha0, double* ut, hipStream_t s) {
   if (n <= 0) return;
-  k_lbfgs_dir<<<nblocks(n), kBlock, 0, s>>>(n, S, Y, g, h, cf, r);
+  k_lbfgs_dir<<<nblocks(n), kBlock, 0, s>>>(n, S, Y, g, h, cf, r, u, alpha0, ut);
 }
 
 // ------------------------------------------------------------------------------------------------------

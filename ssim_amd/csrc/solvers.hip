@@ -348,11 +348,15 @@ static void lbfgs_solve_gram(ssim_ctx* c) {
   double* bv = av + M;
   bool vector_mode = false;
   std::vector<double> SY((size_t)M * M, 0.0), YDY((size_t)M * M, 0.0), rho(M, 0.0), gram(kLbfgsDots);
-  double* u = c->a(SSIM_ARRAY_NEXT_DEFORM);
-  double* g = c->a(SSIM_ARRAY_FORCES);
+  // The iterate and its gradient alternate between the output arrays and the trial buffers: an accepted trial
+  // becomes the iterate by exchanging the pointers, and the previous iterate stays in the trial buffers until the
+  // next Gram pass has formed the new pair from both.  The result is moved to the output arrays at the end.
+  double* const u_out = c->a(SSIM_ARRAY_NEXT_DEFORM);
+  double* const g_out = c->a(SSIM_ARRAY_FORCES);
+  double *u = u_out, *g = g_out, *ut = c->utry.p, *gt = c->gtry.p;
   double* d = c->dir.p;  // r ≈ H⁻¹g; the search direction is −r
   EvalOut cur = eval_energy_grad(c, u, g);
-  int it = 0, hist = 0, head = 0, fresh = -1;  // fresh: slot of the pair written by the last line search
+  int it = 0, hist = 0, head = 0, fresh = -1;  // fresh: slot of the pair of the last accepted step
   auto accept = [&](double ys) {
     if (!(ys > 1e-300)) return false;  // as lbfgs_solve: a pair with yᵀs ≤ 0 is dropped
     rho[fresh] = 1.0 / ys;
@@ -386,7 +390,7 @@ static void lbfgs_solve_gram(ssim_ctx* c) {
       HIPCHK(hipGetLastError());
       gd = -device_dot(c, g, d);
     } else {
-      launch_lbfgs_gram(n, c->lb_S.p, c->lb_Y.p, g, c->hrest.p, fresh, c->lb_part.p, c->lb_scal.p, s);
+      launch_lbfgs_gram(n, c->lb_S.p, c->lb_Y.p, g, c->hrest.p, fresh, u, ut, gt, c->lb_part.p, c->lb_scal.p, s);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(gram.data(), c->lb_scal.p, sizeof(double) * kLbfgsDots, hipMemcpyDeviceToHost, s));
       sync(c);
@@ -438,7 +442,7 @@ static void lbfgs_solve_gram(ssim_ctx* c) {
         cf.mask |= 1u << i;
         gr += -a[i] * YDG[i] + (a[i] - b[i]) * SG[i];
       }
-      launch_lbfgs_dir(n, c->lb_S.p, c->lb_Y.p, g, c->hrest.p, cf, d, s);
+      launch_lbfgs_dir(n, c->lb_S.p, c->lb_Y.p, g, c->hrest.p, cf, d, u, -1.0, ut, s);  // and ut = u − d
       HIPCHK(hipGetLastError());
       gd = -gr;
     }
@@ -446,8 +450,8 @@ static void lbfgs_solve_gram(ssim_ctx* c) {
     EvalOut trial{0, 0};
     bool ok = false;
     for (int ls = 0; ls < c->prm.lbfgs_ls_max; ++ls) {
-      launch_axpy_to(n, u, -alpha, d, c->utry.p, s);
-      trial = eval_energy_grad(c, c->utry.p, c->gtry.p);
+      if (ls > 0 || vector_mode) launch_axpy_to(n, u, -alpha, d, ut, s);
+      trial = eval_energy_grad(c, ut, gt);
       if (trial.E <= cur.E + c->prm.lbfgs_c * alpha * gd) {
         ok = true;
         break;
@@ -458,11 +462,20 @@ static void lbfgs_solve_gram(ssim_ctx* c) {
       c->stats.ls_failures++;
       vector_mode = true;
     }
-    launch_lbfgs_shift(n, c->utry.p, c->gtry.p, u, g, c->lb_S.p + (size_t)head * n, c->lb_Y.p + (size_t)head * n, s);
-    HIPCHK(hipGetLastError());
+    if (vector_mode) {  // the vector recursion reads the pair from its slot: store it and move the iterate
+      launch_lbfgs_shift(n, ut, gt, u, g, c->lb_S.p + (size_t)head * n, c->lb_Y.p + (size_t)head * n, s);
+      HIPCHK(hipGetLastError());
+    } else {  // the next Gram pass forms and stores the pair
+      std::swap(u, ut);
+      std::swap(g, gt);
+    }
     fresh = head;
     c->stats.last_change = std::fabs(cur.E - trial.E);
     cur = trial;
+  }
+  if (u != u_out) {
+    HIPCHK(hipMemcpyAsync(u_out, u, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(g_out, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   }
   c->stats.newton_iterations = it;
 }
