@@ -283,7 +283,9 @@ typedef enum {
 } ssim_kernel_t;
 
 /* Enable (1) or disable (0) sampled kernel timing and clear its accumulators.  When enabled, one launch of each
- * kernel per call site and per PCG chunk is bracketed by two hipEvents on the context's stream. */
+ * kernel per call site and per PCG chunk is timed by two hipEvents on the context's stream; for the single-mesh
+ * element kernels (tile-reduced assemblies) and the PCG kernels the runtime stamps them at the kernel's own start
+ * and end, for the others they bracket the launch. */
 ssim_status_t ssim_set_kernel_timing(ssim_ctx* ctx, int enabled);
 /* average duration in milliseconds and number of sampled launches of one kernel */
 ssim_status_t ssim_get_kernel_timing(const ssim_ctx* ctx, ssim_kernel_t kernel, double* avg_ms, long long* samples);

@@ -118,14 +118,15 @@ struct PcgIter {
 // ---- launchers (kernels.hip) ----
 void launch_rest_shape(const double* X, const int4* cells, int nt, double* B, double* V, hipStream_t s);
 // Reductions: a kernel stores one partial per workgroup; launch_* return the number of partials written.
+// Launchers that take an event pair (e0, e1) have the runtime stamp it at the kernel's own start and end.
 // energy + gradient at deformation u (predictor uhat), reduced per tile of kBlock tets in LDS: one partial per
 // (tile, vertex of the tile); energy partials to part_e
 int launch_elem_grad_tile(const MeshView& m, int tile, const double* u, const double* uhat, double dt2,
                           const ushort4* tslot, const int* gt_off, const int* gt_run, double* gpart, double* part_e,
-                          bool fp32, hipStream_t s);
+                          bool fp32, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // g[v] = Σ of v's tile partials (in tile order), zeroed on Dirichlet dofs; ‖g‖² partials to part_g
 int launch_grad_gather(int nv, const int* gv_seg, const int* gv_pos, const double* gpart, const unsigned char* dof,
-                       double* g, double* part_g, hipStream_t s);
+                       double* g, double* part_g, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // g[v] = Σ per-corner contributions of v (vertex-sorted slots gperm, runs gseg; the adjoint's mass term), zeroed
 // on Dirichlet dofs; ‖g‖² partials to part_g
 int launch_grad_reduce(int nv, const int* gseg, const double* gcontrib, const unsigned char* dof, double* g,
@@ -143,10 +144,11 @@ void launch_hess_reduce(int nub, const int* hseg, const double* hcontrib, const 
 // filter, the blocked-ELL store and the Jacobi inverse diagonal (and the diagonal when diag is not null)
 void launch_elem_hess_tile(const MeshView& m, const double* u, double dt2, int make_spsd, const unsigned* hslot,
                            const int* ht_off, const int* ht_run, double* hpart, long long P, bool fp32,
-                           hipStream_t s);
+                           hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_hess_gather(int nub, const int* hb_seg, const int* hb_pos, const double* hpart, long long P,
                         const int2* ub_rc, const int* ub_pos, const unsigned char* dof, int filter_dbc,
-                        double* bell_val, double* dinv, double* diag, hipStream_t s);
+                        double* bell_val, double* dinv, double* diag, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
 void launch_elem_eval(const MeshView& m, const double* u, int make_spsd, double* psi, double* P, double* H,
                       double* K, bool fp32, hipStream_t s);
 

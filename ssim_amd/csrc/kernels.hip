@@ -21,6 +21,16 @@ __host__ __device__ static inline int nblocks(long long n) { return (int)((n + k
 // ------------------------------------------------------------------------------------------------------
 // rest shape
 
+// With an event pair the launch goes through hipExtLaunchKernelGGL, which stamps the events at the kernel's own
+// start and end (the interval a profiler reports) instead of around the stream commands.
+template <typename K, typename... Args>
+void launch_timed(K kernel, int grid, int block, hipStream_t s, hipEvent_t e0, hipEvent_t e1, Args... args) {
+  if (e0)
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, e0, e1, 0, args...);
+  else
+    kernel<<<grid, block, 0, s>>>(args...);
+}
+
 __global__ __launch_bounds__(kBlock) void k_rest_shape(const double* __restrict__ X, const int4* __restrict__ cells,
                                                        int nt, double* __restrict__ B, double* __restrict__ V) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
@@ -80,21 +90,21 @@ __global__ __launch_bounds__(TILE) void k_elem_grad_tile(MeshView m, const doubl
 
 int launch_elem_grad_tile(const MeshView& m, int tile, const double* u, const double* uhat, double dt2,
                           const ushort4* tslot, const int* gt_off, const int* gt_run, double* gpart, double* part_e,
-                          bool fp32, hipStream_t s) {
+                          bool fp32, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const int nt = m.nt > 0 ? m.nt : 1;
   const int nb = (nt + tile - 1) / tile;
   dispatch_elem(m.model, fp32, [&](auto k) {
     using K = decltype(k);
     const int grid = grid_for(nb, m.xcd_remap);
     if (tile == 64)
-      k_elem_grad_tile<typename K::type, K::model, 64>
-          <<<grid, 64, 0, s>>>(m, u, uhat, dt2, tslot, gt_off, gt_run, gpart, part_e);
+      launch_timed(k_elem_grad_tile<typename K::type, K::model, 64>, grid, 64, s, e0, e1, m, u, uhat, dt2, tslot,
+                   gt_off, gt_run, gpart, part_e);
     else if (tile == 128)
-      k_elem_grad_tile<typename K::type, K::model, 128>
-          <<<grid, 128, 0, s>>>(m, u, uhat, dt2, tslot, gt_off, gt_run, gpart, part_e);
+      launch_timed(k_elem_grad_tile<typename K::type, K::model, 128>, grid, 128, s, e0, e1, m, u, uhat, dt2, tslot,
+                   gt_off, gt_run, gpart, part_e);
     else
-      k_elem_grad_tile<typename K::type, K::model, kBlock>
-          <<<grid, kBlock, 0, s>>>(m, u, uhat, dt2, tslot, gt_off, gt_run, gpart, part_e);
+      launch_timed(k_elem_grad_tile<typename K::type, K::model, kBlock>, grid, kBlock, s, e0, e1, m, u, uhat, dt2,
+                   tslot, gt_off, gt_run, gpart, part_e);
   });
   return nb;
 }
@@ -112,9 +122,9 @@ __global__ __launch_bounds__(kBlock) void k_grad_gather(int nv, const int* __res
 }
 
 int launch_grad_gather(int nv, const int* gv_seg, const int* gv_pos, const double* gpart, const unsigned char* dof,
-                       double* g, double* part_g, hipStream_t s) {
+                       double* g, double* part_g, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const int nb = nblocks(nv > 0 ? nv : 1);
-  k_grad_gather<<<nb, kBlock, 0, s>>>(nv, gv_seg, gv_pos, gpart, dof, g, part_g);
+  launch_timed(k_grad_gather, nb, kBlock, s, e0, e1, nv, gv_seg, gv_pos, gpart, dof, g, part_g);
   return nb;
 }
 
@@ -258,13 +268,13 @@ __global__ __launch_bounds__(kHessTile) void k_elem_hess_tile(MeshView m, const 
 
 void launch_elem_hess_tile(const MeshView& m, const double* u, double dt2, int make_spsd, const unsigned* hslot,
                            const int* ht_off, const int* ht_run, double* hpart, long long P, bool fp32,
-                           hipStream_t s) {
+                           hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   if (m.nt <= 0) return;
   const int nb = (m.nt + kHessTile - 1) / kHessTile;
   dispatch_elem(m.model, fp32, [&](auto k) {
     using K = decltype(k);
-    k_elem_hess_tile<typename K::type, K::model>
-        <<<grid_for(nb, m.xcd_remap), kHessTile, 0, s>>>(m, u, dt2, make_spsd, hslot, ht_off, ht_run, hpart, P);
+    launch_timed(k_elem_hess_tile<typename K::type, K::model>, grid_for(nb, m.xcd_remap), kHessTile, s, e0, e1, m, u,
+                 dt2, make_spsd, hslot, ht_off, ht_run, hpart, P);
   });
 }
 
@@ -286,10 +296,11 @@ __global__ __launch_bounds__(kBlock) void k_hess_gather(int nub, const int* __re
 
 void launch_hess_gather(int nub, const int* hb_seg, const int* hb_pos, const double* hpart, long long P,
                         const int2* ub_rc, const int* ub_pos, const unsigned char* dof, int filter_dbc,
-                        double* bell_val, double* dinv, double* diag, hipStream_t s) {
+                        double* bell_val, double* dinv, double* diag, hipStream_t s, hipEvent_t e0,
+                        hipEvent_t e1) {
   if (nub <= 0) return;
-  k_hess_gather<<<nblocks(nub), kBlock, 0, s>>>(nub, hb_seg, hb_pos, hpart, P, ub_rc, ub_pos, dof, filter_dbc,
-                                                bell_val, dinv, diag);
+  launch_timed(k_hess_gather, nblocks(nub), kBlock, s, e0, e1, nub, hb_seg, hb_pos, hpart, P, ub_rc, ub_pos, dof,
+               filter_dbc, bell_val, dinv, diag);
 }
 
 // per-element outputs for parity checks: Ψ, P, H (full 9x9, projected if make_spsd), K (full 12x12, unscaled)
@@ -925,16 +936,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const double* __re
     }
   }
   store_partials<2>(v, pb_next);
-}
-
-// With an event pair the launch goes through hipExtLaunchKernelGGL, which stamps the events at the kernel's own
-// start and end (the interval a profiler reports) instead of around the stream commands.
-template <typename K, typename... Args>
-void launch_timed(K kernel, int grid, int block, hipStream_t s, hipEvent_t e0, hipEvent_t e1, Args... args) {
-  if (e0)
-    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, e0, e1, 0, args...);
-  else
-    kernel<<<grid, block, 0, s>>>(args...);
 }
 
 int launch_pcg_spmv(const BellView& A, const double* z, double* p, double* q, const double* pb0, const double* pb1,

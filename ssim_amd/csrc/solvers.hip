@@ -23,12 +23,12 @@ EvalOut eval_energy_grad(ssim_ctx* c, const double* u, double* g) {
   hipStream_t s = c->stream;
   const double dt2 = c->dt * c->dt;
   int ne = 0, ng = 0;
-  timed(c, SSIM_KERNEL_ELEM_GRAD, [&] {
+  timed_ext(c, SSIM_KERNEL_ELEM_GRAD, [&](hipEvent_t e0, hipEvent_t e1) {
     ne = launch_elem_grad_tile(c->mesh(), c->grad_tile, u, c->a(SSIM_ARRAY_INERTIA), dt2, c->gt_slot.p, c->gt_off.p,
-                               c->gt_run.p, c->gpart.p, c->part_e(), c->prm.fp32_elements != 0, s);
+                               c->gt_run.p, c->gpart.p, c->part_e(), c->prm.fp32_elements != 0, s, e0, e1);
   });
-  timed(c, SSIM_KERNEL_GRAD_REDUCE, [&] {
-    ng = launch_grad_gather(c->nv, c->gv_seg.p, c->gv_pos.p, c->gpart.p, c->dof.p, g, c->part_g(), s);
+  timed_ext(c, SSIM_KERNEL_GRAD_REDUCE, [&](hipEvent_t e0, hipEvent_t e1) {
+    ng = launch_grad_gather(c->nv, c->gv_seg.p, c->gv_pos.p, c->gpart.p, c->dof.p, g, c->part_g(), s, e0, e1);
   });
   launch_finalize(c->part_e(), ne, 1, c->part_g(), ng, &c->ev.p->energy, s);  // energy, gg
   HIPCHK(hipGetLastError());
@@ -58,13 +58,13 @@ void assemble_into(ssim_ctx* c, const double* u, double dt2, int make_spsd, int 
                    double* dinv, double* diag, bool fp32) {
   hipStream_t s = c->stream;
   if (c->hess_tile) {
-    timed(c, SSIM_KERNEL_ELEM_HESS, [&] {
+    timed_ext(c, SSIM_KERNEL_ELEM_HESS, [&](hipEvent_t e0, hipEvent_t e1) {
       launch_elem_hess_tile(c->mesh(), u, dt2, make_spsd, c->ht_slot.p, c->ht_off.p, c->ht_run.p, c->hpart.p,
-                            c->hparts, fp32, s);
+                            c->hparts, fp32, s, e0, e1);
     });
-    timed(c, SSIM_KERNEL_HESS_REDUCE, [&] {
+    timed_ext(c, SSIM_KERNEL_HESS_REDUCE, [&](hipEvent_t e0, hipEvent_t e1) {
       launch_hess_gather(c->nub, c->hb_seg.p, c->hb_pos.p, c->hpart.p, c->hparts, c->ub_rc.p, c->ub_pos.p, c->dof.p,
-                         filter_dbc, values, dinv, diag, s);
+                         filter_dbc, values, dinv, diag, s, e0, e1);
     });
   } else {
     ensure_hcontrib(c);
