@@ -34,7 +34,8 @@ SSIM_HD void cofactor(const T F[9], T C[9]) {
   C[8] = F[0] * F[4] - F[1] * F[3];
 }
 
-// B = Dm⁻¹ (row-major) and V = |det Dm|/6: def_grad.hpp:229-247, rest_vol.hpp:142-162
+// B = Dm⁻¹ (row-major) and V = |det Dm|/6: def_grad.hpp:229-247 (Eigen's 3x3 inverse, which scales the
+// cofactors by one reciprocal of the determinant, as here), rest_vol.hpp:142-162
 SSIM_HD void rest_shape(const double x0[3], const double x1[3], const double x2[3], const double x3[3],
                         double B[9], double* V) {
   double Dm[9];
@@ -46,8 +47,9 @@ SSIM_HD void rest_shape(const double x0[3], const double x1[3], const double x2[
   double C[9];
   cofactor(Dm, C);  // adj(Dm) = Cᵀ
   const double d = det3(Dm);
+  const double inv = 1.0 / d;
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) B[i * 3 + j] = C[j * 3 + i] / d;
+    for (int j = 0; j < 3; ++j) B[i * 3 + j] = C[j * 3 + i] * inv;
   *V = fabs(d / 6);
 }
 
