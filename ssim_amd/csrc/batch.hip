@@ -38,8 +38,8 @@ struct ssim_batch {
   size_t pin_bytes = 0;
   // compacted PCG grids: slice and vertex-workgroup lists of the meshes still iterating, double-buffered so that
   // a list is rebuilt while the chunk enqueued ahead of the poll still reads the other one
-  DBuf<int> slist[2], wlist[2];
-  int* pin_list[2] = {nullptr, nullptr};  // pinned staging: slices, then vertex workgroups
+  DBuf<int4> slist[2], wlist[2];  // {slice or vertex workgroup, mesh, mesh's first vertex workgroup, count}
+  int4* pin_list[2] = {nullptr, nullptr};  // pinned staging: slices, then vertex workgroups
   int ns_list[2] = {0, 0}, nw_list[2] = {0, 0};
   int compact_pct = 70;  // rebuild when the active slices fall to this percentage of the current list
   // captured chunks of batched PCG iterations, keyed by everything their launches embed
@@ -59,7 +59,7 @@ struct ssim_batch {
   ~ssim_batch() {
     drop_graphs();
     if (pin) (void)hipHostFree(pin);
-    for (int* q : pin_list)
+    for (int4* q : pin_list)
       if (q) (void)hipHostFree(q);
   }
   BatchView view(const DBuf<int>& mask) const {
@@ -181,18 +181,22 @@ void bassemble(ssim_batch* b, const DBuf<int>& mask, const double* u, int make_s
 // only reused after a later poll).
 void build_lists(ssim_batch* b, const std::vector<int>& active, int buf) {
   ssim_ctx* c = &b->c;
-  int* st = b->pin_list[buf];
+  int4* st = b->pin_list[buf];
   int ns = 0, nw = 0;
   for (int m = 0; m < b->nmesh; ++m)
-    if (active[m])
-      for (int w = b->hvwg_off[m]; w < b->hvwg_off[m + 1]; ++w)
-        for (int q = 0; q < kBatchSlices; ++q) st[ns++] = kBatchSlices * w + q;
-  int* wst = st + c->nslices;
+    if (active[m]) {
+      const int w0 = b->hvwg_off[m], nwm = b->hvwg_off[m + 1] - w0;
+      for (int w = w0; w < w0 + nwm; ++w)
+        for (int q = 0; q < kBatchSlices; ++q) st[ns++] = make_int4(kBatchSlices * w + q, m, w0, nwm);
+    }
+  int4* wst = st + c->nslices;
   for (int m = 0; m < b->nmesh; ++m)
-    if (active[m])
-      for (int w = b->hvwg_off[m]; w < b->hvwg_off[m + 1]; ++w) wst[nw++] = w;
-  if (ns) HIPCHK(hipMemcpyAsync(b->slist[buf].p, st, sizeof(int) * ns, hipMemcpyHostToDevice, c->stream));
-  if (nw) HIPCHK(hipMemcpyAsync(b->wlist[buf].p, wst, sizeof(int) * nw, hipMemcpyHostToDevice, c->stream));
+    if (active[m]) {
+      const int w0 = b->hvwg_off[m], nwm = b->hvwg_off[m + 1] - w0;
+      for (int w = w0; w < w0 + nwm; ++w) wst[nw++] = make_int4(w, m, w0, nwm);
+    }
+  if (ns) HIPCHK(hipMemcpyAsync(b->slist[buf].p, st, sizeof(int4) * ns, hipMemcpyHostToDevice, c->stream));
+  if (nw) HIPCHK(hipMemcpyAsync(b->wlist[buf].p, wst, sizeof(int4) * nw, hipMemcpyHostToDevice, c->stream));
   b->ns_list[buf] = ns;
   b->nw_list[buf] = nw;
 }
@@ -215,7 +219,7 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
   sync(c);
   int cur = 0;
   build_lists(b, active, cur);
-  auto launch_iters = [&](int count, const int* sl, int ns, const int* wl, int nw, hipStream_t st) {
+  auto launch_iters = [&](int count, const int4* sl, int ns, const int4* wl, int nw, hipStream_t st) {
     for (int j = 0; j < count; ++j) {
       launch_bpcg_spmv(A, bt, sl, ns, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), b->pcg.p, b->iter.p,
                        c->part_pa(), st);
@@ -249,8 +253,8 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
     return exec;
   };
   auto enqueue = [&](int count, int slot) {
-    const int* sl = b->slist[cur].p;
-    const int* wl = b->wlist[cur].p;
+    const int4* sl = b->slist[cur].p;
+    const int4* wl = b->wlist[cur].p;
     const int ns = b->ns_list[cur], nw = b->nw_list[cur];
     // the chunk's first iteration is launched on its own (it carries the sampled kernel timers)
     timed(c, SSIM_KERNEL_PCG_SPMV, [&] {
@@ -569,7 +573,7 @@ ssim_status_t ssim_batch_create(ssim_batch** out, int nmesh, const int* nv, cons
       b->slist[q].alloc(std::max(c->nslices, 1));
       b->wlist[q].alloc(std::max(b->nvwg, 1));
       HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->pin_list[q]),
-                           sizeof(int) * ((size_t)c->nslices + b->nvwg + 1), hipHostMallocDefault));
+                           sizeof(int4) * ((size_t)c->nslices + b->nvwg + 1), hipHostMallocDefault));
     }
   });
   if (st != SSIM_OK) {

@@ -25,6 +25,50 @@ __device__ __forceinline__ size_t bdof_w(int w, int j) {
   return (size_t)w * (kBVec * kBatchRows) + (size_t)j * kBatchRows + threadIdx.x;
 }
 
+// The totals of two width-2 partial arrays over the same `count` entries, with the same additions as two
+// sum_partials<2, NT, 1> calls (each lane adds its entries in index order, then the workgroup sum), but with a
+// trip's loads from both arrays in flight together: one memory round instead of two.
+template <int NT>
+__device__ __forceinline__ void sum_partials_pair(const double* __restrict__ a, const double* __restrict__ b,
+                                                  int count, double (&ta)[2], double (&tb)[2]) {
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < count; i += NT) {
+    const double a0 = a[2 * (size_t)i], a1 = a[2 * (size_t)i + 1];
+    const double b0 = b[2 * (size_t)i], b1 = b[2 * (size_t)i + 1];
+    t[0] += a0;
+    t[1] += a1;
+    t[2] += b0;
+    t[3] += b1;
+  }
+  block_sum<4, NT>(t);
+  ta[0] = t[0];
+  ta[1] = t[1];
+  tb[0] = t[2];
+  tb[1] = t[3];
+}
+
+// The total of `na` width-1 partials `a` and of the first nb ≤ na width-2 partials `b`, with the same additions as
+// sum_partials<1, kBlock, 1>(a, na) and sum_partials<2, kBlock, 1>(b, nb), in one loop and one workgroup sum.
+__device__ __forceinline__ void sum_partials_1_2(const double* __restrict__ a, int na, const double* __restrict__ b,
+                                                 int nb, double (&ta)[1], double (&tb)[2]) {
+  double t[3] = {0.0, 0.0, 0.0};
+  for (int i = threadIdx.x; i < na; i += kBlock) {
+    const double av = a[i];
+    const bool in = i < nb;
+    const size_t j = 2 * (size_t)(in ? i : 0);
+    const double b0 = b[j], b1 = b[j + 1];
+    t[0] += av;
+    if (in) {
+      t[1] += b0;
+      t[2] += b1;
+    }
+  }
+  block_sum<3, kBlock>(t);
+  ta[0] = t[0];
+  tb[0] = t[1];
+  tb[1] = t[2];
+}
+
 // ------------------------------------------------------------------------------------------------------
 // energy + gradient
 
@@ -298,35 +342,39 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_init_fin(BatchView bt, const do
 
 // kernel A: convergence test on the mesh's ‖r_k‖² (a few dozen partials, so it goes first and a converged
 // mesh skips its SpMV), β, s = A z_k on one 64-row slice, p_k, q_k, p_k·q_k.  The grid covers the slices in
-// slist[0..nlist) (all slices of the meshes still iterating when the host last compacted the list).
+// slist[0..nlist) (all slices of the meshes still iterating when the host last compacted the list).  A list entry
+// names the slice's mesh and the range of the mesh's partial sums, so the slice's index words, the mesh's scalars
+// and both partial sums go out in one round of loads after it.
 template <int LPR>
 __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_bpcg_spmv(
-    BellView A, BatchView bt, const int* __restrict__ slist, int nlist, const double* __restrict__ z,
+    BellView A, BatchView bt, const int4* __restrict__ slist, int nlist, const double* __restrict__ z,
     double* __restrict__ p, double* __restrict__ q, const double* __restrict__ pb0, const double* __restrict__ pb1,
     PcgScalars* sc, PcgIter* it, double* __restrict__ pa) {
   const int k = it->ka;
   if (blockIdx.x == 0 && threadIdx.x == 0) it->kb = k;
   const int li = logical_block(nlist, A.xcd_remap);
   if (li < 0) return;
-  const int lb = slist[li];
+  const int4 le = slist[li];  // slice, mesh, the mesh's first vertex workgroup, its vertex workgroups
+  const int lb = le.x, v0 = le.z, nw = le.w;
   const int v = lb * kSlice + threadIdx.x;
-  // the fixed layout's index words go out with the mesh's scalars and partial sums, which they do not depend on
   FixedRowIndex ix{};
   if (LPR == 0) ix = bell_fixed_index(A, v);
-  const int vw = lb / kBatchSlices;
-  const int mesh = bt.vwg_mesh[vw];
-  PcgScalars* S = sc + mesh;
-  if (S->done) return;
-  double beta = 0.0;
+  PcgScalars* S = sc + le.y;
+  const int done_before = S->done, maxit = S->maxit;
+  const double tol2 = S->tol2;
+  double t_cur[2] = {0.0, 0.0}, t_prev[2] = {0.0, 0.0};
   if (k > 0) {
     const double* pb_cur = (k & 1) ? pb1 : pb0;
     const double* pb_prev = (k & 1) ? pb0 : pb1;
-    const int v0 = bt.vwg_off[mesh], nw = bt.vwg_off[mesh + 1] - v0;
-    double t_cur[2], t_prev[2];
-    sum_partials<2, kSpmvBlock, 1>(pb_cur + 2 * (size_t)v0, nw, t_cur);
-    sum_partials<2, kSpmvBlock, 1>(pb_prev + 2 * (size_t)v0, nw, t_prev);
+    sum_partials_pair<kSpmvBlock>(pb_cur + 2 * (size_t)v0, pb_prev + 2 * (size_t)v0, nw, t_cur, t_prev);
+  }
+  // the empty asm statement takes the scalars and sums as inputs: the compiler keeps their loads above the test
+  asm volatile("" ::"s"(done_before), "s"(maxit), "s"(tol2), "v"(t_cur[0]), "v"(t_cur[1]), "v"(t_prev[0]));
+  if (done_before) return;
+  double beta = 0.0;
+  if (k > 0) {
     const bool nan = !(t_cur[1] == t_cur[1]);
-    const bool done = t_cur[1] <= S->tol2 || k >= S->maxit || nan;
+    const bool done = t_cur[1] <= tol2 || k >= maxit || nan;
     if (lb == v0 * kBatchSlices && threadIdx.x == 0) {  // the mesh's first slice reports
       S->iters = k;
       S->rr = t_cur[1];
@@ -373,7 +421,7 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
 
 // kernel B: α = ρ_k / (p_k·q_k); x += α p, r −= α q, z = D⁻¹ r; partials of (ρ_{k+1}, ‖r_{k+1}‖²).  Workgroup
 // i handles vertex workgroup wlist[i].
-__global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const int* __restrict__ wlist,
+__global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const int4* __restrict__ wlist,
                                                         const double* __restrict__ dinv, double* __restrict__ x,
                                                         double* __restrict__ r, double* __restrict__ z,
                                                         const double* __restrict__ p, const double* __restrict__ q,
@@ -381,10 +429,12 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const int*
                                                         double* __restrict__ pb1, PcgScalars* sc, PcgIter* it) {
   const int k = it->kb;
   if (blockIdx.x == 0 && threadIdx.x == 0) it->ka = k + 1;
-  const int w = wlist[blockIdx.x];
-  const int mesh = bt.vwg_mesh[w];
-  PcgScalars* S = sc + mesh;
-  if (S->done) return;
+  // the list entry names the mesh and the range of its partial sums: the streaming loads, the done flag and the
+  // partial sums then go out in one round (a workgroup of a mesh that finished since the last compaction reads
+  // its share of the vectors for nothing)
+  const int4 le = wlist[blockIdx.x];  // vertex workgroup, mesh, the mesh's first vertex workgroup, its count
+  const int w = le.x, v0 = le.z, nw = le.w;
+  PcgScalars* S = sc + le.y;
   double xv[kBVec], rv[kBVec], pv[kBVec], qv[kBVec], dv[kBVec];
 #pragma unroll
   for (int j = 0; j < kBVec; ++j) {
@@ -395,12 +445,12 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const int*
     qv[j] = q[i];
     dv[j] = dinv[i];
   }
+  const int done_before = S->done;
   const double* pb_cur = (k & 1) ? pb1 : pb0;
   double* pb_next = (k & 1) ? pb0 : pb1;
-  const int v0 = bt.vwg_off[mesh], nw = bt.vwg_off[mesh + 1] - v0;
   double pq[1], rz[2];
-  sum_partials<1, kBlock, 1>(pa + (size_t)v0 * kBatchSlices, nw * kBatchSlices, pq);
-  sum_partials<2, kBlock, 1>(pb_cur + 2 * (size_t)v0, nw, rz);
+  sum_partials_1_2(pa + (size_t)v0 * kBatchSlices, nw * kBatchSlices, pb_cur + 2 * (size_t)v0, nw, pq, rz);
+  if (done_before) return;
   if (!(pq[0] > 0.0)) {
     if (w == v0 && threadIdx.x == 0) {
       S->breakdown = 1;
@@ -547,7 +597,7 @@ void launch_bpcg_init(const BatchView& bt, int nvwg, const double* b, double sig
   if (bt.nmesh > 0) k_bpcg_init_fin<<<bt.nmesh, kBlock, 0, s>>>(bt, pb0, rtol, maxit, sc, it);
 }
 
-void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const int* slist, int nlist, const double* z, double* p,
+void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const int4* slist, int nlist, const double* z, double* p,
                       double* q, const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa,
                       hipStream_t s) {
   if (nlist <= 0) return;
@@ -558,7 +608,7 @@ void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const int* slist, 
     k_bpcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, bt, slist, nlist, z, p, q, pb0, pb1, sc, it, pa);
 }
 
-void launch_bpcg_update(const BatchView& bt, const int* wlist, int nwl, const double* dinv, double* x, double* r,
+void launch_bpcg_update(const BatchView& bt, const int4* wlist, int nwl, const double* dinv, double* x, double* r,
                         double* z, const double* p, const double* q, const double* pa, double* pb0, double* pb1,
                         PcgScalars* sc, PcgIter* it, hipStream_t s) {
   if (nwl <= 0) return;

@@ -277,11 +277,13 @@ void launch_bpcg_init(const BatchView& bt, int nvwg, const double* b, double sig
                       double* r, double* z, double rtol, int maxit, PcgScalars* sc, PcgIter* it, double* pb0,
                       hipStream_t s);
 // iteration kernels over compacted grids: slist[0..nlist) = the 64-row slices, wlist[0..nwl) = the vertex
-// workgroups of the meshes still iterating (a superset is correct: a done mesh's workgroups return at entry)
-void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const int* slist, int nlist, const double* z, double* p,
+// workgroups of the meshes still iterating (a superset is correct: a done mesh's workgroups return at entry).
+// Each entry is {slice or vertex workgroup, mesh, the mesh's first vertex workgroup, its number of vertex
+// workgroups}, so a workgroup has its mesh and the range of the mesh's partial sums in its first load.
+void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const int4* slist, int nlist, const double* z, double* p,
                       double* q, const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa,
                       hipStream_t s);
-void launch_bpcg_update(const BatchView& bt, const int* wlist, int nwl, const double* dinv, double* x, double* r,
+void launch_bpcg_update(const BatchView& bt, const int4* wlist, int nwl, const double* dinv, double* x, double* r,
                         double* z, const double* p, const double* q, const double* pa, double* pb0, double* pb1,
                         PcgScalars* sc, PcgIter* it, hipStream_t s);
 
