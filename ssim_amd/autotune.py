@@ -9,7 +9,7 @@ variants sum in different orders, so results can differ in the last bits.
 The tuner snapshots the state, times one full time step per candidate from that same state (so every
 candidate does the same work), restores the state, and applies the fastest setting.  It searches one knob at a
 time, in the order above, keeping the best value of each before moving to the next (coordinate descent), and
-repeats each timing ``repeats`` times (median).  Results are cached in a JSON file keyed by the mesh layout, so a
+repeats each timing ``repeats`` times (median); a candidate must beat the incumbent by a margin (1 %) to replace it.  Results are cached in a JSON file keyed by the mesh layout, so a
 later process with the same mesh applies them without timing.
 """
 from __future__ import annotations
@@ -47,8 +47,11 @@ def signature(obj) -> str:
     return f"{kind}:nv={info['nv']}:nt={info['nt']}:blocks={info['nonzero_blocks']}:slices={info['slices']}"
 
 
-def coordinate_search(space: dict, measure, start: dict | None = None):
-    """Minimises measure(setting) one knob at a time.  Returns (best setting, {setting repr: time})."""
+def coordinate_search(space: dict, measure, start: dict | None = None, margin: float = 0.0):
+    """Minimises measure(setting) one knob at a time.  Returns (best setting, {setting repr: time}).
+
+    A candidate replaces the incumbent only if it is faster by more than the fraction `margin`, so that timing
+    noise does not move a knob off its first (expected best) value."""
     best = dict(start or {k: v[0] for k, v in space.items()})
     seen = {}
 
@@ -65,7 +68,7 @@ def coordinate_search(space: dict, measure, start: dict | None = None):
                 continue
             trial = dict(best, **{knob: v})
             t = timed(trial)
-            if t < best_t:
+            if t < best_t * (1.0 - margin):
                 best, best_t = trial, t
     return best, seen
 
@@ -78,8 +81,11 @@ def _valid(obj, setting):
         return False
 
 
-def tune(obj, repeats: int = 2, cache: str | None = DEFAULT_CACHE, use_cache: bool = True, verbose: bool = False):
-    """Tunes a TetFiniteElementSolver_Hip or TetFiniteElementBatch_Hip in place and returns the chosen setting."""
+def tune(obj, repeats: int = 2, cache: str | None = DEFAULT_CACHE, use_cache: bool = True, verbose: bool = False,
+         margin: float = 0.01):
+    """Tunes a TetFiniteElementSolver_Hip or TetFiniteElementBatch_Hip in place and returns the chosen setting.
+    A knob leaves its current value only for a candidate faster by more than `margin` (1 %: one time step varies
+    by about that much from run to run)."""
     import torch
 
     key = signature(obj)
@@ -113,7 +119,7 @@ def tune(obj, repeats: int = 2, cache: str | None = DEFAULT_CACHE, use_cache: bo
         return t
 
     start = {k: original.get(k, v[0]) for k, v in space.items()}
-    best, seen = coordinate_search(space, measure, start)
+    best, seen = coordinate_search(space, measure, start, margin=margin)
     restore()
     obj.set_tuning(**best)
     if cache:

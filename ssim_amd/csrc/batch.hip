@@ -519,6 +519,8 @@ ssim_status_t ssim_batch_create(ssim_batch** out, int nmesh, const int* nv, cons
     }
     b->nvwg = b->hvwg_off[nmesh];
     b->ntwg = b->htwg_off[nmesh];
+    c->mesh_rows.assign(b->vbase.begin(), b->vbase.end());
+    c->mesh_rows.push_back((int)pv);
     c->nv = (int)pv;
     c->nt = (int)pt;
     c->n = 3 * c->nv;

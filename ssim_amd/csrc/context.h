@@ -95,6 +95,9 @@ struct ssim_ctx {
   int spmv_lanes = 1;
   bool spmv_lanes_default = true;  // pick the variant from the layout at reset
   bool fixed_layout = false;       // the fixed layout: 7 + 7 off-diagonal slots, 6-value diagonals, 16-bit indices
+  // first rows of the independent meshes whose slot directions are chosen separately (a batch's meshes), then nv;
+  // empty: one mesh
+  std::vector<int> mesh_rows;
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
 
   // device: mesh and rest data

@@ -18,8 +18,8 @@ namespace {
 // Blocked-ELL slots of every block.  By default the upper blocks of a row take slots 0, 1, ... in column order
 // (diagonal first) and the lower references likewise.  In the fixed 8/7 layout the slots are aligned by
 // direction instead: slot k ≥ 1 holds, where it can, the block at the column offset w − v that sits at sorted
-// position k in most full rows (on a structured mesh every row then keeps each neighbour direction in the same
-// slot), and the lower reference to the block stored in slot k of the neighbour's row goes to lower slot 7 − k.
+// position k in most full rows of its mesh (on a structured mesh every row then keeps each neighbour direction in
+// the same slot), and the lower reference to the block stored in slot k of the neighbour's row goes to lower slot 7 − k.
 // The fixed-width SpMV pairs upper slot k with lower slot 7 − k, so that for most rows of a slice the lower
 // reads and the neighbouring slice's own reads of the same lines are issued at the same point of the program
 // and meet in L2.  Slots only move blocks; every block is still read and summed once per row.
@@ -32,49 +32,54 @@ void assign_slots(ssim_ctx* c, const std::vector<int>& len) {
   }
   if (!c->fixed_layout) return;
   static_assert(kFixedWL == kFixedWU - 1, "the slot pairing needs one lower slot per off-diagonal upper slot");
-  // modal column offset of each sorted upper position over the rows with all kFixedWU upper blocks
-  std::vector<std::vector<int>> offs(kFixedWU);
-  for (int v = 0; v < nv; ++v)
-    if (len[v] - c->kdiag[v] == kFixedWU)
-      for (int q = 1; q < kFixedWU; ++q) offs[q].push_back(c->bcol[c->brow[v] + c->kdiag[v] + q] - v);
-  std::vector<long long> modal(kFixedWU, -1);  // −1: no canonical offset (offsets are positive)
-  for (int q = 1; q < kFixedWU; ++q) {
-    auto& o = offs[q];
-    if (o.empty()) continue;
-    std::sort(o.begin(), o.end());
-    int best = o[0], run = 0, best_run = 0;
-    for (size_t i = 0; i < o.size(); ++i) {
-      run = (i > 0 && o[i] == o[i - 1]) ? run + 1 : 1;
-      if (run > best_run) {
-        best_run = run;
-        best = o[i];
+  // Each mesh of a batch picks its own directions: the column offsets of a structured mesh's neighbours depend on
+  // its cross-section, so one choice over a whole batch would align the slots of a few of its meshes only.
+  std::vector<int> seg = c->mesh_rows;
+  if (seg.size() < 2 || seg.front() != 0 || seg.back() != nv) seg = {0, nv};
+  for (size_t g = 0; g + 1 < seg.size(); ++g) {
+    const int r0 = seg[g], r1 = seg[g + 1];
+    // modal column offset of each sorted upper position over the range's rows with all kFixedWU upper blocks
+    std::vector<std::vector<int>> offs(kFixedWU);
+    for (int v = r0; v < r1; ++v)
+      if (len[v] - c->kdiag[v] == kFixedWU)
+        for (int q = 1; q < kFixedWU; ++q) offs[q].push_back(c->bcol[c->brow[v] + c->kdiag[v] + q] - v);
+    std::vector<long long> modal(kFixedWU, -1);  // −1: no canonical offset (offsets are positive)
+    for (int q = 1; q < kFixedWU; ++q) {
+      auto& o = offs[q];
+      if (o.empty()) continue;
+      std::sort(o.begin(), o.end());
+      int best = o[0], run = 0, best_run = 0;
+      for (size_t i = 0; i < o.size(); ++i) {
+        run = (i > 0 && o[i] == o[i - 1]) ? run + 1 : 1;
+        if (run > best_run) {
+          best_run = run;
+          best = o[i];
+        }
       }
+      modal[q] = best;
     }
-    modal[q] = best;
-    o.clear();
-    o.shrink_to_fit();
-  }
-  // upper slots: the diagonal in slot 0, canonical directions in their slot, the rest in the free slots
-  for (int v = 0; v < nv; ++v) {
-    const int kd = c->kdiag[v];
-    bool used[kFixedWU] = {true};
-    int rest[kFixedWU], nrest = 0;
-    c->bslot[c->brow[v] + kd] = 0;
-    for (int k = kd + 1; k < len[v]; ++k) {
-      const long long off = c->bcol[c->brow[v] + k] - v;
-      int q = 1;
-      while (q < kFixedWU && !(modal[q] == off && !used[q])) ++q;
-      if (q < kFixedWU) {
-        c->bslot[c->brow[v] + k] = q;
+    // upper slots: the diagonal in slot 0, canonical directions in their slot, the rest in the free slots
+    for (int v = r0; v < r1; ++v) {
+      const int kd = c->kdiag[v];
+      bool used[kFixedWU] = {true};
+      int rest[kFixedWU], nrest = 0;
+      c->bslot[c->brow[v] + kd] = 0;
+      for (int k = kd + 1; k < len[v]; ++k) {
+        const long long off = c->bcol[c->brow[v] + k] - v;
+        int q = 1;
+        while (q < kFixedWU && !(modal[q] == off && !used[q])) ++q;
+        if (q < kFixedWU) {
+          c->bslot[c->brow[v] + k] = q;
+          used[q] = true;
+        } else {
+          rest[nrest++] = k;
+        }
+      }
+      for (int i = 0, q = 1; i < nrest; ++i) {
+        while (used[q]) ++q;
+        c->bslot[c->brow[v] + rest[i]] = q;
         used[q] = true;
-      } else {
-        rest[nrest++] = k;
       }
-    }
-    for (int i = 0, q = 1; i < nrest; ++i) {
-      while (used[q]) ++q;
-      c->bslot[c->brow[v] + rest[i]] = q;
-      used[q] = true;
     }
   }
   // lower references: the block stored in slot k of row w goes to lower slot kFixedWU − 1 − k where free

@@ -22,6 +22,16 @@ def test_coordinate_search_finds_the_best_value_of_each_knob():
     assert len(calls) <= 1 + sum(len(v) - 1 for v in space.values())
 
 
+def test_a_knob_moves_only_for_a_gain_beyond_the_margin():
+    space = {"x": [1, 0], "y": [0, 1]}
+    cost = {(1, 0): 1.000, (0, 0): 0.995, (1, 1): 0.97}  # x = 0 is 0.5 % faster: within the margin
+
+    best, _ = autotune.coordinate_search(space, lambda s: cost.get((s["x"], s["y"]), 2.0), margin=0.01)
+    assert best == {"x": 1, "y": 1}
+    best, _ = autotune.coordinate_search(space, lambda s: cost.get((s["x"], s["y"]), 2.0))
+    assert best == {"x": 0, "y": 0}  # without a margin any gain moves the knob (then y = 1 is not tried with x = 0)
+
+
 def test_infeasible_settings_never_win():
     space = {"x": [0, 1, 2]}
     best, _ = autotune.coordinate_search(space, lambda s: float("inf") if s["x"] == 2 else 1.0 + s["x"])
