@@ -221,9 +221,9 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
   build_lists(b, active, cur);
   auto launch_iters = [&](int count, const int4* sl, int ns, const int4* wl, int nw, hipStream_t st) {
     for (int j = 0; j < count; ++j) {
-      launch_bpcg_spmv(A, bt, sl, ns, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), b->pcg.p, b->iter.p,
+      launch_bpcg_spmv(A, sl, ns, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), b->pcg.p, b->iter.p,
                        c->part_pa(), st);
-      launch_bpcg_update(bt, wl, nw, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), c->part_pb(0),
+      launch_bpcg_update(wl, nw, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), c->part_pb(0),
                          c->part_pb(1), b->pcg.p, b->iter.p, st);
     }
   };
@@ -258,11 +258,11 @@ void bpcg(ssim_batch* b, const DBuf<int>& mask, const double* rhs, double sign, 
     const int ns = b->ns_list[cur], nw = b->nw_list[cur];
     // the chunk's first iteration is launched on its own (it carries the sampled kernel timers)
     timed(c, SSIM_KERNEL_PCG_SPMV, [&] {
-      launch_bpcg_spmv(A, bt, sl, ns, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), b->pcg.p, b->iter.p,
+      launch_bpcg_spmv(A, sl, ns, c->z.p, c->p.p, c->q.p, c->part_pb(0), c->part_pb(1), b->pcg.p, b->iter.p,
                        c->part_pa(), s);
     });
     timed(c, SSIM_KERNEL_PCG_UPDATE, [&] {
-      launch_bpcg_update(bt, wl, nw, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), c->part_pb(0),
+      launch_bpcg_update(wl, nw, c->dinv.p, x, c->r.p, c->z.p, c->p.p, c->q.p, c->part_pa(), c->part_pb(0),
                          c->part_pb(1), b->pcg.p, b->iter.p, s);
     });
     if (count > 1) {

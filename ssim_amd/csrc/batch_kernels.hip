@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_bpcg_init_fin(BatchView bt, const do
 // and both partial sums go out in one round of loads after it.
 template <int LPR>
 __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_bpcg_spmv(
-    BellView A, BatchView bt, const int4* __restrict__ slist, int nlist, const double* __restrict__ z,
+    BellView A, const int4* __restrict__ slist, int nlist, const double* __restrict__ z,
     double* __restrict__ p, double* __restrict__ q, const double* __restrict__ pb0, const double* __restrict__ pb1,
     PcgScalars* sc, PcgIter* it, double* __restrict__ pa) {
   const int k = it->ka;
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(kSpmvBlock) __attribute__((amdgpu_waves_per_eu(3, 3
 
 // kernel B: α = ρ_k / (p_k·q_k); x += α p, r −= α q, z = D⁻¹ r; partials of (ρ_{k+1}, ‖r_{k+1}‖²).  Workgroup
 // i handles vertex workgroup wlist[i].
-__global__ __launch_bounds__(kBlock) void k_bpcg_update(BatchView bt, const int4* __restrict__ wlist,
+__global__ __launch_bounds__(kBlock) void k_bpcg_update(const int4* __restrict__ wlist,
                                                         const double* __restrict__ dinv, double* __restrict__ x,
                                                         double* __restrict__ r, double* __restrict__ z,
                                                         const double* __restrict__ p, const double* __restrict__ q,
@@ -597,22 +597,22 @@ void launch_bpcg_init(const BatchView& bt, int nvwg, const double* b, double sig
   if (bt.nmesh > 0) k_bpcg_init_fin<<<bt.nmesh, kBlock, 0, s>>>(bt, pb0, rtol, maxit, sc, it);
 }
 
-void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const int4* slist, int nlist, const double* z, double* p,
+void launch_bpcg_spmv(const BellView& A, const int4* slist, int nlist, const double* z, double* p,
                       double* q, const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa,
                       hipStream_t s) {
   if (nlist <= 0) return;
   const int grid = grid_for(nlist, A.xcd_remap);
   if (A.lanes_per_row == 0)
-    k_bpcg_spmv<0><<<grid, kSpmvBlock, 0, s>>>(A, bt, slist, nlist, z, p, q, pb0, pb1, sc, it, pa);
+    k_bpcg_spmv<0><<<grid, kSpmvBlock, 0, s>>>(A, slist, nlist, z, p, q, pb0, pb1, sc, it, pa);
   else
-    k_bpcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, bt, slist, nlist, z, p, q, pb0, pb1, sc, it, pa);
+    k_bpcg_spmv<1><<<grid, kSpmvBlock, 0, s>>>(A, slist, nlist, z, p, q, pb0, pb1, sc, it, pa);
 }
 
-void launch_bpcg_update(const BatchView& bt, const int4* wlist, int nwl, const double* dinv, double* x, double* r,
+void launch_bpcg_update(const int4* wlist, int nwl, const double* dinv, double* x, double* r,
                         double* z, const double* p, const double* q, const double* pa, double* pb0, double* pb1,
                         PcgScalars* sc, PcgIter* it, hipStream_t s) {
   if (nwl <= 0) return;
-  k_bpcg_update<<<nwl, kBlock, 0, s>>>(bt, wlist, dinv, x, r, z, p, q, pa, pb0, pb1, sc, it);
+  k_bpcg_update<<<nwl, kBlock, 0, s>>>(wlist, dinv, x, r, z, p, q, pa, pb0, pb1, sc, it);
 }
 
 }  // namespace ssim_hip

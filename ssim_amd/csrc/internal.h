@@ -280,10 +280,10 @@ void launch_bpcg_init(const BatchView& bt, int nvwg, const double* b, double sig
 // workgroups of the meshes still iterating (a superset is correct: a done mesh's workgroups return at entry).
 // Each entry is {slice or vertex workgroup, mesh, the mesh's first vertex workgroup, its number of vertex
 // workgroups}, so a workgroup has its mesh and the range of the mesh's partial sums in its first load.
-void launch_bpcg_spmv(const BellView& A, const BatchView& bt, const int4* slist, int nlist, const double* z, double* p,
+void launch_bpcg_spmv(const BellView& A, const int4* slist, int nlist, const double* z, double* p,
                       double* q, const double* pb0, const double* pb1, PcgScalars* sc, PcgIter* it, double* pa,
                       hipStream_t s);
-void launch_bpcg_update(const BatchView& bt, const int4* wlist, int nwl, const double* dinv, double* x, double* r,
+void launch_bpcg_update(const int4* wlist, int nwl, const double* dinv, double* x, double* r,
                         double* z, const double* p, const double* q, const double* pa, double* pb0, double* pb1,
                         PcgScalars* sc, PcgIter* it, hipStream_t s);
 
